@@ -1,0 +1,135 @@
+/*
+ * wema_dev.h - C ABI of the MI355X (gfx950) WEMA per-frame amplification pipeline.
+ *
+ * This is the drop-in boundary for the path that curioustorvald/wema drives through
+ * wema_init (and the _color/_iir variants), wema_process_frame and wema_batch_process (src/wema.h:253-283). One handle is one
+ * stream session: its DWT pyramids, IIR state and sliding-window ring live in HBM for the
+ * life of the handle. Frames cross the boundary as RGB24 bytes (or float RGB in [0,1] for
+ * the reference's Frame API); every entry point takes plain pointers and sizes.
+ *
+ * Return values follow the reference: 0 on success, -1 on error (wema.c:437-439); the text
+ * of the last error is available from wema_dev_last_error().
+ *
+ * Threading: a handle is used by one host thread at a time (as WemaContext, wema.h:102).
+ * Work is enqueued on the HIP stream given by the caller (NULL = the handle's own stream)
+ * and is stream-ordered; *_sync entry points wait for completion.
+ */
+#ifndef WEMA_DEV_H
+#define WEMA_DEV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wema_dev wema_dev_t;
+
+/* Options of one session. Mirrors WemaConfig (src/wema.h:192-212) and the fields main.c
+ * writes after wema_init (src/main.c:288-290). */
+typedef struct {
+    int   width, height;
+    float fps;
+    float amp;            /* amplification factor (-a, main.c:96-105) */
+    float f_low, f_high;  /* band edges in Hz (--fl/--fh); IIR mode */
+    int   window;         /* sliding window size (--temporal-window); window mode */
+    int   levels;         /* DWT levels; 0 = the reference's automatic rule (dwt.c:588-596) */
+    int   use_iir;        /* 1 = IIR band-pass (default), 0 = sliding window ("--haar") */
+    int   color;          /* --color: filter Y, Cb, Cr */
+    int   edge_aware;     /* guided filter (default) or 5x5 binomial (--no-edge-aware) */
+    int   bilateral;      /* --bilateral-filter (window mode only) */
+    int   batch_path;     /* 1: wema_batch_process semantics (guided filter always on,
+                             wema_batch.c:452) */
+    int   max_batch;      /* largest number of frames per process call (1..256) */
+    int   device;         /* HIP device ordinal */
+} wema_dev_params;
+
+/* Fills p with the reference CLI defaults (main.c:63-75) for a width x height, fps clip. */
+void wema_dev_params_default(wema_dev_params *p, int width, int height, float fps);
+
+/* Device bytes a session with these options holds (HBM admission; no allocation). */
+int wema_dev_query_bytes(const wema_dev_params *p, size_t *bytes);
+
+/* Replaces wema_init + wema_init_color + wema_init_iir + wema_init_iir_color
+ * (wema.c:359-589): validates the options, designs the band-pass exactly as
+ * iir_design_bandpass does (iir_filter.c:70-101), allocates all device state. */
+int  wema_dev_open(wema_dev_t **out, const wema_dev_params *p);
+void wema_dev_close(wema_dev_t *s);                      /* wema_free (wema.c:591) */
+const char *wema_dev_last_error(void);
+
+/* Number of DWT levels, unique coefficient positions (3 per subband position) and the
+ * reference's position count N6 (dwt_num_positions, dwt.c:769-778). */
+int wema_dev_geometry(const wema_dev_t *s, int *levels, size_t *n_unique, size_t *n6);
+/* IIR warm-up iterations (iir_filter.c:150-151) and the 10 section coefficients
+ * (HP b0 b1 b2 a1 a2, LP b0 b1 b2 a1 a2). */
+int wema_dev_iir_coeffs(const wema_dev_t *s, float *c10, int *warmup);
+/* Frames processed so far by this session. */
+long wema_dev_frames(const wema_dev_t *s);
+
+/* Process n consecutive frames of the session (n <= max_batch), device buffers, RGB24
+ * HWC. Output frame i corresponds to input frame i, as wema_process_frame (wema.c:746)
+ * called n times, or wema_batch_process (wema_batch.c:352) when batch_path is set. */
+int wema_dev_process_u8(wema_dev_t *s, const uint8_t *d_in, uint8_t *d_out, int n,
+                        void *hip_stream);
+/* Same on float RGB in [0,1] (the reference's Frame data, wema.h:28-33). d_out_u8 and
+ * d_out_f32 may each be NULL. */
+int wema_dev_process_f32(wema_dev_t *s, const float *d_in, uint8_t *d_out_u8,
+                         float *d_out_f32, int n, void *hip_stream);
+/* Host-buffer variant (one frame): copies in, processes, copies out, waits. */
+int wema_dev_process_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out);
+int wema_dev_sync(wema_dev_t *s);
+
+/* Pinned host memory for frame slots (hipHostMalloc). */
+void *wema_dev_host_alloc(size_t bytes);
+void  wema_dev_host_free(void *p);
+
+/* ---- test hooks (no counterpart in the reference) --------------------------------
+ * Thresholds used by the last processed frames: thr[f * 3 + ch] (NaN for unused
+ * channels / passthrough frames). Call after wema_dev_sync. */
+int wema_dev_debug_thresholds(wema_dev_t *s, float *thr, int n);
+/* Override the thresholds of the next process call: thr[f * 3 + ch]; NULL clears. */
+int wema_dev_debug_set_thresholds(wema_dev_t *s, const float *thr, int n);
+/* Copy a device buffer of frame f (of the last call) to host:
+ * which 0: original coefficients (unique planes, n_unique floats),
+ *       1: temporal filter output (IIR y / window delta), 2: amplified coefficients,
+ *       3: IIR state (4 * n_unique floats, per position HP w1, HP w2, LP w1, LP w2). */
+int wema_dev_debug_read(wema_dev_t *s, int which, int frame, int ch, float *host, size_t count);
+
+/* ---- synthetic video (deterministic, integer-only; identical on host and device) ---- */
+typedef struct {
+    int      width, height;
+    float    fps;
+    uint32_t seed;
+    int      kind;               /* 0 texture + discs, 1 checkerboard + shear */
+    int      color;              /* independent noise per channel + disc tint */
+    int      ndiscs;             /* <= 4 */
+    float    disc_cx[4], disc_cy[4];     /* centre, fraction of width / height */
+    float    disc_amp[4], disc_freq[4];  /* px, Hz */
+    float    disc_radius;        /* px */
+    float    disc_contrast;      /* levels */
+    float    tint_r, tint_b;     /* extra disc levels on R and B when color */
+    float    noise_sigma;        /* levels */
+    float    shear_amp, shear_freq;       /* px, Hz */
+    int      cell;               /* checkerboard cell size (px) */
+    float    checker_contrast;   /* levels */
+} wema_synth_params;
+
+/* Presets for the workloads: 1 = 320x240@30 disc, 2 = 1920x1080@30 three discs (colour
+ * tint), 3 = 3840x2160@60 checkerboard shear, 4 = disc scene at any size (multi-stream).
+ * seed_offset is added to the base seed 0x57454D41. */
+int  wema_synth_preset(wema_synth_params *p, int preset, int width, int height, float fps,
+                       uint32_t seed_offset);
+/* Static texture of the scene (Q8 levels, width*height uint16). */
+int  wema_synth_texture_host(const wema_synth_params *p, uint16_t *tex);
+/* Frame `frame` on the host (RGB24). */
+int  wema_synth_frame_host(const wema_synth_params *p, const uint16_t *tex, int frame,
+                           uint8_t *rgb);
+/* Frames frame0..frame0+n-1 on the device (RGB24, consecutive), from a device texture. */
+int  wema_synth_frames_device(const wema_synth_params *p, const uint16_t *d_tex, int frame0,
+                              int n, uint8_t *d_rgb, void *hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,337 @@
+"""wema_amd: MI355X (gfx950) implementation of WEMA's per-frame amplification pipeline.
+
+The compute path is libwema_dev.so (HIP kernels behind the C ABI in include/wema_dev.h).
+This module binds it with ctypes and mirrors the reference's C API (src/wema.h:253-283):
+
+    ctx = WemaContext(width, height, fps, amp, f_low, f_high, window)   # wema_init
+    ctx.color_mode = True; ctx.init_color(); ctx.init_iir()             # as main.c:288-322
+    out = ctx.process_frame(frame)                                      # wema_process_frame
+    outs = ctx.batch_process(frames)                                    # wema_batch_process
+
+Frames are numpy or torch arrays of shape (H, W, 3): uint8 RGB24, or float32 RGB in
+[0, 1] as the reference's Frame (src/wema.h:28-33). torch CUDA tensors stay on the device;
+host arrays are staged through the device. There is no CPU fallback: importing this
+module without the built library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwema_dev.so")
+
+
+class WemaError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "wema_amd: %s is not built (run `python -m wema_amd.build` or "
+            "__graft_entry__.build())" % LIB_PATH)
+    return C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+
+
+lib = _load()
+
+
+class Params(C.Structure):
+    """wema_dev_params (include/wema_dev.h)."""
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("fps", C.c_float), ("amp", C.c_float),
+        ("f_low", C.c_float), ("f_high", C.c_float), ("window", C.c_int), ("levels", C.c_int),
+        ("use_iir", C.c_int), ("color", C.c_int), ("edge_aware", C.c_int),
+        ("bilateral", C.c_int), ("batch_path", C.c_int), ("max_batch", C.c_int),
+        ("device", C.c_int),
+    ]
+
+
+class SynthParams(C.Structure):
+    """wema_synth_params (include/wema_dev.h)."""
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("fps", C.c_float), ("seed", C.c_uint32),
+        ("kind", C.c_int), ("color", C.c_int), ("ndiscs", C.c_int),
+        ("disc_cx", C.c_float * 4), ("disc_cy", C.c_float * 4),
+        ("disc_amp", C.c_float * 4), ("disc_freq", C.c_float * 4),
+        ("disc_radius", C.c_float), ("disc_contrast", C.c_float),
+        ("tint_r", C.c_float), ("tint_b", C.c_float), ("noise_sigma", C.c_float),
+        ("shear_amp", C.c_float), ("shear_freq", C.c_float), ("cell", C.c_int),
+        ("checker_contrast", C.c_float),
+    ]
+
+
+_vp = C.c_void_p
+_SIGS = {
+    "wema_dev_params_default": (None, [C.POINTER(Params), C.c_int, C.c_int, C.c_float]),
+    "wema_dev_query_bytes": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_size_t)]),
+    "wema_dev_open": (C.c_int, [C.POINTER(_vp), C.POINTER(Params)]),
+    "wema_dev_close": (None, [_vp]),
+    "wema_dev_last_error": (C.c_char_p, []),
+    "wema_dev_geometry": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "wema_dev_iir_coeffs": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
+    "wema_dev_frames": (C.c_long, [_vp]),
+    "wema_dev_process_u8": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "wema_dev_process_f32": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "wema_dev_process_host_u8": (C.c_int, [_vp, _vp, _vp]),
+    "wema_dev_sync": (C.c_int, [_vp]),
+    "wema_dev_host_alloc": (_vp, [C.c_size_t]),
+    "wema_dev_host_free": (None, [_vp]),
+    "wema_dev_debug_capture": (C.c_int, [_vp, C.c_int]),
+    "wema_dev_debug_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
+    "wema_dev_debug_set_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
+    "wema_dev_debug_read": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_size_t]),
+    "wema_synth_preset": (C.c_int, [C.POINTER(SynthParams), C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint32]),
+    "wema_synth_texture_host": (C.c_int, [C.POINTER(SynthParams), _vp]),
+    "wema_synth_frame_host": (C.c_int, [C.POINTER(SynthParams), _vp, C.c_int, _vp]),
+    "wema_synth_frames_device": (C.c_int, [C.POINTER(SynthParams), _vp, C.c_int, C.c_int, _vp, _vp]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def last_error():
+    msg = lib.wema_dev_last_error()
+    return msg.decode() if msg else ""
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise WemaError("%s: %s" % (what, last_error()))
+
+
+def _ptr(a):
+    """Address of a numpy array or torch tensor."""
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def _stream_handle():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def auto_levels(width, height):
+    """dwt_compute_levels (src/dwt.c:588-596)."""
+    m, lv = min(width, height), 0
+    while m >= 16 and lv < 4:
+        m //= 2
+        lv += 1
+    return max(lv, 1)
+
+
+class Session:
+    """One stream session of the device pipeline (a wema_dev_t handle)."""
+
+    def __init__(self, width, height, fps, amp=120.0, f_low=0.5, f_high=None, window=32,
+                 levels=0, use_iir=True, color=False, edge_aware=True, bilateral=False,
+                 batch_path=False, max_batch=1, device=0, capture=False):
+        p = Params()
+        lib.wema_dev_params_default(C.byref(p), int(width), int(height), float(fps))
+        p.amp = amp
+        p.f_low = f_low
+        if f_high is not None:
+            p.f_high = f_high
+        p.window, p.levels, p.use_iir, p.color = window, levels, int(use_iir), int(color)
+        p.edge_aware, p.bilateral, p.batch_path = int(edge_aware), int(bilateral), int(batch_path)
+        p.max_batch, p.device = max_batch, device
+        self.params = p
+        self.width, self.height = int(width), int(height)
+        h = C.c_void_p()
+        _check(lib.wema_dev_open(C.byref(h), C.byref(p)), "wema_dev_open")
+        self._h = h
+        if capture:
+            _check(lib.wema_dev_debug_capture(h, 1), "wema_dev_debug_capture")
+        lv, nu, n6 = C.c_int(), C.c_size_t(), C.c_size_t()
+        _check(lib.wema_dev_geometry(h, C.byref(lv), C.byref(nu), C.byref(n6)), "geometry")
+        self.levels, self.n_unique, self.n6 = lv.value, nu.value, n6.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.wema_dev_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def query_bytes(width, height, fps, **kw):
+        s = Session.__new__(Session)
+        p = Params()
+        lib.wema_dev_params_default(C.byref(p), int(width), int(height), float(fps))
+        for k, v in kw.items():
+            setattr(p, k, int(v) if isinstance(v, bool) else v)
+        n = C.c_size_t()
+        _check(lib.wema_dev_query_bytes(C.byref(p), C.byref(n)), "wema_dev_query_bytes")
+        del s
+        return n.value
+
+    @property
+    def frames(self):
+        return lib.wema_dev_frames(self._h)
+
+    def iir_coeffs(self):
+        c = (C.c_float * 10)()
+        w = C.c_int()
+        _check(lib.wema_dev_iir_coeffs(self._h, c, C.byref(w)), "iir_coeffs")
+        return np.array(list(c), np.float32), w.value
+
+    def process_device(self, d_in, d_out_u8=None, d_out_f32=None, n=1, stream=None):
+        """Enqueue n frames on device buffers (torch CUDA tensors or raw addresses)."""
+        st = stream if stream is not None else _stream_handle()
+        a = lambda t: 0 if t is None else (t if isinstance(t, int) else t.data_ptr())
+        if getattr(d_in, "dtype", None) is not None and "float" in str(d_in.dtype):
+            rc = lib.wema_dev_process_f32(self._h, a(d_in), a(d_out_u8), a(d_out_f32), n, st)
+        else:
+            rc = lib.wema_dev_process_u8(self._h, a(d_in), a(d_out_u8), n, st)
+        _check(rc, "wema_dev_process")
+
+    def sync(self):
+        _check(lib.wema_dev_sync(self._h), "wema_dev_sync")
+
+    def process_host_u8(self, frame):
+        frame = np.ascontiguousarray(frame, dtype=np.uint8)
+        out = np.empty_like(frame)
+        _check(lib.wema_dev_process_host_u8(self._h, _ptr(frame), _ptr(out)), "process_host_u8")
+        return out
+
+    # ---- test hooks
+    def thresholds(self, n):
+        t = np.zeros(3 * n, np.float32)
+        _check(lib.wema_dev_debug_thresholds(self._h, _ptr(t), n), "debug_thresholds")
+        return t.reshape(n, 3)
+
+    def set_thresholds(self, thr):
+        if thr is None:
+            _check(lib.wema_dev_debug_set_thresholds(self._h, None, 0), "set_thresholds")
+            return
+        t = np.ascontiguousarray(thr, np.float32).reshape(-1, 3)
+        _check(lib.wema_dev_debug_set_thresholds(self._h, _ptr(t), t.shape[0]), "set_thresholds")
+
+    def read(self, which, frame=0, ch=0):
+        n = self.n_unique * (4 if which == 3 else 1)
+        out = np.zeros(n, np.float32)
+        _check(lib.wema_dev_debug_read(self._h, which, frame, ch, _ptr(out), n), "debug_read")
+        return out
+
+
+# ---------------------------------------------------------------------------------------
+# synthetic video
+
+class Synth:
+    """Deterministic synthetic clip (see wema_amd/csrc/synth.h)."""
+
+    def __init__(self, preset, width, height, fps, seed_offset=0):
+        self.p = SynthParams()
+        _check(lib.wema_synth_preset(C.byref(self.p), preset, width, height, float(fps), seed_offset),
+               "wema_synth_preset")
+        self.width, self.height = width, height
+        self.tex = np.zeros(width * height, np.uint16)
+        _check(lib.wema_synth_texture_host(C.byref(self.p), _ptr(self.tex)), "texture")
+        self._dtex = None
+
+    def frame(self, t):
+        out = np.empty((self.height, self.width, 3), np.uint8)
+        _check(lib.wema_synth_frame_host(C.byref(self.p), _ptr(self.tex), t, _ptr(out)), "frame")
+        return out
+
+    def frames_device(self, t0, n, out=None):
+        import torch
+        if self._dtex is None:
+            self._dtex = torch.from_numpy(self.tex.view(np.int16)).cuda()
+        if out is None:
+            out = torch.empty((n, self.height, self.width, 3), dtype=torch.uint8, device="cuda")
+        _check(lib.wema_synth_frames_device(C.byref(self.p), self._dtex.data_ptr(), t0, n,
+                                            out.data_ptr(), _stream_handle()), "frames_device")
+        return out
+
+
+# ---------------------------------------------------------------------------------------
+# mirror of the reference C API (src/wema.h:253-283)
+
+class WemaContext:
+    """Same call sequence as the reference's WemaContext: wema_init, then the field writes
+    of main.c:288-290, then init_color / init_iir / init_iir_color, then process_frame or
+    batch_process. The device session is created lazily on the first frame, once all
+    options are known."""
+
+    def __init__(self, width, height, fps, amp_factor, f_low, f_high, temporal_window,
+                 levels=0, device=0):
+        self.width, self.height, self.fps = width, height, fps
+        self.amp_factor, self.f_low, self.f_high = amp_factor, f_low, f_high
+        self.temporal_window = temporal_window
+        self.num_levels = levels if levels > 0 else auto_levels(width, height)
+        self.edge_aware = True       # src/wema.c:371-373
+        self.bilateral_temp = True
+        self.color_mode = False
+        self.use_iir = False
+        self.device = device
+        self._s = None
+        self._batch = None
+
+    def init_color(self):
+        return 0
+
+    def init_iir(self):
+        self.use_iir = True
+        return 0
+
+    def init_iir_color(self):
+        return 0
+
+    def _session(self, batch_path=False, max_batch=1):
+        key = (batch_path, max_batch)
+        if self._s is None or self._key != key:
+            if self._s is not None:
+                raise WemaError("a context runs either the per-frame or the batch path")
+            self._s = Session(self.width, self.height, self.fps, self.amp_factor, self.f_low,
+                              self.f_high, self.temporal_window, self.num_levels, self.use_iir,
+                              self.color_mode, self.edge_aware, self.bilateral_temp,
+                              batch_path, max_batch, self.device)
+            self._key = key
+        return self._s
+
+    def _run(self, frames, batch_path, max_batch):
+        import torch
+        s = self._session(batch_path, max_batch)
+        x = frames if isinstance(frames, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(frames))
+        on_host = not x.is_cuda
+        x = x.cuda()
+        n = x.shape[0]
+        if x.dtype == torch.uint8:
+            out = torch.empty_like(x)
+            s.process_device(x, d_out_u8=out, n=n)
+        else:
+            x = x.float().contiguous()
+            out = torch.empty_like(x)
+            s.process_device(x, d_out_f32=out, n=n)
+        if on_host:
+            torch.cuda.current_stream().synchronize()
+            return out.cpu().numpy()
+        return out
+
+    def process_frame(self, frame):
+        """wema_process_frame (src/wema.c:746): one (H, W, 3) frame in, one out."""
+        return self._run(frame[None], False, 1)[0]
+
+    def batch_process(self, frames, batch_size=64):
+        """wema_batch_process (src/wema_batch.c:352): (n, H, W, 3) frames, n <= batch_size."""
+        if len(frames) > batch_size:
+            raise WemaError("frame_count exceeds batch size")
+        return self._run(frames, True, batch_size)
+
+    def flush(self):
+        """wema_flush (src/wema.c:954): no buffered frames remain."""
+        return -1
+
+    def close(self):
+        if self._s is not None:
+            self._s.close()
+            self._s = None

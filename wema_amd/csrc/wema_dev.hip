@@ -1,0 +1,673 @@
+// Host side of the C ABI (include/wema_dev.h): session state in HBM, per-call job tables,
+// stage launches, IIR design, synthetic frames.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "../../include/wema_dev.h"
+#include "synth.h"
+#include "wema_host.h"
+#include "wema_jobs.h"
+
+extern "C" {
+hipError_t wema_launch_dwt_fwd(const DwtfJob *, int, int, hipStream_t);
+hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, hipStream_t);
+hipError_t wema_launch_iir(const TemporalJob *, int, int, hipStream_t);
+hipError_t wema_launch_win(const TemporalJob *, int, int, hipStream_t);
+hipError_t wema_launch_egress(int, const EgressJob *, int, int, hipStream_t);
+hipError_t wema_launch_synth(const SynthFrame *, const unsigned short *, unsigned char *, int,
+                             int, int, hipStream_t);
+}
+
+// ---------------------------------------------------------------------------------------
+// errors
+
+static thread_local char g_err[512] = "";
+
+void wema_set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *wema_dev_last_error(void) { return g_err; }
+
+#define HIP_OK(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            wema_set_error("%s failed: %s", #call, hipGetErrorString(e_));                 \
+            return -1;                                                                     \
+        }                                                                                  \
+    } while (0)
+
+// ---------------------------------------------------------------------------------------
+// band-pass design, as the reference's -Ofast build evaluates iir_design_bandpass
+// (src/iir_filter.c:36-101) and the warm-up count (src/iir_filter.c:150-151). This file is
+// compiled with -ffp-contract=off; the fmaf calls are the contractions gcc makes there.
+
+static void design_hp(float *c, float fc, float fs) {
+    const float sqrt2 = 1.41421356237f;
+    const float wc = tanf(((float)M_PI * fc) / fs);
+    const float s2w = sqrt2 * wc;
+    const float norm = 1.0f / (fmaf(wc, wc, s2w) + 1.0f);
+    c[0] = norm;
+    c[1] = -2.0f * norm;
+    c[2] = norm;
+    const float wm1 = fmaf(wc, wc, -1.0f);
+    c[3] = (wm1 + wm1) * norm;
+    c[4] = (fmaf(wc, wc, 1.0f) - s2w) * norm;
+}
+
+static void design_lp(float *c, float fc, float fs) {
+    const float sqrt2 = 1.41421356237f;
+    const float wc = tanf(((float)M_PI * fc) / fs);
+    const float wc2 = wc * wc;
+    const float norm = 1.0f / (fmaf(sqrt2, wc, wc2) + 1.0f);
+    c[0] = wc2 * norm;
+    c[1] = c[0] + c[0];
+    c[2] = c[0];
+    const float wm1 = wc2 - 1.0f;
+    c[3] = (wm1 + wm1) * norm;
+    c[4] = fmaf(-wc, sqrt2, wc2 + 1.0f) * norm;
+}
+
+int wema_design_bandpass(float f_low, float f_high, float fs, float *c10, int *warmup) {
+    if (!(f_low < f_high) || !(f_low > 0) || !(f_high < fs * 0.5f)) return -1;
+    design_hp(c10, f_low, fs);
+    design_lp(c10 + 5, f_high, fs);
+    const float k = 7.0f / (2.0f * (float)M_PI);
+    int wu = (int)((fs * k) / f_low);
+    *warmup = wu < 30 ? 30 : wu;
+    return 0;
+}
+
+// Fade-in after the preseed (src/wema.c:830-839, src/wema_batch.c:426-435).
+float wema_fade_amp(float amp, long global_frame) {
+    const int fade_in_frames = 15;
+    if (global_frame < 0) return 0.0f;
+    if (global_frame < fade_in_frames) {
+        const float t = (float)(global_frame + 1) / (float)fade_in_frames;
+        const float at = amp * t;
+        return at * t;
+    }
+    return amp;
+}
+
+int wema_auto_levels(int width, int height) {
+    // dwt_compute_levels (src/dwt.c:588-596), shipped cap of 4 levels
+    int m = width < height ? width : height, levels = 0;
+    while (m >= 16 && levels < 4) {
+        m /= 2;
+        levels++;
+    }
+    return levels > 0 ? levels : 1;
+}
+
+// ---------------------------------------------------------------------------------------
+// geometry and memory plan
+
+void wema_geometry_init(WemaGeom *g, const wema_dev_params *p) {
+    memset(g, 0, sizeof(*g));
+    g->W = p->width;
+    g->H = p->height;
+    g->L = p->levels > 0 ? p->levels : wema_auto_levels(p->width, p->height);
+    g->nch = p->color ? 3 : 1;
+    g->P = (size_t)g->W * g->H;
+    g->lw[0] = g->W;
+    g->lh[0] = g->H;
+    g->plane_off[0] = 0;
+    g->ll_off[0] = 0;
+    for (int l = 0; l < g->L; l++) {
+        g->lw[l + 1] = (g->lw[l] + 1) / 2;
+        g->lh[l + 1] = (g->lh[l] + 1) / 2;
+        const size_t n = (size_t)g->lw[l + 1] * g->lh[l + 1];
+        g->plane_off[l + 1] = g->plane_off[l] + 3 * n;
+        g->ll_off[l + 1] = g->ll_off[l] + n;
+    }
+    g->nu = g->plane_off[g->L];
+    g->ll_total = g->ll_off[g->L];
+}
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Byte layout of one session's arena.
+struct Layout {
+    size_t coef, ampc, filt, state, ll, rec, ab, acc, amp, thr, thr_ovr, tw, total;
+};
+
+static void plan_layout(const WemaGeom &g, const wema_dev_params &p, int with_filt, Layout *L) {
+    const size_t B = (size_t)p.max_batch, nch = g.nch, nu = g.nu;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    const int ring_slots = p.use_iir ? 0 : p.window + p.max_batch - 1;
+    L->coef = take(sizeof(float) * nu * nch * (p.use_iir ? B : (size_t)ring_slots));
+    L->ampc = take(p.use_iir && !with_filt ? 0 : sizeof(float) * nu * nch * B);
+    L->filt = take(with_filt ? sizeof(float) * nu * nch * B : 0);
+    L->state = take(p.use_iir ? sizeof(float) * 4 * nu * nch : 0);
+    L->ll = take(sizeof(float) * g.ll_total * nch * B);
+    L->rec = take(sizeof(float) * g.P * nch * B);
+    const bool gf = !p.color;
+    L->ab = take(gf ? sizeof(float) * 2 * g.P * B : 0);
+    L->acc = take(sizeof(unsigned long long) * nch * B);
+    L->amp = take(sizeof(float) * nch * B);
+    L->thr = take(sizeof(float) * 3 * B);
+    L->thr_ovr = take(sizeof(float) * 3 * B);
+    L->tw = take(sizeof(float) * 256);
+    L->total = off;
+}
+
+static int validate(const wema_dev_params *p) {
+    if (p->width < 1 || p->height < 1) { wema_set_error("invalid frame size %dx%d", p->width, p->height); return -1; }
+    if (!(p->fps > 0)) { wema_set_error("fps must be positive"); return -1; }
+    if (!(p->amp > 0)) { wema_set_error("amplification factor must be positive"); return -1; }
+    if (p->max_batch < 1 || p->max_batch > WEMA_MAX_BATCH) {
+        wema_set_error("max_batch must be in 1..%d", WEMA_MAX_BATCH);
+        return -1;
+    }
+    int L = p->levels > 0 ? p->levels : wema_auto_levels(p->width, p->height);
+    if (L > WEMA_MAX_LEVELS) { wema_set_error("levels must be <= %d", WEMA_MAX_LEVELS); return -1; }
+    if (!p->use_iir && (p->window < 4 || p->window > 256)) {
+        // src/main.c:135-143
+        wema_set_error("temporal window must be in 4..256");
+        return -1;
+    }
+    if (p->use_iir) {
+        float c[10];
+        int wu;
+        if (wema_design_bandpass(p->f_low, p->f_high, p->fps, c, &wu) < 0) {
+            // iir_design_bandpass rejects these (src/iir_filter.c:74)
+            wema_set_error("invalid band %g..%g Hz at %g fps", p->f_low, p->f_high, p->fps);
+            return -1;
+        }
+    }
+    return 0;
+}
+
+extern "C" void wema_dev_params_default(wema_dev_params *p, int width, int height, float fps) {
+    memset(p, 0, sizeof(*p));
+    p->width = width;
+    p->height = height;
+    p->fps = fps;
+    p->amp = 120.0f;           // src/main.c:66
+    p->f_low = 0.5f;
+    p->f_high = fps / 2.0f - 0.0001f;   // default 1048576 clamped to Nyquist (main.c:68, 248-250)
+    p->window = 32;            // WEMA_TEMPORAL_WINDOW_DEF
+    p->levels = 0;
+    p->use_iir = 1;
+    p->color = 0;
+    p->edge_aware = 1;
+    p->bilateral = 0;
+    p->batch_path = 0;
+    p->max_batch = 1;
+    p->device = 0;
+}
+
+extern "C" int wema_dev_query_bytes(const wema_dev_params *p, size_t *bytes) {
+    if (!p || !bytes) { wema_set_error("null argument"); return -1; }
+    if (validate(p) < 0) return -1;
+    WemaGeom g;
+    wema_geometry_init(&g, p);
+    Layout L;
+    plan_layout(g, *p, 0, &L);
+    *bytes = L.total + sizeof(DwtfJob) * 4096;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// session
+
+int wema_session_alloc(wema_dev *s, int with_filt);
+
+extern "C" int wema_dev_open(wema_dev_t **out, const wema_dev_params *p) {
+    if (!out || !p) { wema_set_error("null argument"); return -1; }
+    *out = nullptr;
+    if (validate(p) < 0) return -1;
+    wema_dev *s = new wema_dev();
+    s->p = *p;
+    wema_geometry_init(&s->g, p);
+    if (p->use_iir) wema_design_bandpass(p->f_low, p->f_high, p->fps, s->c10, &s->warmup);
+    s->ring_slots = p->use_iir ? 0 : p->window + p->max_batch - 1;
+    if (wema_session_alloc(s, 0) < 0) {
+        wema_dev_close(s);
+        return -1;
+    }
+    *out = s;
+    return 0;
+}
+
+int wema_session_alloc(wema_dev *s, int with_filt) {
+    if (hipSetDevice(s->p.device) != hipSuccess) {
+        wema_set_error("hipSetDevice(%d) failed", s->p.device);
+        return -1;
+    }
+    if (!s->stream) HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    Layout L;
+    plan_layout(s->g, s->p, with_filt, &L);
+    if (s->arena) {
+        HIP_OK(hipStreamSynchronize(s->stream));
+        HIP_OK(hipFree(s->arena));
+        s->arena = nullptr;
+    }
+    HIP_OK(hipMalloc(&s->arena, L.total));
+    HIP_OK(hipMemsetAsync(s->arena, 0, L.total, s->stream));
+    char *a = (char *)s->arena;
+    s->coef = (float *)(a + L.coef);
+    s->ampc = (s->p.use_iir && !with_filt) ? nullptr : (float *)(a + L.ampc);
+    s->filt = with_filt ? (float *)(a + L.filt) : nullptr;
+    s->state = s->p.use_iir ? (float4 *)(a + L.state) : nullptr;
+    s->ll = (float *)(a + L.ll);
+    s->rec = (float *)(a + L.rec);
+    s->ab = s->p.color ? nullptr : (float2 *)(a + L.ab);
+    s->acc = (unsigned long long *)(a + L.acc);
+    s->amp_d = (float *)(a + L.amp);
+    s->thr_d = (float *)(a + L.thr);
+    s->thr_ovr_d = (float *)(a + L.thr_ovr);
+    s->tw_d = (float *)(a + L.tw);
+    s->arena_bytes = L.total;
+    // time weights of the bilateral window (src/temporal_filter.c:210-218)
+    if (!s->p.use_iir) {
+        const int ws = s->p.window;
+        std::vector<float> tw(256, 0.0f);
+        const float ts = (float)ws * 0.15f;
+        const float inv = 1.0f / (2.0f * ts * ts);
+        for (int t = 0; t < ws; t++) {
+            const float d = (float)(t - ws / 2);
+            tw[t] = expf(-(d * d) * inv);
+        }
+        HIP_OK(hipMemcpy(s->tw_d, tw.data(), 256 * sizeof(float), hipMemcpyHostToDevice));
+    }
+    s->with_filt = with_filt;
+    for (int k = 0; k < 2; k++) {
+        if (!s->stage_h[k]) HIP_OK(hipHostMalloc(&s->stage_h[k], WEMA_STAGE_BYTES, hipHostMallocDefault));
+        if (!s->stage_d[k]) HIP_OK(hipMalloc(&s->stage_d[k], WEMA_STAGE_BYTES));
+        if (!s->stage_ev[k]) HIP_OK(hipEventCreateWithFlags(&s->stage_ev[k], hipEventDisableTiming));
+    }
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" void wema_dev_close(wema_dev_t *s) {
+    if (!s) return;
+    hipSetDevice(s->p.device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (int k = 0; k < 2; k++) {
+        if (s->stage_h[k]) hipHostFree(s->stage_h[k]);
+        if (s->stage_d[k]) hipFree(s->stage_d[k]);
+        if (s->stage_ev[k]) hipEventDestroy(s->stage_ev[k]);
+    }
+    if (s->arena) hipFree(s->arena);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+extern "C" int wema_dev_geometry(const wema_dev_t *s, int *levels, size_t *n_unique, size_t *n6) {
+    if (!s) { wema_set_error("null session"); return -1; }
+    if (levels) *levels = s->g.L;
+    if (n_unique) *n_unique = s->g.nu;
+    if (n6) *n6 = 2 * s->g.nu;
+    return 0;
+}
+
+extern "C" int wema_dev_iir_coeffs(const wema_dev_t *s, float *c10, int *warmup) {
+    if (!s || !s->p.use_iir) { wema_set_error("not an IIR session"); return -1; }
+    if (c10) memcpy(c10, s->c10, sizeof(s->c10));
+    if (warmup) *warmup = s->warmup;
+    return 0;
+}
+
+extern "C" long wema_dev_frames(const wema_dev_t *s) { return s ? s->frames : -1; }
+
+// ---------------------------------------------------------------------------------------
+// job tables for n frames of one session, appended to a JobBuilder (also used by the pool)
+
+void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,
+                     float *out_f32, int n) {
+    const WemaGeom &g = s->g;
+    const wema_dev_params &p = s->p;
+    const int nch = g.nch, L = g.L;
+    const size_t P = g.P, nu = g.nu;
+    const size_t in_stride = P * 3 * (in_f32 ? sizeof(float) : 1);
+    const long frame0 = s->frames;
+
+    // per-frame effective amplification (channel-major) and threshold overrides
+    for (int c = 0; c < nch; c++)
+        for (int f = 0; f < n; f++) {
+            float a = p.use_iir ? wema_fade_amp(p.amp, frame0 + f) : p.amp;
+            if (c > 0) a = a * 0.5f;   // chroma (src/wema.c:796-799, 898-901)
+            jb.host_amp.push_back(a);
+        }
+
+    auto coef_ptr = [&](int f, int c) -> float * {
+        if (p.use_iir) return s->coef + ((size_t)f * nch + c) * nu;
+        const long gidx = frame0 + f;
+        return s->coef + ((size_t)c * s->ring_slots + (size_t)(gidx % s->ring_slots)) * nu;
+    };
+    auto ll_ptr = [&](int f, int c, int l) -> float * {
+        return s->ll + ((size_t)f * nch + c) * g.ll_total + g.ll_off[l];
+    };
+    auto rec_ptr = [&](int f, int c) -> float * { return s->rec + ((size_t)f * nch + c) * P; };
+
+    // forward DWT levels
+    for (int l = 0; l < L; l++) {
+        const int w = g.lw[l], h = g.lh[l];
+        const int tiles_x = wema_ceil_div(g.lw[l + 1], WEMA_TILE_PAIRS_X);
+        const int tiles = tiles_x * wema_ceil_div(g.lh[l + 1], WEMA_TILE_PAIRS_Y);
+        for (int f = 0; f < n; f++)
+            for (int c = 0; c < nch; c++) {
+                DwtfJob j{};
+                if (l == 0) {
+                    j.src = (const char *)in + (size_t)f * in_stride;
+                    j.src_kind = in_f32 ? (p.color ? SRC_F32_Y + c : SRC_F32_Y)
+                                        : (p.color ? SRC_U8_Y + c : SRC_U8_Y);
+                    j.prev = nullptr;
+                } else {
+                    j.src = ll_ptr(f, c, l - 1);
+                    j.src_kind = SRC_F32_PLANE;
+                    j.prev = coef_ptr(f, c) + g.plane_off[l - 1];
+                }
+                j.ll = ll_ptr(f, c, l);
+                j.planes = coef_ptr(f, c) + g.plane_off[l];
+                j.acc = s->acc + (size_t)f * nch + c;
+                j.w = w;
+                j.h = h;
+                j.tiles_x = tiles_x;
+                jb.add_dwtf(l, j, tiles);
+            }
+    }
+
+    // temporal stage: one job per channel over all n frames
+    const int tiles_t = (int)((nu + 255) / 256);
+    for (int c = 0; c < nch; c++) {
+        TemporalJob j{};
+        j.n = (long long)nu;
+        j.level0_end = (long long)g.plane_off[1];
+        j.acc = s->acc + c;
+        j.acc_stride = nch;
+        j.nframes = n;
+        j.n6f = (float)(2 * nu);
+        j.thr_out = s->thr_d + c;
+        j.thr_override = s->thr_ovr_active ? s->thr_ovr_d + c : nullptr;
+        j.amp_index = (int)(jb.host_amp.size() - (size_t)nch * n + (size_t)c * n);
+        if (p.use_iir) {
+            j.coef = coef_ptr(0, c);
+            j.coef_stride = (long long)(nch * nu);
+            j.out_stride = (long long)(nch * nu);
+            j.amp_out = s->ampc ? s->ampc + (size_t)c * nu : nullptr;
+            j.filt_out = s->filt ? s->filt + (size_t)c * nu : nullptr;
+            j.state = s->state + (size_t)c * nu;
+            memcpy(j.c, s->c10, sizeof(j.c));
+            j.preseed = frame0 == 0;
+            j.warmup = s->warmup;
+        } else {
+            j.coef = nullptr;
+            j.ring = s->coef + (size_t)c * s->ring_slots * nu;
+            j.amp_out = s->ampc + (size_t)c * nu;
+            j.coef_stride = (long long)nu;           // ring plane stride
+            j.out_stride = (long long)(nch * nu);    // amp_out / filt_out frame stride
+            j.filt_out = s->filt ? s->filt + (size_t)c * nu : nullptr;
+            j.window = p.window;
+            j.ring_slots = s->ring_slots;
+            long first = frame0 - p.window + 1;
+            first %= s->ring_slots;
+            if (first < 0) first += s->ring_slots;
+            j.ring_first = (int)first;
+            j.bilateral = p.bilateral;
+            j.tw = s->tw_d;
+        }
+        jb.add_temporal(j, tiles_t);
+    }
+
+    // inverse DWT levels, coarsest first
+    for (int l = L - 1; l >= 0; l--) {
+        const int w = g.lw[l], h = g.lh[l];
+        const int tiles_x = wema_ceil_div(g.lw[l + 1], WEMA_TILE_PAIRS_X);
+        const int tiles = tiles_x * wema_ceil_div(g.lh[l + 1], WEMA_TILE_PAIRS_Y);
+        for (int f = 0; f < n; f++)
+            for (int c = 0; c < nch; c++) {
+                DwtiJob j{};
+                j.ll = ll_ptr(f, c, l);
+                if (!s->ampc) j.planes = coef_ptr(f, c) + g.plane_off[l];
+                else j.planes = s->ampc + ((size_t)f * nch + c) * nu + g.plane_off[l];
+                j.out = l == 0 ? rec_ptr(f, c) : ll_ptr(f, c, l - 1);
+                j.acc_clear = l == L - 1 ? s->acc + (size_t)f * nch + c : nullptr;
+                j.w = w;
+                j.h = h;
+                j.tiles_x = tiles_x;
+                jb.add_dwti(L - 1 - l, j, tiles);
+            }
+    }
+
+    // output stage
+    const int gf_tx = wema_ceil_div(g.W, WEMA_GF_TX);
+    const int gf_tiles = gf_tx * wema_ceil_div(g.H, WEMA_GF_TY);
+    const int px_tiles = (int)((P + 255) / 256), byte_tiles = (int)((3 * P + 255) / 256);
+    for (int f = 0; f < n; f++) {
+        EgressJob j{};
+        j.gray = rec_ptr(f, 0);
+        j.cb = p.color ? rec_ptr(f, 1) : nullptr;
+        j.cr = p.color ? rec_ptr(f, 2) : nullptr;
+        if (in_f32) j.src_f32 = (const float *)((const char *)in + (size_t)f * in_stride);
+        else j.src_u8 = (const uint8_t *)in + (size_t)f * in_stride;
+        j.ab = s->ab ? s->ab + (size_t)f * P : nullptr;
+        j.out_u8 = out_u8 ? out_u8 + (size_t)f * P * 3 : nullptr;
+        j.out_f32 = out_f32 ? out_f32 + (size_t)f * P * 3 : nullptr;
+        j.w = g.W;
+        j.h = g.H;
+        j.tiles_x = gf_tx;
+        const long gidx = frame0 + f;
+        if (!p.use_iir && gidx + 1 < p.window) {
+            jb.add_egress(EG_PASSTHRU, j, byte_tiles);
+        } else if (p.color) {
+            jb.add_egress(EG_YCC, j, px_tiles);
+        } else if (p.edge_aware || p.batch_path) {
+            jb.add_egress(EG_GF1, j, gf_tiles);
+            jb.add_egress(EG_GF2, j, gf_tiles);
+        } else {
+            jb.add_egress(EG_GAUSS, j, gf_tiles);
+        }
+    }
+}
+
+// Uploads the tables of a builder and launches all stages on stream st.
+int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage_ev,
+                  float *amp_d, hipStream_t st) {
+    size_t bytes = 0;
+    if (jb.pack(stage_h, WEMA_STAGE_BYTES, &bytes) < 0) {
+        wema_set_error("job tables exceed %d bytes", WEMA_STAGE_BYTES);
+        return -1;
+    }
+    HIP_OK(hipMemcpyAsync(stage_d, stage_h, bytes, hipMemcpyHostToDevice, st));
+    if (!jb.host_amp.empty())
+        HIP_OK(hipMemcpyAsync(amp_d, (char *)stage_h + jb.amp_offset,
+                              jb.host_amp.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    HIP_OK(hipEventRecord(stage_ev, st));
+    char *base = (char *)stage_d;
+    for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
+        const JobList &jl = jb.dwtf[l];
+        if (jl.count) HIP_OK(wema_launch_dwt_fwd((const DwtfJob *)(base + jl.offset), jl.count, jl.blocks, st));
+    }
+    const JobList &tl = jb.temporal;
+    if (tl.count) {
+        if (jb.window_mode)
+            HIP_OK(wema_launch_win((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
+        else
+            HIP_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
+    }
+    for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
+        const JobList &jl = jb.dwti[l];
+        if (jl.count) HIP_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, st));
+    }
+    for (int k = 0; k < EG_KINDS; k++) {
+        const JobList &jl = jb.egress[k];
+        if (jl.count) HIP_OK(wema_launch_egress(k, (const EgressJob *)(base + jl.offset), jl.count, jl.blocks, st));
+    }
+    return 0;
+}
+
+static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, float *out_f32,
+                   int n, void *stream) {
+    if (!s) { wema_set_error("null session"); return -1; }
+    if (n < 1 || n > s->p.max_batch) {
+        wema_set_error("frame count %d outside 1..%d", n, s->p.max_batch);
+        return -1;
+    }
+    if (!in || (!out_u8 && !out_f32)) { wema_set_error("null frame buffer"); return -1; }
+    HIP_OK(hipSetDevice(s->p.device));
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    const int k = s->stage_cur;
+    s->stage_cur ^= 1;
+    HIP_OK(hipEventSynchronize(s->stage_ev[k]));   // staging buffer k free again
+    JobBuilder jb;
+    jb.window_mode = !s->p.use_iir;
+    wema_build_jobs(s, jb, in, in_f32, out_u8, out_f32, n);
+    jb.fix_amp(s->amp_d);
+    if (wema_run_jobs(jb, s->stage_h[k], s->stage_d[k], s->stage_ev[k], s->amp_d, st) < 0)
+        return -1;
+    s->frames += n;
+    s->last_n = n;
+    s->thr_ovr_active = 0;
+    return 0;
+}
+
+extern "C" int wema_dev_process_u8(wema_dev_t *s, const uint8_t *d_in, uint8_t *d_out, int n,
+                                   void *hip_stream) {
+    return process(s, d_in, 0, d_out, nullptr, n, hip_stream);
+}
+
+extern "C" int wema_dev_process_f32(wema_dev_t *s, const float *d_in, uint8_t *d_out_u8,
+                                    float *d_out_f32, int n, void *hip_stream) {
+    return process(s, d_in, 1, d_out_u8, d_out_f32, n, hip_stream);
+}
+
+extern "C" int wema_dev_sync(wema_dev_t *s) {
+    if (!s) { wema_set_error("null session"); return -1; }
+    HIP_OK(hipSetDevice(s->p.device));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    HIP_OK(hipDeviceSynchronize());
+    return 0;
+}
+
+extern "C" int wema_dev_process_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out) {
+    if (!s || !h_in || !h_out) { wema_set_error("null argument"); return -1; }
+    HIP_OK(hipSetDevice(s->p.device));
+    const size_t bytes = s->g.P * 3;
+    if (!s->io_in) {
+        HIP_OK(hipMalloc(&s->io_in, bytes));
+        HIP_OK(hipMalloc(&s->io_out, bytes));
+    }
+    HIP_OK(hipMemcpyAsync(s->io_in, h_in, bytes, hipMemcpyHostToDevice, s->stream));
+    if (process(s, s->io_in, 0, (uint8_t *)s->io_out, nullptr, 1, s->stream) < 0) return -1;
+    HIP_OK(hipMemcpyAsync(h_out, s->io_out, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" void *wema_dev_host_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        wema_set_error("hipHostMalloc(%zu) failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+extern "C" void wema_dev_host_free(void *p) {
+    if (p) hipHostFree(p);
+}
+
+// ---------------------------------------------------------------------------------------
+// test hooks
+
+extern "C" int wema_dev_debug_capture(wema_dev_t *s, int enable) {
+    if (!s) { wema_set_error("null session"); return -1; }
+    if ((enable != 0) == (s->with_filt != 0)) return 0;
+    if (s->frames > 0) { wema_set_error("capture must be enabled before the first frame"); return -1; }
+    return wema_session_alloc(s, enable != 0);
+}
+
+extern "C" int wema_dev_debug_thresholds(wema_dev_t *s, float *thr, int n) {
+    if (!s || !thr) { wema_set_error("null argument"); return -1; }
+    if (n > s->last_n) n = s->last_n;
+    HIP_OK(hipStreamSynchronize(s->stream));
+    std::vector<float> t((size_t)3 * s->p.max_batch);
+    HIP_OK(hipMemcpy(t.data(), s->thr_d, t.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const long frame0 = s->frames - s->last_n;
+    for (int f = 0; f < n; f++)
+        for (int c = 0; c < 3; c++) {
+            float v = c < s->g.nch ? t[(size_t)f * 3 + c] : NAN;
+            if (!s->p.use_iir && frame0 + f + 1 < s->p.window) v = NAN;
+            thr[f * 3 + c] = v;
+        }
+    return 0;
+}
+
+extern "C" int wema_dev_debug_set_thresholds(wema_dev_t *s, const float *thr, int n) {
+    if (!s) { wema_set_error("null session"); return -1; }
+    if (!thr) { s->thr_ovr_active = 0; return 0; }
+    if (n > s->p.max_batch) { wema_set_error("too many frames"); return -1; }
+    std::vector<float> t((size_t)3 * s->p.max_batch, NAN);
+    memcpy(t.data(), thr, sizeof(float) * 3 * n);
+    HIP_OK(hipMemcpyAsync(s->thr_ovr_d, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    s->thr_ovr_active = 1;
+    return 0;
+}
+
+extern "C" int wema_dev_debug_read(wema_dev_t *s, int which, int frame, int ch, float *host,
+                                   size_t count) {
+    if (!s || !host) { wema_set_error("null argument"); return -1; }
+    if (ch < 0 || ch >= s->g.nch || frame < 0 || frame >= s->last_n) {
+        wema_set_error("frame/channel out of range");
+        return -1;
+    }
+    const size_t nu = s->g.nu, nch = s->g.nch;
+    const long gidx = s->frames - s->last_n + frame;
+    const float *src = nullptr;
+    size_t avail = nu;
+    switch (which) {
+    case 0:
+    case 2:
+        if (which == 2 && s->ampc) {
+            src = s->ampc + ((size_t)frame * nch + ch) * nu;
+        } else if (which == 2 || s->p.use_iir) {
+            if (which == 0 && !s->ampc) {
+                wema_set_error("original coefficients are amplified in place; enable wema_dev_debug_capture");
+                return -1;
+            }
+            src = s->coef + ((size_t)frame * nch + ch) * nu;
+        } else {
+            src = s->coef + ((size_t)ch * s->ring_slots + (size_t)(gidx % s->ring_slots)) * nu;
+        }
+        break;
+    case 1:
+        if (!s->filt) { wema_set_error("enable wema_dev_debug_capture first"); return -1; }
+        src = s->filt + ((size_t)frame * nch + ch) * nu;
+        break;
+    case 3:
+        if (!s->state) { wema_set_error("no IIR state"); return -1; }
+        src = (const float *)(s->state + (size_t)ch * nu);
+        avail = 4 * nu;
+        break;
+    default:
+        wema_set_error("unknown buffer %d", which);
+        return -1;
+    }
+    if (count > avail) count = avail;
+    HIP_OK(hipStreamSynchronize(s->stream));
+    HIP_OK(hipMemcpy(host, src, count * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}

@@ -1,0 +1,134 @@
+// Host-side session state and job-table builder shared by wema_dev.hip (one stream) and
+// wema_pool.hip (many streams per tick).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/wema_dev.h"
+#include "wema_jobs.h"
+
+#define WEMA_MAX_LEVELS 12
+#define WEMA_STAGE_BYTES (8 << 20)
+
+struct WemaGeom {
+    int W, H, L, nch;
+    size_t P;
+    int lw[WEMA_MAX_LEVELS + 1], lh[WEMA_MAX_LEVELS + 1];   // input size of each level
+    size_t plane_off[WEMA_MAX_LEVELS + 1];  // start of level l in the unique-plane layout
+    size_t ll_off[WEMA_MAX_LEVELS + 1];     // start of level l's LL in the LL buffer
+    size_t nu;                               // unique positions (3 per subband position)
+    size_t ll_total;
+};
+
+struct wema_dev {
+    wema_dev_params p;
+    WemaGeom g;
+    float c10[10];
+    int warmup = 0;
+    int ring_slots = 0;
+    long frames = 0;        // frames processed since open
+    int last_n = 0;         // frames of the last call
+    hipStream_t stream = nullptr;
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    float *coef = nullptr;      // IIR: [B][nch][nu] (amplified in place); window: ring
+    float *ampc = nullptr;      // amplified coefficients when kept apart: [B][nch][nu]
+    float *filt = nullptr;      // captured temporal output [B][nch][nu] (test hook)
+    float4 *state = nullptr;    // IIR state [nch][nu]
+    float *ll = nullptr;        // [B][nch][ll_total]
+    float *rec = nullptr;       // level-0 reconstructions [B][nch][P]
+    float2 *ab = nullptr;       // guided filter a, b [B][P]
+    unsigned long long *acc = nullptr;   // [B][nch]
+    float *amp_d = nullptr;     // [nch][B] effective amplification
+    float *thr_d = nullptr;     // [B][3] thresholds used
+    float *thr_ovr_d = nullptr; // [B][3] overrides (NaN = none)
+    float *tw_d = nullptr;      // bilateral time weights [256]
+    int thr_ovr_active = 0;
+    int with_filt = 0;
+    void *stage_h[2] = {nullptr, nullptr};
+    void *stage_d[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    int stage_cur = 0;
+    void *io_in = nullptr, *io_out = nullptr;
+};
+
+enum EgressKind { EG_GF1 = 0, EG_GF2 = 1, EG_GAUSS = 2, EG_YCC = 3, EG_PASSTHRU = 4, EG_KINDS = 5 };
+
+struct JobList {
+    size_t offset = 0;   // byte offset in the packed table
+    int count = 0;
+    int blocks = 0;
+};
+
+struct JobBuilder {
+    std::vector<DwtfJob> dwtf_v[WEMA_MAX_LEVELS];
+    std::vector<TemporalJob> temporal_v;
+    std::vector<DwtiJob> dwti_v[WEMA_MAX_LEVELS];
+    std::vector<EgressJob> egress_v[EG_KINDS];
+    JobList dwtf[WEMA_MAX_LEVELS], temporal, dwti[WEMA_MAX_LEVELS], egress[EG_KINDS];
+    std::vector<float> host_amp;
+    size_t amp_offset = 0;
+    bool window_mode = false;
+
+    void add_dwtf(int l, DwtfJob j, int tiles) {
+        j.tile_begin = dwtf[l].blocks;
+        dwtf[l].blocks += tiles;
+        dwtf_v[l].push_back(j);
+    }
+    void add_temporal(TemporalJob j, int tiles) {
+        j.tile_begin = temporal.blocks;
+        j.tiles = tiles;
+        temporal.blocks += tiles;
+        temporal_v.push_back(j);
+    }
+    void add_dwti(int stage, DwtiJob j, int tiles) {
+        j.tile_begin = dwti[stage].blocks;
+        dwti[stage].blocks += tiles;
+        dwti_v[stage].push_back(j);
+    }
+    void add_egress(int kind, EgressJob j, int tiles) {
+        j.tile_begin = egress[kind].blocks;
+        egress[kind].blocks += tiles;
+        egress_v[kind].push_back(j);
+    }
+    void fix_amp(float *amp_d) {
+        for (auto &j : temporal_v) j.amp = amp_d + j.amp_index;
+    }
+    template <typename T>
+    static bool put(char *dst, size_t cap, size_t &off, const std::vector<T> &v, JobList &jl) {
+        off = (off + 63) & ~(size_t)63;
+        jl.offset = off;
+        jl.count = (int)v.size();
+        const size_t bytes = sizeof(T) * v.size();
+        if (off + bytes > cap) return false;
+        if (bytes) memcpy(dst + off, v.data(), bytes);
+        off += bytes;
+        return true;
+    }
+    int pack(void *dst_v, size_t cap, size_t *used) {
+        char *dst = (char *)dst_v;
+        size_t off = 0;
+        bool ok = true;
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwtf_v[l], dwtf[l]);
+        ok = ok && put(dst, cap, off, temporal_v, temporal);
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_v[l], dwti[l]);
+        for (int k = 0; k < EG_KINDS; k++) ok = ok && put(dst, cap, off, egress_v[k], egress[k]);
+        JobList amp_list;
+        ok = ok && put(dst, cap, off, host_amp, amp_list);
+        amp_offset = amp_list.offset;
+        *used = off;
+        return ok ? 0 : -1;
+    }
+};
+
+void wema_set_error(const char *fmt, ...);
+int wema_design_bandpass(float f_low, float f_high, float fs, float *c10, int *warmup);
+float wema_fade_amp(float amp, long global_frame);
+int wema_auto_levels(int width, int height);
+void wema_geometry_init(WemaGeom *g, const wema_dev_params *p);
+void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,
+                     float *out_f32, int n);
+int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage_ev,
+                  float *amp_d, const float *tw_d, hipStream_t st);

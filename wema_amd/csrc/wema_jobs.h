@@ -1,0 +1,111 @@
+// Work descriptors shared by the host scheduler (wema_dev.hip, wema_pool.hip) and the
+// kernels (wema_kernels.hip).
+//
+// Every stage is one launch over a table of jobs. A job is one (stream, channel, frame)
+// unit of that stage with its own sizes and buffers; the blocks of a launch are dealt out
+// to the jobs in table order (job j owns blocks [tile_begin[j], tile_begin[j+1])). A single
+// stream and a frame batch of many streams use the same kernels and differ only in the
+// table, so a stream's results do not depend on which other streams share the launch.
+#pragma once
+#include <stdint.h>
+
+// CDF 9/7 lifting factors (Daubechies & Sweldens factorisation) and the normalisation the
+// reference uses (src/dwt.c:31-36).
+#define WEMA_ALPHA (-1.586134342059924f)
+#define WEMA_BETA (-0.052980118572961f)
+#define WEMA_GAMMA (0.882911075530934f)
+#define WEMA_DELTA (0.443506852043971f)
+#define WEMA_K (1.149604398860241f)
+#define WEMA_KINV (1.0f / WEMA_K)
+
+// DWT tiles: 64 pairs wide (60 valid outputs + 2 halo pairs per side),
+// WEMA_RP pairs tall (WEMA_RP - 4 valid).
+#define WEMA_TILE_PAIRS_X 60
+#define WEMA_RP 32
+#define WEMA_TILE_PAIRS_Y (WEMA_RP - 4)
+
+// Guided filter tiles (outputs per block).
+#define WEMA_GF_TX 64
+#define WEMA_GF_TY 16
+
+enum SrcKind {
+    SRC_F32_PLANE = 0,   // float image (the previous level's LL)
+    SRC_U8_Y = 1,        // RGB24 -> luma (src/wema.c:42-51)
+    SRC_U8_CB = 2,       // RGB24 -> Cb (src/wema.c:60-78)
+    SRC_U8_CR = 3,
+    SRC_F32_Y = 4,       // float RGB -> luma
+    SRC_F32_CB = 5,
+    SRC_F32_CR = 6,
+};
+
+struct DwtfJob {
+    const void *src;                // level 0: RGB frame; else previous level's LL (w x h)
+    float *ll;                      // LL of this level (sw x sh)
+    float *planes;                  // LH, HH, HL of this level (3 x sw x sh)
+    const float *prev;              // previous level's planes (3 x w x h) or null (level 0)
+    unsigned long long *acc;        // sum of |c| over unique positions, Q32 (or null)
+    int w, h;                       // input size of this level
+    int src_kind;
+    int tiles_x;
+    int tile_begin;
+    int pad_;
+};
+
+struct DwtiJob {
+    const float *ll;                // approximation at this level (sw x sh)
+    const float *planes;            // LH, HH, HL (amplified) of this level
+    float *out;                     // reconstruction (w x h)
+    unsigned long long *acc_clear;  // accumulator to reset once read (or null)
+    int w, h;
+    int tiles_x;
+    int tile_begin;
+};
+
+#define WEMA_MAX_BATCH 256
+
+// Temporal stage (IIR or sliding window) over one (stream, channel) and its frames.
+struct TemporalJob {
+    float *coef;                    // [nframes][coef_stride] coefficients (IIR: in place)
+    float *amp_out;                 // window mode: [nframes][coef_stride] amplified out
+    float *filt_out;                // optional [nframes][coef_stride] temporal output
+    float4 *state;                  // IIR: per position HP w1, HP w2, LP w1, LP w2
+    const float *ring;              // window mode: ring [ring_slots][coef_stride]
+    const unsigned long long *acc;  // [nframes] stride acc_stride
+    const float *thr_override;      // [nframes] stride 3, NaN = compute (or null)
+    float *thr_out;                 // [nframes] stride 3 (or null)
+    const float *amp;               // [nframes] effective amplification of each frame
+    const float *tw;                // window mode: bilateral time weights [ws]
+    long long out_stride;           // frame stride of amp_out / filt_out
+    int amp_index;                  // host-side index of amp in the uploaded amp array
+    float c[10];                    // HP b0 b1 b2 a1 a2, LP b0 b1 b2 a1 a2
+    long long n;                    // unique positions
+    long long level0_end;           // positions below this are level 0 (alpha * 0.5)
+    long long coef_stride;
+    int acc_stride;
+    int nframes;
+    int preseed;                    // run the warm-up on frame 0 first
+    int warmup;
+    float n6f;                      // (float) reference position count
+    int window;                     // window mode: ws
+    int ring_slots;
+    int ring_first;                 // ring slot of the oldest sample of frame 0's window
+    int bilateral;
+    int tile_begin;
+    int tiles;
+};
+
+// Guided filter / Gaussian / colour egress over one output frame.
+struct EgressJob {
+    const float *gray;              // luma reconstruction (or Y for colour)
+    const float *cb, *cr;           // colour mode reconstructions
+    const uint8_t *src_u8;          // source frame (RGB24) or null
+    const float *src_f32;           // source frame (float RGB) or null
+    float2 *ab;                     // guided filter a, b scratch
+    uint8_t *out_u8;                // RGB24 out (or null)
+    float *out_f32;                 // float RGB out (or null)
+    int w, h;
+    int tiles_x;
+    int tile_begin;
+};
+
+__host__ __device__ static inline int wema_ceil_div(int a, int b) { return (a + b - 1) / b; }

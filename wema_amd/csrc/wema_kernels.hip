@@ -1,0 +1,761 @@
+// gfx950 kernels of the WEMA per-frame amplification pipeline.
+//
+// All stages are bandwidth-bound stencils or recurrences (no MFMA). Arithmetic is FP32 with
+// the fused multiply-adds written out explicitly (the library is built with
+// -ffp-contract=off), in the order and contraction shape the reference's -Ofast build uses,
+// so that DWT, IIR and amplification results are bit-identical to it.
+//
+// Stages (one launch per stage over a job table, see wema_jobs.h):
+//   k_dwt_fwd   one 2D CDF 9/7 analysis level: row lifting in registers across the 64
+//               lanes of a wave (neighbours by lane shuffles), column lifting per lane in
+//               registers, LDS tile in between; level 0 converts RGB24 to Y/Cb/Cr on load;
+//               writes LL + LH/HH/HL and adds sum|c| (Q32 integers) to the frame's counter.
+//   k_iir       Butterworth HP+LP biquads (DF2T) per coefficient over the job's frames with
+//               the state kept in registers, warm-up preseed, threshold mask, amplification.
+//   k_win       sliding-window temporal filter (mean removal, optional bilateral weights)
+//               + amplification.
+//   k_dwt_inv   one 2D synthesis level (columns, then rows).
+//   k_gf1/k_gf2 guided filter (r = 4, eps = 0.01) of the luma delta with 9x9 clipped box
+//               means in LDS; k_gf2 adds the smoothed delta to RGB and writes RGB24.
+//   k_gauss     5x5 binomial alternative (--no-edge-aware).
+//   k_ycc       colour mode: Y/Cb/Cr reconstructions -> RGB24.
+//   k_passthru  window mode before the window is full: output = input.
+//   k_synth     synthetic frames (deterministic integer generator).
+#include <hip/hip_runtime.h>
+
+#include "synth.h"
+#include "wema_jobs.h"
+
+// ---------------------------------------------------------------------------------------
+// helpers
+
+template <typename Job>
+__device__ __forceinline__ int find_job(const Job *jobs, int njobs, int b) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_begin <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Whole-sample symmetric extension of index i into [0, n) (src/dwt.c:63-98 boundary rules).
+__device__ __forceinline__ int reflect_idx(int i, int n) {
+    if (n <= 1) return 0;
+    const int period = 2 * n - 2;
+    i %= period;
+    if (i < 0) i += period;
+    return i < n ? i : period - i;
+}
+
+__device__ __forceinline__ float u8f(unsigned v) { return (float)v * (1.0f / 255.0f); }
+
+// Colour conversions with the contraction shapes of the reference's vectorised loops
+// (src/wema.c:42-51, 60-78, 80-102 built with -Ofast).
+__device__ __forceinline__ float luma_of(float r, float g, float b) {
+    return fmaf(0.114f, b, fmaf(0.299f, r, 0.587f * g));
+}
+__device__ __forceinline__ float cb_of(float r, float g, float b) {
+    return fmaf(-0.169f, r, 0.5f * b) + fmaf(-0.331f, g, 0.5f);
+}
+__device__ __forceinline__ float cr_of(float r, float g, float b) {
+    return fmaf(-0.081f, b, fmaf(-0.419f, g, (r + 1.0f) * 0.5f));
+}
+__device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+// src/ffmpeg_io.c:304-310
+__device__ __forceinline__ unsigned char egress_u8(float v) {
+    return (unsigned char)fmaf(clamp01(v), 255.0f, 0.5f);
+}
+
+__device__ __forceinline__ float load_src(const DwtfJob &J, int x, int y) {
+    const size_t idx = (size_t)y * J.w + x;
+    switch (J.src_kind) {
+    case SRC_F32_PLANE:
+        return ((const float *)J.src)[idx];
+    case SRC_U8_Y:
+    case SRC_U8_CB:
+    case SRC_U8_CR: {
+        const unsigned char *p = (const unsigned char *)J.src + idx * 3;
+        float r = u8f(p[0]), g = u8f(p[1]), b = u8f(p[2]);
+        return J.src_kind == SRC_U8_Y ? luma_of(r, g, b)
+             : J.src_kind == SRC_U8_CB ? cb_of(r, g, b) : cr_of(r, g, b);
+    }
+    default: {
+        const float *p = (const float *)J.src + idx * 3;
+        float r = p[0], g = p[1], b = p[2];
+        return J.src_kind == SRC_F32_Y ? luma_of(r, g, b)
+             : J.src_kind == SRC_F32_CB ? cb_of(r, g, b) : cr_of(r, g, b);
+    }
+    }
+}
+
+// |v| rounded to a multiple of 2^-32, as an integer (order-independent threshold sums).
+__device__ __forceinline__ unsigned long long abs_fixed(float v) {
+    const unsigned u = __float_as_uint(v);
+    const unsigned e = (u >> 23) & 0xFFu;
+    if (e == 0) return 0ull;
+    const unsigned long long m = (u & 0x7FFFFFu) | 0x800000u;
+    const int sh = (int)e - 150 + 32;
+    if (sh >= 0) return sh > 38 ? (1ull << 62) : (m << sh);
+    if (sh < -25) return 0ull;
+    return (m + (1ull << (-sh - 1))) >> (-sh);
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// T = 0.2 * (sum over the reference's N6 positions of |c|) / N6, where that sum is twice
+// the exact unique-position sum (src/wema.c:718-723).
+__device__ __forceinline__ float threshold_of(unsigned long long acc, float n6f) {
+    const double s = (double)acc * (1.0 / 4294967296.0);
+    const float mag = (float)(2.0 * s);
+    const float q = mag / n6f;
+    return q * 0.2f;
+}
+
+// One lifting pass over a row held one (even, odd) pair per lane; lanes 0-1 and 62-63 are
+// halo (src/dwt.c:62-98).
+__device__ __forceinline__ void lift_fwd_lanes(float &e, float &o) {
+    float en = __shfl_down(e, 1);
+    o = fmaf(WEMA_ALPHA, e + en, o);
+    float op = __shfl_up(o, 1);
+    e = fmaf(WEMA_BETA, op + o, e);
+    en = __shfl_down(e, 1);
+    o = fmaf(WEMA_GAMMA, e + en, o);
+    op = __shfl_up(o, 1);
+    e = fmaf(WEMA_DELTA, op + o, e);
+}
+
+// Inverse (src/dwt.c:124-161).
+__device__ __forceinline__ void lift_inv_lanes(float &e, float &o) {
+    float op = __shfl_up(o, 1);
+    e = fmaf(-WEMA_DELTA, op + o, e);
+    float en = __shfl_down(e, 1);
+    o = fmaf(-WEMA_GAMMA, e + en, o);
+    op = __shfl_up(o, 1);
+    e = fmaf(-WEMA_BETA, op + o, e);
+    en = __shfl_down(e, 1);
+    o = fmaf(-WEMA_ALPHA, e + en, o);
+}
+
+template <int N>
+__device__ __forceinline__ void lift_fwd_arr(float (&e)[N], float (&o)[N]) {
+#pragma unroll
+    for (int k = 0; k < N - 1; ++k) o[k] = fmaf(WEMA_ALPHA, e[k] + e[k + 1], o[k]);
+#pragma unroll
+    for (int k = 1; k < N; ++k) e[k] = fmaf(WEMA_BETA, o[k - 1] + o[k], e[k]);
+#pragma unroll
+    for (int k = 0; k < N - 1; ++k) o[k] = fmaf(WEMA_GAMMA, e[k] + e[k + 1], o[k]);
+#pragma unroll
+    for (int k = 1; k < N; ++k) e[k] = fmaf(WEMA_DELTA, o[k - 1] + o[k], e[k]);
+}
+
+template <int N>
+__device__ __forceinline__ void lift_inv_arr(float (&e)[N], float (&o)[N]) {
+#pragma unroll
+    for (int k = 1; k < N; ++k) e[k] = fmaf(-WEMA_DELTA, o[k - 1] + o[k], e[k]);
+#pragma unroll
+    for (int k = 0; k < N - 1; ++k) o[k] = fmaf(-WEMA_GAMMA, e[k] + e[k + 1], o[k]);
+#pragma unroll
+    for (int k = 1; k < N; ++k) e[k] = fmaf(-WEMA_BETA, o[k - 1] + o[k], e[k]);
+#pragma unroll
+    for (int k = 0; k < N - 1; ++k) o[k] = fmaf(-WEMA_ALPHA, e[k] + e[k + 1], o[k]);
+}
+
+// ---------------------------------------------------------------------------------------
+// Forward DWT level (src/dwt.c:369-473 + 645-703).
+//
+// Block = 2 waves. Tile = 64 x WEMA_RP sample pairs of the level's input, of which the
+// inner 60 x (WEMA_RP - 4) pairs are outputs; the 2-pair border on each side feeds the
+// 4 lifting steps (each widens the dependency by one pair). Samples outside the image are
+// taken from the whole-sample symmetric extension, which reproduces the reference's
+// per-step boundary rules bit for bit (the extended values are equal sums of the same
+// operands). Row pass first, column pass second, as the reference.
+//
+// Where the parent size is odd, the reference's extraction leaves the last HL/HH column or
+// LH/HH row of the subband unwritten and its scratch buffer keeps the previous level's
+// value at the same flat index (src/dwt.c:459-472, 648-653); those entries are copied from
+// the previous level's plane (0 at level 0) and enter the threshold sum.
+__global__ void __launch_bounds__(128) k_dwt_fwd(const DwtfJob *__restrict__ jobs, int njobs) {
+    constexpr int RP = WEMA_RP;
+    __shared__ float R[2 * RP][128];
+
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const DwtfJob J = jobs[jid];
+    const int t = (int)blockIdx.x - J.tile_begin;
+    const int tx = t % J.tiles_x, ty = t / J.tiles_x;
+    const int w = J.w, h = J.h;
+    const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
+    const int px0 = tx * WEMA_TILE_PAIRS_X, py0 = ty * WEMA_TILE_PAIRS_Y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+    // ---- row pass: one row per wave iteration, one (even, odd) sample pair per lane
+    const int xa = reflect_idx(2 * (px0 - 2 + lane), w);
+    const int xb = reflect_idx(2 * (px0 - 2 + lane) + 1, w);
+    for (int r = wave; r < 2 * RP; r += 2) {
+        const int y = reflect_idx(2 * (py0 - 2) + r, h);
+        float e = load_src(J, xa, y);
+        float o = load_src(J, xb, y);
+        if (w >= 2) {
+            lift_fwd_lanes(e, o);
+            e *= WEMA_KINV;
+            o *= WEMA_K;
+        }
+        R[r][lane] = e;
+        R[r][64 + lane] = o;
+    }
+    __syncthreads();
+
+    // ---- column pass: one column per thread (0..63 low columns, 64..127 high columns)
+    const int c = threadIdx.x;
+    float e[RP], o[RP];
+#pragma unroll
+    for (int k = 0; k < RP; ++k) {
+        e[k] = R[2 * k][c];
+        o[k] = R[2 * k + 1][c];
+    }
+    if (h >= 2) {
+        lift_fwd_arr<RP>(e, o);
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            e[k] *= WEMA_KINV;
+            o[k] *= WEMA_K;
+        }
+    }
+
+    const bool high = c >= 64;
+    const int px = px0 + (c & 63) - 2;
+    const bool col_ok = (c & 63) >= 2 && (c & 63) < 62 && px < sw;
+    const int hw_valid = w >> 1, hh_valid = h >> 1;   // written extents (src/dwt.c:463-469)
+    const size_t n = (size_t)sw * sh;
+    float *LH = J.planes, *HH = J.planes + n, *HL = J.planes + 2 * n;
+    const size_t pn = (size_t)w * h;                  // previous level's plane size
+    unsigned long long part = 0ull;
+    if (col_ok) {
+#pragma unroll
+        for (int k = 2; k < RP - 2; ++k) {
+            const int py = py0 + k - 2;
+            if (py < sh) {
+                const size_t idx = (size_t)py * sw + px;
+                if (!high) {
+                    J.ll[idx] = e[k];
+                    float vlh = o[k];
+                    if (py >= hh_valid) vlh = J.prev ? J.prev[idx] : 0.0f;
+                    LH[idx] = vlh;
+                    part += abs_fixed(vlh);
+                } else {
+                    float vhl = e[k], vhh = o[k];
+                    if (px >= hw_valid) {
+                        vhl = J.prev ? J.prev[2 * pn + idx] : 0.0f;
+                        vhh = J.prev ? J.prev[pn + idx] : 0.0f;
+                    } else if (py >= hh_valid) {
+                        vhh = J.prev ? J.prev[pn + idx] : 0.0f;
+                    }
+                    HL[idx] = vhl;
+                    HH[idx] = vhh;
+                    part += abs_fixed(vhl) + abs_fixed(vhh);
+                }
+            }
+        }
+    }
+    if (J.acc) {
+        part = wave_sum_u64(part);
+        if (lane == 0 && part) atomicAdd(J.acc, part);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Inverse DWT level (src/dwt.c:475-582 + 705-767): packed [LL | HL ; LH | HH] columns
+// first, then rows; each 1D inverse starts by undoing the K scaling.
+__global__ void __launch_bounds__(128) k_dwt_inv(const DwtiJob *__restrict__ jobs, int njobs) {
+    constexpr int RP = WEMA_RP;
+    constexpr int ROWS = 2 * (RP - 4);
+    __shared__ float R[ROWS][128];
+
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const DwtiJob J = jobs[jid];
+    const int t = (int)blockIdx.x - J.tile_begin;
+    if (t == 0 && threadIdx.x == 0 && J.acc_clear) *J.acc_clear = 0ull;
+    const int tx = t % J.tiles_x, ty = t / J.tiles_x;
+    const int w = J.w, h = J.h;
+    const int sw = (w + 1) >> 1;
+    const int px0 = tx * WEMA_TILE_PAIRS_X, py0 = ty * WEMA_TILE_PAIRS_Y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t n = (size_t)sw * ((h + 1) >> 1);
+
+    // ---- column pass
+    const int c = threadIdx.x;
+    const bool high = c >= 64;
+    const int I = px0 - 2 + (c & 63);
+    const int sx = reflect_idx(2 * I + (high ? 1 : 0), w);
+    const int col = sx >> 1;
+    const float *Ep = high ? J.planes + 2 * n : J.ll;   // HL : LL
+    const float *Op = high ? J.planes + n : J.planes;   // HH : LH
+    float e[RP], o[RP];
+#pragma unroll
+    for (int k = 0; k < RP; ++k) {
+        const int jy = py0 - 2 + k;
+        const int ye = reflect_idx(2 * jy, h) >> 1;
+        const int yo = reflect_idx(2 * jy + 1, h) >> 1;
+        e[k] = Ep[(size_t)ye * sw + col];
+        o[k] = Op[(size_t)yo * sw + col];
+    }
+    if (h >= 2) {
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            e[k] *= WEMA_K;
+            o[k] *= WEMA_KINV;
+        }
+        lift_inv_arr<RP>(e, o);
+    }
+#pragma unroll
+    for (int k = 2; k < RP - 2; ++k) {
+        R[2 * (k - 2)][c] = e[k];
+        R[2 * (k - 2) + 1][c] = o[k];
+    }
+    __syncthreads();
+
+    // ---- row pass
+    const int x = 2 * (px0 - 2 + lane);
+    const bool lane_ok = lane >= 2 && lane < 62;
+    for (int r = wave; r < ROWS; r += 2) {
+        const int y = 2 * py0 + r;
+        if (y >= h) break;
+        float ev = R[r][lane], ov = R[r][64 + lane];
+        if (w >= 2) {
+            ev *= WEMA_K;
+            ov *= WEMA_KINV;
+            lift_inv_lanes(ev, ov);
+        }
+        if (lane_ok) {
+            float *row = J.out + (size_t)y * w;
+            if (x < w) row[x] = ev;
+            if (x + 1 < w) row[x + 1] = ov;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// IIR band-pass + amplification (src/iir_filter.c:181-264, 266-370, 376-471;
+// src/wema.c:714-744, 783-800, 830-848; src/wema_batch.c:268-297, 420-440).
+__device__ __forceinline__ float biquad2(const float *c, float4 &st, float x) {
+    // DF2T, FMA sequence of the reference's AVX-512 kernel (src/iir_filter.c:211-219)
+    float y = fmaf(c[0], x, st.x);
+    float nw1 = fmaf(-c[3], y, fmaf(c[1], x, st.y));
+    float nw2 = fmaf(-c[4], y, c[2] * x);
+    st.x = nw1;
+    st.y = nw2;
+    x = y;
+    y = fmaf(c[5], x, st.z);
+    nw1 = fmaf(-c[8], y, fmaf(c[6], x, st.w));
+    nw2 = fmaf(-c[9], y, c[7] * x);
+    st.z = nw1;
+    st.w = nw2;
+    return y;
+}
+
+__global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ jobs, int njobs) {
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const TemporalJob J = jobs[jid];
+    const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
+    float c[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) c[i] = J.c[i];
+    const int nframes = J.nframes;
+    const long long stride = J.coef_stride;
+    if (p == 0 && J.thr_out) {
+        for (int f = 0; f < nframes; ++f) {
+            float T = threshold_of(J.acc[(size_t)f * J.acc_stride], J.n6f);
+            if (J.thr_override && J.thr_override[3 * f] == J.thr_override[3 * f]) T = J.thr_override[3 * f];
+            J.thr_out[3 * f] = T;
+        }
+    }
+    if (p >= J.n) return;
+    float4 st;
+    if (J.preseed) {
+        st = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float x0 = J.coef[p];
+        for (int it = 0; it < J.warmup; ++it) (void)biquad2(c, st, x0);
+    } else {
+        st = J.state[p];
+    }
+    const float lscale = p < J.level0_end ? 0.5f : 1.0f;
+    for (int f = 0; f < nframes; ++f) {
+        float T = threshold_of(J.acc[(size_t)f * J.acc_stride], J.n6f);
+        if (J.thr_override) {
+            const float o = J.thr_override[3 * f];
+            if (o == o) T = o;
+        }
+        const float a = J.amp[f] * lscale;
+        float *cp = J.coef + (size_t)f * stride + p;
+        const float x = *cp;
+        const float y = biquad2(c, st, x);
+        const float out = (fabsf(x) <= T) ? x : fmaf(a, y, x);
+        if (J.amp_out) J.amp_out[(size_t)f * J.out_stride + p] = out;
+        else *cp = out;
+        if (J.filt_out) J.filt_out[(size_t)f * J.out_stride + p] = y;
+    }
+    J.state[p] = st;
+}
+
+// ---------------------------------------------------------------------------------------
+// Sliding-window temporal filter + amplification (src/temporal_filter.c:139-319,
+// src/phase_amp.c:12-56). Window of frame f: ring slots ring_first + f + t (mod ring_slots),
+// t = 0..ws-1, oldest first; tw holds the Gaussian time weights of the bilateral mode.
+__global__ void __launch_bounds__(256) k_win(const TemporalJob *__restrict__ jobs, int njobs) {
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const TemporalJob J = jobs[jid];
+    const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
+    const int nframes = J.nframes, ws = J.window, center = ws / 2;
+    if (p == 0 && J.thr_out) {
+        for (int f = 0; f < nframes; ++f) {
+            float T = threshold_of(J.acc[(size_t)f * J.acc_stride], J.n6f);
+            if (J.thr_override && J.thr_override[3 * f] == J.thr_override[3 * f]) T = J.thr_override[3 * f];
+            J.thr_out[3 * f] = T;
+        }
+    }
+    if (p >= J.n) return;
+    const float inv_ws = 1.0f / (float)ws;
+    const float lscale = p < J.level0_end ? 0.5f : 1.0f;
+    const size_t plane = (size_t)J.coef_stride;
+    const float *tw = J.tw;
+    for (int f = 0; f < nframes; ++f) {
+        const int first = J.ring_first + f;
+        float mean = 0.0f;
+        for (int t = 0; t < ws; ++t) mean += J.ring[(size_t)((first + t) % J.ring_slots) * plane + p];
+        const float cur = J.ring[(size_t)((first + center) % J.ring_slots) * plane + p];
+        const float orig = J.ring[(size_t)((first + ws - 1) % J.ring_slots) * plane + p];
+        mean *= inv_ws;
+        float delta = cur - mean;
+        if (J.bilateral) {
+            float mad = 0.0f;
+            for (int t = 0; t < ws; ++t) {
+                float d = J.ring[(size_t)((first + t) % J.ring_slots) * plane + p] - cur;
+                mad += d > 0.0f ? d : -d;
+            }
+            const float sigma = fmaxf(mad * inv_ws * 0.5f, 0.001f);
+            const float inv2 = 1.0f / (2.0f * sigma * sigma);
+            float wsum = 0.0f;
+            for (int t = 0; t < ws; ++t) {
+                float d = J.ring[(size_t)((first + t) % J.ring_slots) * plane + p] - cur;
+                wsum += expf(-(d * d) * inv2) * tw[t];
+            }
+            if (wsum > 1e-8f) {
+                float acc = 0.0f;
+                for (int t = 0; t < ws; ++t) {
+                    float s = J.ring[(size_t)((first + t) % J.ring_slots) * plane + p];
+                    float d = s - cur;
+                    float wt = expf(-(d * d) * inv2) * tw[t];
+                    acc = fmaf(wt, s - mean, acc);
+                }
+                delta = acc * (1.0f / wsum);
+            }
+        }
+        float T = threshold_of(J.acc[(size_t)f * J.acc_stride], J.n6f);
+        if (J.thr_override) {
+            const float o = J.thr_override[3 * f];
+            if (o == o) T = o;
+        }
+        const float a = J.amp[f] * lscale;
+        J.amp_out[(size_t)f * J.out_stride + p] = (fabsf(orig) <= T) ? orig : fmaf(a, delta, orig);
+        if (J.filt_out) J.filt_out[(size_t)f * J.out_stride + p] = delta;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Guided filter (src/wema.c:114-230, 289-335; src/wema_batch.c:26-132, 311-346).
+// Box means: horizontal 9-tap sum divided by the clipped count, then vertical likewise
+// (src/wema.c:137-167); sums run left to right / top to bottom over the window.
+
+__device__ __forceinline__ float src_luma(const EgressJob &J, size_t idx) {
+    if (J.src_u8) {
+        const unsigned char *p = J.src_u8 + idx * 3;
+        return luma_of(u8f(p[0]), u8f(p[1]), u8f(p[2]));
+    }
+    const float *p = J.src_f32 + idx * 3;
+    return luma_of(p[0], p[1], p[2]);
+}
+
+__device__ __forceinline__ float clip_count(int x, int n) {
+    const int lo = x - 4 > 0 ? x - 4 : 0;
+    const int hi = x + 5 < n ? x + 5 : n;
+    return (float)(hi > lo ? hi - lo : 1);
+}
+
+constexpr int GF_EX = WEMA_GF_TX + 8, GF_EY = WEMA_GF_TY + 8;
+
+__global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs, int njobs) {
+    __shared__ float sI[GF_EY][GF_EX + 1], sP[GF_EY][GF_EX + 1];
+    __shared__ float hI[GF_EY][WEMA_GF_TX], hP[GF_EY][WEMA_GF_TX];
+    __shared__ float hII[GF_EY][WEMA_GF_TX], hIP[GF_EY][WEMA_GF_TX];
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const EgressJob &J = jobs[jid];
+    const int t = (int)blockIdx.x - J.tile_begin;
+    const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
+    const int w = J.w, h = J.h;
+    for (int i = threadIdx.x; i < GF_EY * GF_EX; i += 256) {
+        const int ey = i / GF_EX, ex = i % GF_EX;
+        const int x = x0 - 4 + ex, y = y0 - 4 + ey;
+        float I = 0.0f, p = 0.0f;
+        if (x >= 0 && x < w && y >= 0 && y < h) {
+            const size_t idx = (size_t)y * w + x;
+            I = src_luma(J, idx);
+            p = J.gray[idx] - I;
+        }
+        sI[ey][ex] = I;
+        sP[ey][ex] = p;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < GF_EY * WEMA_GF_TX; i += 256) {
+        const int ey = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
+        const float cnt = clip_count(x0 + x, w);
+        float a = 0.0f, b = 0.0f, c2 = 0.0f, d = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const float I = sI[ey][x + k], p = sP[ey][x + k];
+            a += I;
+            b += p;
+            c2 += I * I;
+            d += I * p;
+        }
+        hI[ey][x] = a / cnt;
+        hP[ey][x] = b / cnt;
+        hII[ey][x] = c2 / cnt;
+        hIP[ey][x] = d / cnt;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < WEMA_GF_TY * WEMA_GF_TX; i += 256) {
+        const int y = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
+        const int gx = x0 + x, gy = y0 + y;
+        if (gx >= w || gy >= h) continue;
+        const float cnt = clip_count(gy, h);
+        float a = 0.0f, b = 0.0f, c2 = 0.0f, d = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            a += hI[y + k][x];
+            b += hP[y + k][x];
+            c2 += hII[y + k][x];
+            d += hIP[y + k][x];
+        }
+        const float mI = a / cnt, mp = b / cnt, cI = c2 / cnt, cIp = d / cnt;
+        const float var = fmaf(-mI, mI, cI);
+        const float cov = fmaf(-mI, mp, cIp);
+        const float av = cov / (var + 0.01f);
+        const float bv = fmaf(-av, mI, mp);
+        J.ab[(size_t)gy * w + gx] = make_float2(av, bv);
+    }
+}
+
+__device__ __forceinline__ void write_rgb(const EgressJob &J, size_t idx, float q) {
+    float s[3];
+    if (J.src_u8) {
+        const unsigned char *p = J.src_u8 + idx * 3;
+        s[0] = u8f(p[0]); s[1] = u8f(p[1]); s[2] = u8f(p[2]);
+    } else {
+        const float *p = J.src_f32 + idx * 3;
+        s[0] = p[0]; s[1] = p[1]; s[2] = p[2];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float v = clamp01(s[k] + q);
+        if (J.out_u8) J.out_u8[idx * 3 + k] = egress_u8(v);
+        if (J.out_f32) J.out_f32[idx * 3 + k] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gf2(const EgressJob *__restrict__ jobs, int njobs) {
+    __shared__ float sA[GF_EY][GF_EX + 1], sB[GF_EY][GF_EX + 1];
+    __shared__ float hA[GF_EY][WEMA_GF_TX], hB[GF_EY][WEMA_GF_TX];
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const EgressJob &J = jobs[jid];
+    const int t = (int)blockIdx.x - J.tile_begin;
+    const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
+    const int w = J.w, h = J.h;
+    for (int i = threadIdx.x; i < GF_EY * GF_EX; i += 256) {
+        const int ey = i / GF_EX, ex = i % GF_EX;
+        const int x = x0 - 4 + ex, y = y0 - 4 + ey;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (x >= 0 && x < w && y >= 0 && y < h) v = J.ab[(size_t)y * w + x];
+        sA[ey][ex] = v.x;
+        sB[ey][ex] = v.y;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < GF_EY * WEMA_GF_TX; i += 256) {
+        const int ey = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
+        const float cnt = clip_count(x0 + x, w);
+        float a = 0.0f, b = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            a += sA[ey][x + k];
+            b += sB[ey][x + k];
+        }
+        hA[ey][x] = a / cnt;
+        hB[ey][x] = b / cnt;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < WEMA_GF_TY * WEMA_GF_TX; i += 256) {
+        const int y = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
+        const int gx = x0 + x, gy = y0 + y;
+        if (gx >= w || gy >= h) continue;
+        const float cnt = clip_count(gy, h);
+        float a = 0.0f, b = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            a += hA[y + k][x];
+            b += hB[y + k][x];
+        }
+        const size_t idx = (size_t)gy * w + gx;
+        const float q = fmaf(a / cnt, src_luma(J, idx), b / cnt);
+        write_rgb(J, idx, q);
+    }
+}
+
+// 5x5 binomial blur of the luma delta with weights renormalised at the borders
+// (src/wema.c:236-283), then RGB transfer.
+__global__ void __launch_bounds__(256) k_gauss(const EgressJob *__restrict__ jobs, int njobs) {
+    constexpr int EX = WEMA_GF_TX + 4, EY = WEMA_GF_TY + 4;
+    __shared__ float sP[EY][EX + 1], sT[EY][WEMA_GF_TX];
+    const float kw[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.0f, 1.0f / 16.0f};
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const EgressJob &J = jobs[jid];
+    const int t = (int)blockIdx.x - J.tile_begin;
+    const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
+    const int w = J.w, h = J.h;
+    for (int i = threadIdx.x; i < EY * EX; i += 256) {
+        const int ey = i / EX, ex = i % EX;
+        const int x = x0 - 2 + ex, y = y0 - 2 + ey;
+        float p = 0.0f;
+        if (x >= 0 && x < w && y >= 0 && y < h) {
+            const size_t idx = (size_t)y * w + x;
+            p = J.gray[idx] - src_luma(J, idx);
+        }
+        sP[ey][ex] = p;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < EY * WEMA_GF_TX; i += 256) {
+        const int ey = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
+        const int gx = x0 + x;
+        float s = 0.0f, wt = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 5; ++d) {
+            const int xx = gx + d - 2;
+            if (xx >= 0 && xx < w) {
+                s = fmaf(kw[d], sP[ey][x + d], s);
+                wt += kw[d];
+            }
+        }
+        sT[ey][x] = wt > 0.0f ? s / wt : 0.0f;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < WEMA_GF_TY * WEMA_GF_TX; i += 256) {
+        const int y = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
+        const int gx = x0 + x, gy = y0 + y;
+        if (gx >= w || gy >= h) continue;
+        float s = 0.0f, wt = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 5; ++d) {
+            const int yy = gy + d - 2;
+            if (yy >= 0 && yy < h) {
+                s = fmaf(kw[d], sT[y + d][x], s);
+                wt += kw[d];
+            }
+        }
+        write_rgb(J, (size_t)gy * w + gx, s / wt);
+    }
+}
+
+// Colour mode output (src/wema.c:80-102).
+__global__ void __launch_bounds__(256) k_ycc(const EgressJob *__restrict__ jobs, int njobs) {
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const EgressJob &J = jobs[jid];
+    const size_t i = (size_t)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
+    if (i >= (size_t)J.w * J.h) return;
+    const float yv = J.gray[i], cbv = J.cb[i] - 0.5f, crv = J.cr[i] - 0.5f;
+    float v[3];
+    v[0] = fmaf(1.402f, crv, yv);
+    v[1] = yv - fmaf(0.344f, cbv, 0.714f * crv);
+    v[2] = fmaf(1.772f, cbv, yv);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float c = clamp01(v[k]);
+        if (J.out_u8) J.out_u8[i * 3 + k] = egress_u8(c);
+        if (J.out_f32) J.out_f32[i * 3 + k] = c;
+    }
+}
+
+// Window mode before the window is full: the input frame is returned (src/wema.c:924-929).
+__global__ void __launch_bounds__(256) k_passthru(const EgressJob *__restrict__ jobs, int njobs) {
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const EgressJob &J = jobs[jid];
+    const size_t i = (size_t)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
+    if (i >= (size_t)J.w * J.h * 3) return;
+    const float v = J.src_u8 ? u8f(J.src_u8[i]) : J.src_f32[i];
+    if (J.out_u8) J.out_u8[i] = egress_u8(v);
+    if (J.out_f32) J.out_f32[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Synthetic frames: one thread per pixel.
+__global__ void __launch_bounds__(256) k_synth(const SynthFrame *__restrict__ frames,
+                                               const unsigned short *__restrict__ tex,
+                                               unsigned char *__restrict__ out, int n) {
+    const SynthFrame &F = frames[blockIdx.y];
+    const size_t P = (size_t)F.width * F.height;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= P || (int)blockIdx.y >= n) return;
+    const int x = (int)(i % F.width), y = (int)(i / F.width);
+    const int tq = tex ? (int)tex[i] : 0;
+    unsigned char *o = out + (size_t)blockIdx.y * P * 3 + i * 3;
+    o[0] = synth_pixel(F, tq, x, y, 0);
+    o[1] = F.color ? synth_pixel(F, tq, x, y, 1) : o[0];
+    o[2] = F.color ? synth_pixel(F, tq, x, y, 2) : o[0];
+}
+
+// ---------------------------------------------------------------------------------------
+// launchers (called by the host scheduler)
+
+extern "C" {
+hipError_t wema_launch_dwt_fwd(const DwtfJob *jobs, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dwt_fwd, dim3(nblocks), dim3(128), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+hipError_t wema_launch_dwt_inv(const DwtiJob *jobs, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dwt_inv, dim3(nblocks), dim3(128), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+hipError_t wema_launch_iir(const TemporalJob *jobs, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_iir, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+hipError_t wema_launch_win(const TemporalJob *jobs, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_win, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+hipError_t wema_launch_egress(int kind, const EgressJob *jobs, int njobs, int nblocks,
+                              hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    switch (kind) {
+    case 0: hipLaunchKernelGGL(k_gf1, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
+    case 1: hipLaunchKernelGGL(k_gf2, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
+    case 2: hipLaunchKernelGGL(k_gauss, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
+    case 3: hipLaunchKernelGGL(k_ycc, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
+    default: hipLaunchKernelGGL(k_passthru, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
+    }
+    return hipGetLastError();
+}
+hipError_t wema_launch_synth(const SynthFrame *frames, const unsigned short *tex,
+                             unsigned char *out, int width, int height, int n, hipStream_t s) {
+    const size_t P = (size_t)width * height;
+    dim3 grid((unsigned)((P + 255) / 256), (unsigned)n);
+    hipLaunchKernelGGL(k_synth, grid, dim3(256), 0, s, frames, tex, out, n);
+    return hipGetLastError();
+}
+}
