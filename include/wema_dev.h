@@ -11,8 +11,9 @@
  * of the last error is available from wema_dev_last_error().
  *
  * Threading: a handle is used by one host thread at a time (as WemaContext, wema.h:102).
- * Work is enqueued on the HIP stream given by the caller (NULL = the handle's own stream)
- * and is stream-ordered; *_sync entry points wait for completion.
+ * Work is enqueued on the HIP stream given by the caller (NULL = the null stream) and is
+ * stream-ordered; consecutive calls on one handle must be ordered (same stream, or
+ * synchronised by the caller). wema_dev_sync waits for the last call's stream.
  */
 #ifndef WEMA_DEV_H
 #define WEMA_DEV_H
@@ -47,6 +48,12 @@ typedef struct {
 
 /* Fills p with the reference CLI defaults (main.c:63-75) for a width x height, fps clip. */
 void wema_dev_params_default(wema_dev_params *p, int width, int height, float fps);
+
+/* Host-only: the band-pass of iir_design_bandpass (iir_filter.c:70-101, order 2: one
+ * high-pass then one low-pass Butterworth biquad), c10 = HP b0 b1 b2 a1 a2, LP b0 b1 b2 a1
+ * a2, and the warm-up count of iir_temporal_init (iir_filter.c:150-151). Same rejections
+ * (f_low <= 0, f_low >= f_high, f_high >= fs / 2) and the same float arithmetic. */
+int wema_dev_design_bandpass(float f_low, float f_high, float fs, float *c10, int *warmup);
 
 /* Device bytes a session with these options holds (HBM admission; no allocation). */
 int wema_dev_query_bytes(const wema_dev_params *p, size_t *bytes);
@@ -84,10 +91,22 @@ int wema_dev_sync(wema_dev_t *s);
 void *wema_dev_host_alloc(size_t bytes);
 void  wema_dev_host_free(void *p);
 
+/* ---- measurement (no counterpart in the reference) --------------------------------
+ * While enabled, HIP events are recorded on the call's stream between the launches of
+ * every process call. wema_dev_timing_report sums them per kernel kind (0 .. kinds-1; it
+ * returns -1 past the last kind): launches, total milliseconds and algorithmic HBM bytes
+ * (compulsory traffic of the launches: inputs read once, outputs written once). */
+int wema_dev_timing(wema_dev_t *s, int enable);
+int wema_dev_timing_report(wema_dev_t *s, int kind, char *name, int name_len, long *launches,
+                           double *total_ms, double *bytes);
+
 /* ---- test hooks (no counterpart in the reference) --------------------------------
  * Thresholds used by the last processed frames: thr[f * 3 + ch] (NaN for unused
  * channels / passthrough frames). Call after wema_dev_sync. */
 int wema_dev_debug_thresholds(wema_dev_t *s, float *thr, int n);
+/* Keep original coefficients and the temporal filter output of every frame for
+ * wema_dev_debug_read (call before the first frame). */
+int wema_dev_debug_capture(wema_dev_t *s, int enable);
 /* Override the thresholds of the next process call: thr[f * 3 + ch]; NULL clears. */
 int wema_dev_debug_set_thresholds(wema_dev_t *s, const float *thr, int n);
 /* Copy a device buffer of frame f (of the last call) to host:

@@ -29,7 +29,7 @@ class WemaError(RuntimeError):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            "wema_amd: %s is not built (run `python -m wema_amd.build` or "
+            "wema_amd: %s is not built (run `python wema_amd/build.py` or "
             "__graft_entry__.build())" % LIB_PATH)
     return C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
 
@@ -66,6 +66,7 @@ _vp = C.c_void_p
 _SIGS = {
     "wema_dev_params_default": (None, [C.POINTER(Params), C.c_int, C.c_int, C.c_float]),
     "wema_dev_query_bytes": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_size_t)]),
+    "wema_dev_design_bandpass": (C.c_int, [C.c_float, C.c_float, C.c_float, _vp, _vp]),
     "wema_dev_open": (C.c_int, [C.POINTER(_vp), C.POINTER(Params)]),
     "wema_dev_close": (None, [_vp]),
     "wema_dev_last_error": (C.c_char_p, []),
@@ -79,6 +80,9 @@ _SIGS = {
     "wema_dev_host_alloc": (_vp, [C.c_size_t]),
     "wema_dev_host_free": (None, [_vp]),
     "wema_dev_debug_capture": (C.c_int, [_vp, C.c_int]),
+    "wema_dev_timing": (C.c_int, [_vp, C.c_int]),
+    "wema_dev_timing_report": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_long),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "wema_dev_debug_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
     "wema_dev_debug_set_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
     "wema_dev_debug_read": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_size_t]),
@@ -113,6 +117,15 @@ def _ptr(a):
 def _stream_handle():
     import torch
     return torch.cuda.current_stream().cuda_stream
+
+
+def design_bandpass(f_low, f_high, fs):
+    """iir_design_bandpass + warm-up (src/iir_filter.c:70-101, 150-151), on the host."""
+    c = np.zeros(10, np.float32)
+    w = C.c_int()
+    _check(lib.wema_dev_design_bandpass(f_low, f_high, fs, c.ctypes.data, C.addressof(w)),
+           "design_bandpass")
+    return c, w.value
 
 
 def auto_levels(width, height):
@@ -201,6 +214,26 @@ class Session:
         out = np.empty_like(frame)
         _check(lib.wema_dev_process_host_u8(self._h, _ptr(frame), _ptr(out)), "process_host_u8")
         return out
+
+    # ---- measurement
+    def timing(self, enable):
+        """Record HIP events between the launches of every call (on the call's stream)."""
+        _check(lib.wema_dev_timing(self._h, int(enable)), "wema_dev_timing")
+
+    def timing_report(self):
+        """Per kernel kind: name, launches, total ms, algorithmic bytes (kinds never
+        launched are left out)."""
+        rows = []
+        k = 0
+        name = C.create_string_buffer(64)
+        while True:
+            la, ms, by = C.c_long(), C.c_double(), C.c_double()
+            if lib.wema_dev_timing_report(self._h, k, name, 64, C.byref(la), C.byref(ms), C.byref(by)) != 0:
+                break
+            if la.value:
+                rows.append(dict(name=name.value.decode(), launches=la.value, ms=ms.value, bytes=by.value))
+            k += 1
+        return rows
 
     # ---- test hooks
     def thresholds(self, n):

@@ -1,6 +1,7 @@
 """Builds the gfx950 pipeline library wema_amd/libwema_dev.so with hipcc.
 
-Usage: python -m wema_amd.build   (or wema_amd.build.build() from Python)
+Usage: python wema_amd/build.py   (a standalone script: importing the package needs the
+library it builds)
 """
 import os
 import subprocess

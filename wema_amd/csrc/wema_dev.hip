@@ -91,6 +91,17 @@ int wema_design_bandpass(float f_low, float f_high, float fs, float *c10, int *w
     return 0;
 }
 
+extern "C" int wema_dev_design_bandpass(float f_low, float f_high, float fs, float *c10,
+                                        int *warmup) {
+    int wu = 0;
+    if (!c10 || wema_design_bandpass(f_low, f_high, fs, c10, &wu) < 0) {
+        wema_set_error("invalid band %g..%g Hz at %g fps", f_low, f_high, fs);
+        return -1;
+    }
+    if (warmup) *warmup = wu;
+    return 0;
+}
+
 // Fade-in after the preseed (src/wema.c:830-839, src/wema_batch.c:426-435).
 float wema_fade_amp(float amp, long global_frame) {
     const int fade_in_frames = 15;
@@ -385,7 +396,11 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.w = w;
                 j.h = h;
                 j.tiles_x = tiles_x;
-                jb.add_dwtf(l, j, tiles);
+                // input read once (RGB24 shared by the channels at level 0) + LL and three
+                // subbands written
+                const double in_b = l == 0 ? 3.0 * w * h / nch * (in_f32 ? 4 : 1) : 4.0 * w * h;
+                const double out_b = 16.0 * g.lw[l + 1] * g.lh[l + 1];
+                jb.add_dwtf(l, j, tiles, in_b + out_b);
             }
     }
 
@@ -428,7 +443,11 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
             j.bilateral = p.bilateral;
             j.tw = s->tw_d;
         }
-        jb.add_temporal(j, tiles_t);
+        // IIR: coefficients read + written per frame, state read + written once;
+        // window: ws ring samples read + amplified coefficient written per frame
+        const double tb = p.use_iir ? (double)nu * (8.0 * n + 32.0)
+                                    : (double)nu * n * (4.0 * p.window + 4.0);
+        jb.add_temporal(j, tiles_t, tb);
     }
 
     // inverse DWT levels, coarsest first
@@ -447,7 +466,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.w = w;
                 j.h = h;
                 j.tiles_x = tiles_x;
-                jb.add_dwti(L - 1 - l, j, tiles);
+                jb.add_dwti(L - 1 - l, j, tiles, 16.0 * g.lw[l + 1] * g.lh[l + 1] + 4.0 * w * h);
             }
     }
 
@@ -469,22 +488,56 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         j.h = g.H;
         j.tiles_x = gf_tx;
         const long gidx = frame0 + f;
+        const double src_b = 3.0 * P * (in_f32 ? 4 : 1);
+        const double out_b = (out_u8 ? 3.0 * P : 0) + (out_f32 ? 12.0 * P : 0);
         if (!p.use_iir && gidx + 1 < p.window) {
-            jb.add_egress(EG_PASSTHRU, j, byte_tiles);
+            jb.add_egress(EG_PASSTHRU, j, byte_tiles, src_b + out_b);
         } else if (p.color) {
-            jb.add_egress(EG_YCC, j, px_tiles);
+            jb.add_egress(EG_YCC, j, px_tiles, 12.0 * P + out_b);
         } else if (p.edge_aware || p.batch_path) {
-            jb.add_egress(EG_GF1, j, gf_tiles);
-            jb.add_egress(EG_GF2, j, gf_tiles);
+            jb.add_egress(EG_GF1, j, gf_tiles, 4.0 * P + src_b + 8.0 * P);
+            jb.add_egress(EG_GF2, j, gf_tiles, 8.0 * P + src_b + out_b);
         } else {
-            jb.add_egress(EG_GAUSS, j, gf_tiles);
+            jb.add_egress(EG_GAUSS, j, gf_tiles, 4.0 * P + src_b + out_b);
         }
     }
 }
 
+const char *const wema_timer_names[TK_COUNT] = {
+    "k_dwt_fwd(level0)", "k_dwt_fwd(levels1+)", "k_iir", "k_win", "k_dwt_inv(levels1+)",
+    "k_dwt_inv(level0)", "k_gf1", "k_gf2", "k_gauss", "k_ycc", "k_passthru"};
+
+static int timer_mark(LaunchTimer *t, int kind, double bytes, hipStream_t st) {
+    if (!t || !t->enabled) return 0;
+    hipEvent_t ev;
+    HIP_OK(hipEventCreate(&ev));
+    HIP_OK(hipEventRecord(ev, st));
+    t->recs.push_back({ev, kind, bytes});
+    return 0;
+}
+
+// Sums the recorded launches: the time between a launch's start marker and the next marker
+// is that launch's duration on the stream.
+int wema_timer_collect(LaunchTimer *t) {
+    for (size_t i = 0; i < t->recs.size(); i++) {
+        if (t->recs[i].kind >= 0 && i > 0) {
+            HIP_OK(hipEventSynchronize(t->recs[i].ev));
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, t->recs[i - 1].ev, t->recs[i].ev));
+            const int k = t->recs[i].kind;
+            t->ms[k] += ms;
+            t->bytes[k] += t->recs[i].bytes;
+            t->launches[k] += 1;
+        }
+    }
+    for (auto &r : t->recs) hipEventDestroy(r.ev);
+    t->recs.clear();
+    return 0;
+}
+
 // Uploads the tables of a builder and launches all stages on stream st.
 int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage_ev,
-                  float *amp_d, hipStream_t st) {
+                  float *amp_d, hipStream_t st, LaunchTimer *timer) {
     size_t bytes = 0;
     if (jb.pack(stage_h, WEMA_STAGE_BYTES, &bytes) < 0) {
         wema_set_error("job tables exceed %d bytes", WEMA_STAGE_BYTES);
@@ -496,9 +549,12 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
                               jb.host_amp.size() * sizeof(float), hipMemcpyHostToDevice, st));
     HIP_OK(hipEventRecord(stage_ev, st));
     char *base = (char *)stage_d;
+    if (timer_mark(timer, -1, 0, st) < 0) return -1;
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
         const JobList &jl = jb.dwtf[l];
-        if (jl.count) HIP_OK(wema_launch_dwt_fwd((const DwtfJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        if (!jl.count) continue;
+        HIP_OK(wema_launch_dwt_fwd((const DwtfJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        if (timer_mark(timer, l == 0 ? TK_DWT_FWD0 : TK_DWT_FWD, jl.bytes, st) < 0) return -1;
     }
     const JobList &tl = jb.temporal;
     if (tl.count) {
@@ -506,14 +562,23 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
             HIP_OK(wema_launch_win((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
         else
             HIP_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
+        if (timer_mark(timer, jb.window_mode ? TK_WIN : TK_IIR, tl.bytes, st) < 0) return -1;
     }
+    int last_inv = -1;
+    for (int l = 0; l < WEMA_MAX_LEVELS; l++)
+        if (jb.dwti[l].count) last_inv = l;
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
         const JobList &jl = jb.dwti[l];
-        if (jl.count) HIP_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        if (!jl.count) continue;
+        HIP_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        if (timer_mark(timer, l == last_inv ? TK_DWT_INV0 : TK_DWT_INV, jl.bytes, st) < 0) return -1;
     }
+    static const int eg_kind[EG_KINDS] = {TK_GF1, TK_GF2, TK_GAUSS, TK_YCC, TK_PASSTHRU};
     for (int k = 0; k < EG_KINDS; k++) {
         const JobList &jl = jb.egress[k];
-        if (jl.count) HIP_OK(wema_launch_egress(k, (const EgressJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        if (!jl.count) continue;
+        HIP_OK(wema_launch_egress(k, (const EgressJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        if (timer_mark(timer, eg_kind[k], jl.bytes, st) < 0) return -1;
     }
     return 0;
 }
@@ -527,7 +592,8 @@ static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, flo
     }
     if (!in || (!out_u8 && !out_f32)) { wema_set_error("null frame buffer"); return -1; }
     HIP_OK(hipSetDevice(s->p.device));
-    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    hipStream_t st = (hipStream_t)stream;    // NULL: the default (null) stream
+    s->last_stream = st;
     const int k = s->stage_cur;
     s->stage_cur ^= 1;
     HIP_OK(hipEventSynchronize(s->stage_ev[k]));   // staging buffer k free again
@@ -535,7 +601,7 @@ static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, flo
     jb.window_mode = !s->p.use_iir;
     wema_build_jobs(s, jb, in, in_f32, out_u8, out_f32, n);
     jb.fix_amp(s->amp_d);
-    if (wema_run_jobs(jb, s->stage_h[k], s->stage_d[k], s->stage_ev[k], s->amp_d, st) < 0)
+    if (wema_run_jobs(jb, s->stage_h[k], s->stage_d[k], s->stage_ev[k], s->amp_d, st, &s->timer) < 0)
         return -1;
     s->frames += n;
     s->last_n = n;
@@ -556,8 +622,8 @@ extern "C" int wema_dev_process_f32(wema_dev_t *s, const float *d_in, uint8_t *d
 extern "C" int wema_dev_sync(wema_dev_t *s) {
     if (!s) { wema_set_error("null session"); return -1; }
     HIP_OK(hipSetDevice(s->p.device));
+    HIP_OK(hipStreamSynchronize(s->last_stream));
     HIP_OK(hipStreamSynchronize(s->stream));
-    HIP_OK(hipDeviceSynchronize());
     return 0;
 }
 
@@ -571,6 +637,7 @@ extern "C" int wema_dev_process_host_u8(wema_dev_t *s, const uint8_t *h_in, uint
     }
     HIP_OK(hipMemcpyAsync(s->io_in, h_in, bytes, hipMemcpyHostToDevice, s->stream));
     if (process(s, s->io_in, 0, (uint8_t *)s->io_out, nullptr, 1, s->stream) < 0) return -1;
+    s->last_stream = s->stream;
     HIP_OK(hipMemcpyAsync(h_out, s->io_out, bytes, hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
     return 0;
@@ -592,6 +659,30 @@ extern "C" void wema_dev_host_free(void *p) {
 // ---------------------------------------------------------------------------------------
 // test hooks
 
+extern "C" int wema_dev_timing(wema_dev_t *s, int enable) {
+    if (!s) { wema_set_error("null session"); return -1; }
+    if (!enable && s->timer.enabled && wema_timer_collect(&s->timer) < 0) return -1;
+    s->timer.enabled = enable != 0;
+    if (enable) {
+        memset(s->timer.ms, 0, sizeof(s->timer.ms));
+        memset(s->timer.bytes, 0, sizeof(s->timer.bytes));
+        memset(s->timer.launches, 0, sizeof(s->timer.launches));
+    }
+    return 0;
+}
+
+extern "C" int wema_dev_timing_report(wema_dev_t *s, int kind, char *name, int name_len,
+                                      long *launches, double *total_ms, double *bytes) {
+    if (!s) { wema_set_error("null session"); return -1; }
+    if (kind < 0 || kind >= TK_COUNT) return -1;
+    if (wema_timer_collect(&s->timer) < 0) return -1;
+    if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", wema_timer_names[kind]);
+    if (launches) *launches = s->timer.launches[kind];
+    if (total_ms) *total_ms = s->timer.ms[kind];
+    if (bytes) *bytes = s->timer.bytes[kind];
+    return 0;
+}
+
 extern "C" int wema_dev_debug_capture(wema_dev_t *s, int enable) {
     if (!s) { wema_set_error("null session"); return -1; }
     if ((enable != 0) == (s->with_filt != 0)) return 0;
@@ -602,7 +693,7 @@ extern "C" int wema_dev_debug_capture(wema_dev_t *s, int enable) {
 extern "C" int wema_dev_debug_thresholds(wema_dev_t *s, float *thr, int n) {
     if (!s || !thr) { wema_set_error("null argument"); return -1; }
     if (n > s->last_n) n = s->last_n;
-    HIP_OK(hipStreamSynchronize(s->stream));
+    HIP_OK(hipStreamSynchronize(s->last_stream));
     std::vector<float> t((size_t)3 * s->p.max_batch);
     HIP_OK(hipMemcpy(t.data(), s->thr_d, t.size() * sizeof(float), hipMemcpyDeviceToHost));
     const long frame0 = s->frames - s->last_n;
@@ -621,8 +712,8 @@ extern "C" int wema_dev_debug_set_thresholds(wema_dev_t *s, const float *thr, in
     if (n > s->p.max_batch) { wema_set_error("too many frames"); return -1; }
     std::vector<float> t((size_t)3 * s->p.max_batch, NAN);
     memcpy(t.data(), thr, sizeof(float) * 3 * n);
-    HIP_OK(hipMemcpyAsync(s->thr_ovr_d, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
-    HIP_OK(hipStreamSynchronize(s->stream));
+    HIP_OK(hipStreamSynchronize(s->last_stream));
+    HIP_OK(hipMemcpy(s->thr_ovr_d, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
     s->thr_ovr_active = 1;
     return 0;
 }
@@ -667,7 +758,7 @@ extern "C" int wema_dev_debug_read(wema_dev_t *s, int which, int frame, int ch, 
         return -1;
     }
     if (count > avail) count = avail;
-    HIP_OK(hipStreamSynchronize(s->stream));
+    HIP_OK(hipStreamSynchronize(s->last_stream));
     HIP_OK(hipMemcpy(host, src, count * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
 }

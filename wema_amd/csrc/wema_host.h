@@ -22,6 +22,33 @@ struct WemaGeom {
     size_t ll_total;
 };
 
+enum EgressKind { EG_GF1 = 0, EG_GF2 = 1, EG_GAUSS = 2, EG_YCC = 3, EG_PASSTHRU = 4, EG_KINDS = 5 };
+
+struct JobList {
+    size_t offset = 0;   // byte offset in the packed table
+    int count = 0;
+    int blocks = 0;
+    double bytes = 0;    // algorithmic HBM bytes of the launch (compulsory traffic)
+};
+
+// Per-launch timing with HIP events recorded between the launches of a call on its
+// stream; read back (and summed per kernel) by wema_dev_timing_report.
+struct LaunchTimer {
+    bool enabled = false;
+    struct Rec {
+        hipEvent_t ev;
+        int kind;        // -1: start marker; otherwise TimerKind of the launch that ended here
+        double bytes;
+    };
+    std::vector<Rec> recs;
+    double ms[16] = {0};
+    double bytes[16] = {0};
+    long launches[16] = {0};
+};
+enum TimerKind { TK_DWT_FWD0 = 0, TK_DWT_FWD = 1, TK_IIR = 2, TK_WIN = 3, TK_DWT_INV = 4,
+                 TK_DWT_INV0 = 5, TK_GF1 = 6, TK_GF2 = 7, TK_GAUSS = 8, TK_YCC = 9,
+                 TK_PASSTHRU = 10, TK_COUNT = 11 };
+
 struct wema_dev {
     wema_dev_params p;
     WemaGeom g;
@@ -30,7 +57,8 @@ struct wema_dev {
     int ring_slots = 0;
     long frames = 0;        // frames processed since open
     int last_n = 0;         // frames of the last call
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // the session's own stream (host-buffer calls)
+    hipStream_t last_stream = nullptr;  // stream of the last process call
     void *arena = nullptr;
     size_t arena_bytes = 0;
     float *coef = nullptr;      // IIR: [B][nch][nu] (amplified in place); window: ring
@@ -51,15 +79,8 @@ struct wema_dev {
     void *stage_d[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     int stage_cur = 0;
+    LaunchTimer timer;
     void *io_in = nullptr, *io_out = nullptr;
-};
-
-enum EgressKind { EG_GF1 = 0, EG_GF2 = 1, EG_GAUSS = 2, EG_YCC = 3, EG_PASSTHRU = 4, EG_KINDS = 5 };
-
-struct JobList {
-    size_t offset = 0;   // byte offset in the packed table
-    int count = 0;
-    int blocks = 0;
 };
 
 struct JobBuilder {
@@ -72,25 +93,29 @@ struct JobBuilder {
     size_t amp_offset = 0;
     bool window_mode = false;
 
-    void add_dwtf(int l, DwtfJob j, int tiles) {
+    void add_dwtf(int l, DwtfJob j, int tiles, double bytes) {
         j.tile_begin = dwtf[l].blocks;
         dwtf[l].blocks += tiles;
+        dwtf[l].bytes += bytes;
         dwtf_v[l].push_back(j);
     }
-    void add_temporal(TemporalJob j, int tiles) {
+    void add_temporal(TemporalJob j, int tiles, double bytes) {
         j.tile_begin = temporal.blocks;
         j.tiles = tiles;
         temporal.blocks += tiles;
+        temporal.bytes += bytes;
         temporal_v.push_back(j);
     }
-    void add_dwti(int stage, DwtiJob j, int tiles) {
+    void add_dwti(int stage, DwtiJob j, int tiles, double bytes) {
         j.tile_begin = dwti[stage].blocks;
         dwti[stage].blocks += tiles;
+        dwti[stage].bytes += bytes;
         dwti_v[stage].push_back(j);
     }
-    void add_egress(int kind, EgressJob j, int tiles) {
+    void add_egress(int kind, EgressJob j, int tiles, double bytes) {
         j.tile_begin = egress[kind].blocks;
         egress[kind].blocks += tiles;
+        egress[kind].bytes += bytes;
         egress_v[kind].push_back(j);
     }
     void fix_amp(float *amp_d) {
@@ -131,4 +156,6 @@ void wema_geometry_init(WemaGeom *g, const wema_dev_params *p);
 void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,
                      float *out_f32, int n);
 int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage_ev,
-                  float *amp_d, const float *tw_d, hipStream_t st);
+                  float *amp_d, hipStream_t st, LaunchTimer *timer);
+int wema_timer_collect(LaunchTimer *t);
+extern const char *const wema_timer_names[TK_COUNT];

@@ -40,6 +40,15 @@ __device__ __forceinline__ int find_job(const Job *jobs, int njobs, int b) {
     return lo;
 }
 
+// Workgroups are dealt to the 8 XCDs of the chip round-robin (block b runs on XCD b % 8),
+// and each XCD has its own L2. Tile kernels re-read a halo around their tile, so blocks
+// that run on one XCD are given consecutive tiles: neighbours then share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n >> 3, r = n & 7;
+    const int x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // Whole-sample symmetric extension of index i into [0, n) (src/dwt.c:63-98 boundary rules).
 __device__ __forceinline__ int reflect_idx(int i, int n) {
     if (n <= 1) return 0;
@@ -184,29 +193,49 @@ __global__ void __launch_bounds__(128) k_dwt_fwd(const DwtfJob *__restrict__ job
     constexpr int RP = WEMA_RP;
     __shared__ float R[2 * RP][128];
 
-    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int jid = find_job(jobs, njobs, bid);
     const DwtfJob J = jobs[jid];
-    const int t = (int)blockIdx.x - J.tile_begin;
+    const int t = bid - J.tile_begin;
     const int tx = t % J.tiles_x, ty = t / J.tiles_x;
     const int w = J.w, h = J.h;
     const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
     const int px0 = tx * WEMA_TILE_PAIRS_X, py0 = ty * WEMA_TILE_PAIRS_Y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-    // ---- row pass: one row per wave iteration, one (even, odd) sample pair per lane
-    const int xa = reflect_idx(2 * (px0 - 2 + lane), w);
-    const int xb = reflect_idx(2 * (px0 - 2 + lane) + 1, w);
-    for (int r = wave; r < 2 * RP; r += 2) {
-        const int y = reflect_idx(2 * (py0 - 2) + r, h);
-        float e = load_src(J, xa, y);
-        float o = load_src(J, xb, y);
-        if (w >= 2) {
-            lift_fwd_lanes(e, o);
-            e *= WEMA_KINV;
-            o *= WEMA_K;
+    // ---- load: the extended tile (2 RP rows x 128 samples) into LDS, all loads independent
+    const int sx0 = 2 * (px0 - 2), sy0 = 2 * (py0 - 2);
+    const int cx = reflect_idx(sx0 + (int)(threadIdx.x & 127), w);
+#pragma unroll 8
+    for (int r = 0; r < 2 * RP; ++r) R[r][threadIdx.x] = load_src(J, cx, reflect_idx(sy0 + r, h));
+    __syncthreads();
+
+    // ---- row pass: one row per wave, one (even, odd) sample pair per lane, 4 rows at a time
+    if (w >= 2) {
+        for (int r0 = wave * 4; r0 < 2 * RP; r0 += 8) {
+            float e[4], o[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float2 v = *reinterpret_cast<const float2 *>(&R[r0 + k][2 * lane]);
+                e[k] = v.x;
+                o[k] = v.y;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lift_fwd_lanes(e[k], o[k]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                R[r0 + k][lane] = e[k] * WEMA_KINV;
+                R[r0 + k][64 + lane] = o[k] * WEMA_K;
+            }
         }
-        R[r][lane] = e;
-        R[r][64 + lane] = o;
+    } else {
+        // n < 2: the row transform is the identity (src/dwt.c:46); keep sample 0 as the
+        // low column and its copy as the (unused) high column
+        for (int r = wave; r < 2 * RP; r += 2) {
+            const float v = R[r][0];
+            R[r][64 + lane] = v;
+            R[r][lane] = v;
+        }
     }
     __syncthreads();
 
@@ -276,9 +305,10 @@ __global__ void __launch_bounds__(128) k_dwt_inv(const DwtiJob *__restrict__ job
     constexpr int ROWS = 2 * (RP - 4);
     __shared__ float R[ROWS][128];
 
-    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int jid = find_job(jobs, njobs, bid);
     const DwtiJob J = jobs[jid];
-    const int t = (int)blockIdx.x - J.tile_begin;
+    const int t = bid - J.tile_begin;
     if (t == 0 && threadIdx.x == 0 && J.acc_clear) *J.acc_clear = 0ull;
     const int tx = t % J.tiles_x, ty = t / J.tiles_x;
     const int w = J.w, h = J.h;
@@ -358,23 +388,33 @@ __device__ __forceinline__ float biquad2(const float *c, float4 &st, float x) {
     return y;
 }
 
+// Thresholds and per-level amplification of the job's frames, computed once per block.
+__device__ __forceinline__ void frame_params(const TemporalJob &J, float *sT, float *sA) {
+    for (int f = threadIdx.x; f < J.nframes; f += blockDim.x) {
+        float T = threshold_of(J.acc[(size_t)f * J.acc_stride], J.n6f);
+        if (J.thr_override) {
+            const float o = J.thr_override[3 * f];
+            if (o == o) T = o;
+        }
+        sT[f] = T;
+        sA[f] = J.amp[f];
+        if (J.thr_out && blockIdx.x == (unsigned)J.tile_begin) J.thr_out[3 * f] = T;
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ jobs, int njobs) {
+    __shared__ float sT[WEMA_MAX_BATCH], sA[WEMA_MAX_BATCH];
     const int jid = find_job(jobs, njobs, (int)blockIdx.x);
     const TemporalJob J = jobs[jid];
+    frame_params(J, sT, sA);
     const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
+    if (p >= J.n) return;
     float c[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) c[i] = J.c[i];
     const int nframes = J.nframes;
     const long long stride = J.coef_stride;
-    if (p == 0 && J.thr_out) {
-        for (int f = 0; f < nframes; ++f) {
-            float T = threshold_of(J.acc[(size_t)f * J.acc_stride], J.n6f);
-            if (J.thr_override && J.thr_override[3 * f] == J.thr_override[3 * f]) T = J.thr_override[3 * f];
-            J.thr_out[3 * f] = T;
-        }
-    }
-    if (p >= J.n) return;
     float4 st;
     if (J.preseed) {
         st = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -385,12 +425,8 @@ __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ job
     }
     const float lscale = p < J.level0_end ? 0.5f : 1.0f;
     for (int f = 0; f < nframes; ++f) {
-        float T = threshold_of(J.acc[(size_t)f * J.acc_stride], J.n6f);
-        if (J.thr_override) {
-            const float o = J.thr_override[3 * f];
-            if (o == o) T = o;
-        }
-        const float a = J.amp[f] * lscale;
+        const float T = sT[f];
+        const float a = sA[f] * lscale;
         float *cp = J.coef + (size_t)f * stride + p;
         const float x = *cp;
         const float y = biquad2(c, st, x);
@@ -407,17 +443,12 @@ __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ job
 // src/phase_amp.c:12-56). Window of frame f: ring slots ring_first + f + t (mod ring_slots),
 // t = 0..ws-1, oldest first; tw holds the Gaussian time weights of the bilateral mode.
 __global__ void __launch_bounds__(256) k_win(const TemporalJob *__restrict__ jobs, int njobs) {
+    __shared__ float sT[WEMA_MAX_BATCH], sA[WEMA_MAX_BATCH];
     const int jid = find_job(jobs, njobs, (int)blockIdx.x);
     const TemporalJob J = jobs[jid];
+    frame_params(J, sT, sA);
     const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
     const int nframes = J.nframes, ws = J.window, center = ws / 2;
-    if (p == 0 && J.thr_out) {
-        for (int f = 0; f < nframes; ++f) {
-            float T = threshold_of(J.acc[(size_t)f * J.acc_stride], J.n6f);
-            if (J.thr_override && J.thr_override[3 * f] == J.thr_override[3 * f]) T = J.thr_override[3 * f];
-            J.thr_out[3 * f] = T;
-        }
-    }
     if (p >= J.n) return;
     const float inv_ws = 1.0f / (float)ws;
     const float lscale = p < J.level0_end ? 0.5f : 1.0f;
@@ -455,12 +486,8 @@ __global__ void __launch_bounds__(256) k_win(const TemporalJob *__restrict__ job
                 delta = acc * (1.0f / wsum);
             }
         }
-        float T = threshold_of(J.acc[(size_t)f * J.acc_stride], J.n6f);
-        if (J.thr_override) {
-            const float o = J.thr_override[3 * f];
-            if (o == o) T = o;
-        }
-        const float a = J.amp[f] * lscale;
+        const float T = sT[f];
+        const float a = sA[f] * lscale;
         J.amp_out[(size_t)f * J.out_stride + p] = (fabsf(orig) <= T) ? orig : fmaf(a, delta, orig);
         if (J.filt_out) J.filt_out[(size_t)f * J.out_stride + p] = delta;
     }
@@ -492,9 +519,10 @@ __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs,
     __shared__ float sI[GF_EY][GF_EX + 1], sP[GF_EY][GF_EX + 1];
     __shared__ float hI[GF_EY][WEMA_GF_TX], hP[GF_EY][WEMA_GF_TX];
     __shared__ float hII[GF_EY][WEMA_GF_TX], hIP[GF_EY][WEMA_GF_TX];
-    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int jid = find_job(jobs, njobs, bid);
     const EgressJob &J = jobs[jid];
-    const int t = (int)blockIdx.x - J.tile_begin;
+    const int t = bid - J.tile_begin;
     const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
     const int w = J.w, h = J.h;
     for (int i = threadIdx.x; i < GF_EY * GF_EX; i += 256) {
@@ -570,9 +598,10 @@ __device__ __forceinline__ void write_rgb(const EgressJob &J, size_t idx, float 
 __global__ void __launch_bounds__(256) k_gf2(const EgressJob *__restrict__ jobs, int njobs) {
     __shared__ float sA[GF_EY][GF_EX + 1], sB[GF_EY][GF_EX + 1];
     __shared__ float hA[GF_EY][WEMA_GF_TX], hB[GF_EY][WEMA_GF_TX];
-    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int jid = find_job(jobs, njobs, bid);
     const EgressJob &J = jobs[jid];
-    const int t = (int)blockIdx.x - J.tile_begin;
+    const int t = bid - J.tile_begin;
     const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
     const int w = J.w, h = J.h;
     for (int i = threadIdx.x; i < GF_EY * GF_EX; i += 256) {
@@ -620,9 +649,10 @@ __global__ void __launch_bounds__(256) k_gauss(const EgressJob *__restrict__ job
     constexpr int EX = WEMA_GF_TX + 4, EY = WEMA_GF_TY + 4;
     __shared__ float sP[EY][EX + 1], sT[EY][WEMA_GF_TX];
     const float kw[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.0f, 1.0f / 16.0f};
-    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int jid = find_job(jobs, njobs, bid);
     const EgressJob &J = jobs[jid];
-    const int t = (int)blockIdx.x - J.tile_begin;
+    const int t = bid - J.tile_begin;
     const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
     const int w = J.w, h = J.h;
     for (int i = threadIdx.x; i < EY * EX; i += 256) {
