@@ -40,6 +40,15 @@ __device__ __forceinline__ int find_job(const Job *jobs, int njobs, int b) {
     return lo;
 }
 
+// Pointers read from job tables are generic; casting them to the global address space lets
+// the compiler emit global (not flat) memory instructions, whose completion does not wait on
+// LDS traffic.
+#define GLOBAL_AS __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ GLOBAL_AS T *gp(T *p) {
+    return (GLOBAL_AS T *)p;
+}
+
 // Workgroups are dealt to the 8 XCDs of the chip round-robin (block b runs on XCD b % 8),
 // and each XCD has its own L2. Tile kernels re-read a halo around their tile, so blocks
 // that run on one XCD are given consecutive tiles: neighbours then share that XCD's L2.
@@ -126,29 +135,30 @@ __device__ __forceinline__ float threshold_of(unsigned long long acc, float n6f)
     return q * 0.2f;
 }
 
+// Value of lane i+1 / lane i-1 of the wave (DPP wave_shl:1 / wave_shr:1; the end lanes get
+// unspecified values, which only reach halo lanes).
+__device__ __forceinline__ float lane_next(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+
 // One lifting pass over a row held one (even, odd) pair per lane; lanes 0-1 and 62-63 are
 // halo (src/dwt.c:62-98).
 __device__ __forceinline__ void lift_fwd_lanes(float &e, float &o) {
-    float en = __shfl_down(e, 1);
-    o = fmaf(WEMA_ALPHA, e + en, o);
-    float op = __shfl_up(o, 1);
-    e = fmaf(WEMA_BETA, op + o, e);
-    en = __shfl_down(e, 1);
-    o = fmaf(WEMA_GAMMA, e + en, o);
-    op = __shfl_up(o, 1);
-    e = fmaf(WEMA_DELTA, op + o, e);
+    o = fmaf(WEMA_ALPHA, e + lane_next(e), o);
+    e = fmaf(WEMA_BETA, lane_prev(o) + o, e);
+    o = fmaf(WEMA_GAMMA, e + lane_next(e), o);
+    e = fmaf(WEMA_DELTA, lane_prev(o) + o, e);
 }
 
 // Inverse (src/dwt.c:124-161).
 __device__ __forceinline__ void lift_inv_lanes(float &e, float &o) {
-    float op = __shfl_up(o, 1);
-    e = fmaf(-WEMA_DELTA, op + o, e);
-    float en = __shfl_down(e, 1);
-    o = fmaf(-WEMA_GAMMA, e + en, o);
-    op = __shfl_up(o, 1);
-    e = fmaf(-WEMA_BETA, op + o, e);
-    en = __shfl_down(e, 1);
-    o = fmaf(-WEMA_ALPHA, e + en, o);
+    e = fmaf(-WEMA_DELTA, lane_prev(o) + o, e);
+    o = fmaf(-WEMA_GAMMA, e + lane_next(e), o);
+    e = fmaf(-WEMA_BETA, lane_prev(o) + o, e);
+    o = fmaf(-WEMA_ALPHA, e + lane_next(e), o);
 }
 
 template <int N>
@@ -178,7 +188,7 @@ __device__ __forceinline__ void lift_inv_arr(float (&e)[N], float (&o)[N]) {
 // ---------------------------------------------------------------------------------------
 // Forward DWT level (src/dwt.c:369-473 + 645-703).
 //
-// Block = 2 waves. Tile = 64 x WEMA_RP sample pairs of the level's input, of which the
+// Block = 4 waves. Tile = 64 x WEMA_RP sample pairs of the level's input, of which the
 // inner 60 x (WEMA_RP - 4) pairs are outputs; the 2-pair border on each side feeds the
 // 4 lifting steps (each widens the dependency by one pair). Samples outside the image are
 // taken from the whole-sample symmetric extension, which reproduces the reference's
@@ -189,7 +199,56 @@ __device__ __forceinline__ void lift_inv_arr(float (&e)[N], float (&o)[N]) {
 // LH/HH row of the subband unwritten and its scratch buffer keeps the previous level's
 // value at the same flat index (src/dwt.c:459-472, 648-653); those entries are copied from
 // the previous level's plane (0 at level 0) and enter the threshold sum.
-__global__ void __launch_bounds__(128) k_dwt_fwd(const DwtfJob *__restrict__ jobs, int njobs) {
+
+constexpr int DWT_THREADS = 256;
+constexpr int HALF_PAIRS = WEMA_RP / 2 + 2;   // pairs a half-column thread lifts
+
+// RGB24 pixel idx as a float channel value. The pixel's 3 bytes come from the aligned
+// dword(s) holding them: one dword when the pixel starts at byte 0 or 1 of a dword, two
+// when it straddles (the second dword then lies inside the next pixel); the last pixel of
+// the buffer is read byte by byte.
+template <int KIND>
+__device__ __forceinline__ float rgb_sample(const unsigned char *frame, size_t idx, bool last) {
+    float r, g, b;
+    if (KIND == SRC_F32_Y || KIND == SRC_F32_CB || KIND == SRC_F32_CR) {
+        const GLOBAL_AS float *p = gp((const float *)frame) + idx * 3;
+        r = p[0]; g = p[1]; b = p[2];
+    } else {
+        const GLOBAL_AS unsigned char *p = gp(frame) + idx * 3;
+        unsigned v;
+        if (!last) {
+            const size_t a = (size_t)(const void *)frame + idx * 3;
+            const unsigned sh = (unsigned)(a & 3u);
+            const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(p - sh);
+            const unsigned lo = q[0];
+            const unsigned hi = sh >= 2 ? q[1] : 0u;
+            v = __builtin_amdgcn_alignbyte(hi, lo, sh);
+        } else {
+            v = (unsigned)p[0] | ((unsigned)p[1] << 8) | ((unsigned)p[2] << 16);
+        }
+        r = u8f(v & 0xFFu); g = u8f((v >> 8) & 0xFFu); b = u8f((v >> 16) & 0xFFu);
+    }
+    if (KIND == SRC_U8_Y || KIND == SRC_F32_Y) return luma_of(r, g, b);
+    if (KIND == SRC_U8_CB || KIND == SRC_F32_CB) return cb_of(r, g, b);
+    return cr_of(r, g, b);
+}
+
+template <int KIND>
+__device__ __forceinline__ void load_tile(float (*R)[128], const DwtfJob &J, int sx0, int sy0) {
+    const int c = threadIdx.x & 127, r0 = threadIdx.x >> 7;
+    const int w = J.w, h = J.h;
+    const int x = reflect_idx(sx0 + c, w);
+#pragma unroll
+    for (int k = 0; k < WEMA_RP; ++k) {
+        const int r = r0 + 2 * k;
+        const int y = reflect_idx(sy0 + r, h);
+        const size_t idx = (size_t)y * w + x;
+        if (KIND == SRC_F32_PLANE) R[r][c] = gp((const float *)J.src)[idx];
+        else R[r][c] = rgb_sample<KIND>((const unsigned char *)J.src, idx, x == w - 1 && y == h - 1);
+    }
+}
+
+__global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restrict__ jobs, int njobs) {
     constexpr int RP = WEMA_RP;
     __shared__ float R[2 * RP][128];
 
@@ -203,16 +262,22 @@ __global__ void __launch_bounds__(128) k_dwt_fwd(const DwtfJob *__restrict__ job
     const int px0 = tx * WEMA_TILE_PAIRS_X, py0 = ty * WEMA_TILE_PAIRS_Y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-    // ---- load: the extended tile (2 RP rows x 128 samples) into LDS, all loads independent
+    // ---- load the extended tile (2 RP rows x 128 samples), all loads independent
     const int sx0 = 2 * (px0 - 2), sy0 = 2 * (py0 - 2);
-    const int cx = reflect_idx(sx0 + (int)(threadIdx.x & 127), w);
-#pragma unroll 8
-    for (int r = 0; r < 2 * RP; ++r) R[r][threadIdx.x] = load_src(J, cx, reflect_idx(sy0 + r, h));
+    switch (J.src_kind) {
+    case SRC_F32_PLANE: load_tile<SRC_F32_PLANE>(R, J, sx0, sy0); break;
+    case SRC_U8_Y: load_tile<SRC_U8_Y>(R, J, sx0, sy0); break;
+    case SRC_U8_CB: load_tile<SRC_U8_CB>(R, J, sx0, sy0); break;
+    case SRC_U8_CR: load_tile<SRC_U8_CR>(R, J, sx0, sy0); break;
+    case SRC_F32_Y: load_tile<SRC_F32_Y>(R, J, sx0, sy0); break;
+    case SRC_F32_CB: load_tile<SRC_F32_CB>(R, J, sx0, sy0); break;
+    default: load_tile<SRC_F32_CR>(R, J, sx0, sy0); break;
+    }
     __syncthreads();
 
-    // ---- row pass: one row per wave, one (even, odd) sample pair per lane, 4 rows at a time
+    // ---- row pass: one row per wave, one (even, odd) pair per lane, 4 rows at a time
     if (w >= 2) {
-        for (int r0 = wave * 4; r0 < 2 * RP; r0 += 8) {
+        for (int r0 = wave * 4; r0 < 2 * RP; r0 += 16) {
             float e[4], o[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -229,9 +294,8 @@ __global__ void __launch_bounds__(128) k_dwt_fwd(const DwtfJob *__restrict__ job
             }
         }
     } else {
-        // n < 2: the row transform is the identity (src/dwt.c:46); keep sample 0 as the
-        // low column and its copy as the (unused) high column
-        for (int r = wave; r < 2 * RP; r += 2) {
+        // n < 2: the row transform is the identity (src/dwt.c:46)
+        for (int r = wave; r < 2 * RP; r += 4) {
             const float v = R[r][0];
             R[r][64 + lane] = v;
             R[r][lane] = v;
@@ -239,18 +303,20 @@ __global__ void __launch_bounds__(128) k_dwt_fwd(const DwtfJob *__restrict__ job
     }
     __syncthreads();
 
-    // ---- column pass: one column per thread (0..63 low columns, 64..127 high columns)
-    const int c = threadIdx.x;
-    float e[RP], o[RP];
+    // ---- column pass: thread = (column, half); half 0 lifts pairs 0..17 and writes
+    // output pairs 2..15, half 1 lifts pairs 14..31 and writes 16..29
+    const int c = threadIdx.x & 127, half = threadIdx.x >> 7;
+    const int k0 = half * (RP / 2 - 2);
+    float e[HALF_PAIRS], o[HALF_PAIRS];
 #pragma unroll
-    for (int k = 0; k < RP; ++k) {
-        e[k] = R[2 * k][c];
-        o[k] = R[2 * k + 1][c];
+    for (int k = 0; k < HALF_PAIRS; ++k) {
+        e[k] = R[2 * (k0 + k)][c];
+        o[k] = R[2 * (k0 + k) + 1][c];
     }
     if (h >= 2) {
-        lift_fwd_arr<RP>(e, o);
+        lift_fwd_arr<HALF_PAIRS>(e, o);
 #pragma unroll
-        for (int k = 0; k < RP; ++k) {
+        for (int k = 0; k < HALF_PAIRS; ++k) {
             e[k] *= WEMA_KINV;
             o[k] *= WEMA_K;
         }
@@ -261,28 +327,30 @@ __global__ void __launch_bounds__(128) k_dwt_fwd(const DwtfJob *__restrict__ job
     const bool col_ok = (c & 63) >= 2 && (c & 63) < 62 && px < sw;
     const int hw_valid = w >> 1, hh_valid = h >> 1;   // written extents (src/dwt.c:463-469)
     const size_t n = (size_t)sw * sh;
-    float *LH = J.planes, *HH = J.planes + n, *HL = J.planes + 2 * n;
+    GLOBAL_AS float *LH = gp(J.planes), *HH = LH + n, *HL = LH + 2 * n;
+    GLOBAL_AS float *LL = gp(J.ll);
+    const GLOBAL_AS float *prev = gp(J.prev);
     const size_t pn = (size_t)w * h;                  // previous level's plane size
     unsigned long long part = 0ull;
     if (col_ok) {
 #pragma unroll
-        for (int k = 2; k < RP - 2; ++k) {
-            const int py = py0 + k - 2;
+        for (int k = 2; k < HALF_PAIRS - 2; ++k) {
+            const int py = py0 + k0 + k - 2;
             if (py < sh) {
                 const size_t idx = (size_t)py * sw + px;
                 if (!high) {
-                    J.ll[idx] = e[k];
+                    LL[idx] = e[k];
                     float vlh = o[k];
-                    if (py >= hh_valid) vlh = J.prev ? J.prev[idx] : 0.0f;
+                    if (py >= hh_valid) vlh = J.prev ? prev[idx] : 0.0f;
                     LH[idx] = vlh;
                     part += abs_fixed(vlh);
                 } else {
                     float vhl = e[k], vhh = o[k];
                     if (px >= hw_valid) {
-                        vhl = J.prev ? J.prev[2 * pn + idx] : 0.0f;
-                        vhh = J.prev ? J.prev[pn + idx] : 0.0f;
+                        vhl = J.prev ? prev[2 * pn + idx] : 0.0f;
+                        vhh = J.prev ? prev[pn + idx] : 0.0f;
                     } else if (py >= hh_valid) {
-                        vhh = J.prev ? J.prev[pn + idx] : 0.0f;
+                        vhh = J.prev ? prev[pn + idx] : 0.0f;
                     }
                     HL[idx] = vhl;
                     HH[idx] = vhh;
@@ -300,7 +368,7 @@ __global__ void __launch_bounds__(128) k_dwt_fwd(const DwtfJob *__restrict__ job
 // ---------------------------------------------------------------------------------------
 // Inverse DWT level (src/dwt.c:475-582 + 705-767): packed [LL | HL ; LH | HH] columns
 // first, then rows; each 1D inverse starts by undoing the K scaling.
-__global__ void __launch_bounds__(128) k_dwt_inv(const DwtiJob *__restrict__ jobs, int njobs) {
+__global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv(const DwtiJob *__restrict__ jobs, int njobs) {
     constexpr int RP = WEMA_RP;
     constexpr int ROWS = 2 * (RP - 4);
     __shared__ float R[ROWS][128];
@@ -317,18 +385,19 @@ __global__ void __launch_bounds__(128) k_dwt_inv(const DwtiJob *__restrict__ job
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t n = (size_t)sw * ((h + 1) >> 1);
 
-    // ---- column pass
-    const int c = threadIdx.x;
+    // ---- column pass: thread = (column, half) as in the forward kernel
+    const int c = threadIdx.x & 127, half = threadIdx.x >> 7;
+    const int k0 = half * (RP / 2 - 2);
     const bool high = c >= 64;
     const int I = px0 - 2 + (c & 63);
     const int sx = reflect_idx(2 * I + (high ? 1 : 0), w);
     const int col = sx >> 1;
-    const float *Ep = high ? J.planes + 2 * n : J.ll;   // HL : LL
-    const float *Op = high ? J.planes + n : J.planes;   // HH : LH
-    float e[RP], o[RP];
+    const GLOBAL_AS float *Ep = high ? gp(J.planes) + 2 * n : gp(J.ll);   // HL : LL
+    const GLOBAL_AS float *Op = high ? gp(J.planes) + n : gp(J.planes);   // HH : LH
+    float e[HALF_PAIRS], o[HALF_PAIRS];
 #pragma unroll
-    for (int k = 0; k < RP; ++k) {
-        const int jy = py0 - 2 + k;
+    for (int k = 0; k < HALF_PAIRS; ++k) {
+        const int jy = py0 - 2 + k0 + k;
         const int ye = reflect_idx(2 * jy, h) >> 1;
         const int yo = reflect_idx(2 * jy + 1, h) >> 1;
         e[k] = Ep[(size_t)ye * sw + col];
@@ -336,35 +405,50 @@ __global__ void __launch_bounds__(128) k_dwt_inv(const DwtiJob *__restrict__ job
     }
     if (h >= 2) {
 #pragma unroll
-        for (int k = 0; k < RP; ++k) {
+        for (int k = 0; k < HALF_PAIRS; ++k) {
             e[k] *= WEMA_K;
             o[k] *= WEMA_KINV;
         }
-        lift_inv_arr<RP>(e, o);
+        lift_inv_arr<HALF_PAIRS>(e, o);
     }
 #pragma unroll
-    for (int k = 2; k < RP - 2; ++k) {
-        R[2 * (k - 2)][c] = e[k];
-        R[2 * (k - 2) + 1][c] = o[k];
+    for (int k = 2; k < HALF_PAIRS - 2; ++k) {
+        const int row = 2 * (k0 + k - 2);
+        R[row][c] = e[k];
+        R[row + 1][c] = o[k];
     }
     __syncthreads();
 
-    // ---- row pass
+    // ---- row pass: 2 rows at a time per wave
     const int x = 2 * (px0 - 2 + lane);
     const bool lane_ok = lane >= 2 && lane < 62;
-    for (int r = wave; r < ROWS; r += 2) {
-        const int y = 2 * py0 + r;
-        if (y >= h) break;
-        float ev = R[r][lane], ov = R[r][64 + lane];
+    for (int r0 = wave * 2; r0 < ROWS; r0 += 8) {
+        const int y0 = 2 * py0 + r0;
+        if (y0 >= h) break;
+        float ev[2], ov[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            ev[k] = R[r0 + k][lane];
+            ov[k] = R[r0 + k][64 + lane];
+        }
         if (w >= 2) {
-            ev *= WEMA_K;
-            ov *= WEMA_KINV;
-            lift_inv_lanes(ev, ov);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                ev[k] *= WEMA_K;
+                ov[k] *= WEMA_KINV;
+                lift_inv_lanes(ev[k], ov[k]);
+            }
         }
         if (lane_ok) {
-            float *row = J.out + (size_t)y * w;
-            if (x < w) row[x] = ev;
-            if (x + 1 < w) row[x + 1] = ov;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int y = y0 + k;
+                if (y < h) {
+                    GLOBAL_AS float *row = gp(J.out) + (size_t)y * w;
+                    if (x < w) row[x] = ev[k];
+                    if (x + 1 < w) row[x + 1] = ov[k];
+                }
+            }
         }
     }
 }
@@ -415,27 +499,42 @@ __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ job
     for (int i = 0; i < 10; ++i) c[i] = J.c[i];
     const int nframes = J.nframes;
     const long long stride = J.coef_stride;
+    GLOBAL_AS float *coef = gp(J.coef);
+    GLOBAL_AS float *sp = gp((float *)J.state) + 4 * p;
     float4 st;
     if (J.preseed) {
         st = make_float4(0.f, 0.f, 0.f, 0.f);
-        const float x0 = J.coef[p];
+        const float x0 = coef[p];
         for (int it = 0; it < J.warmup; ++it) (void)biquad2(c, st, x0);
     } else {
-        st = J.state[p];
+        st = make_float4(sp[0], sp[1], sp[2], sp[3]);
     }
     const float lscale = p < J.level0_end ? 0.5f : 1.0f;
-    for (int f = 0; f < nframes; ++f) {
-        const float T = sT[f];
-        const float a = sA[f] * lscale;
-        float *cp = J.coef + (size_t)f * stride + p;
-        const float x = *cp;
-        const float y = biquad2(c, st, x);
-        const float out = (fabsf(x) <= T) ? x : fmaf(a, y, x);
-        if (J.amp_out) J.amp_out[(size_t)f * J.out_stride + p] = out;
-        else *cp = out;
-        if (J.filt_out) J.filt_out[(size_t)f * J.out_stride + p] = y;
+    // frames in groups of 4 with the coefficient loads issued first (independent of the
+    // state recurrence), so several loads per thread are in flight
+    constexpr int G = 4;
+    for (int f0 = 0; f0 < nframes; f0 += G) {
+        float x[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            x[k] = f0 + k < nframes ? coef[(size_t)(f0 + k) * stride + p] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int f = f0 + k;
+            if (f >= nframes) break;
+            const float T = sT[f];
+            const float a = sA[f] * lscale;
+            const float y = biquad2(c, st, x[k]);
+            const float out = (fabsf(x[k]) <= T) ? x[k] : fmaf(a, y, x[k]);
+            if (J.amp_out) gp(J.amp_out)[(size_t)f * J.out_stride + p] = out;
+            else coef[(size_t)f * stride + p] = out;
+            if (J.filt_out) gp(J.filt_out)[(size_t)f * J.out_stride + p] = y;
+        }
     }
-    J.state[p] = st;
+    sp[0] = st.x;
+    sp[1] = st.y;
+    sp[2] = st.z;
+    sp[3] = st.w;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -507,10 +606,30 @@ __device__ __forceinline__ float src_luma(const EgressJob &J, size_t idx) {
     return luma_of(p[0], p[1], p[2]);
 }
 
-__device__ __forceinline__ float clip_count(int x, int n) {
+// Clipped 9-tap window count at position x of n (src/wema.c:144-147).
+__device__ __forceinline__ int clip_count(int x, int n) {
     const int lo = x - 4 > 0 ? x - 4 : 0;
     const int hi = x + 5 < n ? x + 5 : n;
-    return (float)(hi > lo ? hi - lo : 1);
+    return hi > lo ? hi - lo : 1;
+}
+
+// x / d for a window count d in 1..9, correctly rounded: q = x * RN(1/d) (compile-time
+// constants), then one residual correction (the residual fma(-q, d, x) is exact). Inputs
+// below 2^-100 in magnitude are scaled by 2^64 first and the quotient scaled back (both
+// exact, or flushed like the division's own tiny results). Equal to IEEE x / d for every
+// finite float x and each d in 1..9: checked exhaustively on the device through
+// wema_dev_debug_divcheck.
+__device__ __forceinline__ float div_count(float x, int d) {
+    const float r = d == 9 ? 1.0f / 9.0f : d == 8 ? 1.0f / 8.0f : d == 7 ? 1.0f / 7.0f
+                  : d == 6 ? 1.0f / 6.0f : d == 5 ? 1.0f / 5.0f : d == 4 ? 1.0f / 4.0f
+                  : d == 3 ? 1.0f / 3.0f : d == 2 ? 1.0f / 2.0f : 1.0f;
+    const float df = (float)d;
+    const bool tiny = fabsf(x) < 0x1p-100f;
+    const float xs = tiny ? x * 0x1p64f : x;
+    const float q = xs * r;
+    const float e = fmaf(-q, df, xs);
+    const float q1 = fmaf(e, r, q);
+    return tiny ? q1 * 0x1p-64f : q1;
 }
 
 constexpr int GF_EX = WEMA_GF_TX + 8, GF_EY = WEMA_GF_TY + 8;
@@ -540,7 +659,7 @@ __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs,
     __syncthreads();
     for (int i = threadIdx.x; i < GF_EY * WEMA_GF_TX; i += 256) {
         const int ey = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
-        const float cnt = clip_count(x0 + x, w);
+        const int cnt = clip_count(x0 + x, w);
         float a = 0.0f, b = 0.0f, c2 = 0.0f, d = 0.0f;
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
@@ -550,17 +669,17 @@ __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs,
             c2 += I * I;
             d += I * p;
         }
-        hI[ey][x] = a / cnt;
-        hP[ey][x] = b / cnt;
-        hII[ey][x] = c2 / cnt;
-        hIP[ey][x] = d / cnt;
+        hI[ey][x] = div_count(a, cnt);
+        hP[ey][x] = div_count(b, cnt);
+        hII[ey][x] = div_count(c2, cnt);
+        hIP[ey][x] = div_count(d, cnt);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < WEMA_GF_TY * WEMA_GF_TX; i += 256) {
         const int y = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
         const int gx = x0 + x, gy = y0 + y;
         if (gx >= w || gy >= h) continue;
-        const float cnt = clip_count(gy, h);
+        const int cnt = clip_count(gy, h);
         float a = 0.0f, b = 0.0f, c2 = 0.0f, d = 0.0f;
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
@@ -569,7 +688,8 @@ __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs,
             c2 += hII[y + k][x];
             d += hIP[y + k][x];
         }
-        const float mI = a / cnt, mp = b / cnt, cI = c2 / cnt, cIp = d / cnt;
+        const float mI = div_count(a, cnt), mp = div_count(b, cnt);
+        const float cI = div_count(c2, cnt), cIp = div_count(d, cnt);
         const float var = fmaf(-mI, mI, cI);
         const float cov = fmaf(-mI, mp, cIp);
         const float av = cov / (var + 0.01f);
@@ -615,22 +735,22 @@ __global__ void __launch_bounds__(256) k_gf2(const EgressJob *__restrict__ jobs,
     __syncthreads();
     for (int i = threadIdx.x; i < GF_EY * WEMA_GF_TX; i += 256) {
         const int ey = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
-        const float cnt = clip_count(x0 + x, w);
+        const int cnt = clip_count(x0 + x, w);
         float a = 0.0f, b = 0.0f;
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             a += sA[ey][x + k];
             b += sB[ey][x + k];
         }
-        hA[ey][x] = a / cnt;
-        hB[ey][x] = b / cnt;
+        hA[ey][x] = div_count(a, cnt);
+        hB[ey][x] = div_count(b, cnt);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < WEMA_GF_TY * WEMA_GF_TX; i += 256) {
         const int y = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
         const int gx = x0 + x, gy = y0 + y;
         if (gx >= w || gy >= h) continue;
-        const float cnt = clip_count(gy, h);
+        const int cnt = clip_count(gy, h);
         float a = 0.0f, b = 0.0f;
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
@@ -638,7 +758,7 @@ __global__ void __launch_bounds__(256) k_gf2(const EgressJob *__restrict__ jobs,
             b += hB[y + k][x];
         }
         const size_t idx = (size_t)gy * w + gx;
-        const float q = fmaf(a / cnt, src_luma(J, idx), b / cnt);
+        const float q = fmaf(div_count(a, cnt), src_luma(J, idx), div_count(b, cnt));
         write_rgb(J, idx, q);
     }
 }
@@ -745,18 +865,33 @@ __global__ void __launch_bounds__(256) k_synth(const SynthFrame *__restrict__ fr
     o[2] = F.color ? synth_pixel(F, tq, x, y, 2) : o[0];
 }
 
+// Exhaustive check of div_count against IEEE division for one divisor: every finite float
+// bit pattern, counted per mismatch.
+__global__ void __launch_bounds__(256) k_divcheck(int d, unsigned long long *mism) {
+    unsigned long long bad = 0;
+    const unsigned stride = gridDim.x * 256u;
+    for (unsigned long long u = (unsigned long long)blockIdx.x * 256u + threadIdx.x; u < (1ull << 32); u += stride) {
+        const float x = __uint_as_float((unsigned)u);
+        if (!(fabsf(x) <= 3.4e38f)) continue;   // inf / nan
+        const float a = div_count(x, d), b = x / (float)d;
+        bad += __float_as_uint(a) != __float_as_uint(b);
+    }
+    bad = wave_sum_u64(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(mism, bad);
+}
+
 // ---------------------------------------------------------------------------------------
 // launchers (called by the host scheduler)
 
 extern "C" {
 hipError_t wema_launch_dwt_fwd(const DwtfJob *jobs, int njobs, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dwt_fwd, dim3(nblocks), dim3(128), 0, s, jobs, njobs);
+    hipLaunchKernelGGL(k_dwt_fwd, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 hipError_t wema_launch_dwt_inv(const DwtiJob *jobs, int njobs, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dwt_inv, dim3(nblocks), dim3(128), 0, s, jobs, njobs);
+    hipLaunchKernelGGL(k_dwt_inv, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 hipError_t wema_launch_iir(const TemporalJob *jobs, int njobs, int nblocks, hipStream_t s) {
@@ -779,6 +914,10 @@ hipError_t wema_launch_egress(int kind, const EgressJob *jobs, int njobs, int nb
     case 3: hipLaunchKernelGGL(k_ycc, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     default: hipLaunchKernelGGL(k_passthru, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     }
+    return hipGetLastError();
+}
+hipError_t wema_launch_divcheck(int d, unsigned long long *mism, hipStream_t s) {
+    hipLaunchKernelGGL(k_divcheck, dim3(4096), dim3(256), 0, s, d, mism);
     return hipGetLastError();
 }
 hipError_t wema_launch_synth(const SynthFrame *frames, const unsigned short *tex,
