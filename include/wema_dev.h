@@ -109,9 +109,6 @@ int wema_dev_debug_thresholds(wema_dev_t *s, float *thr, int n);
 int wema_dev_debug_capture(wema_dev_t *s, int enable);
 /* Override the thresholds of the next process call: thr[f * 3 + ch]; NULL clears. */
 int wema_dev_debug_set_thresholds(wema_dev_t *s, const float *thr, int n);
-/* Runs the guided filter's division by a window count d (1..9) on every finite float and
- * counts results that differ from IEEE x / d. */
-int wema_dev_debug_divcheck(int d, unsigned long long *mismatches);
 /* Copy a device buffer of frame f (of the last call) to host:
  * which 0: original coefficients (unique planes, n_unique floats),
  *       1: temporal filter output (IIR y / window delta), 2: amplified coefficients,

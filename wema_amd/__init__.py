@@ -85,7 +85,6 @@ _SIGS = {
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "wema_dev_debug_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
     "wema_dev_debug_set_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
-    "wema_dev_debug_divcheck": (C.c_int, [C.c_int, C.POINTER(C.c_ulonglong)]),
     "wema_dev_debug_read": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_size_t]),
     "wema_synth_preset": (C.c_int, [C.POINTER(SynthParams), C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint32]),
     "wema_synth_texture_host": (C.c_int, [C.POINTER(SynthParams), _vp]),
@@ -254,14 +253,6 @@ class Session:
         out = np.zeros(n, np.float32)
         _check(lib.wema_dev_debug_read(self._h, which, frame, ch, _ptr(out), n), "debug_read")
         return out
-
-
-def divcheck(d):
-    """Mismatches of the device's window-count division against IEEE division (all
-    finite floats)."""
-    m = C.c_ulonglong()
-    _check(lib.wema_dev_debug_divcheck(int(d), C.byref(m)), "divcheck")
-    return m.value
 
 
 # ---------------------------------------------------------------------------------------

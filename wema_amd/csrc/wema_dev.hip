@@ -23,7 +23,6 @@ hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, hipStream_t);
 hipError_t wema_launch_iir(const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_win(const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_egress(int, const EgressJob *, int, int, hipStream_t);
-hipError_t wema_launch_divcheck(int, unsigned long long *, hipStream_t);
 hipError_t wema_launch_synth(const SynthFrame *, const unsigned short *, unsigned char *, int,
                              int, int, hipStream_t);
 }
@@ -689,18 +688,6 @@ extern "C" int wema_dev_debug_capture(wema_dev_t *s, int enable) {
     if ((enable != 0) == (s->with_filt != 0)) return 0;
     if (s->frames > 0) { wema_set_error("capture must be enabled before the first frame"); return -1; }
     return wema_session_alloc(s, enable != 0);
-}
-
-extern "C" int wema_dev_debug_divcheck(int d, unsigned long long *mismatches) {
-    if (!mismatches || d < 1 || d > 9) { wema_set_error("bad argument"); return -1; }
-    unsigned long long *dm = nullptr;
-    HIP_OK(hipMalloc(&dm, sizeof(*dm)));
-    hipError_t e = hipMemset(dm, 0, sizeof(*dm));
-    if (e == hipSuccess) e = wema_launch_divcheck(d, dm, nullptr);
-    if (e == hipSuccess) e = hipMemcpy(mismatches, dm, sizeof(*dm), hipMemcpyDeviceToHost);
-    hipFree(dm);
-    if (e != hipSuccess) { wema_set_error("divcheck: %s", hipGetErrorString(e)); return -1; }
-    return 0;
 }
 
 extern "C" int wema_dev_debug_thresholds(wema_dev_t *s, float *thr, int n) {

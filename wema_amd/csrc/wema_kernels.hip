@@ -59,11 +59,16 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 }
 
 // Whole-sample symmetric extension of index i into [0, n) (src/dwt.c:63-98 boundary rules).
+// Tiles reach at most a few samples past an edge, so the modulo is only needed for tiny n.
 __device__ __forceinline__ int reflect_idx(int i, int n) {
+    if ((unsigned)i < (unsigned)n) return i;
     if (n <= 1) return 0;
+    if (i < 0) i = -i;
+    if (i < n) return i;
+    const int j = 2 * n - 2 - i;
+    if (j >= 0) return j;
     const int period = 2 * n - 2;
     i %= period;
-    if (i < 0) i += period;
     return i < n ? i : period - i;
 }
 
@@ -84,28 +89,6 @@ __device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (v 
 // src/ffmpeg_io.c:304-310
 __device__ __forceinline__ unsigned char egress_u8(float v) {
     return (unsigned char)fmaf(clamp01(v), 255.0f, 0.5f);
-}
-
-__device__ __forceinline__ float load_src(const DwtfJob &J, int x, int y) {
-    const size_t idx = (size_t)y * J.w + x;
-    switch (J.src_kind) {
-    case SRC_F32_PLANE:
-        return ((const float *)J.src)[idx];
-    case SRC_U8_Y:
-    case SRC_U8_CB:
-    case SRC_U8_CR: {
-        const unsigned char *p = (const unsigned char *)J.src + idx * 3;
-        float r = u8f(p[0]), g = u8f(p[1]), b = u8f(p[2]);
-        return J.src_kind == SRC_U8_Y ? luma_of(r, g, b)
-             : J.src_kind == SRC_U8_CB ? cb_of(r, g, b) : cr_of(r, g, b);
-    }
-    default: {
-        const float *p = (const float *)J.src + idx * 3;
-        float r = p[0], g = p[1], b = p[2];
-        return J.src_kind == SRC_F32_Y ? luma_of(r, g, b)
-             : J.src_kind == SRC_F32_CB ? cb_of(r, g, b) : cr_of(r, g, b);
-    }
-    }
 }
 
 // |v| rounded to a multiple of 2^-32, as an integer (order-independent threshold sums).
@@ -185,6 +168,122 @@ __device__ __forceinline__ void lift_inv_arr(float (&e)[N], float (&o)[N]) {
     for (int k = 0; k < N - 1; ++k) o[k] = fmaf(-WEMA_ALPHA, e[k] + e[k + 1], o[k]);
 }
 
+// Channel value of an RGB24 / float RGB pixel.
+template <int KIND>
+__device__ __forceinline__ float chan_of(float r, float g, float b) {
+    if (KIND == SRC_U8_Y || KIND == SRC_F32_Y) return luma_of(r, g, b);
+    if (KIND == SRC_U8_CB || KIND == SRC_F32_CB) return cb_of(r, g, b);
+    return cr_of(r, g, b);
+}
+
+// One input sample of the job (slow path: any position).
+template <int KIND>
+__device__ __forceinline__ float sample_at(const DwtfJob &J, int x, int y) {
+    const size_t idx = (size_t)y * J.w + x;
+    if (KIND == SRC_F32_PLANE) return gp((const float *)J.src)[idx];
+    if (KIND >= SRC_F32_Y) {
+        const GLOBAL_AS float *p = gp((const float *)J.src) + idx * 3;
+        return chan_of<KIND>(p[0], p[1], p[2]);
+    }
+    const GLOBAL_AS unsigned char *p = gp((const unsigned char *)J.src) + idx * 3;
+    return chan_of<KIND>(u8f(p[0]), u8f(p[1]), u8f(p[2]));
+}
+
+// Raw data of the sample pair (x, x + 1) of row y, loaded without using it, so that all
+// pairs of a wave's rows are in flight together. RGB24: the 6 bytes from the aligned
+// dwords that hold them (d2 only when they start at byte 3 of a dword); float: 2 dwords.
+struct PairRaw {
+    unsigned d0, d1, d2, sh;
+};
+
+template <int KIND>
+__device__ __forceinline__ PairRaw pair_load(const DwtfJob &J, int x, int y) {
+    PairRaw r;
+    const size_t idx = (size_t)y * J.w + x;
+    if (KIND == SRC_F32_PLANE) {
+        const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)gp((const float *)J.src) + idx;
+        r.d0 = q[0];
+        r.d1 = q[1];
+        r.d2 = 0u;
+        r.sh = 0u;
+    } else if (KIND >= SRC_F32_Y) {
+        r.d0 = r.d1 = r.d2 = r.sh = 0u;   // float RGB uses the per-sample path
+    } else {
+        const size_t a = (size_t)J.src + idx * 3;
+        r.sh = (unsigned)(a & 3u);
+        const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(a - r.sh);
+        r.d0 = q[0];
+        r.d1 = q[1];
+        r.d2 = r.sh == 3u ? q[2] : 0u;
+    }
+    return r;
+}
+
+template <int KIND>
+__device__ __forceinline__ void pair_decode(const PairRaw &r, float &e, float &o) {
+    if (KIND == SRC_F32_PLANE) {
+        e = __uint_as_float(r.d0);
+        o = __uint_as_float(r.d1);
+    } else {
+        const unsigned v0 = __builtin_amdgcn_alignbyte(r.d1, r.d0, r.sh);
+        const unsigned v1 = __builtin_amdgcn_alignbyte(r.d2, r.d1, r.sh);
+        e = chan_of<KIND>(u8f(v0 & 0xFFu), u8f((v0 >> 8) & 0xFFu), u8f((v0 >> 16) & 0xFFu));
+        o = chan_of<KIND>(u8f(v0 >> 24), u8f(v1 & 0xFFu), u8f((v1 >> 8) & 0xFFu));
+    }
+}
+
+// Row pass of one wave: rows wave, wave + 4, ... of the extended tile; lane = sample pair.
+// All loads of the wave's 16 rows are issued first; the rows are then lifted 4 at a time.
+template <int KIND>
+__device__ __forceinline__ void row_pass(float (*R)[128], const DwtfJob &J, int sx0, int sy0,
+                                         int lane, int wave) {
+    constexpr int NR = 2 * WEMA_RP / 4;
+    const int w = J.w, h = J.h;
+    const int xa = reflect_idx(sx0 + 2 * lane, w), xb = reflect_idx(sx0 + 2 * lane + 1, w);
+    const bool fast_x = KIND != SRC_F32_Y && KIND != SRC_F32_CB && KIND != SRC_F32_CR && xb == xa + 1;
+    PairRaw raw[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int y = reflect_idx(sy0 + wave + 4 * i, h);
+        // the pair's dwords stay inside the buffer unless it ends the last row
+        const bool fast = fast_x && !(y == h - 1 && xb == w - 1);
+        if (fast) raw[i] = pair_load<KIND>(J, xa, y);
+    }
+#pragma unroll
+    for (int g = 0; g < NR; g += 4) {
+        float e[4], o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = g + k;
+            const int y = reflect_idx(sy0 + wave + 4 * i, h);
+            const bool fast = fast_x && !(y == h - 1 && xb == w - 1);
+            if (fast) {
+                pair_decode<KIND>(raw[i], e[k], o[k]);
+            } else {
+                e[k] = sample_at<KIND>(J, xa, y);
+                o[k] = sample_at<KIND>(J, xb, y);
+            }
+        }
+        if (w >= 2) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lift_fwd_lanes(e[k], o[k]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                R[wave + 4 * (g + k)][lane] = e[k] * WEMA_KINV;
+                R[wave + 4 * (g + k)][64 + lane] = o[k] * WEMA_K;
+            }
+        } else {
+            // n < 2: the row transform is the identity (src/dwt.c:46); every sample of
+            // the row is the one pixel
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                R[wave + 4 * (g + k)][lane] = e[k];
+                R[wave + 4 * (g + k)][64 + lane] = e[k];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Forward DWT level (src/dwt.c:369-473 + 645-703).
 //
@@ -193,7 +292,8 @@ __device__ __forceinline__ void lift_inv_arr(float (&e)[N], float (&o)[N]) {
 // 4 lifting steps (each widens the dependency by one pair). Samples outside the image are
 // taken from the whole-sample symmetric extension, which reproduces the reference's
 // per-step boundary rules bit for bit (the extended values are equal sums of the same
-// operands). Row pass first, column pass second, as the reference.
+// operands). Row pass (registers, lane shifts) first, column pass (LDS) second, as the
+// reference.
 //
 // Where the parent size is odd, the reference's extraction leaves the last HL/HH column or
 // LH/HH row of the subband unwritten and its scratch buffer keeps the previous level's
@@ -202,51 +302,6 @@ __device__ __forceinline__ void lift_inv_arr(float (&e)[N], float (&o)[N]) {
 
 constexpr int DWT_THREADS = 256;
 constexpr int HALF_PAIRS = WEMA_RP / 2 + 2;   // pairs a half-column thread lifts
-
-// RGB24 pixel idx as a float channel value. The pixel's 3 bytes come from the aligned
-// dword(s) holding them: one dword when the pixel starts at byte 0 or 1 of a dword, two
-// when it straddles (the second dword then lies inside the next pixel); the last pixel of
-// the buffer is read byte by byte.
-template <int KIND>
-__device__ __forceinline__ float rgb_sample(const unsigned char *frame, size_t idx, bool last) {
-    float r, g, b;
-    if (KIND == SRC_F32_Y || KIND == SRC_F32_CB || KIND == SRC_F32_CR) {
-        const GLOBAL_AS float *p = gp((const float *)frame) + idx * 3;
-        r = p[0]; g = p[1]; b = p[2];
-    } else {
-        const GLOBAL_AS unsigned char *p = gp(frame) + idx * 3;
-        unsigned v;
-        if (!last) {
-            const size_t a = (size_t)(const void *)frame + idx * 3;
-            const unsigned sh = (unsigned)(a & 3u);
-            const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(p - sh);
-            const unsigned lo = q[0];
-            const unsigned hi = sh >= 2 ? q[1] : 0u;
-            v = __builtin_amdgcn_alignbyte(hi, lo, sh);
-        } else {
-            v = (unsigned)p[0] | ((unsigned)p[1] << 8) | ((unsigned)p[2] << 16);
-        }
-        r = u8f(v & 0xFFu); g = u8f((v >> 8) & 0xFFu); b = u8f((v >> 16) & 0xFFu);
-    }
-    if (KIND == SRC_U8_Y || KIND == SRC_F32_Y) return luma_of(r, g, b);
-    if (KIND == SRC_U8_CB || KIND == SRC_F32_CB) return cb_of(r, g, b);
-    return cr_of(r, g, b);
-}
-
-template <int KIND>
-__device__ __forceinline__ void load_tile(float (*R)[128], const DwtfJob &J, int sx0, int sy0) {
-    const int c = threadIdx.x & 127, r0 = threadIdx.x >> 7;
-    const int w = J.w, h = J.h;
-    const int x = reflect_idx(sx0 + c, w);
-#pragma unroll
-    for (int k = 0; k < WEMA_RP; ++k) {
-        const int r = r0 + 2 * k;
-        const int y = reflect_idx(sy0 + r, h);
-        const size_t idx = (size_t)y * w + x;
-        if (KIND == SRC_F32_PLANE) R[r][c] = gp((const float *)J.src)[idx];
-        else R[r][c] = rgb_sample<KIND>((const unsigned char *)J.src, idx, x == w - 1 && y == h - 1);
-    }
-}
 
 __global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restrict__ jobs, int njobs) {
     constexpr int RP = WEMA_RP;
@@ -262,44 +317,16 @@ __global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restri
     const int px0 = tx * WEMA_TILE_PAIRS_X, py0 = ty * WEMA_TILE_PAIRS_Y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-    // ---- load the extended tile (2 RP rows x 128 samples), all loads independent
+    // ---- row pass
     const int sx0 = 2 * (px0 - 2), sy0 = 2 * (py0 - 2);
     switch (J.src_kind) {
-    case SRC_F32_PLANE: load_tile<SRC_F32_PLANE>(R, J, sx0, sy0); break;
-    case SRC_U8_Y: load_tile<SRC_U8_Y>(R, J, sx0, sy0); break;
-    case SRC_U8_CB: load_tile<SRC_U8_CB>(R, J, sx0, sy0); break;
-    case SRC_U8_CR: load_tile<SRC_U8_CR>(R, J, sx0, sy0); break;
-    case SRC_F32_Y: load_tile<SRC_F32_Y>(R, J, sx0, sy0); break;
-    case SRC_F32_CB: load_tile<SRC_F32_CB>(R, J, sx0, sy0); break;
-    default: load_tile<SRC_F32_CR>(R, J, sx0, sy0); break;
-    }
-    __syncthreads();
-
-    // ---- row pass: one row per wave, one (even, odd) pair per lane, 4 rows at a time
-    if (w >= 2) {
-        for (int r0 = wave * 4; r0 < 2 * RP; r0 += 16) {
-            float e[4], o[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float2 v = *reinterpret_cast<const float2 *>(&R[r0 + k][2 * lane]);
-                e[k] = v.x;
-                o[k] = v.y;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) lift_fwd_lanes(e[k], o[k]);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                R[r0 + k][lane] = e[k] * WEMA_KINV;
-                R[r0 + k][64 + lane] = o[k] * WEMA_K;
-            }
-        }
-    } else {
-        // n < 2: the row transform is the identity (src/dwt.c:46)
-        for (int r = wave; r < 2 * RP; r += 4) {
-            const float v = R[r][0];
-            R[r][64 + lane] = v;
-            R[r][lane] = v;
-        }
+    case SRC_F32_PLANE: row_pass<SRC_F32_PLANE>(R, J, sx0, sy0, lane, wave); break;
+    case SRC_U8_Y: row_pass<SRC_U8_Y>(R, J, sx0, sy0, lane, wave); break;
+    case SRC_U8_CB: row_pass<SRC_U8_CB>(R, J, sx0, sy0, lane, wave); break;
+    case SRC_U8_CR: row_pass<SRC_U8_CR>(R, J, sx0, sy0, lane, wave); break;
+    case SRC_F32_Y: row_pass<SRC_F32_Y>(R, J, sx0, sy0, lane, wave); break;
+    case SRC_F32_CB: row_pass<SRC_F32_CB>(R, J, sx0, sy0, lane, wave); break;
+    default: row_pass<SRC_F32_CR>(R, J, sx0, sy0, lane, wave); break;
     }
     __syncthreads();
 
@@ -487,6 +514,51 @@ __device__ __forceinline__ void frame_params(const TemporalJob &J, float *sT, fl
     __syncthreads();
 }
 
+// One frame of one position: biquads, then the threshold mask and amplification
+// (src/wema.c:727-743: |c| <= T keeps c, otherwise fma(alpha_l, y, c)).
+__device__ __forceinline__ float iir_amp(const float *c, float4 &st, float x, float T, float a,
+                                         float &y) {
+    y = biquad2(c, st, x);
+    return (fabsf(x) <= T) ? x : fmaf(a, y, x);
+}
+
+template <bool SEP, bool FILT>
+__device__ __forceinline__ void iir_frames(const TemporalJob &J, const float *c, float4 &st,
+                                           const float *sT, const float *sA, float lscale,
+                                           long long p) {
+    // frames in groups of 4 with the coefficient loads issued first (independent of the
+    // state recurrence), so several loads per thread are in flight
+    const int nframes = J.nframes;
+    const size_t stride = (size_t)J.coef_stride, ostride = (size_t)J.out_stride;
+    GLOBAL_AS float *in = gp(J.coef) + p;
+    GLOBAL_AS float *out = (SEP ? gp(J.amp_out) : gp(J.coef)) + p;
+    GLOBAL_AS float *fo = FILT ? gp(J.filt_out) + p : nullptr;
+    const size_t ostep = SEP ? ostride : stride;
+    int f = 0;
+    for (; f + 4 <= nframes; f += 4) {
+        float x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = in[k * stride];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float y;
+            out[k * ostep] = iir_amp(c, st, x[k], sT[f + k], sA[f + k] * lscale, y);
+            if (FILT) fo[k * ostride] = y;
+        }
+        in += 4 * stride;
+        out += 4 * ostep;
+        if (FILT) fo += 4 * ostride;
+    }
+    for (; f < nframes; ++f) {
+        float y;
+        *out = iir_amp(c, st, *in, sT[f], sA[f] * lscale, y);
+        if (FILT) *fo = y;
+        in += stride;
+        out += ostep;
+        if (FILT) fo += ostride;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ jobs, int njobs) {
     __shared__ float sT[WEMA_MAX_BATCH], sA[WEMA_MAX_BATCH];
     const int jid = find_job(jobs, njobs, (int)blockIdx.x);
@@ -497,39 +569,23 @@ __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ job
     float c[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) c[i] = J.c[i];
-    const int nframes = J.nframes;
-    const long long stride = J.coef_stride;
-    GLOBAL_AS float *coef = gp(J.coef);
     GLOBAL_AS float *sp = gp((float *)J.state) + 4 * p;
     float4 st;
     if (J.preseed) {
+        // warm-up on frame 0 (src/iir_filter.c:376-471), register-resident
         st = make_float4(0.f, 0.f, 0.f, 0.f);
-        const float x0 = coef[p];
+        const float x0 = gp(J.coef)[p];
         for (int it = 0; it < J.warmup; ++it) (void)biquad2(c, st, x0);
     } else {
         st = make_float4(sp[0], sp[1], sp[2], sp[3]);
     }
     const float lscale = p < J.level0_end ? 0.5f : 1.0f;
-    // frames in groups of 4 with the coefficient loads issued first (independent of the
-    // state recurrence), so several loads per thread are in flight
-    constexpr int G = 4;
-    for (int f0 = 0; f0 < nframes; f0 += G) {
-        float x[G];
-#pragma unroll
-        for (int k = 0; k < G; ++k)
-            x[k] = f0 + k < nframes ? coef[(size_t)(f0 + k) * stride + p] : 0.0f;
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-            const int f = f0 + k;
-            if (f >= nframes) break;
-            const float T = sT[f];
-            const float a = sA[f] * lscale;
-            const float y = biquad2(c, st, x[k]);
-            const float out = (fabsf(x[k]) <= T) ? x[k] : fmaf(a, y, x[k]);
-            if (J.amp_out) gp(J.amp_out)[(size_t)f * J.out_stride + p] = out;
-            else coef[(size_t)f * stride + p] = out;
-            if (J.filt_out) gp(J.filt_out)[(size_t)f * J.out_stride + p] = y;
-        }
+    if (J.amp_out) {
+        if (J.filt_out) iir_frames<true, true>(J, c, st, sT, sA, lscale, p);
+        else iir_frames<true, false>(J, c, st, sT, sA, lscale, p);
+    } else {
+        if (J.filt_out) iir_frames<false, true>(J, c, st, sT, sA, lscale, p);
+        else iir_frames<false, false>(J, c, st, sT, sA, lscale, p);
     }
     sp[0] = st.x;
     sp[1] = st.y;
@@ -599,10 +655,10 @@ __global__ void __launch_bounds__(256) k_win(const TemporalJob *__restrict__ job
 
 __device__ __forceinline__ float src_luma(const EgressJob &J, size_t idx) {
     if (J.src_u8) {
-        const unsigned char *p = J.src_u8 + idx * 3;
+        const GLOBAL_AS unsigned char *p = gp(J.src_u8) + idx * 3;
         return luma_of(u8f(p[0]), u8f(p[1]), u8f(p[2]));
     }
-    const float *p = J.src_f32 + idx * 3;
+    const GLOBAL_AS float *p = gp(J.src_f32) + idx * 3;
     return luma_of(p[0], p[1], p[2]);
 }
 
@@ -613,29 +669,24 @@ __device__ __forceinline__ int clip_count(int x, int n) {
     return hi > lo ? hi - lo : 1;
 }
 
-// x / d for a window count d in 1..9, correctly rounded: q = x * RN(1/d) (compile-time
-// constants), then one residual correction (the residual fma(-q, d, x) is exact). Inputs
-// below 2^-100 in magnitude are scaled by 2^64 first and the quotient scaled back (both
-// exact, or flushed like the division's own tiny results). Equal to IEEE x / d for every
-// finite float x and each d in 1..9: checked exhaustively on the device through
-// wema_dev_debug_divcheck.
-__device__ __forceinline__ float div_count(float x, int d) {
-    const float r = d == 9 ? 1.0f / 9.0f : d == 8 ? 1.0f / 8.0f : d == 7 ? 1.0f / 7.0f
-                  : d == 6 ? 1.0f / 6.0f : d == 5 ? 1.0f / 5.0f : d == 4 ? 1.0f / 4.0f
-                  : d == 3 ? 1.0f / 3.0f : d == 2 ? 1.0f / 2.0f : 1.0f;
-    const float df = (float)d;
-    const bool tiny = fabsf(x) < 0x1p-100f;
-    const float xs = tiny ? x * 0x1p64f : x;
-    const float q = xs * r;
-    const float e = fmaf(-q, df, xs);
-    const float q1 = fmaf(e, r, q);
-    return tiny ? q1 * 0x1p-64f : q1;
+constexpr int GF_EX = WEMA_GF_TX + 8, GF_EY = WEMA_GF_TY + 8;
+constexpr int GF_PITCH = GF_EX + 4;   // multiple of 4: rows start 16-byte aligned
+
+// 9-tap sums of 4 consecutive outputs from 12 samples held in registers, each summed
+// left to right from +0 over its own window (the same order as one output at a time).
+__device__ __forceinline__ void sums4(const float (&v)[12], float (&out)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float a = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a += v[j + k];
+        out[j] = a;
+    }
 }
 
-constexpr int GF_EX = WEMA_GF_TX + 8, GF_EY = WEMA_GF_TY + 8;
-
 __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs, int njobs) {
-    __shared__ float sI[GF_EY][GF_EX + 1], sP[GF_EY][GF_EX + 1];
+    __shared__ __attribute__((aligned(16))) float sI[GF_EY][GF_PITCH];
+    __shared__ __attribute__((aligned(16))) float sP[GF_EY][GF_PITCH];
     __shared__ float hI[GF_EY][WEMA_GF_TX], hP[GF_EY][WEMA_GF_TX];
     __shared__ float hII[GF_EY][WEMA_GF_TX], hIP[GF_EY][WEMA_GF_TX];
     const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
@@ -644,57 +695,97 @@ __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs,
     const int t = bid - J.tile_begin;
     const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
     const int w = J.w, h = J.h;
-    for (int i = threadIdx.x; i < GF_EY * GF_EX; i += 256) {
+    const GLOBAL_AS float *gray = gp(J.gray);
+    // all loads of the tile issued before any is used (one memory round trip per block)
+    constexpr int NLD = (GF_EY * GF_EX + 255) / 256;
+    float lI[NLD], lP[NLD];
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+        const int i = threadIdx.x + 256 * j;
         const int ey = i / GF_EX, ex = i % GF_EX;
         const int x = x0 - 4 + ex, y = y0 - 4 + ey;
-        float I = 0.0f, p = 0.0f;
-        if (x >= 0 && x < w && y >= 0 && y < h) {
+        float I = 0.0f, g = 0.0f;
+        if (i < GF_EY * GF_EX && x >= 0 && x < w && y >= 0 && y < h) {
             const size_t idx = (size_t)y * w + x;
             I = src_luma(J, idx);
-            p = J.gray[idx] - I;
+            g = gray[idx];
         }
-        sI[ey][ex] = I;
-        sP[ey][ex] = p;
+        lI[j] = I;
+        lP[j] = g;
+    }
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+        const int i = threadIdx.x + 256 * j;
+        if (i < GF_EY * GF_EX) {
+            const int ey = i / GF_EX, ex = i % GF_EX;
+            const int x = x0 - 4 + ex, y = y0 - 4 + ey;
+            const bool in = x >= 0 && x < w && y >= 0 && y < h;
+            sI[ey][ex] = lI[j];
+            sP[ey][ex] = in ? lP[j] - lI[j] : 0.0f;
+        }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < GF_EY * WEMA_GF_TX; i += 256) {
-        const int ey = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
-        const int cnt = clip_count(x0 + x, w);
-        float a = 0.0f, b = 0.0f, c2 = 0.0f, d = 0.0f;
+    // horizontal: 4 outputs per item, taps read as three 16-byte vectors per quantity
+    for (int i = threadIdx.x; i < GF_EY * (WEMA_GF_TX / 4); i += 256) {
+        const int ey = i / (WEMA_GF_TX / 4), x4 = 4 * (i % (WEMA_GF_TX / 4));
+        float vi[12], vp[12], vii[12], vip[12];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const float I = sI[ey][x + k], p = sP[ey][x + k];
-            a += I;
-            b += p;
-            c2 += I * I;
-            d += I * p;
+        for (int q = 0; q < 3; ++q) {
+            const float4 a = *reinterpret_cast<const float4 *>(&sI[ey][x4 + 4 * q]);
+            const float4 b = *reinterpret_cast<const float4 *>(&sP[ey][x4 + 4 * q]);
+            vi[4 * q] = a.x; vi[4 * q + 1] = a.y; vi[4 * q + 2] = a.z; vi[4 * q + 3] = a.w;
+            vp[4 * q] = b.x; vp[4 * q + 1] = b.y; vp[4 * q + 2] = b.z; vp[4 * q + 3] = b.w;
         }
-        hI[ey][x] = div_count(a, cnt);
-        hP[ey][x] = div_count(b, cnt);
-        hII[ey][x] = div_count(c2, cnt);
-        hIP[ey][x] = div_count(d, cnt);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            vii[k] = vi[k] * vi[k];
+            vip[k] = vi[k] * vp[k];
+        }
+        float si[4], spp[4], sii[4], sip[4];
+        sums4(vi, si);
+        sums4(vp, spp);
+        sums4(vii, sii);
+        sums4(vip, sip);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float cnt = (float)clip_count(x0 + x4 + j, w);
+            hI[ey][x4 + j] = si[j] / cnt;
+            hP[ey][x4 + j] = spp[j] / cnt;
+            hII[ey][x4 + j] = sii[j] / cnt;
+            hIP[ey][x4 + j] = sip[j] / cnt;
+        }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < WEMA_GF_TY * WEMA_GF_TX; i += 256) {
-        const int y = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
-        const int gx = x0 + x, gy = y0 + y;
-        if (gx >= w || gy >= h) continue;
-        const int cnt = clip_count(gy, h);
-        float a = 0.0f, b = 0.0f, c2 = 0.0f, d = 0.0f;
+    // vertical: 4 consecutive rows of one column per item
+    for (int i = threadIdx.x; i < (WEMA_GF_TY / 4) * WEMA_GF_TX; i += 256) {
+        const int y4 = 4 * (i / WEMA_GF_TX), x = i % WEMA_GF_TX;
+        const int gx = x0 + x;
+        if (gx >= w) continue;
+        float vi[12], vp[12], vii[12], vip[12];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            a += hI[y + k][x];
-            b += hP[y + k][x];
-            c2 += hII[y + k][x];
-            d += hIP[y + k][x];
+        for (int k = 0; k < 12; ++k) {
+            vi[k] = hI[y4 + k][x];
+            vp[k] = hP[y4 + k][x];
+            vii[k] = hII[y4 + k][x];
+            vip[k] = hIP[y4 + k][x];
         }
-        const float mI = div_count(a, cnt), mp = div_count(b, cnt);
-        const float cI = div_count(c2, cnt), cIp = div_count(d, cnt);
-        const float var = fmaf(-mI, mI, cI);
-        const float cov = fmaf(-mI, mp, cIp);
-        const float av = cov / (var + 0.01f);
-        const float bv = fmaf(-av, mI, mp);
-        J.ab[(size_t)gy * w + gx] = make_float2(av, bv);
+        float si[4], spp[4], sii[4], sip[4];
+        sums4(vi, si);
+        sums4(vp, spp);
+        sums4(vii, sii);
+        sums4(vip, sip);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gy = y0 + y4 + j;
+            if (gy >= h) break;
+            const float cnt = (float)clip_count(gy, h);
+            const float mI = si[j] / cnt, mp = spp[j] / cnt, cI = sii[j] / cnt, cIp = sip[j] / cnt;
+            const float var = fmaf(-mI, mI, cI);
+            const float cov = fmaf(-mI, mp, cIp);
+            const float av = cov / (var + 0.01f);
+            const float bv = fmaf(-av, mI, mp);
+            J.ab[(size_t)gy * w + gx] = make_float2(av, bv);
+        }
     }
 }
 
@@ -716,7 +807,8 @@ __device__ __forceinline__ void write_rgb(const EgressJob &J, size_t idx, float 
 }
 
 __global__ void __launch_bounds__(256) k_gf2(const EgressJob *__restrict__ jobs, int njobs) {
-    __shared__ float sA[GF_EY][GF_EX + 1], sB[GF_EY][GF_EX + 1];
+    __shared__ __attribute__((aligned(16))) float sA[GF_EY][GF_PITCH];
+    __shared__ __attribute__((aligned(16))) float sB[GF_EY][GF_PITCH];
     __shared__ float hA[GF_EY][WEMA_GF_TX], hB[GF_EY][WEMA_GF_TX];
     const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int jid = find_job(jobs, njobs, bid);
@@ -724,42 +816,75 @@ __global__ void __launch_bounds__(256) k_gf2(const EgressJob *__restrict__ jobs,
     const int t = bid - J.tile_begin;
     const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
     const int w = J.w, h = J.h;
-    for (int i = threadIdx.x; i < GF_EY * GF_EX; i += 256) {
+    const GLOBAL_AS float2 *ab = gp(J.ab);
+    constexpr int NLD = (GF_EY * GF_EX + 255) / 256;
+    float la[NLD], lb[NLD];
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+        const int i = threadIdx.x + 256 * j;
         const int ey = i / GF_EX, ex = i % GF_EX;
         const int x = x0 - 4 + ex, y = y0 - 4 + ey;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (x >= 0 && x < w && y >= 0 && y < h) v = J.ab[(size_t)y * w + x];
-        sA[ey][ex] = v.x;
-        sB[ey][ex] = v.y;
+        float va = 0.0f, vb = 0.0f;
+        if (i < GF_EY * GF_EX && x >= 0 && x < w && y >= 0 && y < h) {
+            const GLOBAL_AS float *q = (const GLOBAL_AS float *)(ab + (size_t)y * w + x);
+            va = q[0];
+            vb = q[1];
+        }
+        la[j] = va;
+        lb[j] = vb;
+    }
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+        const int i = threadIdx.x + 256 * j;
+        if (i < GF_EY * GF_EX) {
+            sA[i / GF_EX][i % GF_EX] = la[j];
+            sB[i / GF_EX][i % GF_EX] = lb[j];
+        }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < GF_EY * WEMA_GF_TX; i += 256) {
-        const int ey = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
-        const int cnt = clip_count(x0 + x, w);
-        float a = 0.0f, b = 0.0f;
+    for (int i = threadIdx.x; i < GF_EY * (WEMA_GF_TX / 4); i += 256) {
+        const int ey = i / (WEMA_GF_TX / 4), x4 = 4 * (i % (WEMA_GF_TX / 4));
+        float va[12], vb[12];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            a += sA[ey][x + k];
-            b += sB[ey][x + k];
+        for (int q = 0; q < 3; ++q) {
+            const float4 a = *reinterpret_cast<const float4 *>(&sA[ey][x4 + 4 * q]);
+            const float4 b = *reinterpret_cast<const float4 *>(&sB[ey][x4 + 4 * q]);
+            va[4 * q] = a.x; va[4 * q + 1] = a.y; va[4 * q + 2] = a.z; va[4 * q + 3] = a.w;
+            vb[4 * q] = b.x; vb[4 * q + 1] = b.y; vb[4 * q + 2] = b.z; vb[4 * q + 3] = b.w;
         }
-        hA[ey][x] = div_count(a, cnt);
-        hB[ey][x] = div_count(b, cnt);
+        float sa[4], sb[4];
+        sums4(va, sa);
+        sums4(vb, sb);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float cnt = (float)clip_count(x0 + x4 + j, w);
+            hA[ey][x4 + j] = sa[j] / cnt;
+            hB[ey][x4 + j] = sb[j] / cnt;
+        }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < WEMA_GF_TY * WEMA_GF_TX; i += 256) {
-        const int y = i / WEMA_GF_TX, x = i % WEMA_GF_TX;
-        const int gx = x0 + x, gy = y0 + y;
-        if (gx >= w || gy >= h) continue;
-        const int cnt = clip_count(gy, h);
-        float a = 0.0f, b = 0.0f;
+    for (int i = threadIdx.x; i < (WEMA_GF_TY / 4) * WEMA_GF_TX; i += 256) {
+        const int y4 = 4 * (i / WEMA_GF_TX), x = i % WEMA_GF_TX;
+        const int gx = x0 + x;
+        if (gx >= w) continue;
+        float va[12], vb[12];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            a += hA[y + k][x];
-            b += hB[y + k][x];
+        for (int k = 0; k < 12; ++k) {
+            va[k] = hA[y4 + k][x];
+            vb[k] = hB[y4 + k][x];
         }
-        const size_t idx = (size_t)gy * w + gx;
-        const float q = fmaf(div_count(a, cnt), src_luma(J, idx), div_count(b, cnt));
-        write_rgb(J, idx, q);
+        float sa[4], sb[4];
+        sums4(va, sa);
+        sums4(vb, sb);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gy = y0 + y4 + j;
+            if (gy >= h) break;
+            const float cnt = (float)clip_count(gy, h);
+            const size_t idx = (size_t)gy * w + gx;
+            const float q = fmaf(sa[j] / cnt, src_luma(J, idx), sb[j] / cnt);
+            write_rgb(J, idx, q);
+        }
     }
 }
 
@@ -775,15 +900,24 @@ __global__ void __launch_bounds__(256) k_gauss(const EgressJob *__restrict__ job
     const int t = bid - J.tile_begin;
     const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
     const int w = J.w, h = J.h;
-    for (int i = threadIdx.x; i < EY * EX; i += 256) {
+    constexpr int NLD = (EY * EX + 255) / 256;
+    float lp[NLD];
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+        const int i = threadIdx.x + 256 * j;
         const int ey = i / EX, ex = i % EX;
         const int x = x0 - 2 + ex, y = y0 - 2 + ey;
         float p = 0.0f;
-        if (x >= 0 && x < w && y >= 0 && y < h) {
+        if (i < EY * EX && x >= 0 && x < w && y >= 0 && y < h) {
             const size_t idx = (size_t)y * w + x;
-            p = J.gray[idx] - src_luma(J, idx);
+            p = gp(J.gray)[idx] - src_luma(J, idx);
         }
-        sP[ey][ex] = p;
+        lp[j] = p;
+    }
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+        const int i = threadIdx.x + 256 * j;
+        if (i < EY * EX) sP[i / EX][i % EX] = lp[j];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < EY * WEMA_GF_TX; i += 256) {
@@ -865,21 +999,6 @@ __global__ void __launch_bounds__(256) k_synth(const SynthFrame *__restrict__ fr
     o[2] = F.color ? synth_pixel(F, tq, x, y, 2) : o[0];
 }
 
-// Exhaustive check of div_count against IEEE division for one divisor: every finite float
-// bit pattern, counted per mismatch.
-__global__ void __launch_bounds__(256) k_divcheck(int d, unsigned long long *mism) {
-    unsigned long long bad = 0;
-    const unsigned stride = gridDim.x * 256u;
-    for (unsigned long long u = (unsigned long long)blockIdx.x * 256u + threadIdx.x; u < (1ull << 32); u += stride) {
-        const float x = __uint_as_float((unsigned)u);
-        if (!(fabsf(x) <= 3.4e38f)) continue;   // inf / nan
-        const float a = div_count(x, d), b = x / (float)d;
-        bad += __float_as_uint(a) != __float_as_uint(b);
-    }
-    bad = wave_sum_u64(bad);
-    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(mism, bad);
-}
-
 // ---------------------------------------------------------------------------------------
 // launchers (called by the host scheduler)
 
@@ -914,10 +1033,6 @@ hipError_t wema_launch_egress(int kind, const EgressJob *jobs, int njobs, int nb
     case 3: hipLaunchKernelGGL(k_ycc, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     default: hipLaunchKernelGGL(k_passthru, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     }
-    return hipGetLastError();
-}
-hipError_t wema_launch_divcheck(int d, unsigned long long *mism, hipStream_t s) {
-    hipLaunchKernelGGL(k_divcheck, dim3(4096), dim3(256), 0, s, d, mism);
     return hipGetLastError();
 }
 hipError_t wema_launch_synth(const SynthFrame *frames, const unsigned short *tex,
