@@ -91,6 +91,63 @@ int wema_dev_sync(wema_dev_t *s);
 void *wema_dev_host_alloc(size_t bytes);
 void  wema_dev_host_free(void *p);
 
+/* ---- multi-stream pool (no counterpart in the reference: it processes one stream) --------
+ * Sessions of many streams share one GPU. wema_pool_add admits a session only if the
+ * pool's device bytes stay within hbm_budget (-1 and "HBM budget exceeded" otherwise).
+ * wema_pool_tick processes one frame of each listed session with one launch per pipeline
+ * stage covering all of them; each session's results are identical to running it alone
+ * (no computation crosses sessions). With graphs enabled, the launch sequence of each tick
+ * shape (set of sessions and their output stage) is captured once in a hipGraph and
+ * replayed. NULL stream = the pool's own stream (graphs run only there). */
+typedef struct wema_pool wema_pool_t;
+int    wema_pool_create(wema_pool_t **out, int device, size_t hbm_budget);
+void   wema_pool_destroy(wema_pool_t *p);
+int    wema_pool_add(wema_pool_t *p, const wema_dev_params *prm, int *id);   /* max_batch 1 */
+int    wema_pool_remove(wema_pool_t *p, int id);
+int    wema_pool_session(wema_pool_t *p, int id, wema_dev_t **s);          /* for queries */
+size_t wema_pool_bytes_in_use(const wema_pool_t *p);
+int    wema_pool_graphs(wema_pool_t *p, int enable);
+int    wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8_t *const *d_in,
+                      uint8_t *const *d_out, void *hip_stream);
+int    wema_pool_sync(wema_pool_t *p);
+void  *wema_pool_stream(wema_pool_t *p);
+int    wema_pool_timing(wema_pool_t *p, int enable);
+int    wema_pool_timing_report(wema_pool_t *p, int kind, char *name, int name_len, long *launches,
+                               double *total_ms, double *bytes);
+
+/* ---- serving loop over a pool (no counterpart in the reference) -----------------------
+ * Sessions arrive as a Poisson process (arrival_rate per second until duration_s; each
+ * lives life_min_s..life_max_s), with a resolution from the mix 8:8:6:2 of 640x480,
+ * 1280x720, 1920x1080, 3840x2160, a frame rate from that mix (fps_from_mix) or 30/60 at
+ * random, a band in 0.3..5 Hz and amplification in 20..150. They are admitted first come
+ * first served while the pool's bytes stay within hbm_budget (and max_sessions, if > 0),
+ * and deliver frames at their frame rate. batched: all due frames run as one pool tick;
+ * otherwise one session at a time (the plain path). saturate: `sessions` sessions admitted
+ * at once, taking the mix in its exact 8:8:6:2 proportions, with a frame always due, for
+ * duration_s (capacity). Latency = frame due time to output complete. */
+typedef struct {
+    int      device;
+    double   duration_s, drain_s;
+    double   arrival_rate;
+    double   life_min_s, life_max_s;
+    size_t   hbm_budget;
+    uint64_t seed;
+    int      saturate, sessions;
+    int      batched, use_graphs;
+    int      max_sessions;
+    int      fps_from_mix;
+} wema_serve_config;
+typedef struct {
+    long   frames;
+    double mpix, wall_s, mpix_per_s;
+    double lat_p50_ms, lat_p99_ms, lat_max_ms;
+    int    sessions_arrived, sessions_admitted, admission_waits, peak_sessions;
+    size_t peak_bytes;
+    long   ticks;
+    double mean_frames_per_tick;
+} wema_serve_stats;
+int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *out);
+
 /* ---- measurement (no counterpart in the reference) --------------------------------
  * While enabled, HIP events are recorded on the call's stream between the launches of
  * every process call. wema_dev_timing_report sums them per kernel kind (0 .. kinds-1; it

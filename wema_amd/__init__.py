@@ -62,6 +62,28 @@ class SynthParams(C.Structure):
     ]
 
 
+class ServeConfig(C.Structure):
+    """wema_serve_config (include/wema_dev.h)."""
+    _fields_ = [
+        ("device", C.c_int), ("duration_s", C.c_double), ("drain_s", C.c_double),
+        ("arrival_rate", C.c_double), ("life_min_s", C.c_double), ("life_max_s", C.c_double),
+        ("hbm_budget", C.c_size_t), ("seed", C.c_uint64), ("saturate", C.c_int),
+        ("sessions", C.c_int), ("batched", C.c_int), ("use_graphs", C.c_int),
+        ("max_sessions", C.c_int), ("fps_from_mix", C.c_int),
+    ]
+
+
+class ServeStats(C.Structure):
+    """wema_serve_stats (include/wema_dev.h)."""
+    _fields_ = [
+        ("frames", C.c_long), ("mpix", C.c_double), ("wall_s", C.c_double),
+        ("mpix_per_s", C.c_double), ("lat_p50_ms", C.c_double), ("lat_p99_ms", C.c_double),
+        ("lat_max_ms", C.c_double), ("sessions_arrived", C.c_int), ("sessions_admitted", C.c_int),
+        ("admission_waits", C.c_int), ("peak_sessions", C.c_int), ("peak_bytes", C.c_size_t),
+        ("ticks", C.c_long), ("mean_frames_per_tick", C.c_double),
+    ]
+
+
 _vp = C.c_void_p
 _SIGS = {
     "wema_dev_params_default": (None, [C.POINTER(Params), C.c_int, C.c_int, C.c_float]),
@@ -86,6 +108,20 @@ _SIGS = {
     "wema_dev_debug_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
     "wema_dev_debug_set_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
     "wema_dev_debug_read": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_size_t]),
+    "wema_pool_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_size_t]),
+    "wema_pool_destroy": (None, [_vp]),
+    "wema_pool_add": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(C.c_int)]),
+    "wema_pool_remove": (C.c_int, [_vp, C.c_int]),
+    "wema_pool_session": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "wema_pool_bytes_in_use": (C.c_size_t, [_vp]),
+    "wema_pool_graphs": (C.c_int, [_vp, C.c_int]),
+    "wema_pool_tick": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "wema_pool_sync": (C.c_int, [_vp]),
+    "wema_pool_stream": (_vp, [_vp]),
+    "wema_pool_timing": (C.c_int, [_vp, C.c_int]),
+    "wema_pool_timing_report": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_long),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "wema_serve_run": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(ServeStats)]),
     "wema_synth_preset": (C.c_int, [C.POINTER(SynthParams), C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint32]),
     "wema_synth_texture_host": (C.c_int, [C.POINTER(SynthParams), _vp]),
     "wema_synth_frame_host": (C.c_int, [C.POINTER(SynthParams), _vp, C.c_int, _vp]),
@@ -253,6 +289,95 @@ class Session:
         out = np.zeros(n, np.float32)
         _check(lib.wema_dev_debug_read(self._h, which, frame, ch, _ptr(out), n), "debug_read")
         return out
+
+
+# ---------------------------------------------------------------------------------------
+# multi-stream pool and serving loop
+
+def make_params(width, height, fps, amp=120.0, f_low=0.5, f_high=None, window=32, levels=0,
+                use_iir=True, color=False, edge_aware=True, bilateral=False, batch_path=False,
+                max_batch=1, device=0):
+    p = Params()
+    lib.wema_dev_params_default(C.byref(p), int(width), int(height), float(fps))
+    p.amp, p.f_low = amp, f_low
+    if f_high is not None:
+        p.f_high = f_high
+    p.window, p.levels, p.use_iir, p.color = window, levels, int(use_iir), int(color)
+    p.edge_aware, p.bilateral, p.batch_path = int(edge_aware), int(bilateral), int(batch_path)
+    p.max_batch, p.device = max_batch, device
+    return p
+
+
+class Pool:
+    """Many stream sessions on one GPU, one launch per stage per tick (wema_pool_*)."""
+
+    def __init__(self, hbm_budget, device=0, graphs=False):
+        h = C.c_void_p()
+        _check(lib.wema_pool_create(C.byref(h), device, int(hbm_budget)), "wema_pool_create")
+        self._h = h
+        self.device = device
+        self.graphs(graphs)
+
+    def graphs(self, enable):
+        _check(lib.wema_pool_graphs(self._h, int(enable)), "wema_pool_graphs")
+
+    def add(self, width, height, fps, **kw):
+        p = make_params(width, height, fps, device=self.device, **kw)
+        i = C.c_int()
+        _check(lib.wema_pool_add(self._h, C.byref(p), C.byref(i)), "wema_pool_add")
+        return i.value
+
+    def remove(self, sid):
+        _check(lib.wema_pool_remove(self._h, sid), "wema_pool_remove")
+
+    @property
+    def bytes_in_use(self):
+        return lib.wema_pool_bytes_in_use(self._h)
+
+    @property
+    def stream(self):
+        return lib.wema_pool_stream(self._h)
+
+    def tick(self, ids, ins, outs, stream=None):
+        """One frame of each session ids[i]: device RGB24 buffers ins[i] -> outs[i]."""
+        n = len(ids)
+        a_ids = (C.c_int * n)(*ids)
+        a_in = (C.c_void_p * n)(*[t.data_ptr() for t in ins])
+        a_out = (C.c_void_p * n)(*[t.data_ptr() for t in outs])
+        _check(lib.wema_pool_tick(self._h, n, a_ids, a_in, a_out, stream), "wema_pool_tick")
+
+    def sync(self):
+        _check(lib.wema_pool_sync(self._h), "wema_pool_sync")
+
+    def thresholds(self, sid):
+        s = C.c_void_p()
+        _check(lib.wema_pool_session(self._h, sid, C.byref(s)), "wema_pool_session")
+        t = np.zeros(3, np.float32)
+        _check(lib.wema_dev_debug_thresholds(s, _ptr(t), 1), "debug_thresholds")
+        return t
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.wema_pool_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve(duration_s=10.0, arrival_rate=4.0, life=(2.0, 20.0), hbm_budget=200e9, batched=True,
+          graphs=True, saturate=False, sessions=0, drain_s=0.0, max_sessions=0,
+          fps_from_mix=False, seed=0, device=0):
+    """Runs the serving loop (wema_serve_run) and returns its statistics as a dict."""
+    c = ServeConfig(device, duration_s, drain_s, arrival_rate, life[0], life[1], int(hbm_budget),
+                    seed, int(saturate), sessions, int(batched), int(graphs), max_sessions,
+                    int(fps_from_mix))
+    st = ServeStats()
+    _check(lib.wema_serve_run(C.byref(c), C.byref(st)), "wema_serve_run")
+    return {k: getattr(st, k) for k, _ in ServeStats._fields_}
 
 
 # ---------------------------------------------------------------------------------------

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwema_dev.so")
 
-SOURCES = ["wema_kernels.hip", "wema_dev.hip", "wema_synth.hip", "wema_pool.hip"]
+SOURCES = ["wema_kernels.hip", "wema_dev.hip", "wema_synth.hip", "wema_pool.hip", "wema_serve.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

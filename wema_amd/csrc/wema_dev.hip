@@ -243,11 +243,12 @@ extern "C" int wema_dev_query_bytes(const wema_dev_params *p, size_t *bytes) {
 
 int wema_session_alloc(wema_dev *s, int with_filt);
 
-extern "C" int wema_dev_open(wema_dev_t **out, const wema_dev_params *p) {
+int wema_dev_open_ex(wema_dev_t **out, const wema_dev_params *p, bool pooled) {
     if (!out || !p) { wema_set_error("null argument"); return -1; }
     *out = nullptr;
     if (validate(p) < 0) return -1;
     wema_dev *s = new wema_dev();
+    s->pooled = pooled;
     s->p = *p;
     wema_geometry_init(&s->g, p);
     if (p->use_iir) wema_design_bandpass(p->f_low, p->f_high, p->fps, s->c10, &s->warmup);
@@ -260,12 +261,16 @@ extern "C" int wema_dev_open(wema_dev_t **out, const wema_dev_params *p) {
     return 0;
 }
 
+extern "C" int wema_dev_open(wema_dev_t **out, const wema_dev_params *p) {
+    return wema_dev_open_ex(out, p, false);
+}
+
 int wema_session_alloc(wema_dev *s, int with_filt) {
     if (hipSetDevice(s->p.device) != hipSuccess) {
         wema_set_error("hipSetDevice(%d) failed", s->p.device);
         return -1;
     }
-    if (!s->stream) HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    if (!s->stream && !s->pooled) HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     Layout L;
     plan_layout(s->g, s->p, with_filt, &L);
     if (s->arena) {
@@ -302,7 +307,8 @@ int wema_session_alloc(wema_dev *s, int with_filt) {
         HIP_OK(hipMemcpy(s->tw_d, tw.data(), 256 * sizeof(float), hipMemcpyHostToDevice));
     }
     s->with_filt = with_filt;
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < 2 && !s->pooled; k++) {
+        // pooled sessions run through the pool's job tables
         if (!s->stage_h[k]) HIP_OK(hipHostMalloc(&s->stage_h[k], WEMA_STAGE_BYTES, hipHostMallocDefault));
         if (!s->stage_d[k]) HIP_OK(hipMalloc(&s->stage_d[k], WEMA_STAGE_BYTES));
         if (!s->stage_ev[k]) HIP_OK(hipEventCreateWithFlags(&s->stage_ev[k], hipEventDisableTiming));
@@ -447,7 +453,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         // window: ws ring samples read + amplified coefficient written per frame
         const double tb = p.use_iir ? (double)nu * (8.0 * n + 32.0)
                                     : (double)nu * n * (4.0 * p.window + 4.0);
-        jb.add_temporal(j, tiles_t, tb);
+        jb.add_temporal(j, tiles_t, tb, !p.use_iir);
     }
 
     // inverse DWT levels, coarsest first
@@ -547,7 +553,7 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
     if (!jb.host_amp.empty())
         HIP_OK(hipMemcpyAsync(amp_d, (char *)stage_h + jb.amp_offset,
                               jb.host_amp.size() * sizeof(float), hipMemcpyHostToDevice, st));
-    HIP_OK(hipEventRecord(stage_ev, st));
+    if (stage_ev) HIP_OK(hipEventRecord(stage_ev, st));
     char *base = (char *)stage_d;
     if (timer_mark(timer, -1, 0, st) < 0) return -1;
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
@@ -558,11 +564,13 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
     }
     const JobList &tl = jb.temporal;
     if (tl.count) {
-        if (jb.window_mode)
-            HIP_OK(wema_launch_win((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
-        else
-            HIP_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
-        if (timer_mark(timer, jb.window_mode ? TK_WIN : TK_IIR, tl.bytes, st) < 0) return -1;
+        HIP_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
+        if (timer_mark(timer, TK_IIR, tl.bytes, st) < 0) return -1;
+    }
+    const JobList &wl = jb.window;
+    if (wl.count) {
+        HIP_OK(wema_launch_win((const TemporalJob *)(base + wl.offset), wl.count, wl.blocks, st));
+        if (timer_mark(timer, TK_WIN, wl.bytes, st) < 0) return -1;
     }
     int last_inv = -1;
     for (int l = 0; l < WEMA_MAX_LEVELS; l++)
@@ -586,6 +594,7 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
 static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, float *out_f32,
                    int n, void *stream) {
     if (!s) { wema_set_error("null session"); return -1; }
+    if (s->pooled) { wema_set_error("a pooled session is processed by wema_pool_tick"); return -1; }
     if (n < 1 || n > s->p.max_batch) {
         wema_set_error("frame count %d outside 1..%d", n, s->p.max_batch);
         return -1;
@@ -598,7 +607,6 @@ static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, flo
     s->stage_cur ^= 1;
     HIP_OK(hipEventSynchronize(s->stage_ev[k]));   // staging buffer k free again
     JobBuilder jb;
-    jb.window_mode = !s->p.use_iir;
     wema_build_jobs(s, jb, in, in_f32, out_u8, out_f32, n);
     jb.fix_amp(s->amp_d);
     if (wema_run_jobs(jb, s->stage_h[k], s->stage_d[k], s->stage_ev[k], s->amp_d, st, &s->timer) < 0)

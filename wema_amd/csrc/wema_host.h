@@ -80,18 +80,19 @@ struct wema_dev {
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     int stage_cur = 0;
     LaunchTimer timer;
+    bool pooled = false;        // owned by a wema_pool (no staging buffers of its own)
     void *io_in = nullptr, *io_out = nullptr;
 };
 
 struct JobBuilder {
     std::vector<DwtfJob> dwtf_v[WEMA_MAX_LEVELS];
-    std::vector<TemporalJob> temporal_v;
+    std::vector<TemporalJob> temporal_v;      // IIR jobs
+    std::vector<TemporalJob> window_v;        // sliding-window jobs
     std::vector<DwtiJob> dwti_v[WEMA_MAX_LEVELS];
     std::vector<EgressJob> egress_v[EG_KINDS];
-    JobList dwtf[WEMA_MAX_LEVELS], temporal, dwti[WEMA_MAX_LEVELS], egress[EG_KINDS];
+    JobList dwtf[WEMA_MAX_LEVELS], temporal, window, dwti[WEMA_MAX_LEVELS], egress[EG_KINDS];
     std::vector<float> host_amp;
     size_t amp_offset = 0;
-    bool window_mode = false;
 
     void add_dwtf(int l, DwtfJob j, int tiles, double bytes) {
         j.tile_begin = dwtf[l].blocks;
@@ -99,12 +100,13 @@ struct JobBuilder {
         dwtf[l].bytes += bytes;
         dwtf_v[l].push_back(j);
     }
-    void add_temporal(TemporalJob j, int tiles, double bytes) {
-        j.tile_begin = temporal.blocks;
+    void add_temporal(TemporalJob j, int tiles, double bytes, bool window_mode) {
+        JobList &jl = window_mode ? window : temporal;
+        j.tile_begin = jl.blocks;
         j.tiles = tiles;
-        temporal.blocks += tiles;
-        temporal.bytes += bytes;
-        temporal_v.push_back(j);
+        jl.blocks += tiles;
+        jl.bytes += bytes;
+        (window_mode ? window_v : temporal_v).push_back(j);
     }
     void add_dwti(int stage, DwtiJob j, int tiles, double bytes) {
         j.tile_begin = dwti[stage].blocks;
@@ -120,6 +122,7 @@ struct JobBuilder {
     }
     void fix_amp(float *amp_d) {
         for (auto &j : temporal_v) j.amp = amp_d + j.amp_index;
+        for (auto &j : window_v) j.amp = amp_d + j.amp_index;
     }
     template <typename T>
     static bool put(char *dst, size_t cap, size_t &off, const std::vector<T> &v, JobList &jl) {
@@ -138,6 +141,7 @@ struct JobBuilder {
         bool ok = true;
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwtf_v[l], dwtf[l]);
         ok = ok && put(dst, cap, off, temporal_v, temporal);
+        ok = ok && put(dst, cap, off, window_v, window);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_v[l], dwti[l]);
         for (int k = 0; k < EG_KINDS; k++) ok = ok && put(dst, cap, off, egress_v[k], egress[k]);
         JobList amp_list;
