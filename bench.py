@@ -13,6 +13,8 @@ Workloads (the default is the single-stream drop-in case, 1920x1080 colour):
        240 frames = (warmup + steps) x 48  (reported under "also" by default)
   c1   320x240 RGB24 @30, 3-level DWT, band 0.5-3 Hz, amp 120, guided filter
   c2w  as c2 in sliding-window mode with bilateral weights (window 32)
+  multistream  24 concurrent streams of mixed resolution, batched ticks vs one at a time
+  serving      Poisson sessions at real-time pace: p50/p99 latency, capacity at p99 <= 33 ms
 
 roofline: the kernel with the largest share of the timed region; achieved = its
 algorithmic bytes per launch / its mean launch duration, from HIP events recorded on the
@@ -168,6 +170,73 @@ def cpu_baseline(cfg, budget_s=20.0, first_frame=0, gpu_sample=None):
                 fps=round(done / t_sum, 3))
 
 
+REALTIME_MIX_MPXS = (8 * 640 * 480 * 30 + 8 * 1280 * 720 * 30 + 6 * 1920 * 1080 * 60
+                     + 2 * 3840 * 2160 * 30) / 1e6   # real-time demand of the 24-stream mix
+
+
+def _serve_summary(st):
+    keys = ("frames", "mpix_per_s", "wall_s", "lat_p50_ms", "lat_p99_ms", "lat_max_ms",
+            "sessions_arrived", "sessions_admitted", "admission_waits", "peak_sessions",
+            "peak_bytes", "ticks", "mean_frames_per_tick")
+    return {k: (round(st[k], 3) if isinstance(st[k], float) else st[k]) for k in keys}
+
+
+def run_multistream(frames_per_session=180):
+    """24 concurrent streams of the 8:8:6:2 mix (640x480@30, 1280x720@30, 1920x1080@60,
+    3840x2160@30), each with its own band in 0.3-5 Hz, amp 20-150 and window 16/32/64, a frame
+    of every stream always due. batched: one launch per stage per tick over all 24 streams,
+    replayed from a hipGraph; plain: the same frames one stream at a time."""
+    import wema_amd
+    common = dict(saturate=True, sessions=24, fps_from_mix=True, hbm_budget=200e9,
+                  frames_per_session=frames_per_session, warmup_rounds=3, duration_s=120.0, seed=11)
+    res = {}
+    for name, kw in (("batched_graphs", dict(batched=True, graphs=True)),
+                     ("batched", dict(batched=True, graphs=False)),
+                     ("plain", dict(batched=False, graphs=False))):
+        res[name] = _serve_summary(wema_amd.serve(**kw, **common))
+    b, p = res["batched_graphs"]["mpix_per_s"], res["plain"]["mpix_per_s"]
+    return {"desc": "24 streams x %d frames, mix 8x640x480@30 8x1280x720@30 6x1920x1080@60 "
+                    "2x3840x2160@30, per-stream band/amp/window" % frames_per_session,
+            "value": round(b, 1), "unit": "Mpixel/s",
+            "realtime_demand_mpix_s": round(REALTIME_MIX_MPXS, 1),
+            "x_realtime": round(b / REALTIME_MIX_MPXS, 2),
+            "batched_over_plain": round(b / p, 2) if p else None, "runs": res}
+
+
+def run_serving(duration_s=4.0, max_mult=64.0):
+    """Poisson sessions at 4/s (life 2-20 s, the 8:8:6:2 resolution mix, 30 or 60 fps),
+    frames delivered at real-time pace, admission under a 200 GB HBM budget, started in
+    steady state. paced: that load, batched (continuous batching) and plain (one session at
+    a time). capacity: the arrival rate is raised in steps of sqrt(2) until p99 latency
+    exceeds 33 ms; the aggregate Mpixel/s of the last load that met it, for both modes."""
+    import wema_amd
+    base = dict(arrival_rate=4.0, life=(2.0, 20.0), hbm_budget=200e9, warm_start=True,
+                duration_s=duration_s, fps_from_mix=False, seed=5)
+    out = {"desc": "Poisson 4/s, life 2-20 s, mix 8:8:6:2 of 640x480..3840x2160 at 30/60 fps, "
+                   "real-time pace, 200 GB HBM budget, %.0f s runs from steady state" % duration_s}
+    out["paced"] = {"batched": _serve_summary(wema_amd.serve(batched=True, graphs=True, **base)),
+                    "plain": _serve_summary(wema_amd.serve(batched=False, graphs=False, **base))}
+    cap = {}
+    for mode, kw in (("batched", dict(batched=True, graphs=True)),
+                     ("plain", dict(batched=False, graphs=False))):
+        best, tried, m = None, [], 1.0
+        while m <= max_mult:
+            st = wema_amd.serve(**kw, **dict(base, arrival_rate=4.0 * m))
+            tried.append({"arrival_rate": round(4.0 * m, 2), "mpix_per_s": round(st["mpix_per_s"], 1),
+                          "lat_p99_ms": round(st["lat_p99_ms"], 2), "peak_sessions": st["peak_sessions"]})
+            if st["lat_p99_ms"] > 33.0:
+                break
+            best = dict(_serve_summary(st), arrival_rate=round(4.0 * m, 2))
+            m *= 2 ** 0.5
+        cap[mode] = {"best": best, "sweep": tried}
+    out["capacity"] = cap
+    b = (cap["batched"]["best"] or {}).get("mpix_per_s")
+    p = (cap["plain"]["best"] or {}).get("mpix_per_s")
+    out["capacity_batched_over_plain"] = round(b / p, 2) if b and p else None
+    out["p99_target_ms"] = 33.0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +247,9 @@ def main():
     ap.add_argument("--also", default="c3", help="comma list of extra workloads ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the multi-stream and serving measurements")
+    ap.add_argument("--serve-duration", type=float, default=4.0, help="seconds per serving run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (for rocprofv3 counter scaling)")
     args = ap.parse_args()
@@ -239,6 +311,9 @@ def main():
                                   for d in rr["kernels"]]}
     if also:
         line["also"] = also
+    if not args.no_serve:
+        line["multistream"] = run_multistream()
+        line["serving"] = run_serving(args.serve_duration)
     print(json.dumps(line), flush=True)
 
 

@@ -124,7 +124,12 @@ int    wema_pool_timing_report(wema_pool_t *p, int kind, char *name, int name_le
  * and deliver frames at their frame rate. batched: all due frames run as one pool tick;
  * otherwise one session at a time (the plain path). saturate: `sessions` sessions admitted
  * at once, taking the mix in its exact 8:8:6:2 proportions, with a frame always due, for
- * duration_s (capacity). Latency = frame due time to output complete. */
+ * duration_s (capacity). warm_start: the Poisson run starts in steady state (the sessions
+ * that arrived in the life_max_s before the start and are still alive are admitted before
+ * the clock starts, each at a random frame phase). frames_per_session > 0: a session leaves
+ * after that many measured frames (a saturate run then ends when all have left).
+ * warmup_rounds: the first rounds (iterations that ran frames) are run but not measured.
+ * Latency = frame due time to output complete. */
 typedef struct {
     int      device;
     double   duration_s, drain_s;
@@ -136,6 +141,9 @@ typedef struct {
     int      batched, use_graphs;
     int      max_sessions;
     int      fps_from_mix;
+    int      warm_start;
+    long     frames_per_session;
+    int      warmup_rounds;
 } wema_serve_config;
 typedef struct {
     long   frames;

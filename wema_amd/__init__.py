@@ -69,7 +69,8 @@ class ServeConfig(C.Structure):
         ("arrival_rate", C.c_double), ("life_min_s", C.c_double), ("life_max_s", C.c_double),
         ("hbm_budget", C.c_size_t), ("seed", C.c_uint64), ("saturate", C.c_int),
         ("sessions", C.c_int), ("batched", C.c_int), ("use_graphs", C.c_int),
-        ("max_sessions", C.c_int), ("fps_from_mix", C.c_int),
+        ("max_sessions", C.c_int), ("fps_from_mix", C.c_int), ("warm_start", C.c_int),
+        ("frames_per_session", C.c_long), ("warmup_rounds", C.c_int),
     ]
 
 
@@ -370,11 +371,12 @@ class Pool:
 
 def serve(duration_s=10.0, arrival_rate=4.0, life=(2.0, 20.0), hbm_budget=200e9, batched=True,
           graphs=True, saturate=False, sessions=0, drain_s=0.0, max_sessions=0,
-          fps_from_mix=False, seed=0, device=0):
+          fps_from_mix=False, seed=0, device=0, warm_start=False, frames_per_session=0,
+          warmup_rounds=0):
     """Runs the serving loop (wema_serve_run) and returns its statistics as a dict."""
     c = ServeConfig(device, duration_s, drain_s, arrival_rate, life[0], life[1], int(hbm_budget),
                     seed, int(saturate), sessions, int(batched), int(graphs), max_sessions,
-                    int(fps_from_mix))
+                    int(fps_from_mix), int(warm_start), int(frames_per_session), int(warmup_rounds))
     st = ServeStats()
     _check(lib.wema_serve_run(C.byref(c), C.byref(st)), "wema_serve_run")
     return {k: getattr(st, k) for k, _ in ServeStats._fields_}

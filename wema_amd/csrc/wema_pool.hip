@@ -52,6 +52,7 @@ struct wema_pool {
     int direct_cur = 0;
     bool graphs = false;
     std::map<std::string, GraphEntry> cache;
+    std::map<std::string, int> seen;   // tick shapes met so far (captured from the 2nd time)
     LaunchTimer timer;
     long ticks = 0;
 };
@@ -226,7 +227,17 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
     for (int i = 0; i < n; i++)
         wema_build_jobs(ss[(size_t)i], jb, d_in[i], 0, d_out[i], nullptr, 1);
 
-    if (!p->graphs || p->timer.enabled || hip_stream) {
+    // a shape is captured the second time it occurs (one-off shapes run directly), and the
+    // cache keeps at most kMaxGraphs shapes
+    constexpr size_t kMaxGraphs = 64;
+    bool use_graph = p->graphs && !p->timer.enabled && !hip_stream;
+    if (use_graph && !p->cache.count(key)) {
+        int &cnt = p->seen[key];
+        cnt++;
+        if (p->seen.size() > 4096) p->seen.clear();
+        if (cnt < 2 || p->cache.size() >= kMaxGraphs) use_graph = false;
+    }
+    if (!use_graph) {
         StageBuf &b = p->direct[p->direct_cur];
         p->direct_cur ^= 1;
         HIP_OK(hipEventSynchronize(b.ev));

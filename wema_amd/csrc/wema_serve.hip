@@ -43,7 +43,7 @@ struct ResClass {
 const ResClass kMix[4] = {{640, 480, 30.f, 8}, {1280, 720, 30.f, 8}, {1920, 1080, 60.f, 6}, {3840, 2160, 30.f, 2}};
 
 struct Planned {
-    double arrive, life;
+    double arrive, life, phase;
     wema_dev_params p;
     int cls;
 };
@@ -74,8 +74,9 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
     std::vector<Planned> plan;
     int wsum = 0;
     for (const ResClass &c : kMix) wsum += c.weight;
-    double t = 0.0;
     const int fixed = cfg->saturate ? cfg->sessions : 0;
+    const bool warm = !fixed && cfg->warm_start;
+    double t = warm ? -cfg->life_max_s : 0.0;
     while (true) {
         if (fixed) {
             if ((int)plan.size() >= fixed) break;
@@ -107,6 +108,13 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         pl.p.device = cfg->device;
         pl.arrive = fixed ? 0.0 : t;
         pl.life = fixed ? 1e30 : rng.uniform(cfg->life_min_s, cfg->life_max_s);
+        pl.phase = rng.uniform();
+        if (pl.arrive < 0.0) {
+            // before the start: alive at t = 0 only if it has not left yet
+            if (pl.arrive + pl.life <= 0.0) continue;
+            pl.life += pl.arrive;
+            pl.arrive = 0.0;
+        }
         plan.push_back(pl);
     }
     out->sessions_arrived = (int)plan.size();
@@ -142,34 +150,22 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
     std::deque<int> waiting;      // planned sessions not yet admitted
     std::vector<Live> live;
     size_t next_plan = 0;
-    long frames = 0, ticks = 0, tick_frames = 0;
+    long frames = 0, ticks = 0, tick_frames = 0, rounds = 0;
     double mpix = 0.0;
     int rc = 0;
-    const auto t0 = std::chrono::steady_clock::now();
+    auto t0 = std::chrono::steady_clock::now();
+    double t_meas = 0.0;          // start of the measured part (after the warm-up rounds)
+    double t_last = 0.0;          // end of the last round
     const double horizon = cfg->saturate ? cfg->duration_s : cfg->duration_s + cfg->drain_s;
+    const long quota = cfg->frames_per_session > 0 ? cfg->frames_per_session + cfg->warmup_rounds : 0;
     std::vector<int> ids;
     std::vector<const uint8_t *> ins;
     std::vector<uint8_t *> outs;
     std::vector<double> due;
     std::vector<size_t> who;
 
-    while (true) {
-        const double now = now_s(t0);
-        if (now >= horizon) break;
-        // arrivals
-        while (next_plan < plan.size() && plan[next_plan].arrive <= now) waiting.push_back((int)next_plan++);
-        // departures
-        for (size_t i = 0; i < live.size();) {
-            if (live[i].end <= now) {
-                hipStreamSynchronize(st);
-                wema_pool_remove(pool, live[i].id);
-                hipFree(live[i].out);
-                live.erase(live.begin() + (long)i);
-            } else {
-                ++i;
-            }
-        }
-        // admission (first come, first served; the head waits until it fits)
+    // admission (first come, first served; the head waits until it fits)
+    auto admit = [&](double now) {
         while (!waiting.empty()) {
             const Planned &pl = plan[(size_t)waiting.front()];
             if (cfg->max_sessions > 0 && (int)live.size() >= cfg->max_sessions) break;
@@ -181,7 +177,8 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
             Live L;
             L.id = id;
             L.cls = pl.cls;
-            L.start = now;
+            // streams that join in steady state run at a random phase of their frame period
+            L.start = now + (warm && pl.arrive == 0.0 ? pl.phase / pl.p.fps : 0.0);
             L.end = now + pl.life;
             L.fps = pl.p.fps;
             L.next = 0;
@@ -189,14 +186,40 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
             if (hipMalloc(&L.out, L.out_bytes) != hipSuccess) {
                 wema_pool_remove(pool, id);
                 wema_set_error("output slot allocation failed");
-                rc = -1;
-                break;
+                return -1;
             }
             live.push_back(L);
             out->sessions_admitted++;
             waiting.pop_front();
         }
-        if (rc < 0) break;
+        return 0;
+    };
+
+    if (warm) {
+        // the steady-state population joins before the clock starts
+        while (next_plan < plan.size() && plan[next_plan].arrive <= 0.0) waiting.push_back((int)next_plan++);
+        if (admit(0.0) < 0) rc = -1;
+        t0 = std::chrono::steady_clock::now();
+    }
+
+    while (rc == 0) {
+        const double now = now_s(t0);
+        if (now >= horizon) break;
+        // arrivals
+        while (next_plan < plan.size() && plan[next_plan].arrive <= now) waiting.push_back((int)next_plan++);
+        // departures
+        for (size_t i = 0; i < live.size();) {
+            if (live[i].end <= now || (quota && live[i].next >= quota)) {
+                hipStreamSynchronize(st);
+                wema_pool_remove(pool, live[i].id);
+                hipFree(live[i].out);
+                live.erase(live.begin() + (long)i);
+            } else {
+                ++i;
+            }
+        }
+        if (admit(now) < 0) { rc = -1; break; }
+        if (live.empty() && waiting.empty() && next_plan == plan.size() && (quota || fixed)) break;
         out->peak_sessions = std::max(out->peak_sessions, (int)live.size());
         out->peak_bytes = std::max(out->peak_bytes, wema_pool_bytes_in_use(pool));
         // frames due now (one per session per tick; a backlog keeps its oldest frame)
@@ -222,30 +245,41 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
             if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(dt, 0.002)));
             continue;
         }
+        const bool measured = rounds >= cfg->warmup_rounds;
         if (cfg->batched) {
             if (wema_pool_tick(pool, (int)ids.size(), ids.data(), ins.data(), outs.data(), nullptr) < 0) { rc = -1; break; }
             if (hipStreamSynchronize(st) != hipSuccess) { wema_set_error("tick failed"); rc = -1; break; }
             const double done = now_s(t0);
-            for (double d : due) lat.push_back(done - d);
-            ticks++;
+            if (measured) {
+                for (double d : due) lat.push_back(done - d);
+                ticks++;
+            }
         } else {
             for (size_t k = 0; k < ids.size(); k++) {
                 if (wema_pool_tick(pool, 1, &ids[k], &ins[k], &outs[k], nullptr) < 0) { rc = -1; break; }
                 if (hipStreamSynchronize(st) != hipSuccess) { wema_set_error("tick failed"); rc = -1; break; }
-                lat.push_back(now_s(t0) - due[k]);
-                ticks++;
+                if (measured) {
+                    lat.push_back(now_s(t0) - due[k]);
+                    ticks++;
+                }
             }
             if (rc < 0) break;
         }
-        tick_frames += (long)ids.size();
         for (size_t k = 0; k < who.size(); k++) {
             Live &L = live[who[k]];
             L.next++;
-            frames++;
-            mpix += (double)kMix[L.cls].w * kMix[L.cls].h / 1e6;
+            if (measured) {
+                frames++;
+                mpix += (double)kMix[L.cls].w * kMix[L.cls].h / 1e6;
+            }
         }
+        if (measured) tick_frames += (long)ids.size();
+        rounds++;
+        t_last = now_s(t0);
+        if (rounds == cfg->warmup_rounds) t_meas = t_last;
     }
-    const double wall = now_s(t0);
+    // a saturate run is measured up to its last round, a paced one over its whole span
+    const double wall = (cfg->saturate ? t_last : now_s(t0)) - t_meas;
     hipStreamSynchronize(st);
     for (Live &L : live) {
         wema_pool_remove(pool, L.id);
