@@ -383,13 +383,14 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         const int w = g.lw[l], h = g.lh[l];
         const int tiles_x = wema_ceil_div(g.lw[l + 1], WEMA_TILE_PAIRS_X);
         const int tiles = tiles_x * wema_ceil_div(g.lh[l + 1], WEMA_TILE_PAIRS_Y);
+        // level 0: one job per frame for all channels (they share the RGB source)
+        const int jch = l == 0 ? nch : 1;
         for (int f = 0; f < n; f++)
-            for (int c = 0; c < nch; c++) {
+            for (int c = 0; c < nch; c += jch) {
                 DwtfJob j{};
                 if (l == 0) {
                     j.src = (const char *)in + (size_t)f * in_stride;
-                    j.src_kind = in_f32 ? (p.color ? SRC_F32_Y + c : SRC_F32_Y)
-                                        : (p.color ? SRC_U8_Y + c : SRC_U8_Y);
+                    j.src_kind = in_f32 ? SRC_F32_Y : SRC_U8_Y;
                     j.prev = nullptr;
                 } else {
                     j.src = ll_ptr(f, c, l - 1);
@@ -399,14 +400,17 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.ll = ll_ptr(f, c, l);
                 j.planes = coef_ptr(f, c) + g.plane_off[l];
                 j.acc = s->acc + (size_t)f * nch + c;
+                j.nch = jch;
+                j.ll_cstride = jch > 1 ? (long long)(ll_ptr(f, 1, l) - ll_ptr(f, 0, l)) : 0;
+                j.planes_cstride = jch > 1 ? (long long)(coef_ptr(f, 1) - coef_ptr(f, 0)) : 0;
                 j.w = w;
                 j.h = h;
                 j.tiles_x = tiles_x;
                 // input read once (RGB24 shared by the channels at level 0) + LL and three
-                // subbands written
-                const double in_b = l == 0 ? 3.0 * w * h / nch * (in_f32 ? 4 : 1) : 4.0 * w * h;
-                const double out_b = 16.0 * g.lw[l + 1] * g.lh[l + 1];
-                jb.add_dwtf(l, j, tiles, in_b + out_b);
+                // subbands written per channel
+                const double in_b = l == 0 ? 3.0 * w * h * (in_f32 ? 4 : 1) : 4.0 * w * h;
+                const double out_b = 16.0 * g.lw[l + 1] * g.lh[l + 1] * jch;
+                jb.add_dwtf(l, j, tiles * jch, in_b + out_b);
             }
     }
 
@@ -479,7 +483,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
     // output stage
     const int gf_tx = wema_ceil_div(g.W, WEMA_GF_TX);
     const int gf_tiles = gf_tx * wema_ceil_div(g.H, WEMA_GF_TY);
-    const int px_tiles = (int)((P + 255) / 256), byte_tiles = (int)((3 * P + 255) / 256);
+    const int px4_tiles = (int)((P + 1023) / 1024), byte_tiles = (int)((3 * P + 255) / 256);
     for (int f = 0; f < n; f++) {
         EgressJob j{};
         j.gray = rec_ptr(f, 0);
@@ -499,7 +503,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         if (!p.use_iir && gidx + 1 < p.window) {
             jb.add_egress(EG_PASSTHRU, j, byte_tiles, src_b + out_b);
         } else if (p.color) {
-            jb.add_egress(EG_YCC, j, px_tiles, 12.0 * P + out_b);
+            jb.add_egress(EG_YCC, j, px4_tiles, 12.0 * P + out_b);
         } else if (p.edge_aware || p.batch_path) {
             jb.add_egress(EG_GF1, j, gf_tiles, 4.0 * P + src_b + 8.0 * P);
             jb.add_egress(EG_GF2, j, gf_tiles, 8.0 * P + src_b + out_b);

@@ -38,17 +38,22 @@ enum SrcKind {
     SRC_F32_CR = 6,
 };
 
+// A forward job covers nch channels of one frame that share the source (level 0 of the
+// colour mode: Y, Cb, Cr from one RGB frame); channel c uses ll + c * ll_cstride,
+// planes + c * planes_cstride, acc + c and source kind src_kind + c. Its blocks take the
+// channels of one tile in turn, so the source tile is re-read from L2.
 struct DwtfJob {
     const void *src;                // level 0: RGB frame; else previous level's LL (w x h)
-    float *ll;                      // LL of this level (sw x sh)
-    float *planes;                  // LH, HH, HL of this level (3 x sw x sh)
+    float *ll;                      // LL of this level (sw x sh), channel 0
+    float *planes;                  // LH, HH, HL of this level (3 x sw x sh), channel 0
     const float *prev;              // previous level's planes (3 x w x h) or null (level 0)
     unsigned long long *acc;        // sum of |c| over unique positions, Q32 (or null)
+    long long ll_cstride, planes_cstride;
     int w, h;                       // input size of this level
     int src_kind;
     int tiles_x;
     int tile_begin;
-    int pad_;
+    int nch;
 };
 
 struct DwtiJob {

@@ -103,6 +103,18 @@ __device__ __forceinline__ unsigned long long abs_fixed(float v) {
     return (m + (1ull << (-sh - 1))) >> (-sh);
 }
 
+// The same without branches for |v| < 2^30: the integer part and the fraction are exact,
+// the fraction times 2^32 is exact, and below 2^23 adding 0.5 then truncating rounds half
+// up. Equal to abs_fixed for every float below 2^30 (checked exhaustively by
+// tests/q32_check.c); abs_fixed saturates from 2^30 on.
+__device__ __forceinline__ unsigned long long abs_q32_small(float v) {
+    const float a = fabsf(v);
+    const unsigned hi = (unsigned)a;
+    const float x = (a - (float)hi) * 4294967296.0f;
+    const unsigned lo = x < 0.5f ? 0u : (unsigned)(x < 8388608.0f ? x + 0.5f : x);
+    return ((unsigned long long)hi << 32) + lo;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -309,8 +321,16 @@ __global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restri
 
     const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int jid = find_job(jobs, njobs, bid);
-    const DwtfJob J = jobs[jid];
-    const int t = bid - J.tile_begin;
+    DwtfJob J = jobs[jid];
+    int t = bid - J.tile_begin;
+    if (J.nch > 1) {
+        const int ch = t % J.nch;
+        t /= J.nch;
+        J.ll += ch * J.ll_cstride;
+        J.planes += ch * J.planes_cstride;
+        J.acc += ch;
+        J.src_kind += ch;
+    }
     const int tx = t % J.tiles_x, ty = t / J.tiles_x;
     const int w = J.w, h = J.h;
     const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
@@ -359,7 +379,30 @@ __global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restri
     const GLOBAL_AS float *prev = gp(J.prev);
     const size_t pn = (size_t)w * h;                  // previous level's plane size
     unsigned long long part = 0ull;
-    if (col_ok) {
+    // tiles inside the written extents need neither bounds checks nor stale entries
+    const bool interior = px0 + WEMA_TILE_PAIRS_X <= hw_valid && py0 + WEMA_TILE_PAIRS_Y <= hh_valid;
+    // the wave's coefficients all below 2^30 in magnitude (and none NaN): the branch-free
+    // conversion applies
+    float mag = 0.0f;
+#pragma unroll
+    for (int k = 2; k < HALF_PAIRS - 2; ++k) mag += fabsf(e[k]) + fabsf(o[k]);
+    const bool small = __all(mag < 1073741824.0f);
+    if (interior && small) {
+        if ((c & 63) >= 2 && (c & 63) < 62) {
+            // c < 64 (waves 0 and 2): LL and LH; c >= 64 (waves 1 and 3): HL and HH
+            GLOBAL_AS float *pe = (high ? HL : LL) + (size_t)(py0 + k0) * sw + px;
+            GLOBAL_AS float *po = (high ? HH : LH) + (size_t)(py0 + k0) * sw + px;
+#pragma unroll
+            for (int k = 2; k < HALF_PAIRS - 2; ++k) {
+                *pe = e[k];
+                *po = o[k];
+                part += abs_q32_small(o[k]);
+                if (high) part += abs_q32_small(e[k]);
+                pe += sw;
+                po += sw;
+            }
+        }
+    } else if (col_ok) {
 #pragma unroll
         for (int k = 2; k < HALF_PAIRS - 2; ++k) {
             const int py = py0 + k0 + k - 2;
@@ -669,6 +712,34 @@ __device__ __forceinline__ int clip_count(int x, int n) {
     return hi > lo ? hi - lo : 1;
 }
 
+// a / c for a window count c in 1..9, given k = the correctly rounded 1/c: q = a * k is
+// within an ulp of a / c, and one correction with the exact residual fma(-q, c, a) rounds
+// to the correctly rounded quotient (Markstein). Equal to IEEE a / c for every normal a
+// (checked exhaustively for c = 1..9 over whole binades of a by tests/div_check.c), and
+// free of the mode switches and range scaling of the general division.
+__device__ __forceinline__ float div_count(float a, float c, float k) {
+    const float q = a * k;
+    return fmaf(fmaf(-q, c, a), k, q);
+}
+
+// a / b for b in [2^-7, 2) and 2^-66 <= |a| < 2^64 (or a = 0): the steps of the IEEE
+// division sequence (reciprocal estimate, one refinement, two corrections with exact
+// residuals) without its range scaling and special-case fix-up, which that range never
+// needs; bit-identical to a / b there (and within an ulp of it for smaller nonzero |a|,
+// whose residuals flush to zero).
+__device__ __forceinline__ float div_mid(float a, float b) {
+    float y = __builtin_amdgcn_rcpf(b);
+    y = fmaf(fmaf(-b, y, 1.0f), y, y);
+    float q = a * y;
+    q = fmaf(fmaf(-b, q, a), y, q);
+    return fmaf(fmaf(-b, q, a), y, q);
+}
+
+// Correctly rounded 1/c for c = 0..15 in LDS (entry 0 unused), filled by the first threads.
+__device__ __forceinline__ void fill_rcp(float *t) {
+    if (threadIdx.x < 16) t[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 0.0f;
+}
+
 constexpr int GF_EX = WEMA_GF_TX + 8, GF_EY = WEMA_GF_TY + 8;
 constexpr int GF_PITCH = GF_EX + 4;   // multiple of 4: rows start 16-byte aligned
 
@@ -684,46 +755,72 @@ __device__ __forceinline__ void sums4(const float (&v)[12], float (&out)[4]) {
     }
 }
 
+// Source pixel of the frame as three floats in [0, 1] (RGB24 or float RGB).
+template <bool U8>
+struct SrcPix {
+    unsigned r, g, b;   // raw bytes or float bits
+    __device__ __forceinline__ void load(const EgressJob &J, size_t idx) {
+        if (U8) {
+            const GLOBAL_AS unsigned char *p = gp(J.src_u8) + idx * 3;
+            r = p[0]; g = p[1]; b = p[2];
+        } else {
+            const GLOBAL_AS float *p = gp(J.src_f32) + idx * 3;
+            r = __float_as_uint(p[0]); g = __float_as_uint(p[1]); b = __float_as_uint(p[2]);
+        }
+    }
+    __device__ __forceinline__ float ch(unsigned v) const { return U8 ? u8f(v) : __uint_as_float(v); }
+    __device__ __forceinline__ float luma() const { return luma_of(ch(r), ch(g), ch(b)); }
+};
+
+// Guide (source luma) and input (reconstruction - guide) of the extended tile into LDS,
+// zero outside the frame. Coordinates are clamped into the frame so that every load is
+// valid and all of them are issued before any is used (one memory round trip per block).
+template <bool U8>
+__device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, float (*sI)[GF_PITCH],
+                                         float (*sP)[GF_PITCH]) {
+    constexpr int NLD = (GF_EY * GF_EX + 255) / 256;
+    const int w = J.w, h = J.h;
+    const GLOBAL_AS float *gray = gp(J.gray);
+    SrcPix<U8> px[NLD];
+    float g[NLD];
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+        const int i = threadIdx.x + 256 * j;
+        const int ey = i / GF_EX, ex = i - ey * GF_EX;
+        const int x = min(max(x0 - 4 + ex, 0), w - 1), y = min(max(y0 - 4 + ey, 0), h - 1);
+        const size_t idx = (size_t)y * w + x;
+        px[j].load(J, idx);
+        g[j] = gray[idx];
+    }
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+        const int i = threadIdx.x + 256 * j;
+        if (i < GF_EY * GF_EX) {
+            const int ey = i / GF_EX, ex = i - ey * GF_EX;
+            const int x = x0 - 4 + ex, y = y0 - 4 + ey;
+            const bool in = x >= 0 && x < w && y >= 0 && y < h;
+            const float I = px[j].luma();
+            sI[ey][ex] = in ? I : 0.0f;
+            sP[ey][ex] = in ? g[j] - I : 0.0f;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs, int njobs) {
     __shared__ __attribute__((aligned(16))) float sI[GF_EY][GF_PITCH];
     __shared__ __attribute__((aligned(16))) float sP[GF_EY][GF_PITCH];
     __shared__ float hI[GF_EY][WEMA_GF_TX], hP[GF_EY][WEMA_GF_TX];
     __shared__ float hII[GF_EY][WEMA_GF_TX], hIP[GF_EY][WEMA_GF_TX];
+    __shared__ float sRcp[16];
+    fill_rcp(sRcp);
     const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int jid = find_job(jobs, njobs, bid);
     const EgressJob &J = jobs[jid];
     const int t = bid - J.tile_begin;
     const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
     const int w = J.w, h = J.h;
-    const GLOBAL_AS float *gray = gp(J.gray);
-    // all loads of the tile issued before any is used (one memory round trip per block)
-    constexpr int NLD = (GF_EY * GF_EX + 255) / 256;
-    float lI[NLD], lP[NLD];
-#pragma unroll
-    for (int j = 0; j < NLD; ++j) {
-        const int i = threadIdx.x + 256 * j;
-        const int ey = i / GF_EX, ex = i % GF_EX;
-        const int x = x0 - 4 + ex, y = y0 - 4 + ey;
-        float I = 0.0f, g = 0.0f;
-        if (i < GF_EY * GF_EX && x >= 0 && x < w && y >= 0 && y < h) {
-            const size_t idx = (size_t)y * w + x;
-            I = src_luma(J, idx);
-            g = gray[idx];
-        }
-        lI[j] = I;
-        lP[j] = g;
-    }
-#pragma unroll
-    for (int j = 0; j < NLD; ++j) {
-        const int i = threadIdx.x + 256 * j;
-        if (i < GF_EY * GF_EX) {
-            const int ey = i / GF_EX, ex = i % GF_EX;
-            const int x = x0 - 4 + ex, y = y0 - 4 + ey;
-            const bool in = x >= 0 && x < w && y >= 0 && y < h;
-            sI[ey][ex] = lI[j];
-            sP[ey][ex] = in ? lP[j] - lI[j] : 0.0f;
-        }
-    }
+    if (J.src_u8) gf1_load<true>(J, x0, y0, sI, sP);
+    else gf1_load<false>(J, x0, y0, sI, sP);
     __syncthreads();
     // horizontal: 4 outputs per item, taps read as three 16-byte vectors per quantity
     for (int i = threadIdx.x; i < GF_EY * (WEMA_GF_TX / 4); i += 256) {
@@ -748,11 +845,12 @@ __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs,
         sums4(vip, sip);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float cnt = (float)clip_count(x0 + x4 + j, w);
-            hI[ey][x4 + j] = si[j] / cnt;
-            hP[ey][x4 + j] = spp[j] / cnt;
-            hII[ey][x4 + j] = sii[j] / cnt;
-            hIP[ey][x4 + j] = sip[j] / cnt;
+            const int n = clip_count(x0 + x4 + j, w);
+            const float cnt = (float)n, k = sRcp[n];
+            hI[ey][x4 + j] = div_count(si[j], cnt, k);
+            hP[ey][x4 + j] = div_count(spp[j], cnt, k);
+            hII[ey][x4 + j] = div_count(sii[j], cnt, k);
+            hIP[ey][x4 + j] = div_count(sip[j], cnt, k);
         }
     }
     __syncthreads();
@@ -778,11 +876,14 @@ __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs,
         for (int j = 0; j < 4; ++j) {
             const int gy = y0 + y4 + j;
             if (gy >= h) break;
-            const float cnt = (float)clip_count(gy, h);
-            const float mI = si[j] / cnt, mp = spp[j] / cnt, cI = sii[j] / cnt, cIp = sip[j] / cnt;
+            const int n = clip_count(gy, h);
+            const float cnt = (float)n, k = sRcp[n];
+            const float mI = div_count(si[j], cnt, k), mp = div_count(spp[j], cnt, k);
+            const float cI = div_count(sii[j], cnt, k), cIp = div_count(sip[j], cnt, k);
             const float var = fmaf(-mI, mI, cI);
             const float cov = fmaf(-mI, mp, cIp);
-            const float av = cov / (var + 0.01f);
+            // var + eps lies in [0.0099, 1.02): I is in [0, 1]
+            const float av = div_mid(cov, var + 0.01f);
             const float bv = fmaf(-av, mI, mp);
             J.ab[(size_t)gy * w + gx] = make_float2(av, bv);
         }
@@ -806,39 +907,53 @@ __device__ __forceinline__ void write_rgb(const EgressJob &J, size_t idx, float 
     }
 }
 
-__global__ void __launch_bounds__(256) k_gf2(const EgressJob *__restrict__ jobs, int njobs) {
-    __shared__ __attribute__((aligned(16))) float sA[GF_EY][GF_PITCH];
-    __shared__ __attribute__((aligned(16))) float sB[GF_EY][GF_PITCH];
-    __shared__ float hA[GF_EY][WEMA_GF_TX], hB[GF_EY][WEMA_GF_TX];
-    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int jid = find_job(jobs, njobs, bid);
-    const EgressJob &J = jobs[jid];
-    const int t = bid - J.tile_begin;
-    const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
+// RGB out = clamp(source + q) (src/wema.c:329-334).
+template <bool U8>
+__device__ __forceinline__ void write_rgb_px(const EgressJob &J, size_t idx, const SrcPix<U8> &sp, float q) {
+    const float s[3] = {sp.ch(sp.r), sp.ch(sp.g), sp.ch(sp.b)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float v = clamp01(s[k] + q);
+        if (J.out_u8) gp(J.out_u8)[idx * 3 + k] = egress_u8(v);
+        if (J.out_f32) gp(J.out_f32)[idx * 3 + k] = v;
+    }
+}
+
+template <bool U8>
+__device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, float (*sA)[GF_PITCH],
+                                         float (*sB)[GF_PITCH], float (*hA)[WEMA_GF_TX],
+                                         float (*hB)[WEMA_GF_TX], const float *sRcp) {
+    static_assert((WEMA_GF_TY / 4) * WEMA_GF_TX == 256, "one vertical item per thread");
     const int w = J.w, h = J.h;
+    // the source pixels of this thread's outputs (one column, 4 rows) and the a, b tile,
+    // all loaded first from coordinates clamped into the frame
+    const int y4 = 4 * ((int)threadIdx.x / WEMA_GF_TX), xo = (int)threadIdx.x % WEMA_GF_TX;
+    const int gx = x0 + xo;
+    SrcPix<U8> src[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        src[j].load(J, (size_t)min(y0 + y4 + j, h - 1) * w + min(gx, w - 1));
     const GLOBAL_AS float2 *ab = gp(J.ab);
     constexpr int NLD = (GF_EY * GF_EX + 255) / 256;
-    float la[NLD], lb[NLD];
+    float2 lab[NLD];
 #pragma unroll
     for (int j = 0; j < NLD; ++j) {
         const int i = threadIdx.x + 256 * j;
-        const int ey = i / GF_EX, ex = i % GF_EX;
-        const int x = x0 - 4 + ex, y = y0 - 4 + ey;
-        float va = 0.0f, vb = 0.0f;
-        if (i < GF_EY * GF_EX && x >= 0 && x < w && y >= 0 && y < h) {
-            const GLOBAL_AS float *q = (const GLOBAL_AS float *)(ab + (size_t)y * w + x);
-            va = q[0];
-            vb = q[1];
-        }
-        la[j] = va;
-        lb[j] = vb;
+        const int ey = i / GF_EX, ex = i - ey * GF_EX;
+        const int x = min(max(x0 - 4 + ex, 0), w - 1), y = min(max(y0 - 4 + ey, 0), h - 1);
+        const GLOBAL_AS float *q = (const GLOBAL_AS float *)(ab + (size_t)y * w + x);
+        lab[j].x = q[0];
+        lab[j].y = q[1];
     }
 #pragma unroll
     for (int j = 0; j < NLD; ++j) {
         const int i = threadIdx.x + 256 * j;
         if (i < GF_EY * GF_EX) {
-            sA[i / GF_EX][i % GF_EX] = la[j];
-            sB[i / GF_EX][i % GF_EX] = lb[j];
+            const int ey = i / GF_EX, ex = i - ey * GF_EX;
+            const int x = x0 - 4 + ex, y = y0 - 4 + ey;
+            const bool in = x >= 0 && x < w && y >= 0 && y < h;
+            sA[ey][ex] = in ? lab[j].x : 0.0f;
+            sB[ey][ex] = in ? lab[j].y : 0.0f;
         }
     }
     __syncthreads();
@@ -857,35 +972,47 @@ __global__ void __launch_bounds__(256) k_gf2(const EgressJob *__restrict__ jobs,
         sums4(vb, sb);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float cnt = (float)clip_count(x0 + x4 + j, w);
-            hA[ey][x4 + j] = sa[j] / cnt;
-            hB[ey][x4 + j] = sb[j] / cnt;
+            const int n = clip_count(x0 + x4 + j, w);
+            const float cnt = (float)n, k = sRcp[n];
+            hA[ey][x4 + j] = div_count(sa[j], cnt, k);
+            hB[ey][x4 + j] = div_count(sb[j], cnt, k);
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < (WEMA_GF_TY / 4) * WEMA_GF_TX; i += 256) {
-        const int y4 = 4 * (i / WEMA_GF_TX), x = i % WEMA_GF_TX;
-        const int gx = x0 + x;
-        if (gx >= w) continue;
-        float va[12], vb[12];
+    if (gx >= w) return;
+    float va[12], vb[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            va[k] = hA[y4 + k][x];
-            vb[k] = hB[y4 + k][x];
-        }
-        float sa[4], sb[4];
-        sums4(va, sa);
-        sums4(vb, sb);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int gy = y0 + y4 + j;
-            if (gy >= h) break;
-            const float cnt = (float)clip_count(gy, h);
-            const size_t idx = (size_t)gy * w + gx;
-            const float q = fmaf(sa[j] / cnt, src_luma(J, idx), sb[j] / cnt);
-            write_rgb(J, idx, q);
-        }
+    for (int k = 0; k < 12; ++k) {
+        va[k] = hA[y4 + k][xo];
+        vb[k] = hB[y4 + k][xo];
     }
+    float sa[4], sb[4];
+    sums4(va, sa);
+    sums4(vb, sb);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int gy = y0 + y4 + j;
+        if (gy >= h) break;
+        const int n = clip_count(gy, h);
+        const float cnt = (float)n, k = sRcp[n];
+        const float q = fmaf(div_count(sa[j], cnt, k), src[j].luma(), div_count(sb[j], cnt, k));
+        write_rgb_px<U8>(J, (size_t)gy * w + gx, src[j], q);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gf2(const EgressJob *__restrict__ jobs, int njobs) {
+    __shared__ __attribute__((aligned(16))) float sA[GF_EY][GF_PITCH];
+    __shared__ __attribute__((aligned(16))) float sB[GF_EY][GF_PITCH];
+    __shared__ float hA[GF_EY][WEMA_GF_TX], hB[GF_EY][WEMA_GF_TX];
+    __shared__ float sRcp[16];
+    fill_rcp(sRcp);
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int jid = find_job(jobs, njobs, bid);
+    const EgressJob &J = jobs[jid];
+    const int t = bid - J.tile_begin;
+    const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
+    if (J.src_u8) gf2_body<true>(J, x0, y0, sA, sB, hA, hB, sRcp);
+    else gf2_body<false>(J, x0, y0, sA, sB, hA, hB, sRcp);
 }
 
 // 5x5 binomial blur of the luma delta with weights renormalised at the borders
@@ -952,22 +1079,65 @@ __global__ void __launch_bounds__(256) k_gauss(const EgressJob *__restrict__ job
     }
 }
 
-// Colour mode output (src/wema.c:80-102).
+// Colour mode output (src/wema.c:80-102), 4 pixels per thread: the three planes are read
+// as 16-byte vectors and the 12 output bytes written as 3 dwords where aligned.
+__device__ __forceinline__ void ycc_px(float yv, float cb, float cr, float (&v)[3]) {
+    const float cbv = cb - 0.5f, crv = cr - 0.5f;
+    v[0] = clamp01(fmaf(1.402f, crv, yv));
+    v[1] = clamp01(yv - fmaf(0.344f, cbv, 0.714f * crv));
+    v[2] = clamp01(fmaf(1.772f, cbv, yv));
+}
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 __global__ void __launch_bounds__(256) k_ycc(const EgressJob *__restrict__ jobs, int njobs) {
     const int jid = find_job(jobs, njobs, (int)blockIdx.x);
     const EgressJob &J = jobs[jid];
-    const size_t i = (size_t)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
-    if (i >= (size_t)J.w * J.h) return;
-    const float yv = J.gray[i], cbv = J.cb[i] - 0.5f, crv = J.cr[i] - 0.5f;
-    float v[3];
-    v[0] = fmaf(1.402f, crv, yv);
-    v[1] = yv - fmaf(0.344f, cbv, 0.714f * crv);
-    v[2] = fmaf(1.772f, cbv, yv);
+    const size_t P = (size_t)J.w * J.h;
+    const size_t i0 = ((size_t)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x) * 4;
+    if (i0 >= P) return;
+    const GLOBAL_AS float *Y = gp(J.gray), *CB = gp(J.cb), *CR = gp(J.cr);
+    const bool vin = i0 + 4 <= P &&
+                     ((((uintptr_t)(Y + i0)) | ((uintptr_t)(CB + i0)) | ((uintptr_t)(CR + i0))) & 15u) == 0;
+    if (vin) {
+        const v4f y4 = *(const GLOBAL_AS v4f *)(Y + i0);
+        const v4f b4 = *(const GLOBAL_AS v4f *)(CB + i0);
+        const v4f r4 = *(const GLOBAL_AS v4f *)(CR + i0);
+        float v[4][3];
+        ycc_px(y4.x, b4.x, r4.x, v[0]);
+        ycc_px(y4.y, b4.y, r4.y, v[1]);
+        ycc_px(y4.z, b4.z, r4.z, v[2]);
+        ycc_px(y4.w, b4.w, r4.w, v[3]);
+        if (J.out_u8) {
+            GLOBAL_AS unsigned char *o = gp(J.out_u8) + i0 * 3;
+            if ((((uintptr_t)o) & 3u) == 0) {
+                unsigned b[12];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float c = clamp01(v[k]);
-        if (J.out_u8) J.out_u8[i * 3 + k] = egress_u8(c);
-        if (J.out_f32) J.out_f32[i * 3 + k] = c;
+                for (int k = 0; k < 12; ++k) b[k] = egress_u8(v[k / 3][k % 3]);
+                GLOBAL_AS unsigned *od = (GLOBAL_AS unsigned *)o;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    od[d] = b[4 * d] | (b[4 * d + 1] << 8) | (b[4 * d + 2] << 16) | (b[4 * d + 3] << 24);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 12; ++k) o[k] = egress_u8(v[k / 3][k % 3]);
+            }
+        }
+        if (J.out_f32) {
+            GLOBAL_AS float *o = gp(J.out_f32) + i0 * 3;
+#pragma unroll
+            for (int k = 0; k < 12; ++k) o[k] = v[k / 3][k % 3];
+        }
+        return;
+    }
+    for (size_t i = i0; i < i0 + 4 && i < P; ++i) {
+        float v[3];
+        ycc_px(Y[i], CB[i], CR[i], v);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (J.out_u8) gp(J.out_u8)[i * 3 + k] = egress_u8(v[k]);
+            if (J.out_f32) gp(J.out_f32)[i * 3 + k] = v[k];
+        }
     }
 }
 
