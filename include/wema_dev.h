@@ -55,6 +55,10 @@ void wema_dev_params_default(wema_dev_params *p, int width, int height, float fp
  * (f_low <= 0, f_low >= f_high, f_high >= fs / 2) and the same float arithmetic. */
 int wema_dev_design_bandpass(float f_low, float f_high, float fs, float *c10, int *warmup);
 
+/* The reference's automatic level count (dwt_compute_levels, dwt.c:588-596: halvings of
+ * the smaller side while it is >= 16, at most 4, at least 1). */
+int wema_dev_auto_levels(int width, int height);
+
 /* Device bytes a session with these options holds (HBM admission; no allocation). */
 int wema_dev_query_bytes(const wema_dev_params *p, size_t *bytes);
 
@@ -85,6 +89,12 @@ int wema_dev_process_f32(wema_dev_t *s, const float *d_in, uint8_t *d_out_u8,
                          float *d_out_f32, int n, void *hip_stream);
 /* Host-buffer variant (one frame): copies in, processes, copies out, waits. */
 int wema_dev_process_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out);
+/* Host-buffer variant on the reference's Frame data (float RGB in [0,1], HWC), n frames
+ * (n <= max_batch) given as arrays of frame pointers: what wema_process_frame
+ * (wema.c:746) and wema_batch_process (wema_batch.c:352) receive and return. Copies in,
+ * processes, copies out (clamped float RGB), waits. Pinned frames (wema_dev_host_alloc)
+ * copy at full link speed. */
+int wema_dev_process_host_f32(wema_dev_t *s, const float *const *h_in, float *const *h_out, int n);
 int wema_dev_sync(wema_dev_t *s);
 
 /* Pinned host memory for frame slots (hipHostMalloc). */

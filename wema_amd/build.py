@@ -1,4 +1,5 @@
-"""Builds the gfx950 pipeline library wema_amd/libwema_dev.so with hipcc.
+"""Builds the gfx950 pipeline library wema_amd/libwema_dev.so with hipcc, and the C11 host
+library wema_amd/libwema.so (the include/wema.h API over it) with gcc.
 
 Usage: python wema_amd/build.py   (a standalone script: importing the package needs the
 library it builds)
@@ -10,6 +11,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwema_dev.so")
+HOST_LIB = os.path.join(HERE, "libwema.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["wema_kernels.hip", "wema_dev.hip", "wema_synth.hip", "wema_pool.hip", "wema_serve.hip"]
 
@@ -46,7 +49,24 @@ def build(verbose=False):
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libwema_dev.so")
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose=False):
+    """The C11 host: plain gcc, linked against libwema_dev.so found next to it."""
+    tmp = HOST_LIB + ".tmp%d" % os.getpid()
+    cmd = ["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-Wpedantic", "-fPIC", "-shared",
+           "-I" + INCLUDE, os.path.join(HERE, "host", "wema_host.c"), "-L" + HERE, "-lwema_dev",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-Bsymbolic", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("gcc failed building libwema.so")
+    os.replace(tmp, HOST_LIB)
+    return HOST_LIB
 
 
 if __name__ == "__main__":

@@ -18,8 +18,8 @@
 #include "wema_jobs.h"
 
 extern "C" {
-hipError_t wema_launch_dwt_fwd(const DwtfJob *, int, int, hipStream_t);
-hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, hipStream_t);
+hipError_t wema_launch_dwt_fwd(const DwtfJob *, int, int, int, hipStream_t);
+hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, int, hipStream_t);
 hipError_t wema_launch_iir(const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_win(const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_egress(int, const EgressJob *, int, int, hipStream_t);
@@ -113,6 +113,8 @@ float wema_fade_amp(float amp, long global_frame) {
     }
     return amp;
 }
+
+extern "C" int wema_dev_auto_levels(int width, int height) { return wema_auto_levels(width, height); }
 
 int wema_auto_levels(int width, int height) {
     // dwt_compute_levels (src/dwt.c:588-596), shipped cap of 4 levels
@@ -327,6 +329,10 @@ extern "C" void wema_dev_close(wema_dev_t *s) {
         if (s->stage_ev[k]) hipEventDestroy(s->stage_ev[k]);
     }
     if (s->arena) hipFree(s->arena);
+    if (s->io_in) hipFree(s->io_in);
+    if (s->io_out) hipFree(s->io_out);
+    if (s->io_in_f32) hipFree(s->io_in_f32);
+    if (s->io_out_f32) hipFree(s->io_out_f32);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
 }
@@ -476,7 +482,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.w = w;
                 j.h = h;
                 j.tiles_x = tiles_x;
-                jb.add_dwti(L - 1 - l, j, tiles, 16.0 * g.lw[l + 1] * g.lh[l + 1] + 4.0 * w * h);
+                jb.add_dwti(WEMA_MAX_LEVELS - 1 - l, j, tiles, 16.0 * g.lw[l + 1] * g.lh[l + 1] + 4.0 * w * h);
             }
     }
 
@@ -514,8 +520,8 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
 }
 
 const char *const wema_timer_names[TK_COUNT] = {
-    "k_dwt_fwd(level0)", "k_dwt_fwd(levels1+)", "k_iir", "k_win", "k_dwt_inv(levels1+)",
-    "k_dwt_inv(level0)", "k_gf1", "k_gf2", "k_gauss", "k_ycc", "k_passthru"};
+    "k_dwt_fwd0", "k_dwt_fwd", "k_iir", "k_win", "k_dwt_inv",
+    "k_dwt_inv0", "k_gf1", "k_gf2", "k_gauss", "k_ycc", "k_passthru"};
 
 static int timer_mark(LaunchTimer *t, int kind, double bytes, hipStream_t st) {
     if (!t || !t->enabled) return 0;
@@ -563,7 +569,7 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
         const JobList &jl = jb.dwtf[l];
         if (!jl.count) continue;
-        HIP_OK(wema_launch_dwt_fwd((const DwtfJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        HIP_OK(wema_launch_dwt_fwd((const DwtfJob *)(base + jl.offset), jl.count, jl.blocks, l == 0, st));
         if (timer_mark(timer, l == 0 ? TK_DWT_FWD0 : TK_DWT_FWD, jl.bytes, st) < 0) return -1;
     }
     const JobList &tl = jb.temporal;
@@ -576,13 +582,13 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
         HIP_OK(wema_launch_win((const TemporalJob *)(base + wl.offset), wl.count, wl.blocks, st));
         if (timer_mark(timer, TK_WIN, wl.bytes, st) < 0) return -1;
     }
-    int last_inv = -1;
-    for (int l = 0; l < WEMA_MAX_LEVELS; l++)
-        if (jb.dwti[l].count) last_inv = l;
+    // inverse stages, coarsest first: level l of every stream is stage WEMA_MAX_LEVELS-1-l,
+    // so the last stage holds exactly the level-0 (full-resolution) jobs
+    const int last_inv = WEMA_MAX_LEVELS - 1;
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
         const JobList &jl = jb.dwti[l];
         if (!jl.count) continue;
-        HIP_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        HIP_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, l == last_inv, st));
         if (timer_mark(timer, l == last_inv ? TK_DWT_INV0 : TK_DWT_INV, jl.bytes, st) < 0) return -1;
     }
     static const int eg_kind[EG_KINDS] = {TK_GF1, TK_GF2, TK_GAUSS, TK_YCC, TK_PASSTHRU};
@@ -651,6 +657,30 @@ extern "C" int wema_dev_process_host_u8(wema_dev_t *s, const uint8_t *h_in, uint
     if (process(s, s->io_in, 0, (uint8_t *)s->io_out, nullptr, 1, s->stream) < 0) return -1;
     s->last_stream = s->stream;
     HIP_OK(hipMemcpyAsync(h_out, s->io_out, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int wema_dev_process_host_f32(wema_dev_t *s, const float *const *h_in, float *const *h_out,
+                                         int n) {
+    if (!s || !h_in || !h_out || n < 1) { wema_set_error("bad argument"); return -1; }
+    if (n > s->p.max_batch) { wema_set_error("%d frames exceed max_batch %d", n, s->p.max_batch); return -1; }
+    for (int i = 0; i < n; i++)
+        if (!h_in[i] || !h_out[i]) { wema_set_error("null frame %d", i); return -1; }
+    HIP_OK(hipSetDevice(s->p.device));
+    const size_t frame = s->g.P * 3;
+    if (!s->io_in_f32) {
+        HIP_OK(hipMalloc(&s->io_in_f32, sizeof(float) * frame * s->p.max_batch));
+        HIP_OK(hipMalloc(&s->io_out_f32, sizeof(float) * frame * s->p.max_batch));
+    }
+    for (int i = 0; i < n; i++)
+        HIP_OK(hipMemcpyAsync(s->io_in_f32 + frame * i, h_in[i], sizeof(float) * frame,
+                              hipMemcpyHostToDevice, s->stream));
+    if (process(s, s->io_in_f32, 1, nullptr, s->io_out_f32, n, s->stream) < 0) return -1;
+    s->last_stream = s->stream;
+    for (int i = 0; i < n; i++)
+        HIP_OK(hipMemcpyAsync(h_out[i], s->io_out_f32 + frame * i, sizeof(float) * frame,
+                              hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
     return 0;
 }

@@ -81,7 +81,8 @@ struct wema_dev {
     int stage_cur = 0;
     LaunchTimer timer;
     bool pooled = false;        // owned by a wema_pool (no staging buffers of its own)
-    void *io_in = nullptr, *io_out = nullptr;
+    void *io_in = nullptr, *io_out = nullptr;           // host-buffer staging (RGB24)
+    float *io_in_f32 = nullptr, *io_out_f32 = nullptr;  // host-buffer staging (float RGB)
 };
 
 struct JobBuilder {

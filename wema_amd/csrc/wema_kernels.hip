@@ -315,9 +315,10 @@ __device__ __forceinline__ void row_pass(float (*R)[128], const DwtfJob &J, int 
 constexpr int DWT_THREADS = 256;
 constexpr int HALF_PAIRS = WEMA_RP / 2 + 2;   // pairs a half-column thread lifts
 
-__global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restrict__ jobs, int njobs) {
+// RGB: level 0 (source kinds from RGB24 / float RGB frames); else the previous level's LL.
+template <bool RGB>
+__device__ __forceinline__ void dwt_fwd_body(float (*R)[128], const DwtfJob *__restrict__ jobs, int njobs) {
     constexpr int RP = WEMA_RP;
-    __shared__ float R[2 * RP][128];
 
     const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int jid = find_job(jobs, njobs, bid);
@@ -339,14 +340,17 @@ __global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restri
 
     // ---- row pass
     const int sx0 = 2 * (px0 - 2), sy0 = 2 * (py0 - 2);
-    switch (J.src_kind) {
-    case SRC_F32_PLANE: row_pass<SRC_F32_PLANE>(R, J, sx0, sy0, lane, wave); break;
-    case SRC_U8_Y: row_pass<SRC_U8_Y>(R, J, sx0, sy0, lane, wave); break;
-    case SRC_U8_CB: row_pass<SRC_U8_CB>(R, J, sx0, sy0, lane, wave); break;
-    case SRC_U8_CR: row_pass<SRC_U8_CR>(R, J, sx0, sy0, lane, wave); break;
-    case SRC_F32_Y: row_pass<SRC_F32_Y>(R, J, sx0, sy0, lane, wave); break;
-    case SRC_F32_CB: row_pass<SRC_F32_CB>(R, J, sx0, sy0, lane, wave); break;
-    default: row_pass<SRC_F32_CR>(R, J, sx0, sy0, lane, wave); break;
+    if (!RGB) {
+        row_pass<SRC_F32_PLANE>(R, J, sx0, sy0, lane, wave);
+    } else {
+        switch (J.src_kind) {
+        case SRC_U8_Y: row_pass<SRC_U8_Y>(R, J, sx0, sy0, lane, wave); break;
+        case SRC_U8_CB: row_pass<SRC_U8_CB>(R, J, sx0, sy0, lane, wave); break;
+        case SRC_U8_CR: row_pass<SRC_U8_CR>(R, J, sx0, sy0, lane, wave); break;
+        case SRC_F32_Y: row_pass<SRC_F32_Y>(R, J, sx0, sy0, lane, wave); break;
+        case SRC_F32_CB: row_pass<SRC_F32_CB>(R, J, sx0, sy0, lane, wave); break;
+        default: row_pass<SRC_F32_CR>(R, J, sx0, sy0, lane, wave); break;
+        }
     }
     __syncthreads();
 
@@ -435,13 +439,22 @@ __global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restri
     }
 }
 
+// Level 0 (from the RGB frame) and the coarser levels are separate kernels.
+__global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd0(const DwtfJob *__restrict__ jobs, int njobs) {
+    __shared__ float R[2 * WEMA_RP][128];
+    dwt_fwd_body<true>(R, jobs, njobs);
+}
+__global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restrict__ jobs, int njobs) {
+    __shared__ float R[2 * WEMA_RP][128];
+    dwt_fwd_body<false>(R, jobs, njobs);
+}
+
 // ---------------------------------------------------------------------------------------
 // Inverse DWT level (src/dwt.c:475-582 + 705-767): packed [LL | HL ; LH | HH] columns
 // first, then rows; each 1D inverse starts by undoing the K scaling.
-__global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv(const DwtiJob *__restrict__ jobs, int njobs) {
+__device__ __forceinline__ void dwt_inv_body(float (*R)[128], const DwtiJob *__restrict__ jobs, int njobs) {
     constexpr int RP = WEMA_RP;
     constexpr int ROWS = 2 * (RP - 4);
-    __shared__ float R[ROWS][128];
 
     const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int jid = find_job(jobs, njobs, bid);
@@ -521,6 +534,17 @@ __global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv(const DwtiJob *__restri
             }
         }
     }
+}
+
+// Level 0 (full-resolution reconstruction) and the coarser levels are separate kernels
+// (the same code), so that profiles tell them apart.
+__global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv0(const DwtiJob *__restrict__ jobs, int njobs) {
+    __shared__ float R[2 * (WEMA_RP - 4)][128];
+    dwt_inv_body(R, jobs, njobs);
+}
+__global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv(const DwtiJob *__restrict__ jobs, int njobs) {
+    __shared__ float R[2 * (WEMA_RP - 4)][128];
+    dwt_inv_body(R, jobs, njobs);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1173,14 +1197,16 @@ __global__ void __launch_bounds__(256) k_synth(const SynthFrame *__restrict__ fr
 // launchers (called by the host scheduler)
 
 extern "C" {
-hipError_t wema_launch_dwt_fwd(const DwtfJob *jobs, int njobs, int nblocks, hipStream_t s) {
+hipError_t wema_launch_dwt_fwd(const DwtfJob *jobs, int njobs, int nblocks, int level0, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dwt_fwd, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
+    if (level0) hipLaunchKernelGGL(k_dwt_fwd0, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
+    else hipLaunchKernelGGL(k_dwt_fwd, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
     return hipGetLastError();
 }
-hipError_t wema_launch_dwt_inv(const DwtiJob *jobs, int njobs, int nblocks, hipStream_t s) {
+hipError_t wema_launch_dwt_inv(const DwtiJob *jobs, int njobs, int nblocks, int level0, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dwt_inv, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
+    if (level0) hipLaunchKernelGGL(k_dwt_inv0, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
+    else hipLaunchKernelGGL(k_dwt_inv, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 hipError_t wema_launch_iir(const TemporalJob *jobs, int njobs, int nblocks, hipStream_t s) {

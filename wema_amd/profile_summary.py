@@ -1,0 +1,113 @@
+#!/usr/bin/env python3
+"""Summarise rocprofv3 CSV output of a bench.py workload into a short markdown table and
+a traffic file (HBM bytes per launch per kernel) that bench.py reads.
+
+Inputs: one `--kernel-trace --stats` directory and one `--pmc FETCH_SIZE` and one
+`--pmc WRITE_SIZE` directory of the same workload, each run with bench.py
+--copy-calibration (the PMC runs), which adds a 1 GiB device copy: its counter readings
+give the scale from counter units to bytes (FETCH_SIZE and WRITE_SIZE are in KiB; the copy
+fetches and writes exactly 1 GiB).
+
+    python -m wema_amd.profile_summary --workload c2 --title "..." \\
+        --trace DIR --fetch DIR --write DIR --cmd "..." --out profiles
+"""
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+
+GIB = float(1 << 30)
+
+
+def _one(pattern):
+    files = glob.glob(pattern)
+    if len(files) != 1:
+        raise SystemExit("expected one file for %s, found %d" % (pattern, len(files)))
+    return files[0]
+
+
+def _short(name):
+    return name.split("(")[0]
+
+
+def kernel_stats(trace_dir):
+    rows = list(csv.DictReader(open(_one(os.path.join(trace_dir, "*_kernel_stats.csv")))))
+    return [(_short(r["Name"]), int(r["Calls"]), float(r["AverageNs"]) / 1e3,
+             float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, float(r["Percentage"])) for r in rows]
+
+
+def counters(pmc_dir, counter):
+    rows = list(csv.DictReader(open(_one(os.path.join(pmc_dir, "*_counter_collection.csv")))))
+    per = collections.defaultdict(list)
+    for r in rows:
+        if r["Counter_Name"] == counter:
+            per[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return per
+
+
+def calibration(per, counter):
+    """Bytes per counter unit, from the 1 GiB copy: the copy dispatches are the device
+    copies whose reading is the largest (the other copies are small table uploads)."""
+    copies = per.get("__amd_rocclr_copyBuffer", [])
+    if not copies:
+        raise SystemExit("no copy kernel in the %s run (bench.py --copy-calibration)" % counter)
+    big = max(copies)
+    return GIB / (big * 1024.0), big
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", required=True)
+    ap.add_argument("--title", required=True)
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch", required=True)
+    ap.add_argument("--write", required=True)
+    ap.add_argument("--cmd", action="append", default=[], help="command lines that made the data")
+    ap.add_argument("--out", default="profiles")
+    ap.add_argument("--md-name", required=True)
+    a = ap.parse_args()
+
+    stats = kernel_stats(a.trace)
+    fetch = counters(a.fetch, "FETCH_SIZE")
+    write = counters(a.write, "WRITE_SIZE")
+    fs, fraw = calibration(fetch, "FETCH_SIZE")
+    ws, wraw = calibration(write, "WRITE_SIZE")
+
+    traffic = {}
+    for name in sorted(set(fetch) | set(write)):
+        if not name.startswith("k_") or name == "k_synth":
+            continue
+        fb = sum(fetch.get(name, [0])) / max(len(fetch.get(name, [])), 1) * 1024.0 * fs
+        wb = sum(write.get(name, [0])) / max(len(write.get(name, [])), 1) * 1024.0 * ws
+        traffic[name] = {"fetch_bytes_per_launch": round(fb), "write_bytes_per_launch": round(wb),
+                         "hbm_bytes_per_launch": round(fb + wb), "launches": len(fetch.get(name, []))}
+    os.makedirs(a.out, exist_ok=True)
+    with open(os.path.join(a.out, "traffic_%s.json" % a.workload), "w") as f:
+        json.dump({"workload": a.workload, "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+                   "kernels": traffic}, f, indent=1)
+        f.write("\n")
+
+    lines = ["# %s" % a.title, "", "Commands (from the repository root on the GPU host):", ""]
+    lines += ["    " + c for c in a.cmd]
+    lines += ["", "## Kernel durations (rocprofv3 --kernel-trace --stats)", "",
+              "| kernel | calls | avg us | min us | max us | % of GPU time |", "|---|---:|---:|---:|---:|---:|"]
+    for n, c, avg, mn, mx, pct in stats:
+        lines.append("| %s | %d | %.1f | %.1f | %.1f | %.2f |" % (n, c, avg, mn, mx, pct))
+    lines += ["", "## HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE)", "",
+              "Scale from the 1 GiB device copy of the same runs: FETCH_SIZE read %.0f KiB "
+              "(x%.3f), WRITE_SIZE read %.0f KiB (x%.3f)." % (fraw, fs, wraw, ws), "",
+              "| kernel | launches | fetch MB | write MB | HBM MB per launch |", "|---|---:|---:|---:|---:|"]
+    for n, t in sorted(traffic.items(), key=lambda kv: -kv[1]["hbm_bytes_per_launch"]):
+        lines.append("| %s | %d | %.1f | %.1f | %.1f |" % (
+            n, t["launches"], t["fetch_bytes_per_launch"] / 1e6, t["write_bytes_per_launch"] / 1e6,
+            t["hbm_bytes_per_launch"] / 1e6))
+    lines.append("")
+    with open(os.path.join(a.out, a.md_name), "w") as f:
+        f.write("\n".join(lines))
+    print("wrote", os.path.join(a.out, a.md_name), "and traffic_%s.json" % a.workload)
+
+
+if __name__ == "__main__":
+    main()
