@@ -27,6 +27,7 @@
 
 #include <stdbool.h>
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -115,6 +116,37 @@ int  wema_batch_process(WemaContext *wema, WemaBatchContext *batch,
 /* src/wema.c:29-40 (pinned host memory here) */
 int  frame_alloc(Frame *frame, int width, int height, int channels);
 void frame_free(Frame *frame);
+
+/* Types of the CLI and its FFmpeg pipes, which live in the same header in the reference
+ * (src/wema.h:166-212); carried with the same fields so that its main.c, ffmpeg_io.c and
+ * async_io.c build against this header unchanged. The pipeline does not use them. */
+typedef struct {
+    int      in_pid, in_fd;
+    int      out_pid, out_fd;
+    int      width, height;
+    float    fps;
+    int64_t  total_frames;
+    char    *input_path;
+    uint8_t *raw_buffer;
+    size_t   buffer_size;
+} FFmpegIO;
+
+typedef struct {
+    char  *input_path;
+    char  *output_path;
+    float  amp_factor;
+    float  f_low, f_high;
+    int    temporal_window;
+    char  *ff_codec;
+    char  *ff_options;
+    bool   verbose;
+    bool   edge_aware;
+    bool   bilateral_temp;
+    bool   color_mode;
+    bool   use_iir;
+    int    batch_size;
+    int    num_threads;
+} WemaConfig;
 
 /* Text of the last error (no counterpart in the reference). */
 const char *wema_last_error(void);
