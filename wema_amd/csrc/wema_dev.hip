@@ -20,6 +20,7 @@
 extern "C" {
 hipError_t wema_launch_dwt_fwd(const DwtfJob *, int, int, int, hipStream_t);
 hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, int, hipStream_t);
+hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *, int, int, hipStream_t);
 hipError_t wema_launch_iir(const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_win(const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_egress(int, const EgressJob *, int, int, hipStream_t);
@@ -466,30 +467,57 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         jb.add_temporal(j, tiles_t, tb, !p.use_iir);
     }
 
+    // frames whose output is the input itself (window mode while the window fills)
+    auto passthrough = [&](int f) { return !p.use_iir && frame0 + f + 1 < p.window; };
+    auto planes_ptr = [&](int f, int c, int l) -> const float * {
+        if (!s->ampc) return coef_ptr(f, c) + g.plane_off[l];
+        return s->ampc + ((size_t)f * nch + c) * nu + g.plane_off[l];
+    };
+    const size_t out_b = (out_u8 ? 3 * P : 0) + (out_f32 ? 12 * P : 0);
+
     // inverse DWT levels, coarsest first
     for (int l = L - 1; l >= 0; l--) {
         const int w = g.lw[l], h = g.lh[l];
         const int tiles_x = wema_ceil_div(g.lw[l + 1], WEMA_TILE_PAIRS_X);
         const int tiles = tiles_x * wema_ceil_div(g.lh[l + 1], WEMA_TILE_PAIRS_Y);
-        for (int f = 0; f < n; f++)
-            for (int c = 0; c < nch; c++) {
+        const double in_b = 16.0 * g.lw[l + 1] * g.lh[l + 1];
+        for (int f = 0; f < n; f++) {
+            if (l == 0 && p.color && !passthrough(f)) {
+                // colour: level 0 of Y, Cb, Cr and the RGB egress in one job
                 DwtiJob j{};
-                j.ll = ll_ptr(f, c, l);
-                if (!s->ampc) j.planes = coef_ptr(f, c) + g.plane_off[l];
-                else j.planes = s->ampc + ((size_t)f * nch + c) * nu + g.plane_off[l];
-                j.out = l == 0 ? rec_ptr(f, c) : ll_ptr(f, c, l - 1);
-                j.acc_clear = l == L - 1 ? s->acc + (size_t)f * nch + c : nullptr;
+                j.ll = ll_ptr(f, 0, 0);
+                j.planes = planes_ptr(f, 0, 0);
+                j.ll_cstride = (long long)(ll_ptr(f, 1, 0) - ll_ptr(f, 0, 0));
+                j.planes_cstride = (long long)(planes_ptr(f, 1, 0) - planes_ptr(f, 0, 0));
+                j.nch = nch;
+                j.out_u8 = out_u8 ? out_u8 + (size_t)f * P * 3 : nullptr;
+                j.out_f32 = out_f32 ? out_f32 + (size_t)f * P * 3 : nullptr;
+                j.acc_clear = L == 1 ? s->acc + (size_t)f * nch : nullptr;
                 j.w = w;
                 j.h = h;
                 j.tiles_x = tiles_x;
-                jb.add_dwti(WEMA_MAX_LEVELS - 1 - l, j, tiles, 16.0 * g.lw[l + 1] * g.lh[l + 1] + 4.0 * w * h);
+                jb.add_dwti_ycc(j, tiles, in_b * nch + (double)out_b);
+                continue;
             }
+            for (int c = 0; c < nch; c++) {
+                DwtiJob j{};
+                j.ll = ll_ptr(f, c, l);
+                j.planes = planes_ptr(f, c, l);
+                j.out = l == 0 ? rec_ptr(f, c) : ll_ptr(f, c, l - 1);
+                j.acc_clear = l == L - 1 ? s->acc + (size_t)f * nch + c : nullptr;
+                j.nch = 1;
+                j.w = w;
+                j.h = h;
+                j.tiles_x = tiles_x;
+                jb.add_dwti(WEMA_MAX_LEVELS - 1 - l, j, tiles, in_b + 4.0 * w * h);
+            }
+        }
     }
 
     // output stage
     const int gf_tx = wema_ceil_div(g.W, WEMA_GF_TX);
     const int gf_tiles = gf_tx * wema_ceil_div(g.H, WEMA_GF_TY);
-    const int px4_tiles = (int)((P + 1023) / 1024), byte_tiles = (int)((3 * P + 255) / 256);
+    const int byte_tiles = (int)((3 * P + 255) / 256);
     for (int f = 0; f < n; f++) {
         EgressJob j{};
         j.gray = rec_ptr(f, 0);
@@ -503,13 +531,11 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         j.w = g.W;
         j.h = g.H;
         j.tiles_x = gf_tx;
-        const long gidx = frame0 + f;
         const double src_b = 3.0 * P * (in_f32 ? 4 : 1);
-        const double out_b = (out_u8 ? 3.0 * P : 0) + (out_f32 ? 12.0 * P : 0);
-        if (!p.use_iir && gidx + 1 < p.window) {
+        if (passthrough(f)) {
             jb.add_egress(EG_PASSTHRU, j, byte_tiles, src_b + out_b);
         } else if (p.color) {
-            jb.add_egress(EG_YCC, j, px4_tiles, 12.0 * P + out_b);
+            continue;   // written by the colour output job of the inverse stage
         } else if (p.edge_aware || p.batch_path) {
             jb.add_egress(EG_GF1, j, gf_tiles, 4.0 * P + src_b + 8.0 * P);
             jb.add_egress(EG_GF2, j, gf_tiles, 8.0 * P + src_b + out_b);
@@ -521,7 +547,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
 
 const char *const wema_timer_names[TK_COUNT] = {
     "k_dwt_fwd0", "k_dwt_fwd", "k_iir", "k_win", "k_dwt_inv",
-    "k_dwt_inv0", "k_gf1", "k_gf2", "k_gauss", "k_ycc", "k_passthru"};
+    "k_dwt_inv0", "k_dwt_inv0_ycc", "k_gf1", "k_gf2", "k_gauss", "k_passthru"};
 
 static int timer_mark(LaunchTimer *t, int kind, double bytes, hipStream_t st) {
     if (!t || !t->enabled) return 0;
@@ -591,7 +617,12 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
         HIP_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, l == last_inv, st));
         if (timer_mark(timer, l == last_inv ? TK_DWT_INV0 : TK_DWT_INV, jl.bytes, st) < 0) return -1;
     }
-    static const int eg_kind[EG_KINDS] = {TK_GF1, TK_GF2, TK_GAUSS, TK_YCC, TK_PASSTHRU};
+    if (jb.dwti_ycc.count) {
+        const JobList &jl = jb.dwti_ycc;
+        HIP_OK(wema_launch_dwt_inv_ycc((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        if (timer_mark(timer, TK_DWT_INV0_YCC, jl.bytes, st) < 0) return -1;
+    }
+    static const int eg_kind[EG_KINDS] = {TK_GF1, TK_GF2, TK_GAUSS, TK_PASSTHRU};
     for (int k = 0; k < EG_KINDS; k++) {
         const JobList &jl = jb.egress[k];
         if (!jl.count) continue;

@@ -22,7 +22,8 @@ struct WemaGeom {
     size_t ll_total;
 };
 
-enum EgressKind { EG_GF1 = 0, EG_GF2 = 1, EG_GAUSS = 2, EG_YCC = 3, EG_PASSTHRU = 4, EG_KINDS = 5 };
+// Output stages after the inverse DWT (colour mode has none: its inverse writes RGB).
+enum EgressKind { EG_GF1 = 0, EG_GF2 = 1, EG_GAUSS = 2, EG_PASSTHRU = 3, EG_KINDS = 4 };
 
 struct JobList {
     size_t offset = 0;   // byte offset in the packed table
@@ -46,7 +47,7 @@ struct LaunchTimer {
     long launches[16] = {0};
 };
 enum TimerKind { TK_DWT_FWD0 = 0, TK_DWT_FWD = 1, TK_IIR = 2, TK_WIN = 3, TK_DWT_INV = 4,
-                 TK_DWT_INV0 = 5, TK_GF1 = 6, TK_GF2 = 7, TK_GAUSS = 8, TK_YCC = 9,
+                 TK_DWT_INV0 = 5, TK_DWT_INV0_YCC = 6, TK_GF1 = 7, TK_GF2 = 8, TK_GAUSS = 9,
                  TK_PASSTHRU = 10, TK_COUNT = 11 };
 
 struct wema_dev {
@@ -90,8 +91,9 @@ struct JobBuilder {
     std::vector<TemporalJob> temporal_v;      // IIR jobs
     std::vector<TemporalJob> window_v;        // sliding-window jobs
     std::vector<DwtiJob> dwti_v[WEMA_MAX_LEVELS];
+    std::vector<DwtiJob> dwti_ycc_v;          // colour output jobs (level 0 + YCbCr -> RGB)
     std::vector<EgressJob> egress_v[EG_KINDS];
-    JobList dwtf[WEMA_MAX_LEVELS], temporal, window, dwti[WEMA_MAX_LEVELS], egress[EG_KINDS];
+    JobList dwtf[WEMA_MAX_LEVELS], temporal, window, dwti[WEMA_MAX_LEVELS], dwti_ycc, egress[EG_KINDS];
     std::vector<float> host_amp;
     size_t amp_offset = 0;
 
@@ -114,6 +116,12 @@ struct JobBuilder {
         dwti[stage].blocks += tiles;
         dwti[stage].bytes += bytes;
         dwti_v[stage].push_back(j);
+    }
+    void add_dwti_ycc(DwtiJob j, int tiles, double bytes) {
+        j.tile_begin = dwti_ycc.blocks;
+        dwti_ycc.blocks += tiles;
+        dwti_ycc.bytes += bytes;
+        dwti_ycc_v.push_back(j);
     }
     void add_egress(int kind, EgressJob j, int tiles, double bytes) {
         j.tile_begin = egress[kind].blocks;
@@ -144,6 +152,7 @@ struct JobBuilder {
         ok = ok && put(dst, cap, off, temporal_v, temporal);
         ok = ok && put(dst, cap, off, window_v, window);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_v[l], dwti[l]);
+        ok = ok && put(dst, cap, off, dwti_ycc_v, dwti_ycc);
         for (int k = 0; k < EG_KINDS; k++) ok = ok && put(dst, cap, off, egress_v[k], egress[k]);
         JobList amp_list;
         ok = ok && put(dst, cap, off, host_amp, amp_list);

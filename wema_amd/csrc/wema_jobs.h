@@ -56,14 +56,22 @@ struct DwtfJob {
     int nch;
 };
 
+// Inverse job. The colour output job (k_dwt_inv0_ycc) synthesises level 0 of Y, Cb and Cr
+// (channel c at ll + c * ll_cstride, planes + c * planes_cstride) and writes RGB24 and/or
+// float RGB directly; the other jobs write one float plane to out.
 struct DwtiJob {
     const float *ll;                // approximation at this level (sw x sh)
     const float *planes;            // LH, HH, HL (amplified) of this level
     float *out;                     // reconstruction (w x h)
-    unsigned long long *acc_clear;  // accumulator to reset once read (or null)
+    unsigned long long *acc_clear;  // accumulators to reset once read (nch of them, or null)
+    uint8_t *out_u8;                // colour output job: RGB24 frame (or null)
+    float *out_f32;                 // colour output job: float RGB frame (or null)
+    long long ll_cstride, planes_cstride;
     int w, h;
     int tiles_x;
     int tile_begin;
+    int nch;
+    int pad_;
 };
 
 #define WEMA_MAX_BATCH 256

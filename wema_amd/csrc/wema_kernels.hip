@@ -6,7 +6,7 @@
 // so that DWT, IIR and amplification results are bit-identical to it.
 //
 // Stages (one launch per stage over a job table, see wema_jobs.h):
-//   k_dwt_fwd   one 2D CDF 9/7 analysis level: row lifting in registers across the 64
+//   k_dwt_fwd0/k_dwt_fwd  one 2D CDF 9/7 analysis level (level 0 / coarser levels): row lifting in registers across the 64
 //               lanes of a wave (neighbours by lane shuffles), column lifting per lane in
 //               registers, LDS tile in between; level 0 converts RGB24 to Y/Cb/Cr on load;
 //               writes LL + LH/HH/HL and adds sum|c| (Q32 integers) to the frame's counter.
@@ -14,11 +14,11 @@
 //               the state kept in registers, warm-up preseed, threshold mask, amplification.
 //   k_win       sliding-window temporal filter (mean removal, optional bilateral weights)
 //               + amplification.
-//   k_dwt_inv   one 2D synthesis level (columns, then rows).
+//   k_dwt_inv   one 2D synthesis level (columns, then rows); k_dwt_inv0 level 0.
+//   k_dwt_inv0_ycc  colour mode: level 0 of Y, Cb, Cr + YCbCr -> RGB24 in one pass.
 //   k_gf1/k_gf2 guided filter (r = 4, eps = 0.01) of the luma delta with 9x9 clipped box
 //               means in LDS; k_gf2 adds the smoothed delta to RGB and writes RGB24.
 //   k_gauss     5x5 binomial alternative (--no-edge-aware).
-//   k_ycc       colour mode: Y/Cb/Cr reconstructions -> RGB24.
 //   k_passthru  window mode before the window is full: output = input.
 //   k_synth     synthetic frames (deterministic integer generator).
 #include <hip/hip_runtime.h>
@@ -452,31 +452,22 @@ __global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restri
 // ---------------------------------------------------------------------------------------
 // Inverse DWT level (src/dwt.c:475-582 + 705-767): packed [LL | HL ; LH | HH] columns
 // first, then rows; each 1D inverse starts by undoing the K scaling.
-__device__ __forceinline__ void dwt_inv_body(float (*R)[128], const DwtiJob *__restrict__ jobs, int njobs) {
+// Column pass of one tile: thread = (column, half) as in the forward kernel; the packed
+// [LL | HL ; LH | HH] columns are read straight from the planes, lifted in registers and
+// left in R as rows of (even | odd) sample pairs.
+__device__ __forceinline__ void inv_col_pass(float (*R)[128], const float *ll, const float *planes,
+                                             int w, int h, int px0, int py0) {
     constexpr int RP = WEMA_RP;
-    constexpr int ROWS = 2 * (RP - 4);
-
-    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int jid = find_job(jobs, njobs, bid);
-    const DwtiJob J = jobs[jid];
-    const int t = bid - J.tile_begin;
-    if (t == 0 && threadIdx.x == 0 && J.acc_clear) *J.acc_clear = 0ull;
-    const int tx = t % J.tiles_x, ty = t / J.tiles_x;
-    const int w = J.w, h = J.h;
     const int sw = (w + 1) >> 1;
-    const int px0 = tx * WEMA_TILE_PAIRS_X, py0 = ty * WEMA_TILE_PAIRS_Y;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t n = (size_t)sw * ((h + 1) >> 1);
-
-    // ---- column pass: thread = (column, half) as in the forward kernel
     const int c = threadIdx.x & 127, half = threadIdx.x >> 7;
     const int k0 = half * (RP / 2 - 2);
     const bool high = c >= 64;
     const int I = px0 - 2 + (c & 63);
     const int sx = reflect_idx(2 * I + (high ? 1 : 0), w);
     const int col = sx >> 1;
-    const GLOBAL_AS float *Ep = high ? gp(J.planes) + 2 * n : gp(J.ll);   // HL : LL
-    const GLOBAL_AS float *Op = high ? gp(J.planes) + n : gp(J.planes);   // HH : LH
+    const GLOBAL_AS float *Ep = high ? gp(planes) + 2 * n : gp(ll);   // HL : LL
+    const GLOBAL_AS float *Op = high ? gp(planes) + n : gp(planes);   // HH : LH
     float e[HALF_PAIRS], o[HALF_PAIRS];
 #pragma unroll
     for (int k = 0; k < HALF_PAIRS; ++k) {
@@ -500,28 +491,44 @@ __device__ __forceinline__ void dwt_inv_body(float (*R)[128], const DwtiJob *__r
         R[row][c] = e[k];
         R[row + 1][c] = o[k];
     }
+}
+
+// Row r of the tile after the row synthesis: samples x = 2 * (px0 - 2 + lane) (ev) and x + 1
+// (ov) of image row 2 * py0 + r.
+__device__ __forceinline__ void inv_row(float (*R)[128], int r, int w, int lane, float &ev, float &ov) {
+    ev = R[r][lane];
+    ov = R[r][64 + lane];
+    if (w >= 2) {
+        ev *= WEMA_K;
+        ov *= WEMA_KINV;
+        lift_inv_lanes(ev, ov);
+    }
+}
+
+constexpr int INV_ROWS = 2 * (WEMA_RP - 4);   // image rows a tile produces
+
+__device__ __forceinline__ void dwt_inv_body(float (*R)[128], const DwtiJob *__restrict__ jobs, int njobs) {
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int jid = find_job(jobs, njobs, bid);
+    const DwtiJob J = jobs[jid];
+    const int t = bid - J.tile_begin;
+    if (t == 0 && threadIdx.x == 0 && J.acc_clear) *J.acc_clear = 0ull;
+    const int w = J.w, h = J.h;
+    const int px0 = (t % J.tiles_x) * WEMA_TILE_PAIRS_X, py0 = (t / J.tiles_x) * WEMA_TILE_PAIRS_Y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+    inv_col_pass(R, J.ll, J.planes, w, h, px0, py0);
     __syncthreads();
 
     // ---- row pass: 2 rows at a time per wave
     const int x = 2 * (px0 - 2 + lane);
     const bool lane_ok = lane >= 2 && lane < 62;
-    for (int r0 = wave * 2; r0 < ROWS; r0 += 8) {
+    for (int r0 = wave * 2; r0 < INV_ROWS; r0 += 8) {
         const int y0 = 2 * py0 + r0;
         if (y0 >= h) break;
         float ev[2], ov[2];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            ev[k] = R[r0 + k][lane];
-            ov[k] = R[r0 + k][64 + lane];
-        }
-        if (w >= 2) {
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                ev[k] *= WEMA_K;
-                ov[k] *= WEMA_KINV;
-                lift_inv_lanes(ev[k], ov[k]);
-            }
-        }
+        for (int k = 0; k < 2; ++k) inv_row(R, r0 + k, w, lane, ev[k], ov[k]);
         if (lane_ok) {
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -539,12 +546,89 @@ __device__ __forceinline__ void dwt_inv_body(float (*R)[128], const DwtiJob *__r
 // Level 0 (full-resolution reconstruction) and the coarser levels are separate kernels
 // (the same code), so that profiles tell them apart.
 __global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv0(const DwtiJob *__restrict__ jobs, int njobs) {
-    __shared__ float R[2 * (WEMA_RP - 4)][128];
+    __shared__ float R[INV_ROWS][128];
     dwt_inv_body(R, jobs, njobs);
 }
 __global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv(const DwtiJob *__restrict__ jobs, int njobs) {
-    __shared__ float R[2 * (WEMA_RP - 4)][128];
+    __shared__ float R[INV_ROWS][128];
     dwt_inv_body(R, jobs, njobs);
+}
+
+// Colour mode output (src/wema.c:80-102).
+__device__ __forceinline__ void ycc_px(float yv, float cb, float cr, float (&v)[3]) {
+    const float cbv = cb - 0.5f, crv = cr - 0.5f;
+    v[0] = clamp01(fmaf(1.402f, crv, yv));
+    v[1] = clamp01(yv - fmaf(0.344f, cbv, 0.714f * crv));
+    v[2] = clamp01(fmaf(1.772f, cbv, yv));
+}
+
+// Level-0 synthesis of Y, Cb and Cr of one tile, then YCbCr -> RGB and egress, without the
+// full-resolution planes ever reaching HBM. The three channels are synthesised in turn
+// through one LDS tile; each thread keeps its Y and Cb rows in registers until Cr is done.
+__global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv0_ycc(const DwtiJob *__restrict__ jobs, int njobs) {
+    __shared__ float R[INV_ROWS][128];
+    constexpr int IT = INV_ROWS / 8;   // row pairs per wave
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int jid = find_job(jobs, njobs, bid);
+    const DwtiJob &J = jobs[jid];
+    const int t = bid - J.tile_begin;
+    if (t == 0 && (int)threadIdx.x < J.nch && J.acc_clear) J.acc_clear[threadIdx.x] = 0ull;
+    const int w = J.w, h = J.h;
+    const int px0 = (t % J.tiles_x) * WEMA_TILE_PAIRS_X, py0 = (t / J.tiles_x) * WEMA_TILE_PAIRS_Y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x = 2 * (px0 - 2 + lane);
+    const bool lane_ok = lane >= 2 && lane < 62 && x < w;
+    const bool two = x + 1 < w;
+    float yv[IT][2][2], cbv[IT][2][2];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        if (ch) __syncthreads();   // the previous channel's rows have been read
+        inv_col_pass(R, J.ll + ch * J.ll_cstride, J.planes + ch * J.planes_cstride, w, h, px0, py0);
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int r0 = wave * 2 + 8 * it;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                float ev, ov;
+                inv_row(R, r0 + k, w, lane, ev, ov);
+                if (ch == 0) {
+                    yv[it][k][0] = ev;
+                    yv[it][k][1] = ov;
+                } else if (ch == 1) {
+                    cbv[it][k][0] = ev;
+                    cbv[it][k][1] = ov;
+                } else {
+                    const int y = 2 * py0 + r0 + k;
+                    if (!lane_ok || y >= h) continue;
+                    float v0[3], v1[3];
+                    ycc_px(yv[it][k][0], cbv[it][k][0], ev, v0);
+                    ycc_px(yv[it][k][1], cbv[it][k][1], ov, v1);
+                    const size_t idx = ((size_t)y * w + x) * 3;
+                    if (J.out_u8) {
+                        GLOBAL_AS unsigned char *o = gp(J.out_u8) + idx;
+                        const unsigned b0 = egress_u8(v0[0]), b1 = egress_u8(v0[1]), b2 = egress_u8(v0[2]);
+                        if (two && (idx & 1) == 0) {
+                            GLOBAL_AS unsigned short *os = (GLOBAL_AS unsigned short *)o;
+                            os[0] = (unsigned short)(b0 | (b1 << 8));
+                            os[1] = (unsigned short)(b2 | ((unsigned)egress_u8(v1[0]) << 8));
+                            os[2] = (unsigned short)(egress_u8(v1[1]) | ((unsigned)egress_u8(v1[2]) << 8));
+                        } else {
+                            o[0] = b0; o[1] = b1; o[2] = b2;
+                            if (two) {
+                                o[3] = egress_u8(v1[0]); o[4] = egress_u8(v1[1]); o[5] = egress_u8(v1[2]);
+                            }
+                        }
+                    }
+                    if (J.out_f32) {
+                        GLOBAL_AS float *o = gp(J.out_f32) + idx;
+                        o[0] = v0[0]; o[1] = v0[1]; o[2] = v0[2];
+                        if (two) { o[3] = v1[0]; o[4] = v1[1]; o[5] = v1[2]; }
+                    }
+                }
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1103,68 +1187,6 @@ __global__ void __launch_bounds__(256) k_gauss(const EgressJob *__restrict__ job
     }
 }
 
-// Colour mode output (src/wema.c:80-102), 4 pixels per thread: the three planes are read
-// as 16-byte vectors and the 12 output bytes written as 3 dwords where aligned.
-__device__ __forceinline__ void ycc_px(float yv, float cb, float cr, float (&v)[3]) {
-    const float cbv = cb - 0.5f, crv = cr - 0.5f;
-    v[0] = clamp01(fmaf(1.402f, crv, yv));
-    v[1] = clamp01(yv - fmaf(0.344f, cbv, 0.714f * crv));
-    v[2] = clamp01(fmaf(1.772f, cbv, yv));
-}
-
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-__global__ void __launch_bounds__(256) k_ycc(const EgressJob *__restrict__ jobs, int njobs) {
-    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
-    const EgressJob &J = jobs[jid];
-    const size_t P = (size_t)J.w * J.h;
-    const size_t i0 = ((size_t)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x) * 4;
-    if (i0 >= P) return;
-    const GLOBAL_AS float *Y = gp(J.gray), *CB = gp(J.cb), *CR = gp(J.cr);
-    const bool vin = i0 + 4 <= P &&
-                     ((((uintptr_t)(Y + i0)) | ((uintptr_t)(CB + i0)) | ((uintptr_t)(CR + i0))) & 15u) == 0;
-    if (vin) {
-        const v4f y4 = *(const GLOBAL_AS v4f *)(Y + i0);
-        const v4f b4 = *(const GLOBAL_AS v4f *)(CB + i0);
-        const v4f r4 = *(const GLOBAL_AS v4f *)(CR + i0);
-        float v[4][3];
-        ycc_px(y4.x, b4.x, r4.x, v[0]);
-        ycc_px(y4.y, b4.y, r4.y, v[1]);
-        ycc_px(y4.z, b4.z, r4.z, v[2]);
-        ycc_px(y4.w, b4.w, r4.w, v[3]);
-        if (J.out_u8) {
-            GLOBAL_AS unsigned char *o = gp(J.out_u8) + i0 * 3;
-            if ((((uintptr_t)o) & 3u) == 0) {
-                unsigned b[12];
-#pragma unroll
-                for (int k = 0; k < 12; ++k) b[k] = egress_u8(v[k / 3][k % 3]);
-                GLOBAL_AS unsigned *od = (GLOBAL_AS unsigned *)o;
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                    od[d] = b[4 * d] | (b[4 * d + 1] << 8) | (b[4 * d + 2] << 16) | (b[4 * d + 3] << 24);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 12; ++k) o[k] = egress_u8(v[k / 3][k % 3]);
-            }
-        }
-        if (J.out_f32) {
-            GLOBAL_AS float *o = gp(J.out_f32) + i0 * 3;
-#pragma unroll
-            for (int k = 0; k < 12; ++k) o[k] = v[k / 3][k % 3];
-        }
-        return;
-    }
-    for (size_t i = i0; i < i0 + 4 && i < P; ++i) {
-        float v[3];
-        ycc_px(Y[i], CB[i], CR[i], v);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (J.out_u8) gp(J.out_u8)[i * 3 + k] = egress_u8(v[k]);
-            if (J.out_f32) gp(J.out_f32)[i * 3 + k] = v[k];
-        }
-    }
-}
-
 // Window mode before the window is full: the input frame is returned (src/wema.c:924-929).
 __global__ void __launch_bounds__(256) k_passthru(const EgressJob *__restrict__ jobs, int njobs) {
     const int jid = find_job(jobs, njobs, (int)blockIdx.x);
@@ -1209,6 +1231,11 @@ hipError_t wema_launch_dwt_inv(const DwtiJob *jobs, int njobs, int nblocks, int 
     else hipLaunchKernelGGL(k_dwt_inv, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
     return hipGetLastError();
 }
+hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *jobs, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dwt_inv0_ycc, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
 hipError_t wema_launch_iir(const TemporalJob *jobs, int njobs, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_iir, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
@@ -1226,7 +1253,6 @@ hipError_t wema_launch_egress(int kind, const EgressJob *jobs, int njobs, int nb
     case 0: hipLaunchKernelGGL(k_gf1, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     case 1: hipLaunchKernelGGL(k_gf2, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     case 2: hipLaunchKernelGGL(k_gauss, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
-    case 3: hipLaunchKernelGGL(k_ycc, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     default: hipLaunchKernelGGL(k_passthru, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     }
     return hipGetLastError();

@@ -202,10 +202,11 @@ extern "C" int wema_pool_timing_report(wema_pool_t *p, int kind, char *name, int
     return 0;
 }
 
-// Egress kind a session's next frame uses (part of a tick's launch shape).
+// Output kind a session's next frame uses (part of a tick's launch shape); EG_KINDS stands
+// for the colour output of the inverse stage.
 static int egress_kind(const wema_dev *s) {
     if (!s->p.use_iir && s->frames + 1 < s->p.window) return EG_PASSTHRU;
-    if (s->p.color) return EG_YCC;
+    if (s->p.color) return EG_KINDS;
     return (s->p.edge_aware || s->p.batch_path) ? EG_GF1 : EG_GAUSS;
 }
 
