@@ -444,37 +444,41 @@ float orc_fade_amp(float amp, int global_frame) {
  * Box / guided / Gaussian (src/wema.c:114-283; src/wema_batch.c:26-132)
  *===========================================================================*/
 
+/* Window sums of one line of n samples (stride st), radius r, samples outside [0, n) taken
+ * as 0. prefix: the reference's FP32 prefix sums (src/wema.c:132-148). Otherwise the
+ * order the GPU kernels use: the line is cut into blocks of 4 outputs starting at multiples
+ * of 4; the first sum of a block adds its window left to right from 0, each next one is
+ * (previous - leaving sample) + entering sample. */
+static void box_line_f32(const float *v, size_t st, int n, int r, int prefix, float *pre,
+                         float *out, size_t ost) {
+    if (prefix) {
+        pre[0] = 0.0f;
+        for (int i = 0; i < n; i++) pre[i + 1] = pre[i] + v[(size_t)i * st];
+    }
+    float s = 0.0f;
+    for (int x = 0; x < n; x++) {
+        const int lo = x - r > 0 ? x - r : 0, hi = x + r + 1 < n ? x + r + 1 : n;
+        if (prefix) {
+            s = pre[hi] - pre[lo];
+        } else if (x % 4 == 0) {
+            s = 0.0f;
+            for (int k = lo; k < hi; k++) s += v[(size_t)k * st];
+        } else {
+            const int leave = x - r - 1, enter = x + r;
+            s = s - (leave >= 0 ? v[(size_t)leave * st] : 0.0f);
+            s = s + (enter < n ? v[(size_t)enter * st] : 0.0f);
+        }
+        out[(size_t)x * ost] = s / (float)(hi - lo);
+    }
+}
+
 void orc_box_f32(const float *in, float *out, int w, int h, int r, int prefix) {
     size_t P = (size_t)w * h;
     float *tmp = malloc(P * sizeof(float));
     float *pre = malloc((size_t)((w > h ? w : h) + 1) * sizeof(float));
-    for (int y = 0; y < h; y++) {
-        const float *row = in + (size_t)y * w;
-        if (prefix) {
-            pre[0] = 0.0f;
-            for (int x = 0; x < w; x++) pre[x + 1] = pre[x] + row[x];
-        }
-        for (int x = 0; x < w; x++) {
-            int lo = x - r > 0 ? x - r : 0, hi = x + r + 1 < w ? x + r + 1 : w;
-            float s;
-            if (prefix) s = pre[hi] - pre[lo];
-            else { s = 0.0f; for (int k = lo; k < hi; k++) s += row[k]; }
-            tmp[(size_t)y * w + x] = s / (float)(hi - lo);
-        }
-    }
-    for (int x = 0; x < w; x++) {
-        if (prefix) {
-            pre[0] = 0.0f;
-            for (int y = 0; y < h; y++) pre[y + 1] = pre[y] + tmp[(size_t)y * w + x];
-        }
-        for (int y = 0; y < h; y++) {
-            int lo = y - r > 0 ? y - r : 0, hi = y + r + 1 < h ? y + r + 1 : h;
-            float s;
-            if (prefix) s = pre[hi] - pre[lo];
-            else { s = 0.0f; for (int k = lo; k < hi; k++) s += tmp[(size_t)k * w + x]; }
-            out[(size_t)y * w + x] = s / (float)(hi - lo);
-        }
-    }
+    for (int y = 0; y < h; y++)
+        box_line_f32(in + (size_t)y * w, 1, w, r, prefix, pre, tmp + (size_t)y * w, 1);
+    for (int x = 0; x < w; x++) box_line_f32(tmp + x, (size_t)w, h, r, prefix, pre, out + x, (size_t)w);
     free(tmp);
     free(pre);
 }

@@ -44,6 +44,7 @@ __device__ __forceinline__ int find_job(const Job *jobs, int njobs, int b) {
 // the compiler emit global (not flat) memory instructions, whose completion does not wait on
 // LDS traffic.
 #define GLOBAL_AS __attribute__((address_space(1)))
+typedef float v4f __attribute__((ext_vector_type(4)));
 template <typename T>
 __device__ __forceinline__ GLOBAL_AS T *gp(T *p) {
     return (GLOBAL_AS T *)p;
@@ -802,7 +803,7 @@ __global__ void __launch_bounds__(256) k_win(const TemporalJob *__restrict__ job
 // ---------------------------------------------------------------------------------------
 // Guided filter (src/wema.c:114-230, 289-335; src/wema_batch.c:26-132, 311-346).
 // Box means: horizontal 9-tap sum divided by the clipped count, then vertical likewise
-// (src/wema.c:137-167); sums run left to right / top to bottom over the window.
+// (src/wema.c:137-167); the sums are formed in blocks of 4 outputs (sums4).
 
 __device__ __forceinline__ float src_luma(const EgressJob &J, size_t idx) {
     if (J.src_u8) {
@@ -851,16 +852,17 @@ __device__ __forceinline__ void fill_rcp(float *t) {
 constexpr int GF_EX = WEMA_GF_TX + 8, GF_EY = WEMA_GF_TY + 8;
 constexpr int GF_PITCH = GF_EX + 4;   // multiple of 4: rows start 16-byte aligned
 
-// 9-tap sums of 4 consecutive outputs from 12 samples held in registers, each summed
-// left to right from +0 over its own window (the same order as one output at a time).
+// 9-tap sums of a block of 4 consecutive outputs (starting at a multiple of 4 of the image)
+// from 12 samples held in registers: the first adds its window left to right from +0, each
+// next one is (previous - leaving sample) + entering sample. Samples outside the image are
+// 0 (exact in both steps).
 __device__ __forceinline__ void sums4(const float (&v)[12], float (&out)[4]) {
+    float a = 0.0f;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        float a = 0.0f;
+    for (int k = 0; k < 9; ++k) a += v[k];
+    out[0] = a;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) a += v[j + k];
-        out[j] = a;
-    }
+    for (int j = 1; j < 4; ++j) out[j] = (out[j - 1] - v[j - 1]) + v[j + 8];
 }
 
 // Source pixel of the frame as three floats in [0, 1] (RGB24 or float RGB).
@@ -914,6 +916,60 @@ __device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, flo
     }
 }
 
+// The same, 4 pixels per load: with the width a multiple of 4 and the buffers aligned, the
+// tile's rows are groups of 4 pixels that lie wholly inside or outside the frame; a group's
+// RGB24 is 3 dwords (float RGB: 3 x 16 bytes) and its guide a float4.
+template <bool U8>
+__device__ __forceinline__ void gf1_load_vec(const EgressJob &J, int x0, int y0, float (*sI)[GF_PITCH],
+                                             float (*sP)[GF_PITCH]) {
+    constexpr int GX = GF_EX / 4, NG = GF_EY * GX, NIT = (NG + 255) / 256;
+    const int w = J.w, h = J.h;
+    unsigned raw[NIT][12];
+    v4f g[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int i = threadIdx.x + 256 * it;
+        const int ey = i / GX, gx = i - ey * GX;
+        const int x = min(max(x0 - 4 + 4 * gx, 0), w - 4), y = min(max(y0 - 4 + ey, 0), h - 1);
+        const size_t idx = (size_t)y * w + x;
+        if (U8) {
+            const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(gp(J.src_u8) + idx * 3);
+            raw[it][0] = q[0]; raw[it][1] = q[1]; raw[it][2] = q[2];
+        } else {
+            const GLOBAL_AS v4f *q = (const GLOBAL_AS v4f *)(gp(J.src_f32) + idx * 3);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const v4f f = q[k];
+                raw[it][4 * k] = __float_as_uint(f.x); raw[it][4 * k + 1] = __float_as_uint(f.y);
+                raw[it][4 * k + 2] = __float_as_uint(f.z); raw[it][4 * k + 3] = __float_as_uint(f.w);
+            }
+        }
+        g[it] = *(const GLOBAL_AS v4f *)(gp(J.gray) + idx);
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int i = threadIdx.x + 256 * it;
+        if (i >= NG) break;
+        const int ey = i / GX, gx = i - ey * GX;
+        const int x = x0 - 4 + 4 * gx, y = y0 - 4 + ey;
+        const bool in = x >= 0 && x < w && y >= 0 && y < h;
+        float c[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            c[k] = U8 ? u8f((raw[it][k / 4] >> (8 * (k % 4))) & 0xFFu) : __uint_as_float(raw[it][k]);
+        const float gv[4] = {g[it].x, g[it].y, g[it].z, g[it].w};
+        float I[4], pv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float l = luma_of(c[3 * j], c[3 * j + 1], c[3 * j + 2]);
+            I[j] = in ? l : 0.0f;
+            pv[j] = in ? gv[j] - l : 0.0f;
+        }
+        *(v4f *)&sI[ey][4 * gx] = v4f{I[0], I[1], I[2], I[3]};
+        *(v4f *)&sP[ey][4 * gx] = v4f{pv[0], pv[1], pv[2], pv[3]};
+    }
+}
+
 __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs, int njobs) {
     __shared__ __attribute__((aligned(16))) float sI[GF_EY][GF_PITCH];
     __shared__ __attribute__((aligned(16))) float sP[GF_EY][GF_PITCH];
@@ -927,8 +983,15 @@ __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs,
     const int t = bid - J.tile_begin;
     const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
     const int w = J.w, h = J.h;
-    if (J.src_u8) gf1_load<true>(J, x0, y0, sI, sP);
-    else gf1_load<false>(J, x0, y0, sI, sP);
+    const uintptr_t al = (uintptr_t)J.gray | (J.src_u8 ? 0 : (uintptr_t)J.src_f32);
+    const bool vec = (w & 3) == 0 && (al & 15) == 0 && (!J.src_u8 || ((uintptr_t)J.src_u8 & 3) == 0);
+    if (vec) {
+        if (J.src_u8) gf1_load_vec<true>(J, x0, y0, sI, sP);
+        else gf1_load_vec<false>(J, x0, y0, sI, sP);
+    } else {
+        if (J.src_u8) gf1_load<true>(J, x0, y0, sI, sP);
+        else gf1_load<false>(J, x0, y0, sI, sP);
+    }
     __syncthreads();
     // horizontal: 4 outputs per item, taps read as three 16-byte vectors per quantity
     for (int i = threadIdx.x; i < GF_EY * (WEMA_GF_TX / 4); i += 256) {
@@ -1042,26 +1105,53 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
     for (int j = 0; j < 4; ++j)
         src[j].load(J, (size_t)min(y0 + y4 + j, h - 1) * w + min(gx, w - 1));
     const GLOBAL_AS float2 *ab = gp(J.ab);
-    constexpr int NLD = (GF_EY * GF_EX + 255) / 256;
-    float2 lab[NLD];
+    if ((w & 3) == 0 && ((uintptr_t)J.ab & 15) == 0) {
+        // groups of 4 pixels wholly inside or outside the frame, two 16-byte loads each
+        constexpr int GX = GF_EX / 4, NG = GF_EY * GX, NIT = (NG + 255) / 256;
+        v4f q[NIT][2];
 #pragma unroll
-    for (int j = 0; j < NLD; ++j) {
-        const int i = threadIdx.x + 256 * j;
-        const int ey = i / GF_EX, ex = i - ey * GF_EX;
-        const int x = min(max(x0 - 4 + ex, 0), w - 1), y = min(max(y0 - 4 + ey, 0), h - 1);
-        const GLOBAL_AS float *q = (const GLOBAL_AS float *)(ab + (size_t)y * w + x);
-        lab[j].x = q[0];
-        lab[j].y = q[1];
-    }
+        for (int it = 0; it < NIT; ++it) {
+            const int i = threadIdx.x + 256 * it;
+            const int ey = i / GX, gx = i - ey * GX;
+            const int x = min(max(x0 - 4 + 4 * gx, 0), w - 4), y = min(max(y0 - 4 + ey, 0), h - 1);
+            const GLOBAL_AS v4f *p = (const GLOBAL_AS v4f *)(ab + (size_t)y * w + x);
+            q[it][0] = p[0];
+            q[it][1] = p[1];
+        }
 #pragma unroll
-    for (int j = 0; j < NLD; ++j) {
-        const int i = threadIdx.x + 256 * j;
-        if (i < GF_EY * GF_EX) {
-            const int ey = i / GF_EX, ex = i - ey * GF_EX;
-            const int x = x0 - 4 + ex, y = y0 - 4 + ey;
+        for (int it = 0; it < NIT; ++it) {
+            const int i = threadIdx.x + 256 * it;
+            if (i >= NG) break;
+            const int ey = i / GX, gx = i - ey * GX;
+            const int x = x0 - 4 + 4 * gx, y = y0 - 4 + ey;
             const bool in = x >= 0 && x < w && y >= 0 && y < h;
-            sA[ey][ex] = in ? lab[j].x : 0.0f;
-            sB[ey][ex] = in ? lab[j].y : 0.0f;
+            const v4f a = {q[it][0].x, q[it][0].z, q[it][1].x, q[it][1].z};
+            const v4f b = {q[it][0].y, q[it][0].w, q[it][1].y, q[it][1].w};
+            *(v4f *)&sA[ey][4 * gx] = in ? a : v4f{0.0f, 0.0f, 0.0f, 0.0f};
+            *(v4f *)&sB[ey][4 * gx] = in ? b : v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    } else {
+        constexpr int NLD = (GF_EY * GF_EX + 255) / 256;
+        float2 lab[NLD];
+#pragma unroll
+        for (int j = 0; j < NLD; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            const int ey = i / GF_EX, ex = i - ey * GF_EX;
+            const int x = min(max(x0 - 4 + ex, 0), w - 1), y = min(max(y0 - 4 + ey, 0), h - 1);
+            const GLOBAL_AS float *q = (const GLOBAL_AS float *)(ab + (size_t)y * w + x);
+            lab[j].x = q[0];
+            lab[j].y = q[1];
+        }
+#pragma unroll
+        for (int j = 0; j < NLD; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            if (i < GF_EY * GF_EX) {
+                const int ey = i / GF_EX, ex = i - ey * GF_EX;
+                const int x = x0 - 4 + ex, y = y0 - 4 + ey;
+                const bool in = x >= 0 && x < w && y >= 0 && y < h;
+                sA[ey][ex] = in ? lab[j].x : 0.0f;
+                sB[ey][ex] = in ? lab[j].y : 0.0f;
+            }
         }
     }
     __syncthreads();
