@@ -98,10 +98,11 @@ def run_gpu(cfg, name, steps, warmup, batch):
     ms = e0.elapsed_time(e1)
     kernels = sess.timing_report()
     sess.timing(False)
+    tile_rows = sess.tile_rows
     frames_timed = steps * batch
     mpix = frames_timed * w * h / 1e6
     res = dict(ms=ms, wall_ms=t_wall * 1e3, frames=frames_timed, mpix_s=mpix / (ms / 1e3),
-               fps=frames_timed / (ms / 1e3), kernels=kernels)
+               fps=frames_timed / (ms / 1e3), kernels=kernels, tile_rows=tile_rows)
     dom = max(kernels, key=lambda k: k["ms"])
     avg_ms = dom["ms"] / dom["launches"]
     bytes_per_launch = dom["bytes"] / dom["launches"]
@@ -289,7 +290,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (deterministic generator, frames resident in HBM)",
         "config": {"workload": args.config, "desc": cfg["desc"], "width": cfg["w"],
-                   "height": cfg["h"], "frames_per_step": batch, "levels": cfg["levels"]},
+                   "height": cfg["h"], "frames_per_step": batch, "levels": cfg["levels"],
+                   "gf_tile_rows": r["tile_rows"] if not cfg["color"] else None},
         "fps": round(r["fps"], 1),
         "pipeline_algorithmic_GBs": r["pipeline_gbs"],
         "roofline": r["roofline"],
@@ -306,6 +308,7 @@ def main():
         also[name] = {"desc": c["desc"], "value": round(rr["mpix_s"], 1), "unit": "Mpixel/s",
                       "fps": round(rr["fps"], 1), "ms_per_step": round(rr["ms"] / args.steps, 3),
                       "frames_per_step": c["batch"], "pipeline_algorithmic_GBs": rr["pipeline_gbs"],
+                      "gf_tile_rows": rr["tile_rows"] if not c["color"] else None,
                       "roofline": rr["roofline"],
                       "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
                                   for d in rr["kernels"]]}

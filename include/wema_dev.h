@@ -97,6 +97,16 @@ int wema_dev_process_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out)
 int wema_dev_process_host_f32(wema_dev_t *s, const float *const *h_in, float *const *h_out, int n);
 int wema_dev_sync(wema_dev_t *s);
 
+/* Tile shapes (no counterpart in the reference). The guided-filter kernels come in tile
+ * heights of 8, 16 and 32 rows (64 columns). A session whose output stage is the guided
+ * filter takes the height an autotuner picked for its frame size: the first session of a
+ * size on a device times the three on frames of that size and keeps the fastest for the
+ * process. Results never depend on the tile shape. wema_dev_autotune(0) turns the timing
+ * off (16 rows); wema_dev_set_tile_rows forces a height for the session's next calls. */
+void wema_dev_autotune(int enable);
+int  wema_dev_tile_rows(const wema_dev_t *s);
+int  wema_dev_set_tile_rows(wema_dev_t *s, int rows);
+
 /* Pinned host memory for frame slots (hipHostMalloc). */
 void *wema_dev_host_alloc(size_t bytes);
 void  wema_dev_host_free(void *p);

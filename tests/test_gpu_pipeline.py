@@ -126,3 +126,28 @@ def test_vs_reference_with_its_thresholds(color):
         d = np.abs(d_out[t].cpu().numpy().astype(int) - r_out.astype(int))
         worst = max(worst, int(d.max()))
     assert worst <= 1, worst
+
+
+@pytest.mark.parametrize("w,h", [(320, 240), (50, 33)])
+def test_tile_rows_do_not_change_results(w, h):
+    """The guided filter's tile height (8, 16, 32 rows; picked by the autotuner) never
+    changes the output bytes, at widths that take the 4-pixel and the 1-pixel load paths."""
+    _, frames = clip(4, w, h, 30.0, 6, seed=4)
+    kw = dict(fps=30.0, amp=100.0, f_low=0.5, f_high=3.0, levels=3)
+    outs = []
+    for rows in (8, 16, 32):
+        sess = wema_amd.Session(w, h, **kw)
+        assert sess.tile_rows in (8, 16, 32)
+        sess.set_tile_rows(rows)
+        d_in = torch.from_numpy(frames).cuda()
+        d_out = torch.empty_like(d_in)
+        for t in range(frames.shape[0]):
+            sess.process_device(d_in[t:t + 1], d_out_u8=d_out[t:t + 1], n=1)
+        sess.sync()
+        outs.append(d_out.cpu().numpy())
+        sess.close()
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+    orc = O.OraclePipe(w, h, **kw)
+    for t in range(frames.shape[0]):
+        o_out, _ = orc.frame(frames[t])
+        assert np.array_equal(outs[1][t], o_out), t

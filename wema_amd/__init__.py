@@ -96,6 +96,9 @@ _SIGS = {
     "wema_dev_geometry": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "wema_dev_iir_coeffs": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     "wema_dev_frames": (C.c_long, [_vp]),
+    "wema_dev_tile_rows": (C.c_int, [_vp]),
+    "wema_dev_set_tile_rows": (C.c_int, [_vp, C.c_int]),
+    "wema_dev_autotune": (None, [C.c_int]),
     "wema_dev_process_u8": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "wema_dev_process_f32": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "wema_dev_process_host_u8": (C.c_int, [_vp, _vp, _vp]),
@@ -154,6 +157,11 @@ def _ptr(a):
 def _stream_handle():
     import torch
     return torch.cuda.current_stream().cuda_stream
+
+
+def autotune(enable):
+    """Turns the guided-filter tile-shape autotuner on or off for sessions opened later."""
+    lib.wema_dev_autotune(int(bool(enable)))
 
 
 def design_bandpass(f_low, f_high, fs):
@@ -226,6 +234,14 @@ class Session:
     @property
     def frames(self):
         return lib.wema_dev_frames(self._h)
+
+    @property
+    def tile_rows(self):
+        """Guided-filter tile height the session uses (picked by the autotuner)."""
+        return lib.wema_dev_tile_rows(self._h)
+
+    def set_tile_rows(self, rows):
+        _check(lib.wema_dev_set_tile_rows(self._h, int(rows)), "wema_dev_set_tile_rows")
 
     def iir_coeffs(self):
         c = (C.c_float * 10)()

@@ -14,7 +14,8 @@ LIB = os.path.join(HERE, "libwema_dev.so")
 HOST_LIB = os.path.join(HERE, "libwema.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["wema_kernels.hip", "wema_dev.hip", "wema_synth.hip", "wema_pool.hip", "wema_serve.hip"]
+SOURCES = ["wema_kernels.hip", "wema_dev.hip", "wema_synth.hip", "wema_pool.hip", "wema_serve.hip",
+           "wema_tune.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -23,7 +23,7 @@ hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, int, hipStream_t);
 hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *, int, int, hipStream_t);
 hipError_t wema_launch_iir(const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_win(const TemporalJob *, int, int, hipStream_t);
-hipError_t wema_launch_egress(int, const EgressJob *, int, int, hipStream_t);
+hipError_t wema_launch_egress(int, int, const EgressJob *, int, int, hipStream_t);
 hipError_t wema_launch_synth(const SynthFrame *, const unsigned short *, unsigned char *, int,
                              int, int, hipStream_t);
 }
@@ -260,6 +260,7 @@ int wema_dev_open_ex(wema_dev_t **out, const wema_dev_params *p, bool pooled) {
         wema_dev_close(s);
         return -1;
     }
+    if (!p->color && (p->edge_aware || p->batch_path)) s->gf_rows = wema_gf_rows_for(p->device, p->width, p->height);
     *out = s;
     return 0;
 }
@@ -516,7 +517,8 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
 
     // output stage
     const int gf_tx = wema_ceil_div(g.W, WEMA_GF_TX);
-    const int gf_tiles = gf_tx * wema_ceil_div(g.H, WEMA_GF_TY);
+    const int gf_tiles = gf_tx * wema_ceil_div(g.H, s->gf_rows);      // guided filter
+    const int gs_tiles = gf_tx * wema_ceil_div(g.H, WEMA_GF_TY);      // 5x5 path
     const int byte_tiles = (int)((3 * P + 255) / 256);
     for (int f = 0; f < n; f++) {
         EgressJob j{};
@@ -537,10 +539,12 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         } else if (p.color) {
             continue;   // written by the colour output job of the inverse stage
         } else if (p.edge_aware || p.batch_path) {
-            jb.add_egress(EG_GF1, j, gf_tiles, 4.0 * P + src_b + 8.0 * P);
-            jb.add_egress(EG_GF2, j, gf_tiles, 8.0 * P + src_b + out_b);
+            const int ri = wema_rows_index(s->gf_rows);
+            jb.add_egress(EG_GF1 + ri, j, gf_tiles, 4.0 * P + src_b + 8.0 * P);
+            jb.add_egress(EG_GF2 + ri, j, gf_tiles, 8.0 * P + src_b + out_b);
         } else {
-            jb.add_egress(EG_GAUSS, j, gf_tiles, 4.0 * P + src_b + out_b);
+            j.tiles_x = gf_tx;
+            jb.add_egress(EG_GAUSS, j, gs_tiles, 4.0 * P + src_b + out_b);
         }
     }
 }
@@ -622,11 +626,13 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
         HIP_OK(wema_launch_dwt_inv_ycc((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, st));
         if (timer_mark(timer, TK_DWT_INV0_YCC, jl.bytes, st) < 0) return -1;
     }
-    static const int eg_kind[EG_KINDS] = {TK_GF1, TK_GF2, TK_GAUSS, TK_PASSTHRU};
+    static const int eg_kind[EG_KINDS] = {TK_GF1, TK_GF1, TK_GF1, TK_GF2, TK_GF2, TK_GF2, TK_GAUSS, TK_PASSTHRU};
     for (int k = 0; k < EG_KINDS; k++) {
         const JobList &jl = jb.egress[k];
         if (!jl.count) continue;
-        HIP_OK(wema_launch_egress(k, (const EgressJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        const int kind = k < EG_GAUSS ? (k < EG_GF2 ? EG_GF1 : EG_GF2) : k;
+        HIP_OK(wema_launch_egress(kind, wema_index_rows((k - kind) % 3), (const EgressJob *)(base + jl.offset),
+                                  jl.count, jl.blocks, st));
         if (timer_mark(timer, eg_kind[k], jl.bytes, st) < 0) return -1;
     }
     return 0;

@@ -76,6 +76,13 @@ struct DwtiJob {
 
 #define WEMA_MAX_BATCH 256
 
+// Output stages after the inverse DWT (colour mode has none: its inverse writes RGB). The
+// guided-filter passes have one job list per tile height (8, 16, 32 rows), all first-pass
+// lists ahead of all second-pass ones.
+enum EgressKind { EG_GF1 = 0, EG_GF2 = 3, EG_GAUSS = 6, EG_PASSTHRU = 7, EG_KINDS = 8 };
+__host__ __device__ static inline int wema_rows_index(int rows) { return rows == 8 ? 0 : (rows == 32 ? 2 : 1); }
+__host__ __device__ static inline int wema_index_rows(int i) { return i == 0 ? 8 : (i == 2 ? 32 : 16); }
+
 // Temporal stage (IIR or sliding window) over one (stream, channel) and its frames.
 struct TemporalJob {
     float *coef;                    // [nframes][coef_stride] coefficients (IIR: in place)

@@ -849,13 +849,15 @@ __device__ __forceinline__ void fill_rcp(float *t) {
     if (threadIdx.x < 16) t[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 0.0f;
 }
 
-constexpr int GF_EX = WEMA_GF_TX + 8, GF_EY = WEMA_GF_TY + 8;
-constexpr int GF_PITCH = GF_EX + 4;   // multiple of 4: rows start 16-byte aligned
+constexpr int GF_TX = WEMA_GF_TX;        // output columns of a tile
+constexpr int GF_EX = GF_TX + 8;         // with the 4-column halo on each side
+constexpr int GF_PITCH = GF_EX + 4;      // multiple of 4: rows start 16-byte aligned
 
 // 9-tap sums of a block of 4 consecutive outputs (starting at a multiple of 4 of the image)
 // from 12 samples held in registers: the first adds its window left to right from +0, each
 // next one is (previous - leaving sample) + entering sample. Samples outside the image are
-// 0 (exact in both steps).
+// 0 (exact in both steps). The blocks are anchored to the image, so results do not depend on
+// the tile shape.
 __device__ __forceinline__ void sums4(const float (&v)[12], float (&out)[4]) {
     float a = 0.0f;
 #pragma unroll
@@ -882,13 +884,13 @@ struct SrcPix {
     __device__ __forceinline__ float luma() const { return luma_of(ch(r), ch(g), ch(b)); }
 };
 
-// Guide (source luma) and input (reconstruction - guide) of the extended tile into LDS,
-// zero outside the frame. Coordinates are clamped into the frame so that every load is
-// valid and all of them are issued before any is used (one memory round trip per block).
-template <bool U8>
+// Guide (source luma) and input (reconstruction - guide) of the extended tile (TY + 8 rows)
+// into LDS, zero outside the frame. Coordinates are clamped into the frame so that every
+// load is valid and all of them are issued before any is used (one memory round trip).
+template <bool U8, int TY>
 __device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, float (*sI)[GF_PITCH],
                                          float (*sP)[GF_PITCH]) {
-    constexpr int NLD = (GF_EY * GF_EX + 255) / 256;
+    constexpr int EY = TY + 8, NLD = (EY * GF_EX + 255) / 256;
     const int w = J.w, h = J.h;
     const GLOBAL_AS float *gray = gp(J.gray);
     SrcPix<U8> px[NLD];
@@ -905,7 +907,7 @@ __device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, flo
 #pragma unroll
     for (int j = 0; j < NLD; ++j) {
         const int i = threadIdx.x + 256 * j;
-        if (i < GF_EY * GF_EX) {
+        if (i < EY * GF_EX) {
             const int ey = i / GF_EX, ex = i - ey * GF_EX;
             const int x = x0 - 4 + ex, y = y0 - 4 + ey;
             const bool in = x >= 0 && x < w && y >= 0 && y < h;
@@ -919,10 +921,10 @@ __device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, flo
 // The same, 4 pixels per load: with the width a multiple of 4 and the buffers aligned, the
 // tile's rows are groups of 4 pixels that lie wholly inside or outside the frame; a group's
 // RGB24 is 3 dwords (float RGB: 3 x 16 bytes) and its guide a float4.
-template <bool U8>
+template <bool U8, int TY>
 __device__ __forceinline__ void gf1_load_vec(const EgressJob &J, int x0, int y0, float (*sI)[GF_PITCH],
                                              float (*sP)[GF_PITCH]) {
-    constexpr int GX = GF_EX / 4, NG = GF_EY * GX, NIT = (NG + 255) / 256;
+    constexpr int GX = GF_EX / 4, NG = (TY + 8) * GX, NIT = (NG + 255) / 256;
     const int w = J.w, h = J.h;
     unsigned raw[NIT][12];
     v4f g[NIT];
@@ -970,32 +972,42 @@ __device__ __forceinline__ void gf1_load_vec(const EgressJob &J, int x0, int y0,
     }
 }
 
-__global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs, int njobs) {
-    __shared__ __attribute__((aligned(16))) float sI[GF_EY][GF_PITCH];
-    __shared__ __attribute__((aligned(16))) float sP[GF_EY][GF_PITCH];
-    __shared__ float hI[GF_EY][WEMA_GF_TX], hP[GF_EY][WEMA_GF_TX];
-    __shared__ float hII[GF_EY][WEMA_GF_TX], hIP[GF_EY][WEMA_GF_TX];
-    __shared__ float sRcp[16];
-    fill_rcp(sRcp);
+// Tile of a guided-filter job: GF_TX x TY outputs.
+template <int TY>
+__device__ __forceinline__ const EgressJob &gf_tile(const EgressJob *__restrict__ jobs, int njobs, int &x0,
+                                                  int &y0) {
     const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int jid = find_job(jobs, njobs, bid);
     const EgressJob &J = jobs[jid];
     const int t = bid - J.tile_begin;
-    const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
+    x0 = (t % J.tiles_x) * GF_TX;
+    y0 = (t / J.tiles_x) * TY;
+    return J;
+}
+
+// First pass: box means of I, p, I^2, I p over the tile, then a and b (src/wema.c:173-219).
+template <int TY>
+__device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int njobs, float (*sI)[GF_PITCH],
+                                        float (*sP)[GF_PITCH], float (*hI)[GF_TX], float (*hP)[GF_TX],
+                                        float (*hII)[GF_TX], float (*hIP)[GF_TX], float *sRcp) {
+    constexpr int EY = TY + 8;
+    fill_rcp(sRcp);
+    int x0, y0;
+    const EgressJob &J = gf_tile<TY>(jobs, njobs, x0, y0);
     const int w = J.w, h = J.h;
     const uintptr_t al = (uintptr_t)J.gray | (J.src_u8 ? 0 : (uintptr_t)J.src_f32);
     const bool vec = (w & 3) == 0 && (al & 15) == 0 && (!J.src_u8 || ((uintptr_t)J.src_u8 & 3) == 0);
     if (vec) {
-        if (J.src_u8) gf1_load_vec<true>(J, x0, y0, sI, sP);
-        else gf1_load_vec<false>(J, x0, y0, sI, sP);
+        if (J.src_u8) gf1_load_vec<true, TY>(J, x0, y0, sI, sP);
+        else gf1_load_vec<false, TY>(J, x0, y0, sI, sP);
     } else {
-        if (J.src_u8) gf1_load<true>(J, x0, y0, sI, sP);
-        else gf1_load<false>(J, x0, y0, sI, sP);
+        if (J.src_u8) gf1_load<true, TY>(J, x0, y0, sI, sP);
+        else gf1_load<false, TY>(J, x0, y0, sI, sP);
     }
     __syncthreads();
     // horizontal: 4 outputs per item, taps read as three 16-byte vectors per quantity
-    for (int i = threadIdx.x; i < GF_EY * (WEMA_GF_TX / 4); i += 256) {
-        const int ey = i / (WEMA_GF_TX / 4), x4 = 4 * (i % (WEMA_GF_TX / 4));
+    for (int i = threadIdx.x; i < EY * (GF_TX / 4); i += 256) {
+        const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
         float vi[12], vp[12], vii[12], vip[12];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -1026,8 +1038,8 @@ __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs,
     }
     __syncthreads();
     // vertical: 4 consecutive rows of one column per item
-    for (int i = threadIdx.x; i < (WEMA_GF_TY / 4) * WEMA_GF_TX; i += 256) {
-        const int y4 = 4 * (i / WEMA_GF_TX), x = i % WEMA_GF_TX;
+    for (int i = threadIdx.x; i < (TY / 4) * GF_TX; i += 256) {
+        const int y4 = 4 * (i / GF_TX), x = i % GF_TX;
         const int gx = x0 + x;
         if (gx >= w) continue;
         float vi[12], vp[12], vii[12], vip[12];
@@ -1061,23 +1073,6 @@ __global__ void __launch_bounds__(256) k_gf1(const EgressJob *__restrict__ jobs,
     }
 }
 
-__device__ __forceinline__ void write_rgb(const EgressJob &J, size_t idx, float q) {
-    float s[3];
-    if (J.src_u8) {
-        const unsigned char *p = J.src_u8 + idx * 3;
-        s[0] = u8f(p[0]); s[1] = u8f(p[1]); s[2] = u8f(p[2]);
-    } else {
-        const float *p = J.src_f32 + idx * 3;
-        s[0] = p[0]; s[1] = p[1]; s[2] = p[2];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float v = clamp01(s[k] + q);
-        if (J.out_u8) J.out_u8[idx * 3 + k] = egress_u8(v);
-        if (J.out_f32) J.out_f32[idx * 3 + k] = v;
-    }
-}
-
 // RGB out = clamp(source + q) (src/wema.c:329-334).
 template <bool U8>
 __device__ __forceinline__ void write_rgb_px(const EgressJob &J, size_t idx, const SrcPix<U8> &sp, float q) {
@@ -1090,24 +1085,29 @@ __device__ __forceinline__ void write_rgb_px(const EgressJob &J, size_t idx, con
     }
 }
 
-template <bool U8>
+// Second pass: box means of a and b, q = mean(a) I + mean(b), RGB out (src/wema.c:221-229,
+// 289-335).
+template <bool U8, int TY>
 __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, float (*sA)[GF_PITCH],
-                                         float (*sB)[GF_PITCH], float (*hA)[WEMA_GF_TX],
-                                         float (*hB)[WEMA_GF_TX], const float *sRcp) {
-    static_assert((WEMA_GF_TY / 4) * WEMA_GF_TX == 256, "one vertical item per thread");
+                                         float (*sB)[GF_PITCH], float (*hA)[GF_TX], float (*hB)[GF_TX],
+                                         const float *sRcp) {
+    constexpr int EY = TY + 8, NV = (TY / 4) * GF_TX, VI = (NV + 255) / 256;
     const int w = J.w, h = J.h;
-    // the source pixels of this thread's outputs (one column, 4 rows) and the a, b tile,
+    // the source pixels of this thread's outputs (VI columns of 4 rows) and the a, b tile,
     // all loaded first from coordinates clamped into the frame
-    const int y4 = 4 * ((int)threadIdx.x / WEMA_GF_TX), xo = (int)threadIdx.x % WEMA_GF_TX;
-    const int gx = x0 + xo;
-    SrcPix<U8> src[4];
+    SrcPix<U8> src[VI][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-        src[j].load(J, (size_t)min(y0 + y4 + j, h - 1) * w + min(gx, w - 1));
+    for (int v = 0; v < VI; ++v) {
+        const int i = threadIdx.x + 256 * v;
+        const int y4 = 4 * (i / GF_TX), gx = x0 + i % GF_TX;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            src[v][j].load(J, (size_t)min(y0 + y4 + j, h - 1) * w + min(gx, w - 1));
+    }
     const GLOBAL_AS float2 *ab = gp(J.ab);
     if ((w & 3) == 0 && ((uintptr_t)J.ab & 15) == 0) {
         // groups of 4 pixels wholly inside or outside the frame, two 16-byte loads each
-        constexpr int GX = GF_EX / 4, NG = GF_EY * GX, NIT = (NG + 255) / 256;
+        constexpr int GX = GF_EX / 4, NG = EY * GX, NIT = (NG + 255) / 256;
         v4f q[NIT][2];
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
@@ -1131,7 +1131,7 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
             *(v4f *)&sB[ey][4 * gx] = in ? b : v4f{0.0f, 0.0f, 0.0f, 0.0f};
         }
     } else {
-        constexpr int NLD = (GF_EY * GF_EX + 255) / 256;
+        constexpr int NLD = (EY * GF_EX + 255) / 256;
         float2 lab[NLD];
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
@@ -1145,7 +1145,7 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
             const int i = threadIdx.x + 256 * j;
-            if (i < GF_EY * GF_EX) {
+            if (i < EY * GF_EX) {
                 const int ey = i / GF_EX, ex = i - ey * GF_EX;
                 const int x = x0 - 4 + ex, y = y0 - 4 + ey;
                 const bool in = x >= 0 && x < w && y >= 0 && y < h;
@@ -1155,8 +1155,8 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < GF_EY * (WEMA_GF_TX / 4); i += 256) {
-        const int ey = i / (WEMA_GF_TX / 4), x4 = 4 * (i % (WEMA_GF_TX / 4));
+    for (int i = threadIdx.x; i < EY * (GF_TX / 4); i += 256) {
+        const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
         float va[12], vb[12];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -1177,40 +1177,81 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
         }
     }
     __syncthreads();
-    if (gx >= w) return;
-    float va[12], vb[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-        va[k] = hA[y4 + k][xo];
-        vb[k] = hB[y4 + k][xo];
-    }
-    float sa[4], sb[4];
-    sums4(va, sa);
-    sums4(vb, sb);
+    for (int v = 0; v < VI; ++v) {
+        const int i = threadIdx.x + 256 * v;
+        const int y4 = 4 * (i / GF_TX), xo = i % GF_TX;
+        const int gx = x0 + xo;
+        if (i >= NV || gx >= w) break;
+        float va[12], vb[12];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int gy = y0 + y4 + j;
-        if (gy >= h) break;
-        const int n = clip_count(gy, h);
-        const float cnt = (float)n, k = sRcp[n];
-        const float q = fmaf(div_count(sa[j], cnt, k), src[j].luma(), div_count(sb[j], cnt, k));
-        write_rgb_px<U8>(J, (size_t)gy * w + gx, src[j], q);
+        for (int k = 0; k < 12; ++k) {
+            va[k] = hA[y4 + k][xo];
+            vb[k] = hB[y4 + k][xo];
+        }
+        float sa[4], sb[4];
+        sums4(va, sa);
+        sums4(vb, sb);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gy = y0 + y4 + j;
+            if (gy >= h) break;
+            const int n = clip_count(gy, h);
+            const float cnt = (float)n, k = sRcp[n];
+            const float q = fmaf(div_count(sa[j], cnt, k), src[v][j].luma(), div_count(sb[j], cnt, k));
+            write_rgb_px<U8>(J, (size_t)gy * w + gx, src[v][j], q);
+        }
     }
 }
 
-__global__ void __launch_bounds__(256) k_gf2(const EgressJob *__restrict__ jobs, int njobs) {
-    __shared__ __attribute__((aligned(16))) float sA[GF_EY][GF_PITCH];
-    __shared__ __attribute__((aligned(16))) float sB[GF_EY][GF_PITCH];
-    __shared__ float hA[GF_EY][WEMA_GF_TX], hB[GF_EY][WEMA_GF_TX];
-    __shared__ float sRcp[16];
+template <int TY>
+__device__ __forceinline__ void gf2_run(const EgressJob *__restrict__ jobs, int njobs, float (*sA)[GF_PITCH],
+                                        float (*sB)[GF_PITCH], float (*hA)[GF_TX], float (*hB)[GF_TX],
+                                        float *sRcp) {
     fill_rcp(sRcp);
-    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int jid = find_job(jobs, njobs, bid);
-    const EgressJob &J = jobs[jid];
-    const int t = bid - J.tile_begin;
-    const int x0 = (t % J.tiles_x) * WEMA_GF_TX, y0 = (t / J.tiles_x) * WEMA_GF_TY;
-    if (J.src_u8) gf2_body<true>(J, x0, y0, sA, sB, hA, hB, sRcp);
-    else gf2_body<false>(J, x0, y0, sA, sB, hA, hB, sRcp);
+    int x0, y0;
+    const EgressJob &J = gf_tile<TY>(jobs, njobs, x0, y0);
+    if (J.src_u8) gf2_body<true, TY>(J, x0, y0, sA, sB, hA, hB, sRcp);
+    else gf2_body<false, TY>(J, x0, y0, sA, sB, hA, hB, sRcp);
+}
+
+// One kernel per tile height (GF_TX x TY outputs per block); the session's tuner picks one.
+#define WEMA_GF_KERNELS(TY)                                                                    \
+    __global__ void __launch_bounds__(256) k_gf1_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
+        __shared__ __attribute__((aligned(16))) float sI[TY + 8][GF_PITCH];                    \
+        __shared__ __attribute__((aligned(16))) float sP[TY + 8][GF_PITCH];                    \
+        __shared__ float hI[TY + 8][GF_TX], hP[TY + 8][GF_TX];                                 \
+        __shared__ float hII[TY + 8][GF_TX], hIP[TY + 8][GF_TX];                               \
+        __shared__ float sRcp[16];                                                             \
+        gf1_run<TY>(jobs, njobs, sI, sP, hI, hP, hII, hIP, sRcp);                              \
+    }                                                                                          \
+    __global__ void __launch_bounds__(256) k_gf2_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
+        __shared__ __attribute__((aligned(16))) float sA[TY + 8][GF_PITCH];                    \
+        __shared__ __attribute__((aligned(16))) float sB[TY + 8][GF_PITCH];                    \
+        __shared__ float hA[TY + 8][GF_TX], hB[TY + 8][GF_TX];                                 \
+        __shared__ float sRcp[16];                                                             \
+        gf2_run<TY>(jobs, njobs, sA, sB, hA, hB, sRcp);                                        \
+    }
+WEMA_GF_KERNELS(8)
+WEMA_GF_KERNELS(16)
+WEMA_GF_KERNELS(32)
+
+// RGB out = clamp(source + q) for the 5x5 path (src/wema.c:329-334).
+__device__ __forceinline__ void write_rgb(const EgressJob &J, size_t idx, float q) {
+    float s[3];
+    if (J.src_u8) {
+        const GLOBAL_AS unsigned char *p = gp(J.src_u8) + idx * 3;
+        s[0] = u8f(p[0]); s[1] = u8f(p[1]); s[2] = u8f(p[2]);
+    } else {
+        const GLOBAL_AS float *p = gp(J.src_f32) + idx * 3;
+        s[0] = p[0]; s[1] = p[1]; s[2] = p[2];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float v = clamp01(s[k] + q);
+        if (J.out_u8) gp(J.out_u8)[idx * 3 + k] = egress_u8(v);
+        if (J.out_f32) gp(J.out_f32)[idx * 3 + k] = v;
+    }
 }
 
 // 5x5 binomial blur of the luma delta with weights renormalised at the borders
@@ -1336,13 +1377,21 @@ hipError_t wema_launch_win(const TemporalJob *jobs, int njobs, int nblocks, hipS
     hipLaunchKernelGGL(k_win, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError();
 }
-hipError_t wema_launch_egress(int kind, const EgressJob *jobs, int njobs, int nblocks,
+hipError_t wema_launch_egress(int kind, int rows, const EgressJob *jobs, int njobs, int nblocks,
                               hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     switch (kind) {
-    case 0: hipLaunchKernelGGL(k_gf1, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
-    case 1: hipLaunchKernelGGL(k_gf2, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
-    case 2: hipLaunchKernelGGL(k_gauss, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
+    case EG_GF1:
+        if (rows == 8) hipLaunchKernelGGL(k_gf1_r8, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+        else if (rows == 32) hipLaunchKernelGGL(k_gf1_r32, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+        else hipLaunchKernelGGL(k_gf1_r16, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+        break;
+    case EG_GF2:
+        if (rows == 8) hipLaunchKernelGGL(k_gf2_r8, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+        else if (rows == 32) hipLaunchKernelGGL(k_gf2_r32, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+        else hipLaunchKernelGGL(k_gf2_r16, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+        break;
+    case EG_GAUSS: hipLaunchKernelGGL(k_gauss, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     default: hipLaunchKernelGGL(k_passthru, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     }
     return hipGetLastError();

@@ -17,6 +17,7 @@ import csv
 import glob
 import json
 import os
+import re
 
 GIB = float(1 << 30)
 
@@ -29,7 +30,12 @@ def _one(pattern):
 
 
 def _short(name):
-    return name.split("(")[0]
+    """Kernel name without its argument list; the guided-filter tile variants (k_gf1_r16 ...)
+    are reported under the pass name bench.py times them by (k_gf1, k_gf2)."""
+    n = name.split("(")[0]
+    if n.startswith("void "):
+        n = n[5:]
+    return re.sub(r"^(k_gf[12])_r\d+$", r"\1", n)
 
 
 def kernel_stats(trace_dir):
