@@ -66,7 +66,7 @@ def read_traffic(kernel, workload):
         return None
 
 
-def run_gpu(cfg, name, steps, warmup, batch):
+def run_gpu(cfg, name, steps, warmup, batch, tile_rows=0):
     import torch
     import wema_amd
 
@@ -81,6 +81,8 @@ def run_gpu(cfg, name, steps, warmup, batch):
                             f_high=cfg["f_high"], window=cfg["window"], levels=cfg["levels"],
                             use_iir=cfg["use_iir"], color=cfg["color"],
                             bilateral=cfg["bilateral"], max_batch=batch)
+    if tile_rows:
+        sess.set_tile_rows(tile_rows)
     torch.cuda.synchronize()
     for k in range(warmup):
         sess.process_device(frames[k * batch:(k + 1) * batch], d_out_u8=out[k * batch:(k + 1) * batch], n=batch)
@@ -251,6 +253,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the multi-stream and serving measurements")
     ap.add_argument("--serve-duration", type=float, default=4.0, help="seconds per serving run")
+    ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16, 32],
+                    help="guided-filter tile height (0: the autotuner's pick; a fixed height "
+                         "skips the tuner's timing launches, e.g. under a profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (for rocprofv3 counter scaling)")
     args = ap.parse_args()
@@ -273,9 +278,12 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
 
+    if args.tile_rows:
+        import wema_amd
+        wema_amd.autotune(False)
     cfg = CONFIGS[args.config]
     batch = args.batch or cfg["batch"]
-    r = run_gpu(cfg, args.config, args.steps, args.warmup, batch)
+    r = run_gpu(cfg, args.config, args.steps, args.warmup, batch, args.tile_rows)
     line = {
         "metric": METRIC,
         "value": round(r["mpix_s"], 1),
@@ -304,7 +312,7 @@ def main():
     also = {}
     for name in [a for a in args.also.split(",") if a and a != args.config]:
         c = CONFIGS[name]
-        rr = run_gpu(c, name, args.steps, args.warmup, c["batch"])
+        rr = run_gpu(c, name, args.steps, args.warmup, c["batch"], args.tile_rows)
         also[name] = {"desc": c["desc"], "value": round(rr["mpix_s"], 1), "unit": "Mpixel/s",
                       "fps": round(rr["fps"], 1), "ms_per_step": round(rr["ms"] / args.steps, 3),
                       "frames_per_step": c["batch"], "pipeline_algorithmic_GBs": rr["pipeline_gbs"],

@@ -39,9 +39,15 @@ def _short(name):
 
 
 def kernel_stats(trace_dir):
-    rows = list(csv.DictReader(open(_one(os.path.join(trace_dir, "*_kernel_stats.csv")))))
-    return [(_short(r["Name"]), int(r["Calls"]), float(r["AverageNs"]) / 1e3,
-             float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, float(r["Percentage"])) for r in rows]
+    """Per (kernel, grid size): calls, mean / min / max duration in us, share of GPU time.
+    The DWT kernels run one grid size per level, so each level gets its own row."""
+    per = collections.defaultdict(list)
+    for r in csv.DictReader(open(_one(os.path.join(trace_dir, "*_kernel_trace.csv")))):
+        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        per[(_short(r["Kernel_Name"]), grid)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    total = sum(sum(v) for v in per.values())
+    rows = [(n, g, len(v), sum(v) / len(v), min(v), max(v), 100.0 * sum(v) / total) for (n, g), v in per.items()]
+    return sorted(rows, key=lambda r: -r[3] * r[2])
 
 
 def counters(pmc_dir, counter):
@@ -98,9 +104,10 @@ def main():
     lines = ["# %s" % a.title, "", "Commands (from the repository root on the GPU host):", ""]
     lines += ["    " + c for c in a.cmd]
     lines += ["", "## Kernel durations (rocprofv3 --kernel-trace --stats)", "",
-              "| kernel | calls | avg us | min us | max us | % of GPU time |", "|---|---:|---:|---:|---:|---:|"]
-    for n, c, avg, mn, mx, pct in stats:
-        lines.append("| %s | %d | %.1f | %.1f | %.1f | %.2f |" % (n, c, avg, mn, mx, pct))
+              "| kernel | grid (threads) | calls | avg us | min us | max us | % of GPU time |",
+              "|---|---:|---:|---:|---:|---:|---:|"]
+    for n, g, c, avg, mn, mx, pct in stats:
+        lines.append("| %s | %d | %d | %.1f | %.1f | %.1f | %.2f |" % (n, g, c, avg, mn, mx, pct))
     lines += ["", "## HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE)", "",
               "Scale from the 1 GiB device copy of the same runs: FETCH_SIZE read %.0f KiB "
               "(x%.3f), WRITE_SIZE read %.0f KiB (x%.3f)." % (fraw, fs, wraw, ws), "",
