@@ -164,6 +164,37 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
     std::vector<double> due;
     std::vector<size_t> who;
 
+    // batched ticks are pipelined: a tick is launched before the previous one is waited
+    // for, so the host gathers and builds the next tick while the GPU runs this one
+    struct Pending {
+        hipEvent_t ev = nullptr;
+        std::vector<double> due;
+        bool measured = false;
+    };
+    Pending slots[2];
+    for (Pending &q : slots)
+        if (hipEventCreateWithFlags(&q.ev, hipEventDisableTiming) != hipSuccess) {
+            wema_set_error("event creation failed");
+            return -1;
+        }
+    std::deque<Pending *> inflight;
+    int slot = 0;
+    auto drain_one = [&]() {
+        Pending *q = inflight.front();
+        inflight.pop_front();
+        if (hipEventSynchronize(q->ev) != hipSuccess) { wema_set_error("tick failed"); return -1; }
+        const double done = now_s(t0);
+        if (q->measured)
+            for (double d : q->due) lat.push_back(done - d);
+        t_last = done;
+        return 0;
+    };
+    auto drain_all = [&]() {
+        while (!inflight.empty())
+            if (drain_one() < 0) return -1;
+        return 0;
+    };
+
     // admission (first come, first served; the head waits until it fits)
     auto admit = [&](double now) {
         while (!waiting.empty()) {
@@ -210,6 +241,7 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         // departures
         for (size_t i = 0; i < live.size();) {
             if (live[i].end <= now || (quota && live[i].next >= quota)) {
+                if (drain_all() < 0) { rc = -1; break; }
                 hipStreamSynchronize(st);
                 wema_pool_remove(pool, live[i].id);
                 hipFree(live[i].out);
@@ -239,6 +271,11 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
             }
         }
         if (ids.empty()) {
+            if (!inflight.empty()) {
+                // nothing due: see the running tick finish
+                if (drain_one() < 0) { rc = -1; break; }
+                continue;
+            }
             double wake = next_due;
             if (next_plan < plan.size()) wake = std::min(wake, plan[next_plan].arrive);
             const double dt = std::min(wake, horizon) - now_s(t0);
@@ -247,13 +284,16 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         }
         const bool measured = rounds >= cfg->warmup_rounds;
         if (cfg->batched) {
+            Pending &q = slots[slot];
+            slot ^= 1;
+            if (!inflight.empty() && inflight.front() == &q && drain_one() < 0) { rc = -1; break; }
             if (wema_pool_tick(pool, (int)ids.size(), ids.data(), ins.data(), outs.data(), nullptr) < 0) { rc = -1; break; }
-            if (hipStreamSynchronize(st) != hipSuccess) { wema_set_error("tick failed"); rc = -1; break; }
-            const double done = now_s(t0);
-            if (measured) {
-                for (double d : due) lat.push_back(done - d);
-                ticks++;
-            }
+            if (hipEventRecord(q.ev, st) != hipSuccess) { wema_set_error("event record failed"); rc = -1; break; }
+            q.due = due;
+            q.measured = measured;
+            inflight.push_back(&q);
+            if (inflight.size() > 1 && drain_one() < 0) { rc = -1; break; }
+            if (measured) ticks++;
         } else {
             for (size_t k = 0; k < ids.size(); k++) {
                 if (wema_pool_tick(pool, 1, &ids[k], &ins[k], &outs[k], nullptr) < 0) { rc = -1; break; }
@@ -275,12 +315,14 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         }
         if (measured) tick_frames += (long)ids.size();
         rounds++;
-        t_last = now_s(t0);
-        if (rounds == cfg->warmup_rounds) t_meas = t_last;
+        if (!cfg->batched) t_last = now_s(t0);
+        if (rounds == cfg->warmup_rounds) t_meas = now_s(t0);
     }
+    if (drain_all() < 0) rc = -1;
     // a saturate run is measured up to its last round, a paced one over its whole span
     const double wall = (cfg->saturate ? t_last : now_s(t0)) - t_meas;
     hipStreamSynchronize(st);
+    for (Pending &q : slots) hipEventDestroy(q.ev);
     for (Live &L : live) {
         wema_pool_remove(pool, L.id);
         hipFree(L.out);
