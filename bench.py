@@ -12,7 +12,7 @@ Workloads (the default is the single-stream drop-in case, 1920x1080 colour):
   c3   3840x2160 RGB24 @60, 6-level DWT, band 0.8-2.0 Hz, amp 50, guided filter,
        240 frames = (warmup + steps) x 48  (reported under "also" by default)
   c1   320x240 RGB24 @30, 3-level DWT, band 0.5-3 Hz, amp 120, guided filter
-  c2w  as c2 in sliding-window mode with bilateral weights (window 32)
+  c2w  as c2 in sliding-window mode with bilateral weights (window 32); also reported
   multistream  24 concurrent streams of mixed resolution, batched ticks vs one at a time
   serving      Poisson sessions at real-time pace: p50/p99 latency, capacity at p99 <= 33 ms
 
@@ -264,7 +264,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--batch", type=int, default=0, help="frames per step (default per config)")
-    ap.add_argument("--also", default="c3", help="comma list of extra workloads ('' = none)")
+    ap.add_argument("--also", default="c3,c2w", help="comma list of extra workloads ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true",

@@ -22,7 +22,7 @@ hipError_t wema_launch_dwt_fwd(const DwtfJob *, int, int, int, hipStream_t);
 hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, int, hipStream_t);
 hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *, int, int, hipStream_t);
 hipError_t wema_launch_iir(const TemporalJob *, int, int, hipStream_t);
-hipError_t wema_launch_win(const TemporalJob *, int, int, hipStream_t);
+hipError_t wema_launch_win(int, const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_egress(int, int, const EgressJob *, int, int, hipStream_t);
 hipError_t wema_launch_synth(const SynthFrame *, const unsigned short *, unsigned char *, int,
                              int, int, hipStream_t);
@@ -461,10 +461,11 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
             j.bilateral = p.bilateral;
             j.tw = s->tw_d;
         }
-        // IIR: coefficients read + written per frame, state read + written once;
-        // window: ws ring samples read + amplified coefficient written per frame
+        // IIR: coefficients read + written per frame, state read + written once; window:
+        // the ws - 1 + n ring samples the call's windows cover, read once, and one amplified
+        // coefficient written per frame
         const double tb = p.use_iir ? (double)nu * (8.0 * n + 32.0)
-                                    : (double)nu * n * (4.0 * p.window + 4.0);
+                                    : (double)nu * (4.0 * (p.window - 1 + n) + 4.0 * n);
         jb.add_temporal(j, tiles_t, tb, !p.use_iir);
     }
 
@@ -607,9 +608,10 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
         HIP_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
         if (timer_mark(timer, TK_IIR, tl.bytes, st) < 0) return -1;
     }
-    const JobList &wl = jb.window;
-    if (wl.count) {
-        HIP_OK(wema_launch_win((const TemporalJob *)(base + wl.offset), wl.count, wl.blocks, st));
+    for (int v = 0; v < 3; v++) {
+        const JobList &wl = jb.window[v];
+        if (!wl.count) continue;
+        HIP_OK(wema_launch_win(v, (const TemporalJob *)(base + wl.offset), wl.count, wl.blocks, st));
         if (timer_mark(timer, TK_WIN, wl.bytes, st) < 0) return -1;
     }
     // inverse stages, coarsest first: level l of every stream is stage WEMA_MAX_LEVELS-1-l,

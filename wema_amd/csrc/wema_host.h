@@ -89,11 +89,11 @@ struct wema_dev {
 struct JobBuilder {
     std::vector<DwtfJob> dwtf_v[WEMA_MAX_LEVELS];
     std::vector<TemporalJob> temporal_v;      // IIR jobs
-    std::vector<TemporalJob> window_v;        // sliding-window jobs
+    std::vector<TemporalJob> window_v[3];     // sliding-window jobs: ws <= 32, <= 64, any
     std::vector<DwtiJob> dwti_v[WEMA_MAX_LEVELS];
     std::vector<DwtiJob> dwti_ycc_v;          // colour output jobs (level 0 + YCbCr -> RGB)
     std::vector<EgressJob> egress_v[EG_KINDS];
-    JobList dwtf[WEMA_MAX_LEVELS], temporal, window, dwti[WEMA_MAX_LEVELS], dwti_ycc, egress[EG_KINDS];
+    JobList dwtf[WEMA_MAX_LEVELS], temporal, window[3], dwti[WEMA_MAX_LEVELS], dwti_ycc, egress[EG_KINDS];
     std::vector<float> host_amp;
     size_t amp_offset = 0;
 
@@ -104,12 +104,13 @@ struct JobBuilder {
         dwtf_v[l].push_back(j);
     }
     void add_temporal(TemporalJob j, int tiles, double bytes, bool window_mode) {
-        JobList &jl = window_mode ? window : temporal;
+        const int wv = !window_mode ? 0 : (j.window <= 32 ? 0 : (j.window <= 64 ? 1 : 2));
+        JobList &jl = window_mode ? window[wv] : temporal;
         j.tile_begin = jl.blocks;
         j.tiles = tiles;
         jl.blocks += tiles;
         jl.bytes += bytes;
-        (window_mode ? window_v : temporal_v).push_back(j);
+        (window_mode ? window_v[wv] : temporal_v).push_back(j);
     }
     void add_dwti(int stage, DwtiJob j, int tiles, double bytes) {
         j.tile_begin = dwti[stage].blocks;
@@ -131,7 +132,8 @@ struct JobBuilder {
     }
     void fix_amp(float *amp_d) {
         for (auto &j : temporal_v) j.amp = amp_d + j.amp_index;
-        for (auto &j : window_v) j.amp = amp_d + j.amp_index;
+        for (auto &v : window_v)
+            for (auto &j : v) j.amp = amp_d + j.amp_index;
     }
     template <typename T>
     static bool put(char *dst, size_t cap, size_t &off, const std::vector<T> &v, JobList &jl) {
@@ -150,7 +152,7 @@ struct JobBuilder {
         bool ok = true;
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwtf_v[l], dwtf[l]);
         ok = ok && put(dst, cap, off, temporal_v, temporal);
-        ok = ok && put(dst, cap, off, window_v, window);
+        for (int k = 0; k < 3; k++) ok = ok && put(dst, cap, off, window_v[k], window[k]);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_v[l], dwti[l]);
         ok = ok && put(dst, cap, off, dwti_ycc_v, dwti_ycc);
         for (int k = 0; k < EG_KINDS; k++) ok = ok && put(dst, cap, off, egress_v[k], egress[k]);

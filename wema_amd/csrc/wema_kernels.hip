@@ -749,6 +749,82 @@ __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ job
 // Sliding-window temporal filter + amplification (src/temporal_filter.c:139-319,
 // src/phase_amp.c:12-56). Window of frame f: ring slots ring_first + f + t (mod ring_slots),
 // t = 0..ws-1, oldest first; tw holds the Gaussian time weights of the bilateral mode.
+//
+// One thread per coefficient position runs all frames of the job. Per frame the window is
+// read twice, oldest first: once for the mean (and, bilateral, the mean absolute deviation
+// from the centre sample), once for the bilateral weights and the weighted sum (the
+// reference's two weight loops read the same values with the same weights, so one pass
+// with one exponential per sample gives the same sums). The weights use the hardware
+// exp2 (relative error of a few 1e-7 where it matters, against about 6e-8 for libm's expf).
+
+// Temporal output of one frame from its window, given in order by win.each(fn(t, v)).
+template <class Win>
+__device__ __forceinline__ float win_delta(const Win &win, int ws, float inv_ws, float cur, bool bilateral,
+                                           const float *tw, float &mean_out) {
+    float mean = 0.0f, mad = 0.0f;
+    win.each([&](int, float v) {
+        mean += v;
+        if (bilateral) {
+            const float d = v - cur;
+            mad += d > 0.0f ? d : -d;
+        }
+    });
+    mean *= inv_ws;
+    mean_out = mean;
+    float delta = cur - mean;
+    if (bilateral) {
+        const float sigma = fmaxf(mad * inv_ws * 0.5f, 0.001f);
+        const float inv2 = 1.0f / (2.0f * sigma * sigma);
+        float wsum = 0.0f, acc = 0.0f;
+        win.each([&](int t, float v) {
+            const float d = v - cur;
+            const float wt = __expf(-(d * d) * inv2) * tw[t];
+            wsum += wt;
+            acc = fmaf(wt, v - mean, acc);
+        });
+        if (wsum > 1e-8f) delta = acc * (1.0f / wsum);
+    }
+    return delta;
+}
+
+// Window read from the ring in global memory.
+struct WinGlobal {
+    const float *ring;
+    size_t plane;
+    long long p;
+    int first, slots, ws;
+    template <class F>
+    __device__ __forceinline__ void each(F &&fn) const {
+        int slot = first;
+        for (int t = 0; t < ws; ++t) {
+            fn(t, gp(ring)[(size_t)slot * plane + p]);
+            if (++slot == slots) slot = 0;
+        }
+    }
+};
+
+// Window held in LDS, one column per thread: entries head..ws-1 then 0..head-1.
+template <int WSMAX>
+struct WinLds {
+    float (*W)[256];
+    int head, ws;
+    template <class F>
+    __device__ __forceinline__ void each(F &&fn) const {
+        int t = 0;
+        for (int j = head; j < ws; ++j) fn(t++, W[j][threadIdx.x]);
+        for (int j = 0; j < head; ++j) fn(t++, W[j][threadIdx.x]);
+    }
+};
+
+__device__ __forceinline__ void win_store(const TemporalJob &J, int f, long long p, float orig, float delta,
+                                          const float *sT, const float *sA, float lscale) {
+    const float T = sT[f];
+    const float a = sA[f] * lscale;
+    J.amp_out[(size_t)f * J.out_stride + p] = (fabsf(orig) <= T) ? orig : fmaf(a, delta, orig);
+    if (J.filt_out) J.filt_out[(size_t)f * J.out_stride + p] = delta;
+}
+
+// Any window size: the window is read from the ring for every frame.
 __global__ void __launch_bounds__(256) k_win(const TemporalJob *__restrict__ jobs, int njobs) {
     __shared__ float sT[WEMA_MAX_BATCH], sA[WEMA_MAX_BATCH];
     const int jid = find_job(jobs, njobs, (int)blockIdx.x);
@@ -760,45 +836,69 @@ __global__ void __launch_bounds__(256) k_win(const TemporalJob *__restrict__ job
     const float inv_ws = 1.0f / (float)ws;
     const float lscale = p < J.level0_end ? 0.5f : 1.0f;
     const size_t plane = (size_t)J.coef_stride;
-    const float *tw = J.tw;
     for (int f = 0; f < nframes; ++f) {
-        const int first = J.ring_first + f;
-        float mean = 0.0f;
-        for (int t = 0; t < ws; ++t) mean += J.ring[(size_t)((first + t) % J.ring_slots) * plane + p];
-        const float cur = J.ring[(size_t)((first + center) % J.ring_slots) * plane + p];
-        const float orig = J.ring[(size_t)((first + ws - 1) % J.ring_slots) * plane + p];
-        mean *= inv_ws;
-        float delta = cur - mean;
-        if (J.bilateral) {
-            float mad = 0.0f;
-            for (int t = 0; t < ws; ++t) {
-                float d = J.ring[(size_t)((first + t) % J.ring_slots) * plane + p] - cur;
-                mad += d > 0.0f ? d : -d;
-            }
-            const float sigma = fmaxf(mad * inv_ws * 0.5f, 0.001f);
-            const float inv2 = 1.0f / (2.0f * sigma * sigma);
-            float wsum = 0.0f;
-            for (int t = 0; t < ws; ++t) {
-                float d = J.ring[(size_t)((first + t) % J.ring_slots) * plane + p] - cur;
-                wsum += expf(-(d * d) * inv2) * tw[t];
-            }
-            if (wsum > 1e-8f) {
-                float acc = 0.0f;
-                for (int t = 0; t < ws; ++t) {
-                    float s = J.ring[(size_t)((first + t) % J.ring_slots) * plane + p];
-                    float d = s - cur;
-                    float wt = expf(-(d * d) * inv2) * tw[t];
-                    acc = fmaf(wt, s - mean, acc);
-                }
-                delta = acc * (1.0f / wsum);
-            }
-        }
-        const float T = sT[f];
-        const float a = sA[f] * lscale;
-        J.amp_out[(size_t)f * J.out_stride + p] = (fabsf(orig) <= T) ? orig : fmaf(a, delta, orig);
-        if (J.filt_out) J.filt_out[(size_t)f * J.out_stride + p] = delta;
+        const int first = (J.ring_first + f) % J.ring_slots;
+        const WinGlobal win{J.ring, plane, p, first, J.ring_slots, ws};
+        const float cur = gp(J.ring)[(size_t)((first + center) % J.ring_slots) * plane + p];
+        const float orig = gp(J.ring)[(size_t)((first + ws - 1) % J.ring_slots) * plane + p];
+        float mean;
+        const float delta = win_delta(win, ws, inv_ws, cur, J.bilateral != 0, J.tw, mean);
+        win_store(J, f, p, orig, delta, sT, sA, lscale);
     }
 }
+
+// Windows of up to WSMAX samples: each position's window slides through an LDS column, so
+// every ring sample is read from HBM once per launch (ws - 1 + nframes reads per position
+// instead of ws per frame).
+template <int WSMAX>
+__device__ __forceinline__ void win_lds_body(const TemporalJob *__restrict__ jobs, int njobs, float (*W)[256],
+                                             float *sT, float *sA, float *sTw) {
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const TemporalJob J = jobs[jid];
+    frame_params(J, sT, sA);
+    const int nframes = J.nframes, ws = J.window, center = ws / 2;
+    if (J.bilateral)
+        for (int t = threadIdx.x; t < ws; t += blockDim.x) sTw[t] = J.tw[t];
+    __syncthreads();
+    const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
+    if (p >= J.n) return;
+    const float inv_ws = 1.0f / (float)ws;
+    const float lscale = p < J.level0_end ? 0.5f : 1.0f;
+    const size_t plane = (size_t)J.coef_stride;
+    const GLOBAL_AS float *ring = gp(J.ring);
+    // the first ws - 1 samples of frame 0's window
+    int slot = J.ring_first;
+    for (int t = 0; t < ws - 1; ++t) {
+        W[t][threadIdx.x] = ring[(size_t)slot * plane + p];
+        if (++slot == J.ring_slots) slot = 0;
+    }
+    int head = 0;   // LDS entry of the oldest sample of the current window
+    for (int f = 0; f < nframes; ++f) {
+        // the newest sample of frame f's window (ring slot ring_first + f + ws - 1)
+        const float orig = ring[(size_t)slot * plane + p];
+        if (++slot == J.ring_slots) slot = 0;
+        int tail = head + ws - 1;
+        if (tail >= ws) tail -= ws;
+        W[tail][threadIdx.x] = orig;
+        int c = head + center;
+        if (c >= ws) c -= ws;
+        const float cur = W[c][threadIdx.x];
+        const WinLds<WSMAX> win{W, head, ws};
+        float mean;
+        const float delta = win_delta(win, ws, inv_ws, cur, J.bilateral != 0, sTw, mean);
+        win_store(J, f, p, orig, delta, sT, sA, lscale);
+        if (++head == ws) head = 0;
+    }
+}
+
+#define WEMA_WIN_LDS_KERNEL(WSMAX)                                                              \
+    __global__ void __launch_bounds__(256) k_win_l##WSMAX(const TemporalJob *__restrict__ jobs, int njobs) { \
+        __shared__ float W[WSMAX][256];                                                         \
+        __shared__ float sT[WEMA_MAX_BATCH], sA[WEMA_MAX_BATCH], sTw[WSMAX];                    \
+        win_lds_body<WSMAX>(jobs, njobs, W, sT, sA, sTw);                                       \
+    }
+WEMA_WIN_LDS_KERNEL(32)
+WEMA_WIN_LDS_KERNEL(64)
 
 // ---------------------------------------------------------------------------------------
 // Guided filter (src/wema.c:114-230, 289-335; src/wema_batch.c:26-132, 311-346).
@@ -1372,9 +1472,12 @@ hipError_t wema_launch_iir(const TemporalJob *jobs, int njobs, int nblocks, hipS
     hipLaunchKernelGGL(k_iir, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError();
 }
-hipError_t wema_launch_win(const TemporalJob *jobs, int njobs, int nblocks, hipStream_t s) {
+// variant: 0 = windows up to 32 samples (LDS), 1 = up to 64 (LDS), 2 = any (ring reads)
+hipError_t wema_launch_win(int variant, const TemporalJob *jobs, int njobs, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_win, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    if (variant == 0) hipLaunchKernelGGL(k_win_l32, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    else if (variant == 1) hipLaunchKernelGGL(k_win_l64, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    else hipLaunchKernelGGL(k_win, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 hipError_t wema_launch_egress(int kind, int rows, const EgressJob *jobs, int njobs, int nblocks,
