@@ -121,6 +121,44 @@ def run_gpu(cfg, name, steps, warmup, batch, tile_rows=0):
     return res
 
 
+def run_host_pipeline(cfg, steps, batch):
+    """The same workload from pinned host memory: each step's frames are uploaded, processed
+    and downloaded with the asynchronous pipeline (two batches in flight, copies on their
+    own streams), so the rate includes the PCIe transfers both ways."""
+    import wema_amd
+    w, h = cfg["w"], cfg["h"]
+    synth = wema_amd.Synth(cfg["preset"], w, h, cfg["fps"])
+    sess = wema_amd.Session(w, h, cfg["fps"], amp=cfg["amp"], f_low=cfg["f_low"],
+                            f_high=cfg["f_high"], window=cfg["window"], levels=cfg["levels"],
+                            use_iir=cfg["use_iir"], color=cfg["color"],
+                            bilateral=cfg["bilateral"], max_batch=batch)
+    pin_in = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(2)]
+    pin_out = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(2)]
+    for k in range(2):
+        for i in range(batch):
+            pin_in[k].array[i] = synth.frame(k * batch + i)
+    # warm-up batch, then `steps` timed batches, alternating the two slots
+    sess.wait(sess.submit_host(pin_in[0].array, pin_out[0].array, batch))
+    t0 = time.perf_counter()
+    tickets = []
+    for k in range(steps):
+        if len(tickets) == 2:
+            sess.wait(tickets.pop(0))
+        tickets.append(sess.submit_host(pin_in[k % 2].array, pin_out[k % 2].array, batch))
+    for t in tickets:
+        sess.wait(t)
+    dt = time.perf_counter() - t0
+    sess.close()
+    for a in pin_in + pin_out:
+        a.close()
+    frames = steps * batch
+    return {"desc": "the same frames from pinned host memory through the asynchronous pipeline "
+                    "(RGB24 up and down over PCIe, overlapped with processing)",
+            "value": round(frames * w * h / 1e6 / dt, 1), "unit": "Mpixel/s",
+            "fps": round(frames / dt, 1), "steps": steps, "frames_per_step": batch,
+            "link_GBs": round(2 * frames * w * h * 3 / dt / 1e9, 1)}
+
+
 def cpu_baseline(cfg, budget_s=20.0, first_frame=0, gpu_sample=None):
     """The compiled reference on this host: colour and window modes run the per-frame path
     (src/main.c:355: colour never batches; single-threaded), luma IIR the OpenMP batch path."""
@@ -324,6 +362,7 @@ def main():
                     for d in r["kernels"]],
         "cpu_baseline": None,
     }
+    line["host_pipeline"] = run_host_pipeline(cfg, max(args.steps, 4), batch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
     also = {}

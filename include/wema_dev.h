@@ -97,6 +97,16 @@ int wema_dev_process_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out)
 int wema_dev_process_host_f32(wema_dev_t *s, const float *const *h_in, float *const *h_out, int n);
 int wema_dev_sync(wema_dev_t *s);
 
+/* Asynchronous host-buffer pipeline, the GPU counterpart of the reader/writer slots of
+ * async_io.c (async_io.c:183-272): n RGB24 frames (n <= max_batch) are uploaded on a copy
+ * stream, processed on the session's stream and downloaded on a second copy stream; the
+ * call returns at once with a ticket. Two batches can be in flight, so the uploads and
+ * downloads of one overlap the processing of the other. h_in and h_out must stay valid
+ * until wema_dev_wait_ticket returns for the ticket; with pinned memory
+ * (wema_dev_host_alloc) the copies run at full link speed. */
+int wema_dev_submit_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out, int n, long *ticket);
+int wema_dev_wait_ticket(wema_dev_t *s, long ticket);
+
 /* Tile shapes (no counterpart in the reference). The guided-filter kernels come in tile
  * heights of 8, 16 and 32 rows (64 columns). A session whose output stage is the guided
  * filter takes the height an autotuner picked for its frame size: the first session of a

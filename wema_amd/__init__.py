@@ -104,6 +104,8 @@ _SIGS = {
     "wema_dev_process_host_u8": (C.c_int, [_vp, _vp, _vp]),
     "wema_dev_sync": (C.c_int, [_vp]),
     "wema_dev_host_alloc": (_vp, [C.c_size_t]),
+    "wema_dev_submit_host_u8": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_long)]),
+    "wema_dev_wait_ticket": (C.c_int, [_vp, C.c_long]),
     "wema_dev_host_free": (None, [_vp]),
     "wema_dev_debug_capture": (C.c_int, [_vp, C.c_int]),
     "wema_dev_timing": (C.c_int, [_vp, C.c_int]),
@@ -157,6 +159,26 @@ def _ptr(a):
 def _stream_handle():
     import torch
     return torch.cuda.current_stream().cuda_stream
+
+
+class PinnedArray:
+    """A numpy array over pinned host memory (wema_dev_host_alloc); close() releases it."""
+
+    def __init__(self, shape, dtype=np.uint8):
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        self._p = lib.wema_dev_host_alloc(n)
+        if not self._p:
+            raise WemaError(last_error())
+        self.array = np.frombuffer((C.c_uint8 * n).from_address(self._p), dtype=dtype).reshape(shape)
+
+    def close(self):
+        if self._p:
+            self.array = None
+            lib.wema_dev_host_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
 
 
 def autotune(enable):
@@ -267,6 +289,18 @@ class Session:
         out = np.empty_like(frame)
         _check(lib.wema_dev_process_host_u8(self._h, _ptr(frame), _ptr(out)), "process_host_u8")
         return out
+
+    def submit_host(self, h_in, h_out, n):
+        """Asynchronous host pipeline: uploads n RGB24 frames from h_in, processes them and
+        downloads them into h_out (arrays that stay alive until wait(ticket)); returns the
+        ticket. Pinned arrays (PinnedArray) copy at full link speed."""
+        t = C.c_long()
+        _check(lib.wema_dev_submit_host_u8(self._h, _ptr(h_in), _ptr(h_out), int(n), C.byref(t)),
+               "submit_host_u8")
+        return t.value
+
+    def wait(self, ticket):
+        _check(lib.wema_dev_wait_ticket(self._h, int(ticket)), "wait_ticket")
 
     # ---- measurement
     def timing(self, enable):

@@ -335,6 +335,16 @@ extern "C" void wema_dev_close(wema_dev_t *s) {
     if (s->io_out) hipFree(s->io_out);
     if (s->io_in_f32) hipFree(s->io_in_f32);
     if (s->io_out_f32) hipFree(s->io_out_f32);
+    for (auto &a : s->aslot) {
+        if (a.ev_out) hipEventSynchronize(a.ev_out);
+        if (a.d_in) hipFree(a.d_in);
+        if (a.d_out) hipFree(a.d_out);
+        if (a.ev_in) hipEventDestroy(a.ev_in);
+        if (a.ev_comp) hipEventDestroy(a.ev_comp);
+        if (a.ev_out) hipEventDestroy(a.ev_out);
+    }
+    if (s->s_up) hipStreamDestroy(s->s_up);
+    if (s->s_down) hipStreamDestroy(s->s_down);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
 }
@@ -721,6 +731,50 @@ extern "C" int wema_dev_process_host_f32(wema_dev_t *s, const float *const *h_in
         HIP_OK(hipMemcpyAsync(h_out[i], s->io_out_f32 + frame * i, sizeof(float) * frame,
                               hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+extern "C" int wema_dev_submit_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out, int n,
+                                       long *ticket) {
+    if (!s || !h_in || !h_out || n < 1) { wema_set_error("bad argument"); return -1; }
+    if (s->pooled) { wema_set_error("a pooled session is processed by wema_pool_tick"); return -1; }
+    if (n > s->p.max_batch) { wema_set_error("%d frames exceed max_batch %d", n, s->p.max_batch); return -1; }
+    HIP_OK(hipSetDevice(s->p.device));
+    const size_t bytes = s->g.P * 3 * (size_t)n;
+    if (!s->s_up) {
+        const size_t cap = s->g.P * 3 * (size_t)s->p.max_batch;
+        HIP_OK(hipStreamCreateWithFlags(&s->s_up, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&s->s_down, hipStreamNonBlocking));
+        for (auto &a : s->aslot) {
+            HIP_OK(hipMalloc(&a.d_in, cap));
+            HIP_OK(hipMalloc(&a.d_out, cap));
+            HIP_OK(hipEventCreateWithFlags(&a.ev_in, hipEventDisableTiming));
+            HIP_OK(hipEventCreateWithFlags(&a.ev_comp, hipEventDisableTiming));
+            HIP_OK(hipEventCreateWithFlags(&a.ev_out, hipEventDisableTiming));
+        }
+    }
+    const long t = s->next_ticket++;
+    auto &a = s->aslot[t & 1];
+    // the slot's previous batch has left the device before its buffers are overwritten
+    if (a.ticket >= 0) HIP_OK(hipStreamWaitEvent(s->s_up, a.ev_out, 0));
+    HIP_OK(hipMemcpyAsync(a.d_in, h_in, bytes, hipMemcpyHostToDevice, s->s_up));
+    HIP_OK(hipEventRecord(a.ev_in, s->s_up));
+    HIP_OK(hipStreamWaitEvent(s->stream, a.ev_in, 0));
+    if (process(s, a.d_in, 0, a.d_out, nullptr, n, s->stream) < 0) return -1;
+    HIP_OK(hipEventRecord(a.ev_comp, s->stream));
+    HIP_OK(hipStreamWaitEvent(s->s_down, a.ev_comp, 0));
+    HIP_OK(hipMemcpyAsync(h_out, a.d_out, bytes, hipMemcpyDeviceToHost, s->s_down));
+    HIP_OK(hipEventRecord(a.ev_out, s->s_down));
+    a.ticket = t;
+    if (ticket) *ticket = t;
+    return 0;
+}
+
+extern "C" int wema_dev_wait_ticket(wema_dev_t *s, long ticket) {
+    if (!s || ticket < 0 || ticket >= s->next_ticket) { wema_set_error("bad ticket"); return -1; }
+    // the slot's event marks this batch's download or a later one, and downloads run in
+    // order on one stream
+    HIP_OK(hipEventSynchronize(s->aslot[ticket & 1].ev_out));
     return 0;
 }
 

@@ -84,6 +84,15 @@ struct wema_dev {
     bool pooled = false;        // owned by a wema_pool (no staging buffers of its own)
     void *io_in = nullptr, *io_out = nullptr;           // host-buffer staging (RGB24)
     float *io_in_f32 = nullptr, *io_out_f32 = nullptr;  // host-buffer staging (float RGB)
+    // asynchronous host pipeline: two device slots of max_batch RGB24 frames in and out,
+    // uploads and downloads on their own streams
+    struct AsyncSlot {
+        uint8_t *d_in = nullptr, *d_out = nullptr;
+        hipEvent_t ev_in = nullptr, ev_comp = nullptr, ev_out = nullptr;
+        long ticket = -1;
+    } aslot[2];
+    hipStream_t s_up = nullptr, s_down = nullptr;
+    long next_ticket = 0;
 };
 
 struct JobBuilder {
