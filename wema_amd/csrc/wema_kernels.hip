@@ -247,9 +247,11 @@ __device__ __forceinline__ void pair_decode(const PairRaw &r, float &e, float &o
 
 // Row pass of one wave: rows wave, wave + 4, ... of the extended tile; lane = sample pair.
 // All loads of the wave's 16 rows are issued first; the rows are then lifted 4 at a time.
-template <int KIND>
-__device__ __forceinline__ void row_pass(float (*R)[128], const DwtfJob &J, int sx0, int sy0,
-                                         int lane, int wave) {
+// ROWS_IN: every row of the tile lies inside the image and none is its last row, so rows
+// need no reflection and no end-of-buffer check.
+template <int KIND, bool ROWS_IN>
+__device__ __forceinline__ void row_pass_t(float (*R)[128], const DwtfJob &J, int sx0, int sy0,
+                                           int lane, int wave) {
     constexpr int NR = 2 * WEMA_RP / 4;
     const int w = J.w, h = J.h;
     const int xa = reflect_idx(sx0 + 2 * lane, w), xb = reflect_idx(sx0 + 2 * lane + 1, w);
@@ -257,9 +259,9 @@ __device__ __forceinline__ void row_pass(float (*R)[128], const DwtfJob &J, int 
     PairRaw raw[NR];
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
-        const int y = reflect_idx(sy0 + wave + 4 * i, h);
+        const int y = ROWS_IN ? sy0 + wave + 4 * i : reflect_idx(sy0 + wave + 4 * i, h);
         // the pair's dwords stay inside the buffer unless it ends the last row
-        const bool fast = fast_x && !(y == h - 1 && xb == w - 1);
+        const bool fast = ROWS_IN ? fast_x : fast_x && !(y == h - 1 && xb == w - 1);
         if (fast) raw[i] = pair_load<KIND>(J, xa, y);
     }
 #pragma unroll
@@ -268,8 +270,8 @@ __device__ __forceinline__ void row_pass(float (*R)[128], const DwtfJob &J, int 
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int i = g + k;
-            const int y = reflect_idx(sy0 + wave + 4 * i, h);
-            const bool fast = fast_x && !(y == h - 1 && xb == w - 1);
+            const int y = ROWS_IN ? sy0 + wave + 4 * i : reflect_idx(sy0 + wave + 4 * i, h);
+            const bool fast = ROWS_IN ? fast_x : fast_x && !(y == h - 1 && xb == w - 1);
             if (fast) {
                 pair_decode<KIND>(raw[i], e[k], o[k]);
             } else {
@@ -295,6 +297,16 @@ __device__ __forceinline__ void row_pass(float (*R)[128], const DwtfJob &J, int 
             }
         }
     }
+}
+
+template <int KIND>
+__device__ __forceinline__ void row_pass(float (*R)[128], const DwtfJob &J, int sx0, int sy0,
+                                         int lane, int wave) {
+    // (the coarser levels' float planes keep the general form: measured faster there)
+    if (KIND != SRC_F32_PLANE && sy0 >= 0 && sy0 + 2 * WEMA_RP < J.h)
+        row_pass_t<KIND, true>(R, J, sx0, sy0, lane, wave);
+    else
+        row_pass_t<KIND, false>(R, J, sx0, sy0, lane, wave);
 }
 
 // ---------------------------------------------------------------------------------------
