@@ -181,3 +181,27 @@ def test_async_host_pipeline_equals_device_path():
     sess.close()
     for a in pin_in + pin_out:
         a.close()
+
+
+@pytest.mark.parametrize("w,h,levels,mode", [(1, 1, 1, "iir"), (3, 2, 1, "iir"), (7, 5, 2, "iir_color"),
+                                             (17, 1, 3, "iir"), (16, 9, 3, "window"),
+                                             (5, 19, 2, "gauss")])
+def test_tiny_and_degenerate_shapes_vs_oracle(w, h, levels, mode):
+    """Frames smaller than a tile, single rows and columns (the reference's n < 2 identity,
+    src/dwt.c:46) and levels whose subbands shrink to one sample: output bytes equal to the
+    oracle."""
+    fps, n = 30.0, 10
+    kw = dict(fps=fps, amp=90.0, f_low=0.5, f_high=3.0, levels=levels, window=4)
+    if mode == "iir_color":
+        kw.update(color=True)
+    elif mode == "window":
+        kw.update(use_iir=False)
+    elif mode == "gauss":
+        kw.update(edge_aware=False)
+    rng = np.random.default_rng(w * 100 + h)
+    frames = rng.integers(0, 256, size=(n, h, w, 3), dtype=np.uint8)
+    _, g_out, _ = run_gpu(frames, **kw)
+    orc = O.OraclePipe(w, h, **kw)
+    for t in range(n):
+        o_out, _ = orc.frame(frames[t])
+        assert np.array_equal(g_out[t], o_out), (t, np.abs(g_out[t].astype(int) - o_out.astype(int)).max())
