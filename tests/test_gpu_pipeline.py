@@ -205,3 +205,24 @@ def test_tiny_and_degenerate_shapes_vs_oracle(w, h, levels, mode):
     for t in range(n):
         o_out, _ = orc.frame(frames[t])
         assert np.array_equal(g_out[t], o_out), (t, np.abs(g_out[t].astype(int) - o_out.astype(int)).max())
+
+
+@pytest.mark.parametrize("preset,w,h,fps,kw", [
+    (2, 1920, 1080, 30.0, dict(levels=5, amp=120.0, f_low=0.5, f_high=3.0, color=True)),
+    (3, 3840, 2160, 60.0, dict(levels=6, amp=50.0, f_low=0.8, f_high=2.0)),
+])
+def test_full_size_workloads_vs_oracle(preset, w, h, fps, kw):
+    """The 1080p colour and 4K guided-filter workloads at their full sizes (including the
+    odd-parent level 135 -> 68 where stale entries enter the threshold): thresholds and
+    output bytes equal to the oracle's for the first frames, batched as the benchmark runs
+    them."""
+    n = 3 if w < 3000 else 2   # the oracle's 4K frame 0 runs the IIR warm-up: ~10 s on a CPU
+    syn = wema_amd.Synth(preset, w, h, fps)
+    frames = np.stack([syn.frame(t) for t in range(n)])
+    sess, g_out, g_thr = run_gpu(frames, batch=n, fps=fps, **kw)
+    orc = O.OraclePipe(w, h, fps=fps, **kw)
+    for t in range(n):
+        o_out, o_thr = orc.frame(frames[t])
+        nch = 3 if kw.get("color") else 1
+        assert np.array_equal(g_thr[t][:nch], o_thr[:nch]), (t, g_thr[t], o_thr)
+        assert np.array_equal(g_out[t], o_out), (t, int(np.abs(g_out[t].astype(int) - o_out.astype(int)).max()))
