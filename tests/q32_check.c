@@ -1,11 +1,10 @@
 /* Exhaustive check of the short conversion of |v| to a multiple of 2^-32 (round half up)
  * used by the forward-DWT kernel for |v| < 2^30: the integer part hi is exact by
- * truncation, the fraction a - hi is exact, x = fraction * 2^32 is exact; below 0.5 the
- * low word is 0; in [0.5, 2^23) adding 0.5 can only drop a last bit below the integers,
- * and truncation then rounds half up; from 2^23 on x is an integer (as it always is when
- * |v| >= 1). It must equal the general conversion (exponent / mantissa shifts) for every
- * float below 2^30 (subnormals count as 0, as the kernels flush them; from 2^30 on the
- * general conversion saturates). */
+ * truncation, the fraction a - hi is exact, x = fraction * 2^32 is exact and below 2^32;
+ * the low word is x truncated, plus one where x - floor(x) >= 0.5 (the kernel's
+ * v_fract_f32, exact for x >= 0). It must equal the general conversion (exponent /
+ * mantissa shifts) for every float below 2^30 (subnormals count as 0, as the kernels
+ * flush them; from 2^30 on the general conversion saturates). */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -30,8 +29,8 @@ static uint64_t q32_short(float v) {
     const float a = fabsf(v);
     const uint32_t hi = (uint32_t)a;          /* integer part (truncation) */
     const float x = (a - (float)hi) * 4294967296.0f;
-    uint32_t lo = 0;
-    if (x >= 0.5f) lo = (uint32_t)(x < 8388608.0f ? x + 0.5f : x);
+    const float fr = x - floorf(x);
+    const uint32_t lo = (uint32_t)x + (fr >= 0.5f ? 1u : 0u);
     return ((uint64_t)hi << 32) + lo;
 }
 

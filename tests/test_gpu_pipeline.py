@@ -226,3 +226,23 @@ def test_full_size_workloads_vs_oracle(preset, w, h, fps, kw):
         nch = 3 if kw.get("color") else 1
         assert np.array_equal(g_thr[t][:nch], o_thr[:nch]), (t, g_thr[t], o_thr)
         assert np.array_equal(g_out[t], o_out), (t, int(np.abs(g_out[t].astype(int) - o_out.astype(int)).max()))
+
+
+@pytest.mark.parametrize("w,h,levels,color,batch", [(642, 481, 4, True, 3), (1000, 701, 5, False, 2),
+                                                    (320, 240, 3, True, 4)])
+def test_medium_shapes_batched_vs_oracle(w, h, levels, color, batch):
+    """Batched frames of medium sizes: widths that are not a multiple of 4 (RGB24 rows and
+    frames that do not start on a dword), odd heights (stale entries at several levels) and
+    strips that stream whole 32-pair columns: thresholds and output bytes equal to the
+    oracle's."""
+    fps, n = 30.0, 2 * batch
+    kw = dict(fps=fps, amp=100.0, f_low=0.5, f_high=3.0, levels=levels, color=color)
+    rng = np.random.default_rng(w + h)
+    frames = rng.integers(0, 256, size=(n, h, w, 3), dtype=np.uint8)
+    _, g_out, g_thr = run_gpu(frames, batch=batch, **kw)
+    orc = O.OraclePipe(w, h, **kw)
+    nch = 3 if color else 1
+    for t in range(n):
+        o_out, o_thr = orc.frame(frames[t])
+        assert np.array_equal(g_thr[t][:nch], o_thr[:nch]), (t, g_thr[t], o_thr)
+        assert np.array_equal(g_out[t], o_out), (t, int(np.abs(g_out[t].astype(int) - o_out.astype(int)).max()))

@@ -24,6 +24,22 @@
 #define WEMA_RP 32
 #define WEMA_TILE_PAIRS_Y (WEMA_RP - 4)
 
+// Forward DWT strips: one wave computes 60 output pairs across (64 lanes less the 2-pair
+// border on each side) and sp output pairs down (sp = 32, 16 or 8, per launch), streaming
+// sp + 4 input pairs; a block holds WEMA_STRIPS_PER_BLOCK strips of a job.
+#define WEMA_SP_MAX 32
+#define WEMA_SP_MIN 8
+#define WEMA_STRIPS_PER_BLOCK 4
+#ifndef WEMA_FWD_STRIPS
+#define WEMA_FWD_STRIPS 1
+#endif
+#ifndef WEMA_FWD_COLOUR_WAVE
+#define WEMA_FWD_COLOUR_WAVE 1
+#endif
+#ifndef WEMA_STRIP_TARGET
+#define WEMA_STRIP_TARGET 16384
+#endif
+
 // Guided filter tiles (outputs per block).
 #define WEMA_GF_TX 64
 #define WEMA_GF_TY 16
@@ -54,6 +70,7 @@ struct DwtfJob {
     int tiles_x;
     int tile_begin;
     int nch;
+    int sp;                         // strip height in output pairs
 };
 
 // Inverse job. The colour output job (k_dwt_inv0_ycc) synthesises level 0 of Y, Cb and Cr

@@ -105,14 +105,14 @@ __device__ __forceinline__ unsigned long long abs_fixed(float v) {
 }
 
 // The same without branches for |v| < 2^30: the integer part and the fraction are exact,
-// the fraction times 2^32 is exact, and below 2^23 adding 0.5 then truncating rounds half
-// up. Equal to abs_fixed for every float below 2^30 (checked exhaustively by
-// tests/q32_check.c); abs_fixed saturates from 2^30 on.
+// the fraction times 2^32 (x, below 2^32) is exact, and x rounded half up is its truncation
+// plus one where its fractional part is at least 0.5. Equal to abs_fixed for every float
+// below 2^30 (checked exhaustively by tests/q32_check.c); abs_fixed saturates from 2^30 on.
 __device__ __forceinline__ unsigned long long abs_q32_small(float v) {
     const float a = fabsf(v);
     const unsigned hi = (unsigned)a;
     const float x = (a - (float)hi) * 4294967296.0f;
-    const unsigned lo = x < 0.5f ? 0u : (unsigned)(x < 8388608.0f ? x + 0.5f : x);
+    const unsigned lo = (unsigned)x + (__builtin_amdgcn_fractf(x) >= 0.5f ? 1u : 0u);
     return ((unsigned long long)hi << 32) + lo;
 }
 
@@ -299,11 +299,65 @@ __device__ __forceinline__ void row_pass_t(float (*R)[128], const DwtfJob &J, in
     }
 }
 
+// Row pass of a tile that lies wholly inside the image, none of its rows the last one:
+// no reflection and no per-row conditions. Loads take a 32-bit offset from the source's
+// base (sources below 2^29 pixels, starting on a dword), and an RGB24 pair's byte alignment
+// is the same in every row of the wave (4 rows are 12·w bytes), so each row is one unconditional
+// 3-dword load (it ends at most 11 bytes after the pair's start, inside the next row).
+template <int KIND>
+__device__ __forceinline__ void row_pass_in(float (*R)[128], const DwtfJob &J, int sx0, int sy0,
+                                            int lane, int wave) {
+    constexpr int NR = 2 * WEMA_RP / 4;
+    const unsigned w = (unsigned)J.w;
+    const unsigned first = (unsigned)(sy0 + wave) * w + (unsigned)(sx0 + 2 * lane);
+    PairRaw raw[NR];
+    if (KIND == SRC_F32_PLANE) {
+        const GLOBAL_AS unsigned *base = (const GLOBAL_AS unsigned *)gp((const float *)J.src);
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const GLOBAL_AS unsigned *q = base + (first + (unsigned)i * 4u * w);
+            raw[i].d0 = q[0];
+            raw[i].d1 = q[1];
+            raw[i].d2 = 0u;
+            raw[i].sh = 0u;
+        }
+    } else {
+        const GLOBAL_AS unsigned char *base = gp((const unsigned char *)J.src);
+        const unsigned b0 = first * 3u, sh = b0 & 3u;
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const GLOBAL_AS unsigned *q =
+                (const GLOBAL_AS unsigned *)(base + (b0 - sh + (unsigned)i * 12u * w));
+            raw[i].d0 = q[0];
+            raw[i].d1 = q[1];
+            raw[i].d2 = q[2];
+            raw[i].sh = sh;
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < NR; g += 4) {
+        float e[4], o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pair_decode<KIND>(raw[g + k], e[k], o[k]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lift_fwd_lanes(e[k], o[k]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            R[wave + 4 * (g + k)][lane] = e[k] * WEMA_KINV;
+            R[wave + 4 * (g + k)][64 + lane] = o[k] * WEMA_K;
+        }
+    }
+}
+
 template <int KIND>
 __device__ __forceinline__ void row_pass(float (*R)[128], const DwtfJob &J, int sx0, int sy0,
                                          int lane, int wave) {
-    // (the coarser levels' float planes keep the general form: measured faster there)
-    if (KIND != SRC_F32_PLANE && sy0 >= 0 && sy0 + 2 * WEMA_RP < J.h)
+    const bool rows_in = sy0 >= 0 && sy0 + 2 * WEMA_RP < J.h;
+    if (KIND != SRC_F32_Y && KIND != SRC_F32_CB && KIND != SRC_F32_CR && rows_in && sx0 >= 0 &&
+        sx0 + 128 <= J.w && (long long)J.w * J.h < (1ll << 29) && ((size_t)J.src & 3) == 0)
+        row_pass_in<KIND>(R, J, sx0, sy0, lane, wave);
+    else if (KIND != SRC_F32_PLANE && rows_in)
+        // (the coarser levels' float planes keep the general form: measured faster there)
         row_pass_t<KIND, true>(R, J, sx0, sy0, lane, wave);
     else
         row_pass_t<KIND, false>(R, J, sx0, sy0, lane, wave);
@@ -368,8 +422,10 @@ __device__ __forceinline__ void dwt_fwd_body(float (*R)[128], const DwtfJob *__r
     __syncthreads();
 
     // ---- column pass: thread = (column, half); half 0 lifts pairs 0..17 and writes
-    // output pairs 2..15, half 1 lifts pairs 14..31 and writes 16..29
-    const int c = threadIdx.x & 127, half = threadIdx.x >> 7;
+    // output pairs 2..15, half 1 lifts pairs 14..31 and writes 16..29 (half and high are
+    // the same for the whole wave: taken as scalars)
+    const int c = threadIdx.x & 127;
+    const int half = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);
     const int k0 = half * (RP / 2 - 2);
     float e[HALF_PAIRS], o[HALF_PAIRS];
 #pragma unroll
@@ -386,7 +442,7 @@ __device__ __forceinline__ void dwt_fwd_body(float (*R)[128], const DwtfJob *__r
         }
     }
 
-    const bool high = c >= 64;
+    const bool high = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) & 1;   // c >= 64
     const int px = px0 + (c & 63) - 2;
     const bool col_ok = (c & 63) >= 2 && (c & 63) < 62 && px < sw;
     const int hw_valid = w >> 1, hh_valid = h >> 1;   // written extents (src/dwt.c:463-469)
@@ -397,26 +453,41 @@ __device__ __forceinline__ void dwt_fwd_body(float (*R)[128], const DwtfJob *__r
     const size_t pn = (size_t)w * h;                  // previous level's plane size
     unsigned long long part = 0ull;
     // tiles inside the written extents need neither bounds checks nor stale entries
-    const bool interior = px0 + WEMA_TILE_PAIRS_X <= hw_valid && py0 + WEMA_TILE_PAIRS_Y <= hh_valid;
+    const bool interior = px0 + WEMA_TILE_PAIRS_X <= hw_valid && py0 + WEMA_TILE_PAIRS_Y <= hh_valid &&
+                          n < (1ull << 29);
     // the wave's coefficients all below 2^30 in magnitude (and none NaN): the branch-free
-    // conversion applies
-    float mag = 0.0f;
+    // conversion applies. From RGB24 samples (in [0, 1]) every coefficient is below 64:
+    // each 1-D analysis at most multiplies the largest magnitude by 7.73 (the four lifting
+    // steps with |alpha|, |beta|, |gamma|, |delta| and the K scaling).
+    bool small = true;
+    if (!(RGB && J.src_kind >= SRC_U8_Y && J.src_kind <= SRC_U8_CR)) {
+        float mag = 0.0f;
 #pragma unroll
-    for (int k = 2; k < HALF_PAIRS - 2; ++k) mag += fabsf(e[k]) + fabsf(o[k]);
-    const bool small = __all(mag < 1073741824.0f);
+        for (int k = 2; k < HALF_PAIRS - 2; ++k) mag += fabsf(e[k]) + fabsf(o[k]);
+        small = __all(mag < 1073741824.0f);
+    }
     if (interior && small) {
         if ((c & 63) >= 2 && (c & 63) < 62) {
             // c < 64 (waves 0 and 2): LL and LH; c >= 64 (waves 1 and 3): HL and HH
-            GLOBAL_AS float *pe = (high ? HL : LL) + (size_t)(py0 + k0) * sw + px;
-            GLOBAL_AS float *po = (high ? HH : LH) + (size_t)(py0 + k0) * sw + px;
+            // byte offsets from the wave's (scalar) plane bases; a plane is below 2^31 bytes
+            GLOBAL_AS char *pe = (GLOBAL_AS char *)(high ? HL : LL);
+            GLOBAL_AS char *po = (GLOBAL_AS char *)(high ? HH : LH);
+            const unsigned off = 4u * ((unsigned)(py0 + k0) * (unsigned)sw + (unsigned)px);
+            const unsigned step = 4u * (unsigned)sw;
+            if (high) {
 #pragma unroll
-            for (int k = 2; k < HALF_PAIRS - 2; ++k) {
-                *pe = e[k];
-                *po = o[k];
-                part += abs_q32_small(o[k]);
-                if (high) part += abs_q32_small(e[k]);
-                pe += sw;
-                po += sw;
+                for (int k = 2; k < HALF_PAIRS - 2; ++k) {
+                    *(GLOBAL_AS float *)(pe + (off + (unsigned)(k - 2) * step)) = e[k];
+                    *(GLOBAL_AS float *)(po + (off + (unsigned)(k - 2) * step)) = o[k];
+                    part += abs_q32_small(o[k]) + abs_q32_small(e[k]);
+                }
+            } else {
+#pragma unroll
+                for (int k = 2; k < HALF_PAIRS - 2; ++k) {
+                    *(GLOBAL_AS float *)(pe + (off + (unsigned)(k - 2) * step)) = e[k];
+                    *(GLOBAL_AS float *)(po + (off + (unsigned)(k - 2) * step)) = o[k];
+                    part += abs_q32_small(o[k]);
+                }
             }
         }
     } else if (col_ok) {
@@ -460,6 +531,412 @@ __global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd0(const DwtfJob *__restr
 __global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restrict__ jobs, int njobs) {
     __shared__ float R[2 * WEMA_RP][128];
     dwt_fwd_body<false>(R, jobs, njobs);
+}
+
+// ---------------------------------------------------------------------------------------
+// Forward DWT level as strips (src/dwt.c:369-473 + 645-703).
+//
+// A wave computes output pairs [px0, px0 + 60) x [py0, py0 + sp) of one job, for all
+// of the job's channels (Y, Cb and Cr of an RGB frame are decoded from one load). Lane l
+// owns input pair column px0 - 2 + l, so the row lifting runs across lanes (DPP shifts) as
+// in the tile kernels. The wave streams input pair rows py0 - 2 .. py0 + sp + 1 from
+// top to bottom, and the column lifting runs as a pipeline in registers: when input pair m
+// arrives, steps 1 and 2 of pair m - 1 and steps 3 and 4 of pair m - 2 are computed, which
+// makes output pair m - 2 final. Each step is the same fused multiply-add of the same
+// operands as the in-place lifting (lift_fwd_arr), so the coefficients are bit-identical;
+// no LDS and no barrier are needed, and the loads of the next two input pairs are in
+// flight while a pair is computed. Samples outside the image come from the same
+// whole-sample symmetric extension as in the tile kernels.
+
+// Source families of a strip job.
+enum { FAM_U8_RGB = 0, FAM_F32_RGB = 1, FAM_F32_PLANE = 2 };
+
+// One input row's data for a lane's sample pair (xa, xb).
+template <int FAM> struct StripRaw;
+template <> struct StripRaw<FAM_U8_RGB> { unsigned d0, d1, d2, sh; };   // 6 bytes at byte sh of d0
+template <> struct StripRaw<FAM_F32_RGB> { float ra, ga, ba, rb, gb, bb; };
+template <> struct StripRaw<FAM_F32_PLANE> { float a, b; };
+
+// Input pairs a strip keeps in flight.
+#define SLOTS_OF(NCH) ((NCH) == 1 ? 4 : 2)
+
+// Per-lane constants of a strip.
+struct StripLane {
+    int xa, xb;        // reflected sample columns of the lane's pair
+    unsigned boff;     // fast path: byte offset of the pair in its row (RGB24: aligned down)
+    unsigned sh;       // fast path, RGB24: byte offset of the pair in its first dword
+    bool fast;         // the pair is two adjacent samples (no reflection)
+};
+
+// Loads row y. FAST: every lane's pair is two adjacent samples and its wide load stays
+// inside the source (uniform over the strip); the row's offset is then a 32-bit scalar.
+template <int FAM, bool FAST>
+__device__ __forceinline__ StripRaw<FAM> strip_load(const DwtfJob &J, const StripLane &L, int y) {
+    StripRaw<FAM> r;
+    const int w = J.w;
+    if (FAM == FAM_U8_RGB) {
+        unsigned *d = (unsigned *)&r;
+        if (FAST) {
+            const GLOBAL_AS unsigned char *row = gp((const unsigned char *)J.src) + (unsigned)y * (unsigned)(3 * w);
+            const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(row + L.boff);
+            d[0] = q[0];
+            d[1] = q[1];
+            d[2] = q[2];
+            d[3] = L.sh;
+        } else {
+            const GLOBAL_AS unsigned char *row = gp((const unsigned char *)J.src) + (size_t)y * w * 3;
+            if (L.fast && !(y == J.h - 1 && L.xa + 4 > w)) {
+                // (rows need not start on a dword: the alignment is taken per row)
+                const size_t a = (size_t)(row + (size_t)L.xa * 3);
+                const unsigned sh = (unsigned)(a & 3u);
+                const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(a - sh);
+                d[0] = q[0];
+                d[1] = q[1];
+                d[2] = q[2];
+                d[3] = sh;
+            } else {
+                // reflected columns, or the end of the frame: bytes one by one
+                const GLOBAL_AS unsigned char *pa = row + (size_t)L.xa * 3, *pb = row + (size_t)L.xb * 3;
+                d[0] = (unsigned)pa[0] | ((unsigned)pa[1] << 8) | ((unsigned)pa[2] << 16) |
+                       ((unsigned)pb[0] << 24);
+                d[1] = (unsigned)pb[1] | ((unsigned)pb[2] << 8);
+                d[2] = 0u;
+                d[3] = 0u;
+            }
+        }
+    } else if (FAM == FAM_F32_PLANE) {
+        float *d = (float *)&r;
+        if (FAST) {
+            const GLOBAL_AS float *row = gp((const float *)J.src) + (unsigned)y * (unsigned)w;
+            const GLOBAL_AS float *q = row + L.xa;
+            d[0] = q[0];
+            d[1] = q[1];
+        } else {
+            const GLOBAL_AS float *row = gp((const float *)J.src) + (size_t)y * w;
+            d[0] = row[L.xa];
+            d[1] = row[L.xb];
+        }
+    } else {
+        const GLOBAL_AS float *row = gp((const float *)J.src) + (size_t)y * w * 3;
+        const GLOBAL_AS float *pa = row + (size_t)L.xa * 3, *pb = row + (size_t)L.xb * 3;
+        float *d = (float *)&r;
+        d[0] = pa[0];
+        d[1] = pa[1];
+        d[2] = pa[2];
+        d[3] = pb[0];
+        d[4] = pb[1];
+        d[5] = pb[2];
+    }
+    return r;
+}
+
+// Float RGB of both samples of a row (RGB families).
+template <int FAM>
+__device__ __forceinline__ void strip_rgb(const StripRaw<FAM> &r, float (&a)[3], float (&b)[3]) {
+    if constexpr (FAM == FAM_U8_RGB) {
+        const unsigned v0 = __builtin_amdgcn_alignbyte(r.d1, r.d0, r.sh);
+        const unsigned v1 = __builtin_amdgcn_alignbyte(r.d2, r.d1, r.sh);
+        a[0] = u8f(v0 & 0xFFu);
+        a[1] = u8f((v0 >> 8) & 0xFFu);
+        a[2] = u8f((v0 >> 16) & 0xFFu);
+        b[0] = u8f(v0 >> 24);
+        b[1] = u8f(v1 & 0xFFu);
+        b[2] = u8f((v1 >> 8) & 0xFFu);
+    } else if constexpr (FAM == FAM_F32_RGB) {
+        a[0] = r.ra;
+        a[1] = r.ga;
+        a[2] = r.ba;
+        b[0] = r.rb;
+        b[1] = r.gb;
+        b[2] = r.bb;
+    }
+}
+
+// Channel c (0 = Y, 1 = Cb, 2 = Cr) of a float RGB sample.
+__device__ __forceinline__ float chan_idx(int c, const float (&p)[3]) {
+    return c == 0 ? chan_of<SRC_U8_Y>(p[0], p[1], p[2])
+                  : c == 1 ? chan_of<SRC_U8_CB>(p[0], p[1], p[2]) : chan_of<SRC_U8_CR>(p[0], p[1], p[2]);
+}
+
+// Column-lifting pipeline of one lane column: raw e, o of the last pair, o after step 1
+// and e after step 2 of the pair before, o after step 3 of the pair before that.
+struct VPipe {
+    float e, o, o1, e2, o3;
+};
+
+// Input pair m arrives (e, o); returns e after step 4 and o after step 3 of pair m - 2.
+__device__ __forceinline__ void vpipe_step(VPipe &p, float e, float o, float &e4, float &o3) {
+    const float o1 = fmaf(WEMA_ALPHA, p.e + e, p.o);       // o1[m-1]
+    const float e2 = fmaf(WEMA_BETA, p.o1 + o1, p.e);      // e2[m-1]
+    o3 = fmaf(WEMA_GAMMA, p.e2 + e2, p.o1);                 // o3[m-2]
+    e4 = fmaf(WEMA_DELTA, p.o3 + o3, p.e2);                 // e4[m-2]
+    p.e = e;
+    p.o = o;
+    p.o1 = o1;
+    p.e2 = e2;
+    p.o3 = o3;
+}
+
+// C0: colour channel of the job's first channel (RGB families).
+template <int FAM, int NCH, int C0, bool FAST, bool STALE>
+__device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int lane) {
+    const int SP = J.sp;
+    const int w = J.w, h = J.h;
+    const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
+    const int sx = strip % J.tiles_x, sy = strip / J.tiles_x;
+    const int px0 = sx * WEMA_TILE_PAIRS_X, py0 = sy * SP;
+    const int sx0 = 2 * (px0 - 2);
+
+    StripLane L;
+    L.xa = reflect_idx(sx0 + 2 * lane, w);
+    L.xb = reflect_idx(sx0 + 2 * lane + 1, w);
+    L.fast = L.xb == L.xa + 1;
+    if (FAM == FAM_U8_RGB) {   // (FAST strips: rows start on a dword)
+        const unsigned b0 = 3u * (unsigned)L.xa;
+        L.sh = b0 & 3u;
+        L.boff = b0 - L.sh;
+    } else {
+        L.sh = 0u;
+        L.boff = 0u;
+    }
+
+    const int hw_valid = w >> 1, hh_valid = h >> 1;   // written extents (src/dwt.c:463-469)
+    const size_t n = (size_t)sw * sh;
+    const size_t pn = (size_t)w * h;
+    const int px = px0 + lane - 2;
+    const bool col_ok = lane >= 2 && lane < 62 && px < sw;
+    const bool col_stale = px >= hw_valid;
+    const GLOBAL_AS float *prev = gp(J.prev);
+
+    VPipe pl[NCH], ph[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        pl[c] = VPipe{0.f, 0.f, 0.f, 0.f, 0.f};
+        ph[c] = VPipe{0.f, 0.f, 0.f, 0.f, 0.f};
+    }
+    unsigned long long part[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) part[c] = 0ull;
+
+    // output pair row py (uniform) of channel c from its four final values
+    auto emit = [&](int c, int py, float ll, float lh, float hl, float hh) {
+        // level 0 has no previous level (stale entries are 0); at the coarser levels only
+        // strips that reach past the written extents read the previous level's planes, and
+        // only lanes that write an output (px in [0, sw)) read them
+        if ((FAM != FAM_F32_PLANE || STALE) && col_ok) {
+            if (py >= hh_valid) {
+                lh = FAM == FAM_F32_PLANE ? prev[(size_t)py * sw + px] : 0.0f;
+                hh = FAM == FAM_F32_PLANE ? prev[pn + (size_t)py * sw + px] : 0.0f;
+            }
+            if (col_stale) {
+                hl = FAM == FAM_F32_PLANE ? prev[2 * pn + (size_t)py * sw + px] : 0.0f;
+                hh = FAM == FAM_F32_PLANE ? prev[pn + (size_t)py * sw + px] : 0.0f;
+            }
+        }
+        GLOBAL_AS float *LLc = gp(J.ll) + c * J.ll_cstride + (size_t)py * sw;
+        GLOBAL_AS float *LHc = gp(J.planes) + c * J.planes_cstride + (size_t)py * sw;
+        if (col_ok) {
+            LLc[px] = ll;
+            LHc[px] = lh;
+            LHc[n + px] = hh;
+            LHc[2 * n + px] = hl;
+        }
+        bool small = true;
+        if (FAM != FAM_U8_RGB)   // (RGB24 samples give coefficients below 64)
+            small = __all(!col_ok || fabsf(lh) + fabsf(hl) + fabsf(hh) < 1073741824.0f);
+        if (col_ok) {
+            if (small) part[c] += abs_q32_small(lh) + abs_q32_small(hl) + abs_q32_small(hh);
+            else part[c] += abs_fixed(lh) + abs_fixed(hl) + abs_fixed(hh);
+        }
+    };
+
+    // one input pair (rows ra, rb) through the row lifting and the column pipelines
+    auto step = [&](int m, bool out, const StripRaw<FAM> &ra, const StripRaw<FAM> &rb) {
+        float a0[3], b0[3], a1[3], b1[3];
+        if constexpr (FAM != FAM_F32_PLANE) {
+            strip_rgb<FAM>(ra, a0, b0);
+            strip_rgb<FAM>(rb, a1, b1);
+        }
+        const int py = py0 + m - 4;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            float e0, o0, e1, o1;
+            if constexpr (FAM == FAM_F32_PLANE) {
+                const float *fa = (const float *)&ra, *fb = (const float *)&rb;
+                e0 = fa[0];
+                o0 = fa[1];
+                e1 = fb[0];
+                o1 = fb[1];
+            } else {
+                e0 = chan_idx(C0 + c, a0);
+                o0 = chan_idx(C0 + c, b0);
+                e1 = chan_idx(C0 + c, a1);
+                o1 = chan_idx(C0 + c, b1);
+            }
+            float l0, g0, l1, g1;   // row-lifted L (low) and H (high) samples of both rows
+            if (FAST || w >= 2) {
+                lift_fwd_lanes(e0, o0);
+                lift_fwd_lanes(e1, o1);
+                l0 = e0 * WEMA_KINV;
+                g0 = o0 * WEMA_K;
+                l1 = e1 * WEMA_KINV;
+                g1 = o1 * WEMA_K;
+            } else {
+                // n < 2: the row transform is the identity (src/dwt.c:46)
+                l0 = g0 = e0;
+                l1 = g1 = e1;
+            }
+            float le4, lo3, he4, ho3;
+            vpipe_step(pl[c], l0, l1, le4, lo3);
+            vpipe_step(ph[c], g0, g1, he4, ho3);
+            if (out) emit(c, py, le4 * WEMA_KINV, lo3 * WEMA_K, he4 * WEMA_KINV, ho3 * WEMA_K);
+        }
+    };
+
+    if (h < 2) {
+        // one output pair row: the column transform is the identity (src/dwt.c:46), and
+        // both rows of the pair are row 0
+        const StripRaw<FAM> r = strip_load<FAM, FAST>(J, L, 0);
+        float a0[3], b0[3];
+        if constexpr (FAM != FAM_F32_PLANE) strip_rgb<FAM>(r, a0, b0);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            float e0, o0;
+            if constexpr (FAM == FAM_F32_PLANE) {
+                e0 = ((const float *)&r)[0];
+                o0 = ((const float *)&r)[1];
+            } else {
+                e0 = chan_idx(C0 + c, a0);
+                o0 = chan_idx(C0 + c, b0);
+            }
+            float l0 = e0, g0 = e0;
+            if (w >= 2) {
+                lift_fwd_lanes(e0, o0);
+                l0 = e0 * WEMA_KINV;
+                g0 = o0 * WEMA_K;
+            }
+            emit(c, 0, l0, l0, g0, g0);
+        }
+    } else {
+        // input pairs py0 - 2 + m for m < M; output pair rows py0 .. py0 + M - 5
+        const int M = min(SP, sh - py0) + 4;
+        auto load = [&](int m, int r) {
+            int y = 2 * (py0 - 2 + m) + r;
+            // (FAST strips: h >= 8, so one reflection reaches every row a strip reads)
+            y = FAST ? (y < 0 ? -y : y >= h ? 2 * h - 2 - y : y) : reflect_idx(y, h);
+            return strip_load<FAM, FAST>(J, L, y);
+        };
+        // SLOTS input pairs in flight: a slot is reloaded right after its pair is used, so
+        // its registers are rewritten in place while the other slots' pairs are computed.
+        // Pairs 0-3 only fill the column pipelines. FAST strips have M >= 4 + 2 * SLOTS, so
+        // the loads and steps of the warm-up and of the main loop need no conditions (which
+        // keeps the loads in flight across iterations); the tail takes the rest.
+        auto reload = [&](StripRaw<FAM> &a, StripRaw<FAM> &b, int m) {
+            a = load(m, 0);
+            b = load(m, 1);
+        };
+        StripRaw<FAM> sa[SLOTS_OF(NCH)], sb[SLOTS_OF(NCH)];
+        constexpr int SLOTS = SLOTS_OF(NCH);
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) reload(sa[k], sb[k], k);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            step(m, false, sa[m % SLOTS], sb[m % SLOTS]);
+            if (FAST || m + SLOTS < M) reload(sa[m % SLOTS], sb[m % SLOTS], m + SLOTS);
+        }
+        int m = 4;
+        if (FAST) {
+            for (; m + 2 * SLOTS <= M; m += SLOTS) {
+#pragma unroll
+                for (int k = 0; k < SLOTS; ++k) {
+                    step(m + k, true, sa[k], sb[k]);
+                    reload(sa[k], sb[k], m + k + SLOTS);
+                }
+            }
+        }
+        for (; m < M; m += SLOTS) {
+#pragma unroll
+            for (int k = 0; k < SLOTS; ++k) {
+                if (m + k < M) {
+                    step(m + k, true, sa[k], sb[k]);
+                    if (m + k + SLOTS < M) reload(sa[k], sb[k], m + k + SLOTS);
+                }
+            }
+        }
+    }
+    if (J.acc) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const unsigned long long s = wave_sum_u64(part[c]);
+            if (lane == 0 && s) atomicAdd(J.acc + c, s);
+        }
+    }
+}
+
+template <int FAM, int NCH, int C0>
+__device__ __forceinline__ void dwt_fwd_strip(const DwtfJob &J, int strip, int lane) {
+    const int w = J.w, h = J.h, sh = (h + 1) >> 1;
+    const int sx0 = 2 * ((strip % J.tiles_x) * WEMA_TILE_PAIRS_X - 2);
+    const int py0 = (strip / J.tiles_x) * J.sp;
+    if constexpr (FAM == FAM_F32_RGB) {
+        // (float RGB frames: the general loads only)
+        dwt_fwd_strip_t<FAM, NCH, C0, false, false>(J, strip, lane);
+    } else if constexpr (FAM == FAM_F32_PLANE) {
+        // every lane's pair adjacent and inside the row, sources below 2^29 samples (32-bit
+        // row offsets); strips that reach the unwritten last column or row of an odd-sized
+        // parent take those entries from the previous level
+        const int px0 = (strip % J.tiles_x) * WEMA_TILE_PAIRS_X;
+        const bool stale = px0 + WEMA_TILE_PAIRS_X > (w >> 1) || py0 + J.sp > (h >> 1);
+        const bool fast = sx0 >= 0 && sx0 + 128 <= w && h >= 8 && (long long)w * h < (1ll << 29) &&
+                          min(J.sp, sh - py0) >= 2 * SLOTS_OF(NCH);
+        if (stale) dwt_fwd_strip_t<FAM, NCH, C0, false, true>(J, strip, lane);
+        else if (fast) dwt_fwd_strip_t<FAM, NCH, C0, true, false>(J, strip, lane);
+        else dwt_fwd_strip_t<FAM, NCH, C0, false, false>(J, strip, lane);
+    } else {
+        // RGB24 as above, and rows that start on a dword (frame aligned, width a multiple
+        // of 4); the 3-dword loads end at most 6 bytes past the pair, so the strip must not
+        // reach the frame's last bytes
+        const int last_row = 2 * (min(J.sp, sh - py0) + py0 + 1) + 1;   // last input row (unreflected)
+        const bool fast = sx0 >= 0 && sx0 + 128 <= w && h >= 8 && (long long)w * h < (1ll << 29) &&
+                          min(J.sp, sh - py0) >= 2 * SLOTS_OF(NCH) && (((size_t)J.src | (size_t)w) & 3) == 0 &&
+                          (sx0 + 132 <= w || last_row < h - 1);
+        if (fast) dwt_fwd_strip_t<FAM, NCH, C0, true, false>(J, strip, lane);
+        else dwt_fwd_strip_t<FAM, NCH, C0, false, false>(J, strip, lane);
+    }
+}
+
+// Block = WEMA_STRIPS_PER_BLOCK waves, each an independent strip of the block's job.
+template <bool LEVEL0>
+__device__ __forceinline__ void dwt_fwd_strips(const DwtfJob *__restrict__ jobs, int njobs) {
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int jid = find_job(jobs, njobs, bid);
+    const DwtfJob J = jobs[jid];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int strip = (bid - J.tile_begin) * WEMA_STRIPS_PER_BLOCK + wave;
+    if (strip >= J.tiles_x * wema_ceil_div((J.h + 1) >> 1, J.sp)) return;
+    if (!LEVEL0) {
+        dwt_fwd_strip<FAM_F32_PLANE, 1, 0>(J, strip, lane);
+    } else if (J.nch == 3) {
+        if (J.src_kind == SRC_U8_Y) dwt_fwd_strip<FAM_U8_RGB, 3, 0>(J, strip, lane);
+        else dwt_fwd_strip<FAM_F32_RGB, 3, 0>(J, strip, lane);
+    } else {
+        switch (J.src_kind) {
+        case SRC_U8_Y: dwt_fwd_strip<FAM_U8_RGB, 1, 0>(J, strip, lane); break;
+        case SRC_U8_CB: dwt_fwd_strip<FAM_U8_RGB, 1, 1>(J, strip, lane); break;
+        case SRC_U8_CR: dwt_fwd_strip<FAM_U8_RGB, 1, 2>(J, strip, lane); break;
+        case SRC_F32_Y: dwt_fwd_strip<FAM_F32_RGB, 1, 0>(J, strip, lane); break;
+        case SRC_F32_CB: dwt_fwd_strip<FAM_F32_RGB, 1, 1>(J, strip, lane); break;
+        default: dwt_fwd_strip<FAM_F32_RGB, 1, 2>(J, strip, lane); break;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_fwd0s(const DwtfJob *__restrict__ jobs, int njobs) {
+    dwt_fwd_strips<true>(jobs, njobs);
+}
+__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_fwds(const DwtfJob *__restrict__ jobs, int njobs) {
+    dwt_fwd_strips<false>(jobs, njobs);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1464,8 +1941,13 @@ __global__ void __launch_bounds__(256) k_synth(const SynthFrame *__restrict__ fr
 extern "C" {
 hipError_t wema_launch_dwt_fwd(const DwtfJob *jobs, int njobs, int nblocks, int level0, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
+#if WEMA_FWD_STRIPS
+    if (level0) hipLaunchKernelGGL(k_dwt_fwd0s, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
+    else hipLaunchKernelGGL(k_dwt_fwds, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
+#else
     if (level0) hipLaunchKernelGGL(k_dwt_fwd0, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
     else hipLaunchKernelGGL(k_dwt_fwd, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
+#endif
     return hipGetLastError();
 }
 hipError_t wema_launch_dwt_inv(const DwtiJob *jobs, int njobs, int nblocks, int level0, hipStream_t s) {
