@@ -155,30 +155,34 @@ struct JobBuilder {
         off += bytes;
         return true;
     }
-    // Strip height of each forward level, from all jobs of its launch: the tallest of 32, 16
-    // and 8 output pairs that still gives at least WEMA_STRIP_TARGET waves (shorter strips
-    // re-read more halo rows); block ranges follow.
+    // Strip height of each DWT launch, from all of its jobs: the tallest of 32, 16 and 8
+    // pair rows that still gives at least WEMA_STRIP_TARGET waves (shorter strips re-read
+    // more halo rows); block ranges follow.
+    template <typename Job>
+    static void strips_of(std::vector<Job> &v, JobList &jl) {
+        if (v.empty()) return;
+        auto strips = [&](const Job &j, int sp) { return (long)j.tiles_x * wema_ceil_div((j.h + 1) >> 1, sp); };
+        int sp = WEMA_SP_MAX;
+        for (; sp > WEMA_SP_MIN; sp /= 2) {
+            long waves = 0;
+            for (const auto &j : v) waves += strips(j, sp);
+            if (waves >= WEMA_STRIP_TARGET) break;
+        }
+        int blocks = 0;
+        for (auto &j : v) {
+            j.sp = sp;
+            j.tile_begin = blocks;
+            blocks += wema_ceil_div((int)strips(j, sp), WEMA_STRIPS_PER_BLOCK);
+        }
+        jl.blocks = blocks;
+    }
     void assign_strips() {
 #if WEMA_FWD_STRIPS
-        for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
-            if (dwtf_v[l].empty()) continue;
-            auto strips = [&](const DwtfJob &j, int sp) {
-                return (long)j.tiles_x * wema_ceil_div((j.h + 1) >> 1, sp);
-            };
-            int sp = WEMA_SP_MAX;
-            for (; sp > WEMA_SP_MIN; sp /= 2) {
-                long waves = 0;
-                for (const auto &j : dwtf_v[l]) waves += strips(j, sp);
-                if (waves >= WEMA_STRIP_TARGET) break;
-            }
-            int blocks = 0;
-            for (auto &j : dwtf_v[l]) {
-                j.sp = sp;
-                j.tile_begin = blocks;
-                blocks += wema_ceil_div((int)strips(j, sp), WEMA_STRIPS_PER_BLOCK);
-            }
-            dwtf[l].blocks = blocks;
-        }
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++) strips_of(dwtf_v[l], dwtf[l]);
+#endif
+#if WEMA_INV_STRIPS
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++) strips_of(dwti_v[l], dwti[l]);
+        strips_of(dwti_ycc_v, dwti_ycc);
 #endif
     }
     int pack(void *dst_v, size_t cap, size_t *used) {

@@ -33,6 +33,9 @@
 #ifndef WEMA_FWD_STRIPS
 #define WEMA_FWD_STRIPS 1
 #endif
+#ifndef WEMA_INV_STRIPS
+#define WEMA_INV_STRIPS 1
+#endif
 #ifndef WEMA_FWD_COLOUR_WAVE
 #define WEMA_FWD_COLOUR_WAVE 1
 #endif
@@ -88,7 +91,7 @@ struct DwtiJob {
     int tiles_x;
     int tile_begin;
     int nch;
-    int pad_;
+    int sp;                         // strip height in coefficient pair rows
 };
 
 #define WEMA_MAX_BATCH 256

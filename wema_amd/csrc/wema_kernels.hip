@@ -1122,6 +1122,246 @@ __global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv0_ycc(const DwtiJob *__r
 }
 
 // ---------------------------------------------------------------------------------------
+// Inverse DWT level as strips (src/dwt.c:475-582 + 705-767).
+//
+// A wave reconstructs image columns 2 px0 .. 2 px0 + 119 and rows 2 py0 .. 2 (py0 + sp) - 1
+// of one job (all channels of a colour job). Lane l owns coefficient column
+// I = px0 - 2 + l of the packed [LL | HL ; LH | HH] layout: its low column (LL over LH) and
+// its high column (HL over HH). The wave streams coefficient pair rows py0 - 2 ..
+// py0 + sp + 1 from top to bottom through two column pipelines per channel: when pair m
+// arrives, step 1 of pair m, steps 2 and 3 of pair m - 1 and step 4 of pair m - 2 are
+// computed (the steps of lift_inv_arr with the same operands, so the samples are
+// bit-identical), which makes image rows 2 (m - 2) and 2 (m - 2) + 1 final. Each such row
+// then goes through the row synthesis across lanes (DPP shifts) and is stored. Coefficients
+// outside the subbands come from the same whole-sample symmetric extension as in the tile
+// kernels; every load is in bounds.
+
+// Column-synthesis pipeline of one lane column: o (K-scaled) of the last pair, e after
+// step 1 of the last pair, o after step 2 and e after step 3 of the pair before.
+struct IPipe {
+    float o, e1, o2, e3;
+};
+
+// Pair m arrives (e, o scaled); returns the even-row sample (e after step 3) and the
+// odd-row sample (o after step 4) of pair m - 2.
+__device__ __forceinline__ void ipipe_step(IPipe &p, float e, float o, float &ev, float &od) {
+    const float e1 = fmaf(-WEMA_DELTA, p.o + o, e);        // e1[m]
+    const float o2 = fmaf(-WEMA_GAMMA, p.e1 + e1, p.o);    // o2[m-1]
+    const float e3 = fmaf(-WEMA_BETA, p.o2 + o2, p.e1);    // e3[m-1]
+    od = fmaf(-WEMA_ALPHA, p.e3 + e3, p.o2);                // o4[m-2]
+    ev = p.e3;                                              // e3[m-2]
+    p.o = o;
+    p.e1 = e1;
+    p.o2 = o2;
+    p.e3 = e3;
+}
+
+// One coefficient pair row of every channel for a lane: LL, LH (low column) and HL, HH
+// (high column).
+template <int NCH>
+struct IStripRaw {
+    float ll[NCH], lh[NCH], hl[NCH], hh[NCH];
+};
+
+#define ISLOTS_OF(NCH) ((NCH) == 1 ? 4 : 2)
+
+// OUT: 0 = one float plane (J.out), 1 = colour egress (RGB24 and/or float RGB).
+template <int NCH, int OUT, bool FAST>
+__device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int lane) {
+    const int SP = J.sp;
+    const int w = J.w, h = J.h;
+    const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
+    const size_t n = (size_t)sw * sh;
+    const int px0 = (strip % J.tiles_x) * WEMA_TILE_PAIRS_X, py0 = (strip / J.tiles_x) * SP;
+    const int I = px0 - 2 + lane;
+    // coefficient columns of the lane's low and high columns (FAST: I itself)
+    const unsigned colL = FAST ? (unsigned)I : (unsigned)(reflect_idx(2 * I, w) >> 1);
+    const unsigned colH = FAST ? (unsigned)I : (unsigned)(reflect_idx(2 * I + 1, w) >> 1);
+    const int x = 2 * I;
+    const bool lane_ok = lane >= 2 && lane < 62 && x < w;
+    const bool two = x + 1 < w;
+    const bool vert = h >= 2, horz = w >= 2, even_w = (w & 1) == 0;
+
+    IPipe pl[NCH], ph[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        pl[c] = IPipe{0.f, 0.f, 0.f, 0.f};
+        ph[c] = IPipe{0.f, 0.f, 0.f, 0.f};
+    }
+
+    // coefficient pair row q (unextended) of every channel
+    auto load = [&](int q) {
+        IStripRaw<NCH> r;
+        int ye2 = 2 * q, yo2 = 2 * q + 1;
+        // (FAST strips: h >= 8, so one reflection reaches every row a strip reads)
+        ye2 = FAST ? (ye2 < 0 ? -ye2 : ye2 >= h ? 2 * h - 2 - ye2 : ye2) : reflect_idx(ye2, h);
+        yo2 = FAST ? (yo2 < 0 ? -yo2 : yo2 >= h ? 2 * h - 2 - yo2 : yo2) : reflect_idx(yo2, h);
+        const unsigned re = (unsigned)(ye2 >> 1) * (unsigned)sw, ro = (unsigned)(yo2 >> 1) * (unsigned)sw;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const GLOBAL_AS float *LL = gp(J.ll) + c * J.ll_cstride;
+            const GLOBAL_AS float *LH = gp(J.planes) + c * J.planes_cstride;
+            r.ll[c] = LL[re + colL];
+            r.lh[c] = LH[ro + colL];
+            r.hl[c] = LH[2 * n + re + colH];
+            r.hh[c] = LH[n + ro + colH];
+        }
+        return r;
+    };
+
+    // image row y (uniform) of every channel: row synthesis and store
+    auto put_row = [&](int y, const float (&lo)[NCH], const float (&hi)[NCH]) {
+        float ev[NCH], ov[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            ev[c] = lo[c];
+            ov[c] = hi[c];
+            if (FAST || horz) {
+                ev[c] *= WEMA_K;
+                ov[c] *= WEMA_KINV;
+                lift_inv_lanes(ev[c], ov[c]);
+            }
+        }
+        if (!lane_ok || y >= h) return;
+        if constexpr (OUT == 0) {
+            GLOBAL_AS float *row = gp(J.out) + (size_t)y * w;
+            if (even_w && two) {
+                // (even width: the pair starts on 8 bytes)
+                typedef float v2f __attribute__((ext_vector_type(2)));
+                *(GLOBAL_AS v2f *)(row + x) = v2f{ev[0], ov[0]};
+            } else {
+                row[x] = ev[0];
+                if (two) row[x + 1] = ov[0];
+            }
+        } else {
+            float v0[3], v1[3];
+            ycc_px(ev[0], ev[1], ev[2], v0);
+            ycc_px(ov[0], ov[1], ov[2], v1);
+            const size_t idx = ((size_t)y * w + x) * 3;
+            if (J.out_u8) {
+                GLOBAL_AS unsigned char *o = gp(J.out_u8) + idx;
+                const unsigned b0 = egress_u8(v0[0]), b1 = egress_u8(v0[1]), b2 = egress_u8(v0[2]);
+                const size_t a = (size_t)o;
+                if (two && (a & 1) == 0) {
+                    // the 6 bytes as one dword and one short, both aligned: bytes 0-3 and
+                    // 4-5 when the pair starts on a dword, else bytes 0-1 and 2-5
+                    const unsigned lo = b0 | (b1 << 8) | (b2 << 16) | ((unsigned)egress_u8(v1[0]) << 24);
+                    const unsigned hi = (unsigned)egress_u8(v1[1]) | ((unsigned)egress_u8(v1[2]) << 8);
+                    const bool d0 = (a & 2) == 0;
+                    const unsigned dw = d0 ? lo : (lo >> 16) | (hi << 16);
+                    const unsigned short sv = (unsigned short)(d0 ? hi : lo);
+                    *(GLOBAL_AS unsigned *)(a + (d0 ? 0 : 2)) = dw;
+                    *(GLOBAL_AS unsigned short *)(a + (d0 ? 4 : 0)) = sv;
+                } else {
+                    o[0] = b0;
+                    o[1] = b1;
+                    o[2] = b2;
+                    if (two) {
+                        o[3] = egress_u8(v1[0]);
+                        o[4] = egress_u8(v1[1]);
+                        o[5] = egress_u8(v1[2]);
+                    }
+                }
+            }
+            if (J.out_f32) {
+                GLOBAL_AS float *o = gp(J.out_f32) + idx;
+                o[0] = v0[0];
+                o[1] = v0[1];
+                o[2] = v0[2];
+                if (two) {
+                    o[3] = v1[0];
+                    o[4] = v1[1];
+                    o[5] = v1[2];
+                }
+            }
+        }
+    };
+
+    // one coefficient pair row through the column pipelines; out: store pair m - 2
+    auto step = [&](int m, bool out, const IStripRaw<NCH> &r) {
+        float le[NCH], lo[NCH], he[NCH], ho[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            ipipe_step(pl[c], r.ll[c] * WEMA_K, r.lh[c] * WEMA_KINV, le[c], lo[c]);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            ipipe_step(ph[c], r.hl[c] * WEMA_K, r.hh[c] * WEMA_KINV, he[c], ho[c]);
+        if (out) {
+            const int y = 2 * (py0 + m - 4);
+            put_row(y, le, he);
+            put_row(y + 1, lo, ho);
+        }
+    };
+
+    if (!vert) {
+        // one coefficient row: the column synthesis is the identity (src/dwt.c:46), and
+        // image row 0 is the even sample
+        const IStripRaw<NCH> r = load(0);
+        put_row(0, r.ll, r.hl);
+        return;
+    }
+    const int M = min(SP, sh - py0) + 4;
+    constexpr int SLOTS = ISLOTS_OF(NCH);
+    IStripRaw<NCH> s[SLOTS];
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) s[k] = load(py0 - 2 + k);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        step(m, false, s[m % SLOTS]);
+        if (FAST || m + SLOTS < M) s[m % SLOTS] = load(py0 - 2 + m + SLOTS);
+    }
+    int m = 4;
+    if (FAST) {
+        for (; m + 2 * SLOTS <= M; m += SLOTS) {
+#pragma unroll
+            for (int k = 0; k < SLOTS; ++k) {
+                step(m + k, true, s[k]);
+                s[k] = load(py0 - 2 + m + k + SLOTS);
+            }
+        }
+    }
+    for (; m < M; m += SLOTS) {
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+            if (m + k < M) {
+                step(m + k, true, s[k]);
+                if (m + k + SLOTS < M) s[k] = load(py0 - 2 + m + k + SLOTS);
+            }
+        }
+    }
+}
+
+template <int NCH, int OUT>
+__device__ __forceinline__ void dwt_inv_strips(const DwtiJob *__restrict__ jobs, int njobs) {
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int jid = find_job(jobs, njobs, bid);
+    const DwtiJob &J = jobs[jid];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int strip = (bid - J.tile_begin) * WEMA_STRIPS_PER_BLOCK + wave;
+    if (strip == 0 && lane < J.nch && J.acc_clear) J.acc_clear[lane] = 0ull;
+    const int w = J.w, h = J.h, sw = (w + 1) >> 1, sh = (h + 1) >> 1;
+    if (strip >= J.tiles_x * wema_ceil_div(sh, J.sp)) return;
+    const int px0 = (strip % J.tiles_x) * WEMA_TILE_PAIRS_X, py0 = (strip / J.tiles_x) * J.sp;
+    // every lane's columns unreflected (I in [0, sw) with 2 I + 1 < w), one reflection
+    // reaches every row, full-length strips, 32-bit plane offsets
+    const bool fast = px0 >= 2 && 2 * (px0 + 61) + 1 < w && h >= 8 &&
+                      min(J.sp, sh - py0) >= 2 * ISLOTS_OF(NCH) && (long long)sw * sh < (1ll << 28);
+    if (fast) dwt_inv_strip_t<NCH, OUT, true>(J, strip, lane);
+    else dwt_inv_strip_t<NCH, OUT, false>(J, strip, lane);
+}
+
+__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv0s(const DwtiJob *__restrict__ jobs, int njobs) {
+    dwt_inv_strips<1, 0>(jobs, njobs);
+}
+__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_invs(const DwtiJob *__restrict__ jobs, int njobs) {
+    dwt_inv_strips<1, 0>(jobs, njobs);
+}
+__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv0_yccs(const DwtiJob *__restrict__ jobs, int njobs) {
+    dwt_inv_strips<3, 1>(jobs, njobs);
+}
+
+// ---------------------------------------------------------------------------------------
 // IIR band-pass + amplification (src/iir_filter.c:181-264, 266-370, 376-471;
 // src/wema.c:714-744, 783-800, 830-848; src/wema_batch.c:268-297, 420-440).
 __device__ __forceinline__ float biquad2(const float *c, float4 &st, float x) {
@@ -1952,13 +2192,22 @@ hipError_t wema_launch_dwt_fwd(const DwtfJob *jobs, int njobs, int nblocks, int 
 }
 hipError_t wema_launch_dwt_inv(const DwtiJob *jobs, int njobs, int nblocks, int level0, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
+#if WEMA_INV_STRIPS
+    if (level0) hipLaunchKernelGGL(k_dwt_inv0s, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
+    else hipLaunchKernelGGL(k_dwt_invs, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
+#else
     if (level0) hipLaunchKernelGGL(k_dwt_inv0, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
     else hipLaunchKernelGGL(k_dwt_inv, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
+#endif
     return hipGetLastError();
 }
 hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *jobs, int njobs, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
+#if WEMA_INV_STRIPS
+    hipLaunchKernelGGL(k_dwt_inv0_yccs, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
+#else
     hipLaunchKernelGGL(k_dwt_inv0_ycc, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
+#endif
     return hipGetLastError();
 }
 hipError_t wema_launch_iir(const TemporalJob *jobs, int njobs, int nblocks, hipStream_t s) {
