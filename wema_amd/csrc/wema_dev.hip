@@ -400,24 +400,16 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
     for (int l = 0; l < L; l++) {
         const int w = g.lw[l], h = g.lh[l];
         const int tiles_x = wema_ceil_div(g.lw[l + 1], WEMA_TILE_PAIRS_X);
-#if WEMA_FWD_STRIPS
         // level 0: one job per frame for all channels (they share the RGB source; each wave
         // computes its strip of every channel); strip heights and block ranges are set when
         // the launch's table is packed
-        const int jch = l == 0 && WEMA_FWD_COLOUR_WAVE ? nch : 1;
-        const int tiles_per_job = 0;
-#else
-        // level 0: one job per frame for all channels (they share the RGB source)
         const int jch = l == 0 ? nch : 1;
-        const int tiles = tiles_x * wema_ceil_div(g.lh[l + 1], WEMA_TILE_PAIRS_Y);
-        const int tiles_per_job = tiles * jch;
-#endif
         for (int f = 0; f < n; f++)
             for (int c = 0; c < nch; c += jch) {
                 DwtfJob j{};
                 if (l == 0) {
                     j.src = (const char *)in + (size_t)f * in_stride;
-                    j.src_kind = (in_f32 ? SRC_F32_Y : SRC_U8_Y) + (jch == 1 ? c : 0);
+                    j.src_kind = in_f32 ? SRC_F32_Y : SRC_U8_Y;
                     j.prev = nullptr;
                 } else {
                     j.src = ll_ptr(f, c, l - 1);
@@ -435,9 +427,9 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.tiles_x = tiles_x;
                 // input read once (RGB24 shared by the channels at level 0) + LL and three
                 // subbands written per channel
-                const double in_b = l == 0 ? (c == 0 ? 3.0 * w * h * (in_f32 ? 4 : 1) : 0.0) : 4.0 * w * h;
+                const double in_b = l == 0 ? 3.0 * w * h * (in_f32 ? 4 : 1) : 4.0 * w * h;
                 const double out_b = 16.0 * g.lw[l + 1] * g.lh[l + 1] * jch;
-                jb.add_dwtf(l, j, tiles_per_job, in_b + out_b);
+                jb.add_dwtf(l, j, 0, in_b + out_b);
             }
     }
 
@@ -500,7 +492,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
     for (int l = L - 1; l >= 0; l--) {
         const int w = g.lw[l], h = g.lh[l];
         const int tiles_x = wema_ceil_div(g.lw[l + 1], WEMA_TILE_PAIRS_X);
-        const int tiles = tiles_x * wema_ceil_div(g.lh[l + 1], WEMA_TILE_PAIRS_Y);
+        const int tiles = 0;   // (strip heights and block ranges are set when the table is packed)
         const double in_b = 16.0 * g.lw[l + 1] * g.lh[l + 1];
         for (int f = 0; f < n; f++) {
             if (l == 0 && p.color && !passthrough(f)) {

@@ -177,13 +177,9 @@ struct JobBuilder {
         jl.blocks = blocks;
     }
     void assign_strips() {
-#if WEMA_FWD_STRIPS
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) strips_of(dwtf_v[l], dwtf[l]);
-#endif
-#if WEMA_INV_STRIPS
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) strips_of(dwti_v[l], dwti[l]);
         strips_of(dwti_ycc_v, dwti_ycc);
-#endif
     }
     int pack(void *dst_v, size_t cap, size_t *used) {
         assign_strips();

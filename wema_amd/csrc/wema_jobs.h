@@ -18,27 +18,13 @@
 #define WEMA_K (1.149604398860241f)
 #define WEMA_KINV (1.0f / WEMA_K)
 
-// DWT tiles: 64 pairs wide (60 valid outputs + 2 halo pairs per side),
-// WEMA_RP pairs tall (WEMA_RP - 4 valid).
+// DWT strips: one wave computes WEMA_TILE_PAIRS_X = 60 pair columns (64 lanes less the
+// 2-pair border on each side) and sp pair rows (sp = 32, 16 or 8, per launch) of a level,
+// streaming sp + 4 pair rows; a block holds WEMA_STRIPS_PER_BLOCK strips of a job.
 #define WEMA_TILE_PAIRS_X 60
-#define WEMA_RP 32
-#define WEMA_TILE_PAIRS_Y (WEMA_RP - 4)
-
-// Forward DWT strips: one wave computes 60 output pairs across (64 lanes less the 2-pair
-// border on each side) and sp output pairs down (sp = 32, 16 or 8, per launch), streaming
-// sp + 4 input pairs; a block holds WEMA_STRIPS_PER_BLOCK strips of a job.
 #define WEMA_SP_MAX 32
 #define WEMA_SP_MIN 8
 #define WEMA_STRIPS_PER_BLOCK 4
-#ifndef WEMA_FWD_STRIPS
-#define WEMA_FWD_STRIPS 1
-#endif
-#ifndef WEMA_INV_STRIPS
-#define WEMA_INV_STRIPS 1
-#endif
-#ifndef WEMA_FWD_COLOUR_WAVE
-#define WEMA_FWD_COLOUR_WAVE 1
-#endif
 #ifndef WEMA_STRIP_TARGET
 #define WEMA_STRIP_TARGET 16384
 #endif
@@ -59,8 +45,8 @@ enum SrcKind {
 
 // A forward job covers nch channels of one frame that share the source (level 0 of the
 // colour mode: Y, Cb, Cr from one RGB frame); channel c uses ll + c * ll_cstride,
-// planes + c * planes_cstride, acc + c and source kind src_kind + c. Its blocks take the
-// channels of one tile in turn, so the source tile is re-read from L2.
+// planes + c * planes_cstride and acc + c. Each wave computes its strip of every channel
+// from one load of the source.
 struct DwtfJob {
     const void *src;                // level 0: RGB frame; else previous level's LL (w x h)
     float *ll;                      // LL of this level (sw x sh), channel 0

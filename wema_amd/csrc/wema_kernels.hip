@@ -6,15 +6,17 @@
 // so that DWT, IIR and amplification results are bit-identical to it.
 //
 // Stages (one launch per stage over a job table, see wema_jobs.h):
-//   k_dwt_fwd0/k_dwt_fwd  one 2D CDF 9/7 analysis level (level 0 / coarser levels): row lifting in registers across the 64
-//               lanes of a wave (neighbours by lane shuffles), column lifting per lane in
-//               registers, LDS tile in between; level 0 converts RGB24 to Y/Cb/Cr on load;
-//               writes LL + LH/HH/HL and adds sum|c| (Q32 integers) to the frame's counter.
+//   k_dwt_fwd0/k_dwt_fwd  one 2D CDF 9/7 analysis level (level 0 / coarser levels) as
+//               strips: a wave streams a 60-pair-wide column strip top to bottom, row
+//               lifting across its 64 lanes (DPP lane shifts), column lifting as a register
+//               pipeline; level 0 converts RGB24 to Y (and Cb, Cr) on load; writes LL +
+//               LH/HH/HL and adds sum|c| (Q32 integers) to the frame's counter.
 //   k_iir       Butterworth HP+LP biquads (DF2T) per coefficient over the job's frames with
 //               the state kept in registers, warm-up preseed, threshold mask, amplification.
 //   k_win       sliding-window temporal filter (mean removal, optional bilateral weights)
 //               + amplification.
-//   k_dwt_inv   one 2D synthesis level (columns, then rows); k_dwt_inv0 level 0.
+//   k_dwt_inv   one 2D synthesis level as strips (column synthesis as a register
+//               pipeline, then rows across lanes); k_dwt_inv0 level 0.
 //   k_dwt_inv0_ycc  colour mode: level 0 of Y, Cb, Cr + YCbCr -> RGB24 in one pass.
 //   k_gf1/k_gf2 guided filter (r = 4, eps = 0.01) of the luma delta with 9x9 clipped box
 //               means in LDS; k_gf2 adds the smoothed delta to RGB and writes RGB24.
@@ -157,30 +159,6 @@ __device__ __forceinline__ void lift_inv_lanes(float &e, float &o) {
     o = fmaf(-WEMA_ALPHA, e + lane_next(e), o);
 }
 
-template <int N>
-__device__ __forceinline__ void lift_fwd_arr(float (&e)[N], float (&o)[N]) {
-#pragma unroll
-    for (int k = 0; k < N - 1; ++k) o[k] = fmaf(WEMA_ALPHA, e[k] + e[k + 1], o[k]);
-#pragma unroll
-    for (int k = 1; k < N; ++k) e[k] = fmaf(WEMA_BETA, o[k - 1] + o[k], e[k]);
-#pragma unroll
-    for (int k = 0; k < N - 1; ++k) o[k] = fmaf(WEMA_GAMMA, e[k] + e[k + 1], o[k]);
-#pragma unroll
-    for (int k = 1; k < N; ++k) e[k] = fmaf(WEMA_DELTA, o[k - 1] + o[k], e[k]);
-}
-
-template <int N>
-__device__ __forceinline__ void lift_inv_arr(float (&e)[N], float (&o)[N]) {
-#pragma unroll
-    for (int k = 1; k < N; ++k) e[k] = fmaf(-WEMA_DELTA, o[k - 1] + o[k], e[k]);
-#pragma unroll
-    for (int k = 0; k < N - 1; ++k) o[k] = fmaf(-WEMA_GAMMA, e[k] + e[k + 1], o[k]);
-#pragma unroll
-    for (int k = 1; k < N; ++k) e[k] = fmaf(-WEMA_BETA, o[k - 1] + o[k], e[k]);
-#pragma unroll
-    for (int k = 0; k < N - 1; ++k) o[k] = fmaf(-WEMA_ALPHA, e[k] + e[k + 1], o[k]);
-}
-
 // Channel value of an RGB24 / float RGB pixel.
 template <int KIND>
 __device__ __forceinline__ float chan_of(float r, float g, float b) {
@@ -189,364 +167,28 @@ __device__ __forceinline__ float chan_of(float r, float g, float b) {
     return cr_of(r, g, b);
 }
 
-// One input sample of the job (slow path: any position).
-template <int KIND>
-__device__ __forceinline__ float sample_at(const DwtfJob &J, int x, int y) {
-    const size_t idx = (size_t)y * J.w + x;
-    if (KIND == SRC_F32_PLANE) return gp((const float *)J.src)[idx];
-    if (KIND >= SRC_F32_Y) {
-        const GLOBAL_AS float *p = gp((const float *)J.src) + idx * 3;
-        return chan_of<KIND>(p[0], p[1], p[2]);
-    }
-    const GLOBAL_AS unsigned char *p = gp((const unsigned char *)J.src) + idx * 3;
-    return chan_of<KIND>(u8f(p[0]), u8f(p[1]), u8f(p[2]));
-}
-
-// Raw data of the sample pair (x, x + 1) of row y, loaded without using it, so that all
-// pairs of a wave's rows are in flight together. RGB24: the 6 bytes from the aligned
-// dwords that hold them (d2 only when they start at byte 3 of a dword); float: 2 dwords.
-struct PairRaw {
-    unsigned d0, d1, d2, sh;
-};
-
-template <int KIND>
-__device__ __forceinline__ PairRaw pair_load(const DwtfJob &J, int x, int y) {
-    PairRaw r;
-    const size_t idx = (size_t)y * J.w + x;
-    if (KIND == SRC_F32_PLANE) {
-        const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)gp((const float *)J.src) + idx;
-        r.d0 = q[0];
-        r.d1 = q[1];
-        r.d2 = 0u;
-        r.sh = 0u;
-    } else if (KIND >= SRC_F32_Y) {
-        r.d0 = r.d1 = r.d2 = r.sh = 0u;   // float RGB uses the per-sample path
-    } else {
-        const size_t a = (size_t)J.src + idx * 3;
-        r.sh = (unsigned)(a & 3u);
-        const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(a - r.sh);
-        r.d0 = q[0];
-        r.d1 = q[1];
-        r.d2 = r.sh == 3u ? q[2] : 0u;
-    }
-    return r;
-}
-
-template <int KIND>
-__device__ __forceinline__ void pair_decode(const PairRaw &r, float &e, float &o) {
-    if (KIND == SRC_F32_PLANE) {
-        e = __uint_as_float(r.d0);
-        o = __uint_as_float(r.d1);
-    } else {
-        const unsigned v0 = __builtin_amdgcn_alignbyte(r.d1, r.d0, r.sh);
-        const unsigned v1 = __builtin_amdgcn_alignbyte(r.d2, r.d1, r.sh);
-        e = chan_of<KIND>(u8f(v0 & 0xFFu), u8f((v0 >> 8) & 0xFFu), u8f((v0 >> 16) & 0xFFu));
-        o = chan_of<KIND>(u8f(v0 >> 24), u8f(v1 & 0xFFu), u8f((v1 >> 8) & 0xFFu));
-    }
-}
-
-// Row pass of one wave: rows wave, wave + 4, ... of the extended tile; lane = sample pair.
-// All loads of the wave's 16 rows are issued first; the rows are then lifted 4 at a time.
-// ROWS_IN: every row of the tile lies inside the image and none is its last row, so rows
-// need no reflection and no end-of-buffer check.
-template <int KIND, bool ROWS_IN>
-__device__ __forceinline__ void row_pass_t(float (*R)[128], const DwtfJob &J, int sx0, int sy0,
-                                           int lane, int wave) {
-    constexpr int NR = 2 * WEMA_RP / 4;
-    const int w = J.w, h = J.h;
-    const int xa = reflect_idx(sx0 + 2 * lane, w), xb = reflect_idx(sx0 + 2 * lane + 1, w);
-    const bool fast_x = KIND != SRC_F32_Y && KIND != SRC_F32_CB && KIND != SRC_F32_CR && xb == xa + 1;
-    PairRaw raw[NR];
-#pragma unroll
-    for (int i = 0; i < NR; ++i) {
-        const int y = ROWS_IN ? sy0 + wave + 4 * i : reflect_idx(sy0 + wave + 4 * i, h);
-        // the pair's dwords stay inside the buffer unless it ends the last row
-        const bool fast = ROWS_IN ? fast_x : fast_x && !(y == h - 1 && xb == w - 1);
-        if (fast) raw[i] = pair_load<KIND>(J, xa, y);
-    }
-#pragma unroll
-    for (int g = 0; g < NR; g += 4) {
-        float e[4], o[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int i = g + k;
-            const int y = ROWS_IN ? sy0 + wave + 4 * i : reflect_idx(sy0 + wave + 4 * i, h);
-            const bool fast = ROWS_IN ? fast_x : fast_x && !(y == h - 1 && xb == w - 1);
-            if (fast) {
-                pair_decode<KIND>(raw[i], e[k], o[k]);
-            } else {
-                e[k] = sample_at<KIND>(J, xa, y);
-                o[k] = sample_at<KIND>(J, xb, y);
-            }
-        }
-        if (w >= 2) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) lift_fwd_lanes(e[k], o[k]);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                R[wave + 4 * (g + k)][lane] = e[k] * WEMA_KINV;
-                R[wave + 4 * (g + k)][64 + lane] = o[k] * WEMA_K;
-            }
-        } else {
-            // n < 2: the row transform is the identity (src/dwt.c:46); every sample of
-            // the row is the one pixel
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                R[wave + 4 * (g + k)][lane] = e[k];
-                R[wave + 4 * (g + k)][64 + lane] = e[k];
-            }
-        }
-    }
-}
-
-// Row pass of a tile that lies wholly inside the image, none of its rows the last one:
-// no reflection and no per-row conditions. Loads take a 32-bit offset from the source's
-// base (sources below 2^29 pixels, starting on a dword), and an RGB24 pair's byte alignment
-// is the same in every row of the wave (4 rows are 12·w bytes), so each row is one unconditional
-// 3-dword load (it ends at most 11 bytes after the pair's start, inside the next row).
-template <int KIND>
-__device__ __forceinline__ void row_pass_in(float (*R)[128], const DwtfJob &J, int sx0, int sy0,
-                                            int lane, int wave) {
-    constexpr int NR = 2 * WEMA_RP / 4;
-    const unsigned w = (unsigned)J.w;
-    const unsigned first = (unsigned)(sy0 + wave) * w + (unsigned)(sx0 + 2 * lane);
-    PairRaw raw[NR];
-    if (KIND == SRC_F32_PLANE) {
-        const GLOBAL_AS unsigned *base = (const GLOBAL_AS unsigned *)gp((const float *)J.src);
-#pragma unroll
-        for (int i = 0; i < NR; ++i) {
-            const GLOBAL_AS unsigned *q = base + (first + (unsigned)i * 4u * w);
-            raw[i].d0 = q[0];
-            raw[i].d1 = q[1];
-            raw[i].d2 = 0u;
-            raw[i].sh = 0u;
-        }
-    } else {
-        const GLOBAL_AS unsigned char *base = gp((const unsigned char *)J.src);
-        const unsigned b0 = first * 3u, sh = b0 & 3u;
-#pragma unroll
-        for (int i = 0; i < NR; ++i) {
-            const GLOBAL_AS unsigned *q =
-                (const GLOBAL_AS unsigned *)(base + (b0 - sh + (unsigned)i * 12u * w));
-            raw[i].d0 = q[0];
-            raw[i].d1 = q[1];
-            raw[i].d2 = q[2];
-            raw[i].sh = sh;
-        }
-    }
-#pragma unroll
-    for (int g = 0; g < NR; g += 4) {
-        float e[4], o[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) pair_decode<KIND>(raw[g + k], e[k], o[k]);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) lift_fwd_lanes(e[k], o[k]);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            R[wave + 4 * (g + k)][lane] = e[k] * WEMA_KINV;
-            R[wave + 4 * (g + k)][64 + lane] = o[k] * WEMA_K;
-        }
-    }
-}
-
-template <int KIND>
-__device__ __forceinline__ void row_pass(float (*R)[128], const DwtfJob &J, int sx0, int sy0,
-                                         int lane, int wave) {
-    const bool rows_in = sy0 >= 0 && sy0 + 2 * WEMA_RP < J.h;
-    if (KIND != SRC_F32_Y && KIND != SRC_F32_CB && KIND != SRC_F32_CR && rows_in && sx0 >= 0 &&
-        sx0 + 128 <= J.w && (long long)J.w * J.h < (1ll << 29) && ((size_t)J.src & 3) == 0)
-        row_pass_in<KIND>(R, J, sx0, sy0, lane, wave);
-    else if (KIND != SRC_F32_PLANE && rows_in)
-        // (the coarser levels' float planes keep the general form: measured faster there)
-        row_pass_t<KIND, true>(R, J, sx0, sy0, lane, wave);
-    else
-        row_pass_t<KIND, false>(R, J, sx0, sy0, lane, wave);
-}
-
-// ---------------------------------------------------------------------------------------
-// Forward DWT level (src/dwt.c:369-473 + 645-703).
-//
-// Block = 4 waves. Tile = 64 x WEMA_RP sample pairs of the level's input, of which the
-// inner 60 x (WEMA_RP - 4) pairs are outputs; the 2-pair border on each side feeds the
-// 4 lifting steps (each widens the dependency by one pair). Samples outside the image are
-// taken from the whole-sample symmetric extension, which reproduces the reference's
-// per-step boundary rules bit for bit (the extended values are equal sums of the same
-// operands). Row pass (registers, lane shifts) first, column pass (LDS) second, as the
-// reference.
-//
-// Where the parent size is odd, the reference's extraction leaves the last HL/HH column or
-// LH/HH row of the subband unwritten and its scratch buffer keeps the previous level's
-// value at the same flat index (src/dwt.c:459-472, 648-653); those entries are copied from
-// the previous level's plane (0 at level 0) and enter the threshold sum.
-
-constexpr int DWT_THREADS = 256;
-constexpr int HALF_PAIRS = WEMA_RP / 2 + 2;   // pairs a half-column thread lifts
-
-// RGB: level 0 (source kinds from RGB24 / float RGB frames); else the previous level's LL.
-template <bool RGB>
-__device__ __forceinline__ void dwt_fwd_body(float (*R)[128], const DwtfJob *__restrict__ jobs, int njobs) {
-    constexpr int RP = WEMA_RP;
-
-    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int jid = find_job(jobs, njobs, bid);
-    DwtfJob J = jobs[jid];
-    int t = bid - J.tile_begin;
-    if (J.nch > 1) {
-        const int ch = t % J.nch;
-        t /= J.nch;
-        J.ll += ch * J.ll_cstride;
-        J.planes += ch * J.planes_cstride;
-        J.acc += ch;
-        J.src_kind += ch;
-    }
-    const int tx = t % J.tiles_x, ty = t / J.tiles_x;
-    const int w = J.w, h = J.h;
-    const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
-    const int px0 = tx * WEMA_TILE_PAIRS_X, py0 = ty * WEMA_TILE_PAIRS_Y;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-
-    // ---- row pass
-    const int sx0 = 2 * (px0 - 2), sy0 = 2 * (py0 - 2);
-    if (!RGB) {
-        row_pass<SRC_F32_PLANE>(R, J, sx0, sy0, lane, wave);
-    } else {
-        switch (J.src_kind) {
-        case SRC_U8_Y: row_pass<SRC_U8_Y>(R, J, sx0, sy0, lane, wave); break;
-        case SRC_U8_CB: row_pass<SRC_U8_CB>(R, J, sx0, sy0, lane, wave); break;
-        case SRC_U8_CR: row_pass<SRC_U8_CR>(R, J, sx0, sy0, lane, wave); break;
-        case SRC_F32_Y: row_pass<SRC_F32_Y>(R, J, sx0, sy0, lane, wave); break;
-        case SRC_F32_CB: row_pass<SRC_F32_CB>(R, J, sx0, sy0, lane, wave); break;
-        default: row_pass<SRC_F32_CR>(R, J, sx0, sy0, lane, wave); break;
-        }
-    }
-    __syncthreads();
-
-    // ---- column pass: thread = (column, half); half 0 lifts pairs 0..17 and writes
-    // output pairs 2..15, half 1 lifts pairs 14..31 and writes 16..29 (half and high are
-    // the same for the whole wave: taken as scalars)
-    const int c = threadIdx.x & 127;
-    const int half = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);
-    const int k0 = half * (RP / 2 - 2);
-    float e[HALF_PAIRS], o[HALF_PAIRS];
-#pragma unroll
-    for (int k = 0; k < HALF_PAIRS; ++k) {
-        e[k] = R[2 * (k0 + k)][c];
-        o[k] = R[2 * (k0 + k) + 1][c];
-    }
-    if (h >= 2) {
-        lift_fwd_arr<HALF_PAIRS>(e, o);
-#pragma unroll
-        for (int k = 0; k < HALF_PAIRS; ++k) {
-            e[k] *= WEMA_KINV;
-            o[k] *= WEMA_K;
-        }
-    }
-
-    const bool high = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) & 1;   // c >= 64
-    const int px = px0 + (c & 63) - 2;
-    const bool col_ok = (c & 63) >= 2 && (c & 63) < 62 && px < sw;
-    const int hw_valid = w >> 1, hh_valid = h >> 1;   // written extents (src/dwt.c:463-469)
-    const size_t n = (size_t)sw * sh;
-    GLOBAL_AS float *LH = gp(J.planes), *HH = LH + n, *HL = LH + 2 * n;
-    GLOBAL_AS float *LL = gp(J.ll);
-    const GLOBAL_AS float *prev = gp(J.prev);
-    const size_t pn = (size_t)w * h;                  // previous level's plane size
-    unsigned long long part = 0ull;
-    // tiles inside the written extents need neither bounds checks nor stale entries
-    const bool interior = px0 + WEMA_TILE_PAIRS_X <= hw_valid && py0 + WEMA_TILE_PAIRS_Y <= hh_valid &&
-                          n < (1ull << 29);
-    // the wave's coefficients all below 2^30 in magnitude (and none NaN): the branch-free
-    // conversion applies. From RGB24 samples (in [0, 1]) every coefficient is below 64:
-    // each 1-D analysis at most multiplies the largest magnitude by 7.73 (the four lifting
-    // steps with |alpha|, |beta|, |gamma|, |delta| and the K scaling).
-    bool small = true;
-    if (!(RGB && J.src_kind >= SRC_U8_Y && J.src_kind <= SRC_U8_CR)) {
-        float mag = 0.0f;
-#pragma unroll
-        for (int k = 2; k < HALF_PAIRS - 2; ++k) mag += fabsf(e[k]) + fabsf(o[k]);
-        small = __all(mag < 1073741824.0f);
-    }
-    if (interior && small) {
-        if ((c & 63) >= 2 && (c & 63) < 62) {
-            // c < 64 (waves 0 and 2): LL and LH; c >= 64 (waves 1 and 3): HL and HH
-            // byte offsets from the wave's (scalar) plane bases; a plane is below 2^31 bytes
-            GLOBAL_AS char *pe = (GLOBAL_AS char *)(high ? HL : LL);
-            GLOBAL_AS char *po = (GLOBAL_AS char *)(high ? HH : LH);
-            const unsigned off = 4u * ((unsigned)(py0 + k0) * (unsigned)sw + (unsigned)px);
-            const unsigned step = 4u * (unsigned)sw;
-            if (high) {
-#pragma unroll
-                for (int k = 2; k < HALF_PAIRS - 2; ++k) {
-                    *(GLOBAL_AS float *)(pe + (off + (unsigned)(k - 2) * step)) = e[k];
-                    *(GLOBAL_AS float *)(po + (off + (unsigned)(k - 2) * step)) = o[k];
-                    part += abs_q32_small(o[k]) + abs_q32_small(e[k]);
-                }
-            } else {
-#pragma unroll
-                for (int k = 2; k < HALF_PAIRS - 2; ++k) {
-                    *(GLOBAL_AS float *)(pe + (off + (unsigned)(k - 2) * step)) = e[k];
-                    *(GLOBAL_AS float *)(po + (off + (unsigned)(k - 2) * step)) = o[k];
-                    part += abs_q32_small(o[k]);
-                }
-            }
-        }
-    } else if (col_ok) {
-#pragma unroll
-        for (int k = 2; k < HALF_PAIRS - 2; ++k) {
-            const int py = py0 + k0 + k - 2;
-            if (py < sh) {
-                const size_t idx = (size_t)py * sw + px;
-                if (!high) {
-                    LL[idx] = e[k];
-                    float vlh = o[k];
-                    if (py >= hh_valid) vlh = J.prev ? prev[idx] : 0.0f;
-                    LH[idx] = vlh;
-                    part += abs_fixed(vlh);
-                } else {
-                    float vhl = e[k], vhh = o[k];
-                    if (px >= hw_valid) {
-                        vhl = J.prev ? prev[2 * pn + idx] : 0.0f;
-                        vhh = J.prev ? prev[pn + idx] : 0.0f;
-                    } else if (py >= hh_valid) {
-                        vhh = J.prev ? prev[pn + idx] : 0.0f;
-                    }
-                    HL[idx] = vhl;
-                    HH[idx] = vhh;
-                    part += abs_fixed(vhl) + abs_fixed(vhh);
-                }
-            }
-        }
-    }
-    if (J.acc) {
-        part = wave_sum_u64(part);
-        if (lane == 0 && part) atomicAdd(J.acc, part);
-    }
-}
-
-// Level 0 (from the RGB frame) and the coarser levels are separate kernels.
-__global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd0(const DwtfJob *__restrict__ jobs, int njobs) {
-    __shared__ float R[2 * WEMA_RP][128];
-    dwt_fwd_body<true>(R, jobs, njobs);
-}
-__global__ void __launch_bounds__(DWT_THREADS) k_dwt_fwd(const DwtfJob *__restrict__ jobs, int njobs) {
-    __shared__ float R[2 * WEMA_RP][128];
-    dwt_fwd_body<false>(R, jobs, njobs);
-}
-
 // ---------------------------------------------------------------------------------------
 // Forward DWT level as strips (src/dwt.c:369-473 + 645-703).
 //
 // A wave computes output pairs [px0, px0 + 60) x [py0, py0 + sp) of one job, for all
 // of the job's channels (Y, Cb and Cr of an RGB frame are decoded from one load). Lane l
-// owns input pair column px0 - 2 + l, so the row lifting runs across lanes (DPP shifts) as
-// in the tile kernels. The wave streams input pair rows py0 - 2 .. py0 + sp + 1 from
+// owns input pair column px0 - 2 + l, so the row lifting runs across lanes (DPP shifts);
+// the 2-pair border on each side feeds the 4 lifting steps, each of which widens the
+// dependency by one pair. The wave streams input pair rows py0 - 2 .. py0 + sp + 1 from
 // top to bottom, and the column lifting runs as a pipeline in registers: when input pair m
 // arrives, steps 1 and 2 of pair m - 1 and steps 3 and 4 of pair m - 2 are computed, which
 // makes output pair m - 2 final. Each step is the same fused multiply-add of the same
-// operands as the in-place lifting (lift_fwd_arr), so the coefficients are bit-identical;
-// no LDS and no barrier are needed, and the loads of the next two input pairs are in
-// flight while a pair is computed. Samples outside the image come from the same
-// whole-sample symmetric extension as in the tile kernels.
+// operands as the reference's lifting (src/dwt.c:45-107), so the coefficients are
+// bit-identical; no LDS and no barrier are needed, and the loads of the next input pairs
+// are in flight while a pair is computed.
+//
+// Samples outside the image are taken from the whole-sample symmetric extension, which
+// reproduces the reference's per-step boundary rules bit for bit (the extended values are
+// equal sums of the same operands). Where the parent size is odd, the reference's
+// extraction leaves the last HL/HH column or LH/HH row of the subband unwritten and its
+// scratch buffer keeps the previous level's value at the same flat index
+// (src/dwt.c:459-472, 648-653); those entries are copied from the previous level's plane
+// (0 at level 0) and enter the threshold sum.
 
 // Source families of a strip job.
 enum { FAM_U8_RGB = 0, FAM_F32_RGB = 1, FAM_F32_PLANE = 2 };
@@ -677,8 +319,7 @@ __device__ __forceinline__ void vpipe_step(VPipe &p, float e, float o, float &e4
     p.o3 = o3;
 }
 
-// C0: colour channel of the job's first channel (RGB families).
-template <int FAM, int NCH, int C0, bool FAST, bool STALE>
+template <int FAM, int NCH, bool FAST, bool STALE>
 __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int lane) {
     const int SP = J.sp;
     const int w = J.w, h = J.h;
@@ -768,10 +409,10 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
                 e1 = fb[0];
                 o1 = fb[1];
             } else {
-                e0 = chan_idx(C0 + c, a0);
-                o0 = chan_idx(C0 + c, b0);
-                e1 = chan_idx(C0 + c, a1);
-                o1 = chan_idx(C0 + c, b1);
+                e0 = chan_idx(c, a0);
+                o0 = chan_idx(c, b0);
+                e1 = chan_idx(c, a1);
+                o1 = chan_idx(c, b1);
             }
             float l0, g0, l1, g1;   // row-lifted L (low) and H (high) samples of both rows
             if (FAST || w >= 2) {
@@ -806,8 +447,8 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
                 e0 = ((const float *)&r)[0];
                 o0 = ((const float *)&r)[1];
             } else {
-                e0 = chan_idx(C0 + c, a0);
-                o0 = chan_idx(C0 + c, b0);
+                e0 = chan_idx(c, a0);
+                o0 = chan_idx(c, b0);
             }
             float l0 = e0, g0 = e0;
             if (w >= 2) {
@@ -873,14 +514,14 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
     }
 }
 
-template <int FAM, int NCH, int C0>
+template <int FAM, int NCH>
 __device__ __forceinline__ void dwt_fwd_strip(const DwtfJob &J, int strip, int lane) {
     const int w = J.w, h = J.h, sh = (h + 1) >> 1;
     const int sx0 = 2 * ((strip % J.tiles_x) * WEMA_TILE_PAIRS_X - 2);
     const int py0 = (strip / J.tiles_x) * J.sp;
     if constexpr (FAM == FAM_F32_RGB) {
         // (float RGB frames: the general loads only)
-        dwt_fwd_strip_t<FAM, NCH, C0, false, false>(J, strip, lane);
+        dwt_fwd_strip_t<FAM, NCH, false, false>(J, strip, lane);
     } else if constexpr (FAM == FAM_F32_PLANE) {
         // every lane's pair adjacent and inside the row, sources below 2^29 samples (32-bit
         // row offsets); strips that reach the unwritten last column or row of an odd-sized
@@ -889,9 +530,9 @@ __device__ __forceinline__ void dwt_fwd_strip(const DwtfJob &J, int strip, int l
         const bool stale = px0 + WEMA_TILE_PAIRS_X > (w >> 1) || py0 + J.sp > (h >> 1);
         const bool fast = sx0 >= 0 && sx0 + 128 <= w && h >= 8 && (long long)w * h < (1ll << 29) &&
                           min(J.sp, sh - py0) >= 2 * SLOTS_OF(NCH);
-        if (stale) dwt_fwd_strip_t<FAM, NCH, C0, false, true>(J, strip, lane);
-        else if (fast) dwt_fwd_strip_t<FAM, NCH, C0, true, false>(J, strip, lane);
-        else dwt_fwd_strip_t<FAM, NCH, C0, false, false>(J, strip, lane);
+        if (stale) dwt_fwd_strip_t<FAM, NCH, false, true>(J, strip, lane);
+        else if (fast) dwt_fwd_strip_t<FAM, NCH, true, false>(J, strip, lane);
+        else dwt_fwd_strip_t<FAM, NCH, false, false>(J, strip, lane);
     } else {
         // RGB24 as above, and rows that start on a dword (frame aligned, width a multiple
         // of 4); the 3-dword loads end at most 6 bytes past the pair, so the strip must not
@@ -900,8 +541,8 @@ __device__ __forceinline__ void dwt_fwd_strip(const DwtfJob &J, int strip, int l
         const bool fast = sx0 >= 0 && sx0 + 128 <= w && h >= 8 && (long long)w * h < (1ll << 29) &&
                           min(J.sp, sh - py0) >= 2 * SLOTS_OF(NCH) && (((size_t)J.src | (size_t)w) & 3) == 0 &&
                           (sx0 + 132 <= w || last_row < h - 1);
-        if (fast) dwt_fwd_strip_t<FAM, NCH, C0, true, false>(J, strip, lane);
-        else dwt_fwd_strip_t<FAM, NCH, C0, false, false>(J, strip, lane);
+        if (fast) dwt_fwd_strip_t<FAM, NCH, true, false>(J, strip, lane);
+        else dwt_fwd_strip_t<FAM, NCH, false, false>(J, strip, lane);
     }
 }
 
@@ -915,133 +556,19 @@ __device__ __forceinline__ void dwt_fwd_strips(const DwtfJob *__restrict__ jobs,
     const int lane = threadIdx.x & 63;
     const int strip = (bid - J.tile_begin) * WEMA_STRIPS_PER_BLOCK + wave;
     if (strip >= J.tiles_x * wema_ceil_div((J.h + 1) >> 1, J.sp)) return;
-    if (!LEVEL0) {
-        dwt_fwd_strip<FAM_F32_PLANE, 1, 0>(J, strip, lane);
-    } else if (J.nch == 3) {
-        if (J.src_kind == SRC_U8_Y) dwt_fwd_strip<FAM_U8_RGB, 3, 0>(J, strip, lane);
-        else dwt_fwd_strip<FAM_F32_RGB, 3, 0>(J, strip, lane);
-    } else {
-        switch (J.src_kind) {
-        case SRC_U8_Y: dwt_fwd_strip<FAM_U8_RGB, 1, 0>(J, strip, lane); break;
-        case SRC_U8_CB: dwt_fwd_strip<FAM_U8_RGB, 1, 1>(J, strip, lane); break;
-        case SRC_U8_CR: dwt_fwd_strip<FAM_U8_RGB, 1, 2>(J, strip, lane); break;
-        case SRC_F32_Y: dwt_fwd_strip<FAM_F32_RGB, 1, 0>(J, strip, lane); break;
-        case SRC_F32_CB: dwt_fwd_strip<FAM_F32_RGB, 1, 1>(J, strip, lane); break;
-        default: dwt_fwd_strip<FAM_F32_RGB, 1, 2>(J, strip, lane); break;
-        }
-    }
+    // (level-0 jobs take Y, or Y, Cb and Cr, from RGB24 or float RGB frames)
+    if (!LEVEL0) dwt_fwd_strip<FAM_F32_PLANE, 1>(J, strip, lane);
+    else if (J.src_kind == SRC_U8_Y && J.nch == 3) dwt_fwd_strip<FAM_U8_RGB, 3>(J, strip, lane);
+    else if (J.src_kind == SRC_U8_Y) dwt_fwd_strip<FAM_U8_RGB, 1>(J, strip, lane);
+    else if (J.nch == 3) dwt_fwd_strip<FAM_F32_RGB, 3>(J, strip, lane);
+    else dwt_fwd_strip<FAM_F32_RGB, 1>(J, strip, lane);
 }
 
-__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_fwd0s(const DwtfJob *__restrict__ jobs, int njobs) {
+__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_fwd0(const DwtfJob *__restrict__ jobs, int njobs) {
     dwt_fwd_strips<true>(jobs, njobs);
 }
-__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_fwds(const DwtfJob *__restrict__ jobs, int njobs) {
+__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_fwd(const DwtfJob *__restrict__ jobs, int njobs) {
     dwt_fwd_strips<false>(jobs, njobs);
-}
-
-// ---------------------------------------------------------------------------------------
-// Inverse DWT level (src/dwt.c:475-582 + 705-767): packed [LL | HL ; LH | HH] columns
-// first, then rows; each 1D inverse starts by undoing the K scaling.
-// Column pass of one tile: thread = (column, half) as in the forward kernel; the packed
-// [LL | HL ; LH | HH] columns are read straight from the planes, lifted in registers and
-// left in R as rows of (even | odd) sample pairs.
-__device__ __forceinline__ void inv_col_pass(float (*R)[128], const float *ll, const float *planes,
-                                             int w, int h, int px0, int py0) {
-    constexpr int RP = WEMA_RP;
-    const int sw = (w + 1) >> 1;
-    const size_t n = (size_t)sw * ((h + 1) >> 1);
-    const int c = threadIdx.x & 127, half = threadIdx.x >> 7;
-    const int k0 = half * (RP / 2 - 2);
-    const bool high = c >= 64;
-    const int I = px0 - 2 + (c & 63);
-    const int sx = reflect_idx(2 * I + (high ? 1 : 0), w);
-    const int col = sx >> 1;
-    const GLOBAL_AS float *Ep = high ? gp(planes) + 2 * n : gp(ll);   // HL : LL
-    const GLOBAL_AS float *Op = high ? gp(planes) + n : gp(planes);   // HH : LH
-    float e[HALF_PAIRS], o[HALF_PAIRS];
-#pragma unroll
-    for (int k = 0; k < HALF_PAIRS; ++k) {
-        const int jy = py0 - 2 + k0 + k;
-        const int ye = reflect_idx(2 * jy, h) >> 1;
-        const int yo = reflect_idx(2 * jy + 1, h) >> 1;
-        e[k] = Ep[(size_t)ye * sw + col];
-        o[k] = Op[(size_t)yo * sw + col];
-    }
-    if (h >= 2) {
-#pragma unroll
-        for (int k = 0; k < HALF_PAIRS; ++k) {
-            e[k] *= WEMA_K;
-            o[k] *= WEMA_KINV;
-        }
-        lift_inv_arr<HALF_PAIRS>(e, o);
-    }
-#pragma unroll
-    for (int k = 2; k < HALF_PAIRS - 2; ++k) {
-        const int row = 2 * (k0 + k - 2);
-        R[row][c] = e[k];
-        R[row + 1][c] = o[k];
-    }
-}
-
-// Row r of the tile after the row synthesis: samples x = 2 * (px0 - 2 + lane) (ev) and x + 1
-// (ov) of image row 2 * py0 + r.
-__device__ __forceinline__ void inv_row(float (*R)[128], int r, int w, int lane, float &ev, float &ov) {
-    ev = R[r][lane];
-    ov = R[r][64 + lane];
-    if (w >= 2) {
-        ev *= WEMA_K;
-        ov *= WEMA_KINV;
-        lift_inv_lanes(ev, ov);
-    }
-}
-
-constexpr int INV_ROWS = 2 * (WEMA_RP - 4);   // image rows a tile produces
-
-__device__ __forceinline__ void dwt_inv_body(float (*R)[128], const DwtiJob *__restrict__ jobs, int njobs) {
-    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int jid = find_job(jobs, njobs, bid);
-    const DwtiJob J = jobs[jid];
-    const int t = bid - J.tile_begin;
-    if (t == 0 && threadIdx.x == 0 && J.acc_clear) *J.acc_clear = 0ull;
-    const int w = J.w, h = J.h;
-    const int px0 = (t % J.tiles_x) * WEMA_TILE_PAIRS_X, py0 = (t / J.tiles_x) * WEMA_TILE_PAIRS_Y;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-
-    inv_col_pass(R, J.ll, J.planes, w, h, px0, py0);
-    __syncthreads();
-
-    // ---- row pass: 2 rows at a time per wave
-    const int x = 2 * (px0 - 2 + lane);
-    const bool lane_ok = lane >= 2 && lane < 62;
-    for (int r0 = wave * 2; r0 < INV_ROWS; r0 += 8) {
-        const int y0 = 2 * py0 + r0;
-        if (y0 >= h) break;
-        float ev[2], ov[2];
-#pragma unroll
-        for (int k = 0; k < 2; ++k) inv_row(R, r0 + k, w, lane, ev[k], ov[k]);
-        if (lane_ok) {
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int y = y0 + k;
-                if (y < h) {
-                    GLOBAL_AS float *row = gp(J.out) + (size_t)y * w;
-                    if (x < w) row[x] = ev[k];
-                    if (x + 1 < w) row[x + 1] = ov[k];
-                }
-            }
-        }
-    }
-}
-
-// Level 0 (full-resolution reconstruction) and the coarser levels are separate kernels
-// (the same code), so that profiles tell them apart.
-__global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv0(const DwtiJob *__restrict__ jobs, int njobs) {
-    __shared__ float R[INV_ROWS][128];
-    dwt_inv_body(R, jobs, njobs);
-}
-__global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv(const DwtiJob *__restrict__ jobs, int njobs) {
-    __shared__ float R[INV_ROWS][128];
-    dwt_inv_body(R, jobs, njobs);
 }
 
 // Colour mode output (src/wema.c:80-102).
@@ -1050,75 +577,6 @@ __device__ __forceinline__ void ycc_px(float yv, float cb, float cr, float (&v)[
     v[0] = clamp01(fmaf(1.402f, crv, yv));
     v[1] = clamp01(yv - fmaf(0.344f, cbv, 0.714f * crv));
     v[2] = clamp01(fmaf(1.772f, cbv, yv));
-}
-
-// Level-0 synthesis of Y, Cb and Cr of one tile, then YCbCr -> RGB and egress, without the
-// full-resolution planes ever reaching HBM. The three channels are synthesised in turn
-// through one LDS tile; each thread keeps its Y and Cb rows in registers until Cr is done.
-__global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv0_ycc(const DwtiJob *__restrict__ jobs, int njobs) {
-    __shared__ float R[INV_ROWS][128];
-    constexpr int IT = INV_ROWS / 8;   // row pairs per wave
-    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int jid = find_job(jobs, njobs, bid);
-    const DwtiJob &J = jobs[jid];
-    const int t = bid - J.tile_begin;
-    if (t == 0 && (int)threadIdx.x < J.nch && J.acc_clear) J.acc_clear[threadIdx.x] = 0ull;
-    const int w = J.w, h = J.h;
-    const int px0 = (t % J.tiles_x) * WEMA_TILE_PAIRS_X, py0 = (t / J.tiles_x) * WEMA_TILE_PAIRS_Y;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x = 2 * (px0 - 2 + lane);
-    const bool lane_ok = lane >= 2 && lane < 62 && x < w;
-    const bool two = x + 1 < w;
-    float yv[IT][2][2], cbv[IT][2][2];
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        if (ch) __syncthreads();   // the previous channel's rows have been read
-        inv_col_pass(R, J.ll + ch * J.ll_cstride, J.planes + ch * J.planes_cstride, w, h, px0, py0);
-        __syncthreads();
-#pragma unroll
-        for (int it = 0; it < IT; ++it) {
-            const int r0 = wave * 2 + 8 * it;
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                float ev, ov;
-                inv_row(R, r0 + k, w, lane, ev, ov);
-                if (ch == 0) {
-                    yv[it][k][0] = ev;
-                    yv[it][k][1] = ov;
-                } else if (ch == 1) {
-                    cbv[it][k][0] = ev;
-                    cbv[it][k][1] = ov;
-                } else {
-                    const int y = 2 * py0 + r0 + k;
-                    if (!lane_ok || y >= h) continue;
-                    float v0[3], v1[3];
-                    ycc_px(yv[it][k][0], cbv[it][k][0], ev, v0);
-                    ycc_px(yv[it][k][1], cbv[it][k][1], ov, v1);
-                    const size_t idx = ((size_t)y * w + x) * 3;
-                    if (J.out_u8) {
-                        GLOBAL_AS unsigned char *o = gp(J.out_u8) + idx;
-                        const unsigned b0 = egress_u8(v0[0]), b1 = egress_u8(v0[1]), b2 = egress_u8(v0[2]);
-                        if (two && (idx & 1) == 0) {
-                            GLOBAL_AS unsigned short *os = (GLOBAL_AS unsigned short *)o;
-                            os[0] = (unsigned short)(b0 | (b1 << 8));
-                            os[1] = (unsigned short)(b2 | ((unsigned)egress_u8(v1[0]) << 8));
-                            os[2] = (unsigned short)(egress_u8(v1[1]) | ((unsigned)egress_u8(v1[2]) << 8));
-                        } else {
-                            o[0] = b0; o[1] = b1; o[2] = b2;
-                            if (two) {
-                                o[3] = egress_u8(v1[0]); o[4] = egress_u8(v1[1]); o[5] = egress_u8(v1[2]);
-                            }
-                        }
-                    }
-                    if (J.out_f32) {
-                        GLOBAL_AS float *o = gp(J.out_f32) + idx;
-                        o[0] = v0[0]; o[1] = v0[1]; o[2] = v0[2];
-                        if (two) { o[3] = v1[0]; o[4] = v1[1]; o[5] = v1[2]; }
-                    }
-                }
-            }
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1130,11 +588,12 @@ __global__ void __launch_bounds__(DWT_THREADS) k_dwt_inv0_ycc(const DwtiJob *__r
 // its high column (HL over HH). The wave streams coefficient pair rows py0 - 2 ..
 // py0 + sp + 1 from top to bottom through two column pipelines per channel: when pair m
 // arrives, step 1 of pair m, steps 2 and 3 of pair m - 1 and step 4 of pair m - 2 are
-// computed (the steps of lift_inv_arr with the same operands, so the samples are
+// computed (each 1D inverse first undoes the K scaling, then runs the reference's four
+// inverse lifting steps with the same operands, src/dwt.c:124-161, so the samples are
 // bit-identical), which makes image rows 2 (m - 2) and 2 (m - 2) + 1 final. Each such row
 // then goes through the row synthesis across lanes (DPP shifts) and is stored. Coefficients
-// outside the subbands come from the same whole-sample symmetric extension as in the tile
-// kernels; every load is in bounds.
+// outside the subbands come from the whole-sample symmetric extension of the interleaved
+// signal (as in the forward DWT); every load is in bounds.
 
 // Column-synthesis pipeline of one lane column: o (K-scaled) of the last pair, e after
 // step 1 of the last pair, o after step 2 and e after step 3 of the pair before.
@@ -1351,13 +810,13 @@ __device__ __forceinline__ void dwt_inv_strips(const DwtiJob *__restrict__ jobs,
     else dwt_inv_strip_t<NCH, OUT, false>(J, strip, lane);
 }
 
-__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv0s(const DwtiJob *__restrict__ jobs, int njobs) {
+__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv0(const DwtiJob *__restrict__ jobs, int njobs) {
     dwt_inv_strips<1, 0>(jobs, njobs);
 }
-__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_invs(const DwtiJob *__restrict__ jobs, int njobs) {
+__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv(const DwtiJob *__restrict__ jobs, int njobs) {
     dwt_inv_strips<1, 0>(jobs, njobs);
 }
-__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv0_yccs(const DwtiJob *__restrict__ jobs, int njobs) {
+__global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv0_ycc(const DwtiJob *__restrict__ jobs, int njobs) {
     dwt_inv_strips<3, 1>(jobs, njobs);
 }
 
@@ -2181,33 +1640,19 @@ __global__ void __launch_bounds__(256) k_synth(const SynthFrame *__restrict__ fr
 extern "C" {
 hipError_t wema_launch_dwt_fwd(const DwtfJob *jobs, int njobs, int nblocks, int level0, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-#if WEMA_FWD_STRIPS
-    if (level0) hipLaunchKernelGGL(k_dwt_fwd0s, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
-    else hipLaunchKernelGGL(k_dwt_fwds, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
-#else
-    if (level0) hipLaunchKernelGGL(k_dwt_fwd0, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
-    else hipLaunchKernelGGL(k_dwt_fwd, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
-#endif
+    if (level0) hipLaunchKernelGGL(k_dwt_fwd0, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
+    else hipLaunchKernelGGL(k_dwt_fwd, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 hipError_t wema_launch_dwt_inv(const DwtiJob *jobs, int njobs, int nblocks, int level0, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-#if WEMA_INV_STRIPS
-    if (level0) hipLaunchKernelGGL(k_dwt_inv0s, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
-    else hipLaunchKernelGGL(k_dwt_invs, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
-#else
-    if (level0) hipLaunchKernelGGL(k_dwt_inv0, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
-    else hipLaunchKernelGGL(k_dwt_inv, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
-#endif
+    if (level0) hipLaunchKernelGGL(k_dwt_inv0, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
+    else hipLaunchKernelGGL(k_dwt_inv, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *jobs, int njobs, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-#if WEMA_INV_STRIPS
-    hipLaunchKernelGGL(k_dwt_inv0_yccs, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
-#else
-    hipLaunchKernelGGL(k_dwt_inv0_ycc, dim3(nblocks), dim3(DWT_THREADS), 0, s, jobs, njobs);
-#endif
+    hipLaunchKernelGGL(k_dwt_inv0_ycc, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 hipError_t wema_launch_iir(const TemporalJob *jobs, int njobs, int nblocks, hipStream_t s) {
