@@ -149,6 +149,7 @@ void wema_geometry_init(WemaGeom *g, const wema_dev_params *p) {
         g->ll_off[l + 1] = g->ll_off[l] + n;
     }
     g->nu = g->plane_off[g->L];
+    g->nus = (g->nu + 63) / 64 * 64;
     g->ll_total = g->ll_off[g->L];
 }
 
@@ -160,7 +161,7 @@ struct Layout {
 };
 
 static void plan_layout(const WemaGeom &g, const wema_dev_params &p, int with_filt, Layout *L) {
-    const size_t B = (size_t)p.max_batch, nch = g.nch, nu = g.nu;
+    const size_t B = (size_t)p.max_batch, nch = g.nch, nu = g.nus;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         size_t o = off;
@@ -374,7 +375,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
     const WemaGeom &g = s->g;
     const wema_dev_params &p = s->p;
     const int nch = g.nch, L = g.L;
-    const size_t P = g.P, nu = g.nu;
+    const size_t P = g.P, nu = g.nu, nus = g.nus;
     const size_t in_stride = P * 3 * (in_f32 ? sizeof(float) : 1);
     const long frame0 = s->frames;
 
@@ -387,9 +388,9 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         }
 
     auto coef_ptr = [&](int f, int c) -> float * {
-        if (p.use_iir) return s->coef + ((size_t)f * nch + c) * nu;
+        if (p.use_iir) return s->coef + ((size_t)f * nch + c) * nus;
         const long gidx = frame0 + f;
-        return s->coef + ((size_t)c * s->ring_slots + (size_t)(gidx % s->ring_slots)) * nu;
+        return s->coef + ((size_t)c * s->ring_slots + (size_t)(gidx % s->ring_slots)) * nus;
     };
     auto ll_ptr = [&](int f, int c, int l) -> float * {
         return s->ll + ((size_t)f * nch + c) * g.ll_total + g.ll_off[l];
@@ -448,21 +449,21 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         j.amp_index = (int)(jb.host_amp.size() - (size_t)nch * n + (size_t)c * n);
         if (p.use_iir) {
             j.coef = coef_ptr(0, c);
-            j.coef_stride = (long long)(nch * nu);
-            j.out_stride = (long long)(nch * nu);
-            j.amp_out = s->ampc ? s->ampc + (size_t)c * nu : nullptr;
-            j.filt_out = s->filt ? s->filt + (size_t)c * nu : nullptr;
-            j.state = s->state + (size_t)c * nu;
+            j.coef_stride = (long long)(nch * nus);
+            j.out_stride = (long long)(nch * nus);
+            j.amp_out = s->ampc ? s->ampc + (size_t)c * nus : nullptr;
+            j.filt_out = s->filt ? s->filt + (size_t)c * nus : nullptr;
+            j.state = s->state + (size_t)c * nus;
             memcpy(j.c, s->c10, sizeof(j.c));
             j.preseed = frame0 == 0;
             j.warmup = s->warmup;
         } else {
             j.coef = nullptr;
-            j.ring = s->coef + (size_t)c * s->ring_slots * nu;
-            j.amp_out = s->ampc + (size_t)c * nu;
-            j.coef_stride = (long long)nu;           // ring plane stride
-            j.out_stride = (long long)(nch * nu);    // amp_out / filt_out frame stride
-            j.filt_out = s->filt ? s->filt + (size_t)c * nu : nullptr;
+            j.ring = s->coef + (size_t)c * s->ring_slots * nus;
+            j.amp_out = s->ampc + (size_t)c * nus;
+            j.coef_stride = (long long)nus;          // ring plane stride
+            j.out_stride = (long long)(nch * nus);   // amp_out / filt_out frame stride
+            j.filt_out = s->filt ? s->filt + (size_t)c * nus : nullptr;
             j.window = p.window;
             j.ring_slots = s->ring_slots;
             long first = frame0 - p.window + 1;
@@ -484,7 +485,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
     auto passthrough = [&](int f) { return !p.use_iir && frame0 + f + 1 < p.window; };
     auto planes_ptr = [&](int f, int c, int l) -> const float * {
         if (!s->ampc) return coef_ptr(f, c) + g.plane_off[l];
-        return s->ampc + ((size_t)f * nch + c) * nu + g.plane_off[l];
+        return s->ampc + ((size_t)f * nch + c) * nus + g.plane_off[l];
     };
     const size_t out_b = (out_u8 ? 3 * P : 0) + (out_f32 ? 12 * P : 0);
 
@@ -861,7 +862,7 @@ extern "C" int wema_dev_debug_read(wema_dev_t *s, int which, int frame, int ch, 
         wema_set_error("frame/channel out of range");
         return -1;
     }
-    const size_t nu = s->g.nu, nch = s->g.nch;
+    const size_t nu = s->g.nu, nus = s->g.nus, nch = s->g.nch;
     const long gidx = s->frames - s->last_n + frame;
     const float *src = nullptr;
     size_t avail = nu;
@@ -869,24 +870,24 @@ extern "C" int wema_dev_debug_read(wema_dev_t *s, int which, int frame, int ch, 
     case 0:
     case 2:
         if (which == 2 && s->ampc) {
-            src = s->ampc + ((size_t)frame * nch + ch) * nu;
+            src = s->ampc + ((size_t)frame * nch + ch) * nus;
         } else if (which == 2 || s->p.use_iir) {
             if (which == 0 && !s->ampc) {
                 wema_set_error("original coefficients are amplified in place; enable wema_dev_debug_capture");
                 return -1;
             }
-            src = s->coef + ((size_t)frame * nch + ch) * nu;
+            src = s->coef + ((size_t)frame * nch + ch) * nus;
         } else {
-            src = s->coef + ((size_t)ch * s->ring_slots + (size_t)(gidx % s->ring_slots)) * nu;
+            src = s->coef + ((size_t)ch * s->ring_slots + (size_t)(gidx % s->ring_slots)) * nus;
         }
         break;
     case 1:
         if (!s->filt) { wema_set_error("enable wema_dev_debug_capture first"); return -1; }
-        src = s->filt + ((size_t)frame * nch + ch) * nu;
+        src = s->filt + ((size_t)frame * nch + ch) * nus;
         break;
     case 3:
         if (!s->state) { wema_set_error("no IIR state"); return -1; }
-        src = (const float *)(s->state + (size_t)ch * nu);
+        src = (const float *)(s->state + (size_t)ch * nus);
         avail = 4 * nu;
         break;
     default:

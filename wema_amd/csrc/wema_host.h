@@ -19,6 +19,8 @@ struct WemaGeom {
     size_t plane_off[WEMA_MAX_LEVELS + 1];  // start of level l in the unique-plane layout
     size_t ll_off[WEMA_MAX_LEVELS + 1];     // start of level l's LL in the LL buffer
     size_t nu;                               // unique positions (3 per subband position)
+    size_t nus;                              // their stride in the coefficient buffers (nu
+                                             // rounded up to 64: planes start on 256 bytes)
     size_t ll_total;
 };
 
@@ -62,10 +64,10 @@ struct wema_dev {
     hipStream_t last_stream = nullptr;  // stream of the last process call
     void *arena = nullptr;
     size_t arena_bytes = 0;
-    float *coef = nullptr;      // IIR: [B][nch][nu] (amplified in place); window: ring
-    float *ampc = nullptr;      // amplified coefficients when kept apart: [B][nch][nu]
-    float *filt = nullptr;      // captured temporal output [B][nch][nu] (test hook)
-    float4 *state = nullptr;    // IIR state [nch][nu]
+    float *coef = nullptr;      // IIR: [B][nch][nus] (amplified in place); window: ring
+    float *ampc = nullptr;      // amplified coefficients when kept apart: [B][nch][nus]
+    float *filt = nullptr;      // captured temporal output [B][nch][nus] (test hook)
+    float4 *state = nullptr;    // IIR state [nch][nus]
     float *ll = nullptr;        // [B][nch][ll_total]
     float *rec = nullptr;       // level-0 reconstructions [B][nch][P]
     float2 *ab = nullptr;       // guided filter a, b [B][P]
