@@ -311,6 +311,8 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16, 32],
                     help="guided-filter tile height (0: the autotuner's pick; a fixed height "
                          "skips the tuner's timing launches, e.g. under a profiler)")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the pinned-host pipeline measurement (e.g. under a profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (for rocprofv3 counter scaling)")
     args = ap.parse_args()
@@ -362,7 +364,8 @@ def main():
                     for d in r["kernels"]],
         "cpu_baseline": None,
     }
-    line["host_pipeline"] = run_host_pipeline(cfg, max(args.steps, 4), batch)
+    if not args.no_host:
+        line["host_pipeline"] = run_host_pipeline(cfg, max(args.steps, 4), batch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
     also = {}

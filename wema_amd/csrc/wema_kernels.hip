@@ -24,6 +24,7 @@
 //   k_passthru  window mode before the window is full: output = input.
 //   k_synth     synthetic frames (deterministic integer generator).
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "synth.h"
 #include "wema_jobs.h"
@@ -1137,6 +1138,22 @@ __device__ __forceinline__ void fill_rcp(float *t) {
     if (threadIdx.x < 16) t[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 0.0f;
 }
 
+// Window count at x of n and its reciprocal: inside (IN: the whole tile's windows lie in
+// the frame) 9 and the constant 1/9 (the correctly rounded value, as in the table), else
+// the clipped count and the table entry.
+template <bool IN>
+__device__ __forceinline__ void count_at(int x, int n, const float *sRcp, float &cnt, float &k) {
+    if (IN) {
+        cnt = 9.0f;
+        k = 1.0f / 9.0f;
+    } else {
+        const int c = clip_count(x, n);
+        cnt = (float)c;
+        k = sRcp[c];
+    }
+}
+template <bool B> using bool_c = std::integral_constant<bool, B>;
+
 constexpr int GF_TX = WEMA_GF_TX;        // output columns of a tile
 constexpr int GF_EX = GF_TX + 8;         // with the 4-column halo on each side
 constexpr int GF_PITCH = GF_EX + 4;      // multiple of 4: rows start 16-byte aligned
@@ -1375,7 +1392,11 @@ __device__ __forceinline__ void write_rgb_px(const EgressJob &J, size_t idx, con
 
 // Second pass: box means of a and b, q = mean(a) I + mean(b), RGB out (src/wema.c:221-229,
 // 289-335).
-template <bool U8, int TY>
+// VOUT (RGB24 in and out only, width a multiple of 4, frames on a dword): the vertical pass
+// leaves mean(a) and mean(b) in LDS, and the output stage takes 4 consecutive pixels of a
+// row per thread, reading and writing their RGB24 as 3 dwords; otherwise each thread
+// finishes its own column of 4 rows pixel by pixel.
+template <bool U8, int TY, bool VOUT>
 __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, float (*sA)[GF_PITCH],
                                          float (*sB)[GF_PITCH], float (*hA)[GF_TX], float (*hB)[GF_TX],
                                          const float *sRcp) {
@@ -1383,14 +1404,16 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
     const int w = J.w, h = J.h;
     // the source pixels of this thread's outputs (VI columns of 4 rows) and the a, b tile,
     // all loaded first from coordinates clamped into the frame
-    SrcPix<U8> src[VI][4];
+    SrcPix<U8> src[VOUT ? 1 : VI][4];
+    if (!VOUT) {
 #pragma unroll
-    for (int v = 0; v < VI; ++v) {
-        const int i = threadIdx.x + 256 * v;
-        const int y4 = 4 * (i / GF_TX), gx = x0 + i % GF_TX;
+        for (int v = 0; v < VI; ++v) {
+            const int i = threadIdx.x + 256 * v;
+            const int y4 = 4 * (i / GF_TX), gx = x0 + i % GF_TX;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            src[v][j].load(J, (size_t)min(y0 + y4 + j, h - 1) * w + min(gx, w - 1));
+            for (int j = 0; j < 4; ++j)
+                src[v][j].load(J, (size_t)min(y0 + y4 + j, h - 1) * w + min(gx, w - 1));
+        }
     }
     const GLOBAL_AS float2 *ab = gp(J.ab);
     if ((w & 3) == 0 && ((uintptr_t)J.ab & 15) == 0) {
@@ -1443,6 +1466,10 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
         }
     }
     __syncthreads();
+    // windows wholly inside the frame across the tile (x) and down it (y)
+    const bool in_x = x0 >= 4 && x0 + GF_TX + 4 <= w, in_y = y0 >= 4 && y0 + TY + 4 <= h;
+    auto hpass = [&](auto inx) {
+    constexpr bool INX = decltype(inx)::value;
     for (int i = threadIdx.x; i < EY * (GF_TX / 4); i += 256) {
         const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
         float va[12], vb[12];
@@ -1458,13 +1485,18 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
         sums4(vb, sb);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int n = clip_count(x0 + x4 + j, w);
-            const float cnt = (float)n, k = sRcp[n];
+            float cnt, k;
+            count_at<INX>(x0 + x4 + j, w, sRcp, cnt, k);
             hA[ey][x4 + j] = div_count(sa[j], cnt, k);
             hB[ey][x4 + j] = div_count(sb[j], cnt, k);
         }
     }
+    };
+    if (in_x) hpass(bool_c<true>{});
+    else hpass(bool_c<false>{});
     __syncthreads();
+    auto vpass = [&](auto iny) {
+    constexpr bool INY = decltype(iny)::value;
 #pragma unroll
     for (int v = 0; v < VI; ++v) {
         const int i = threadIdx.x + 256 * v;
@@ -1483,11 +1515,50 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int gy = y0 + y4 + j;
-            if (gy >= h) break;
-            const int n = clip_count(gy, h);
-            const float cnt = (float)n, k = sRcp[n];
-            const float q = fmaf(div_count(sa[j], cnt, k), src[v][j].luma(), div_count(sb[j], cnt, k));
-            write_rgb_px<U8>(J, (size_t)gy * w + gx, src[v][j], q);
+            if (!INY && gy >= h) break;
+            float cnt, k;
+            count_at<INY>(gy, h, sRcp, cnt, k);
+            if (VOUT) {
+                // (sA, sB were last read by the horizontal pass, before the barrier)
+                sA[y4 + j][xo] = div_count(sa[j], cnt, k);
+                sB[y4 + j][xo] = div_count(sb[j], cnt, k);
+            } else {
+                const float q = fmaf(div_count(sa[j], cnt, k), src[v][j].luma(), div_count(sb[j], cnt, k));
+                write_rgb_px<U8>(J, (size_t)gy * w + gx, src[v][j], q);
+            }
+        }
+    }
+    };
+    if (in_y) vpass(bool_c<true>{});
+    else vpass(bool_c<false>{});
+    if (VOUT) {
+        __syncthreads();
+        // 4 pixels of a row per thread: q = mean(a) I + mean(b), out = clamp(src + q)
+        for (int i = threadIdx.x; i < TY * (GF_TX / 4); i += 256) {
+            const int r = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
+            const int gy = y0 + r, gx = x0 + x4;
+            if (gy >= h || gx >= w) continue;
+            const size_t b0 = ((size_t)gy * w + gx) * 3;
+            const GLOBAL_AS unsigned *qs = (const GLOBAL_AS unsigned *)(gp(J.src_u8) + b0);
+            const unsigned d[3] = {qs[0], qs[1], qs[2]};
+            const float4 ma = *reinterpret_cast<const float4 *>(&sA[r][x4]);
+            const float4 mb = *reinterpret_cast<const float4 *>(&sB[r][x4]);
+            const float mav[4] = {ma.x, ma.y, ma.z, ma.w}, mbv[4] = {mb.x, mb.y, mb.z, mb.w};
+            unsigned o[3] = {0u, 0u, 0u};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float c[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) c[k] = u8f((d[(3 * j + k) / 4] >> (8 * ((3 * j + k) % 4))) & 0xFFu);
+                const float q = fmaf(mav[j], luma_of(c[0], c[1], c[2]), mbv[j]);
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    o[(3 * j + k) / 4] |= (unsigned)egress_u8(clamp01(c[k] + q)) << (8 * ((3 * j + k) % 4));
+            }
+            GLOBAL_AS unsigned *os = (GLOBAL_AS unsigned *)(gp(J.out_u8) + b0);
+            os[0] = o[0];
+            os[1] = o[1];
+            os[2] = o[2];
         }
     }
 }
@@ -1499,8 +1570,11 @@ __device__ __forceinline__ void gf2_run(const EgressJob *__restrict__ jobs, int 
     fill_rcp(sRcp);
     int x0, y0;
     const EgressJob &J = gf_tile<TY>(jobs, njobs, x0, y0);
-    if (J.src_u8) gf2_body<true, TY>(J, x0, y0, sA, sB, hA, hB, sRcp);
-    else gf2_body<false, TY>(J, x0, y0, sA, sB, hA, hB, sRcp);
+    const bool vout = J.src_u8 && J.out_u8 && !J.out_f32 && (J.w & 3) == 0 &&
+                      (((uintptr_t)J.src_u8 | (uintptr_t)J.out_u8) & 3) == 0;
+    if (vout) gf2_body<true, TY, true>(J, x0, y0, sA, sB, hA, hB, sRcp);
+    else if (J.src_u8) gf2_body<true, TY, false>(J, x0, y0, sA, sB, hA, hB, sRcp);
+    else gf2_body<false, TY, false>(J, x0, y0, sA, sB, hA, hB, sRcp);
 }
 
 // One kernel per tile height (GF_TX x TY outputs per block); the session's tuner picks one.
