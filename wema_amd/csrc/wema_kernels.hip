@@ -1310,9 +1310,19 @@ __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int 
         else gf1_load<false, TY>(J, x0, y0, sI, sP);
     }
     __syncthreads();
-    // horizontal: 4 outputs per item, taps read as three 16-byte vectors per quantity
+    // horizontal: 4 outputs per item, taps read as three 16-byte vectors per quantity; a
+    // thread's items share their 4 columns (256 is a multiple of GF_TX / 4), so the window
+    // counts and reciprocals are taken once
+    const int hx4 = 4 * ((int)threadIdx.x % (GF_TX / 4));
+    float hcnt[4], hk[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int n = clip_count(x0 + hx4 + j, w);
+        hcnt[j] = (float)n;
+        hk[j] = sRcp[n];
+    }
     for (int i = threadIdx.x; i < EY * (GF_TX / 4); i += 256) {
-        const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
+        const int ey = i / (GF_TX / 4), x4 = hx4;
         float vi[12], vp[12], vii[12], vip[12];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -1333,12 +1343,10 @@ __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int 
         sums4(vip, sip);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int n = clip_count(x0 + x4 + j, w);
-            const float cnt = (float)n, k = sRcp[n];
-            hI[ey][x4 + j] = div_count(si[j], cnt, k);
-            hP[ey][x4 + j] = div_count(spp[j], cnt, k);
-            hII[ey][x4 + j] = div_count(sii[j], cnt, k);
-            hIP[ey][x4 + j] = div_count(sip[j], cnt, k);
+            hI[ey][x4 + j] = div_count(si[j], hcnt[j], hk[j]);
+            hP[ey][x4 + j] = div_count(spp[j], hcnt[j], hk[j]);
+            hII[ey][x4 + j] = div_count(sii[j], hcnt[j], hk[j]);
+            hIP[ey][x4 + j] = div_count(sip[j], hcnt[j], hk[j]);
         }
     }
     __syncthreads();
