@@ -81,6 +81,11 @@ struct DwtiJob {
 };
 
 #define WEMA_MAX_BATCH 256
+// Frames whose coefficient loads the IIR kernel issues together (the state recurrence
+// then runs through them).
+#ifndef WEMA_IIR_G
+#define WEMA_IIR_G 16
+#endif
 
 // Output stages after the inverse DWT (colour mode has none: its inverse writes RGB). The
 // guided-filter passes have one job list per tile height (8, 16, 32 rows), all first-pass

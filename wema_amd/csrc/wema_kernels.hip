@@ -867,28 +867,29 @@ template <bool SEP, bool FILT>
 __device__ __forceinline__ void iir_frames(const TemporalJob &J, const float *c, float4 &st,
                                            const float *sT, const float *sA, float lscale,
                                            long long p) {
-    // frames in groups of 4 with the coefficient loads issued first (independent of the
-    // state recurrence), so several loads per thread are in flight
+    // frames in groups of WEMA_IIR_G with the coefficient loads issued first (independent
+    // of the state recurrence), so several loads per thread are in flight
     const int nframes = J.nframes;
     const size_t stride = (size_t)J.coef_stride, ostride = (size_t)J.out_stride;
     GLOBAL_AS float *in = gp(J.coef) + p;
     GLOBAL_AS float *out = (SEP ? gp(J.amp_out) : gp(J.coef)) + p;
     GLOBAL_AS float *fo = FILT ? gp(J.filt_out) + p : nullptr;
     const size_t ostep = SEP ? ostride : stride;
+    constexpr int G = WEMA_IIR_G;
     int f = 0;
-    for (; f + 4 <= nframes; f += 4) {
-        float x[4];
+    for (; f + G <= nframes; f += G) {
+        float x[G];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) x[k] = in[k * stride];
+        for (int k = 0; k < G; ++k) x[k] = in[k * stride];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < G; ++k) {
             float y;
             out[k * ostep] = iir_amp(c, st, x[k], sT[f + k], sA[f + k] * lscale, y);
             if (FILT) fo[k * ostride] = y;
         }
-        in += 4 * stride;
-        out += 4 * ostep;
-        if (FILT) fo += 4 * ostride;
+        in += G * stride;
+        out += G * ostep;
+        if (FILT) fo += G * ostride;
     }
     for (; f < nframes; ++f) {
         float y;
