@@ -158,17 +158,17 @@ struct JobBuilder {
         return true;
     }
     // Strip height of each DWT launch, from all of its jobs: the tallest of 32, 16 and 8
-    // pair rows that still gives at least WEMA_STRIP_TARGET waves (shorter strips re-read
-    // more halo rows); block ranges follow.
+    // pair rows that still gives the target count of channel-strips (shorter strips re-read
+    // more halo rows, taller ones leave the chip under-filled); block ranges follow.
     template <typename Job>
-    static void strips_of(std::vector<Job> &v, JobList &jl) {
+    static void strips_of(std::vector<Job> &v, JobList &jl, long target) {
         if (v.empty()) return;
         auto strips = [&](const Job &j, int sp) { return (long)j.tiles_x * wema_ceil_div((j.h + 1) >> 1, sp); };
         int sp = WEMA_SP_MAX;
         for (; sp > WEMA_SP_MIN; sp /= 2) {
-            long waves = 0;
-            for (const auto &j : v) waves += strips(j, sp);
-            if (waves >= WEMA_STRIP_TARGET) break;
+            long work = 0;   // channel-strips: a wave of a colour job computes 3 channels
+            for (const auto &j : v) work += strips(j, sp) * (j.nch > 1 ? j.nch : 1);
+            if (work >= target) break;
         }
         int blocks = 0;
         for (auto &j : v) {
@@ -179,9 +179,9 @@ struct JobBuilder {
         jl.blocks = blocks;
     }
     void assign_strips() {
-        for (int l = 0; l < WEMA_MAX_LEVELS; l++) strips_of(dwtf_v[l], dwtf[l]);
-        for (int l = 0; l < WEMA_MAX_LEVELS; l++) strips_of(dwti_v[l], dwti[l]);
-        strips_of(dwti_ycc_v, dwti_ycc);
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++) strips_of(dwtf_v[l], dwtf[l], WEMA_STRIP_TARGET);
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++) strips_of(dwti_v[l], dwti[l], WEMA_STRIP_TARGET_INV);
+        strips_of(dwti_ycc_v, dwti_ycc, WEMA_STRIP_TARGET_INV);
     }
     int pack(void *dst_v, size_t cap, size_t *used) {
         assign_strips();

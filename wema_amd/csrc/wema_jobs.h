@@ -25,8 +25,15 @@
 #define WEMA_SP_MAX 32
 #define WEMA_SP_MIN 8
 #define WEMA_STRIPS_PER_BLOCK 4
+// Strip height per launch: the tallest that still gives this many channel-strips (a colour
+// strip counts 3). Measured on 1080p colour and 4K luma: the forward DWT balances halo
+// re-reads against parallelism at about 64 K; the inverse is fastest with the shortest
+// strips (8 pair rows) unless a launch is far larger.
 #ifndef WEMA_STRIP_TARGET
-#define WEMA_STRIP_TARGET 16384
+#define WEMA_STRIP_TARGET 65536
+#endif
+#ifndef WEMA_STRIP_TARGET_INV
+#define WEMA_STRIP_TARGET_INV 262144
 #endif
 
 // Guided filter tiles (outputs per block).
