@@ -479,6 +479,8 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
         };
         StripRaw<FAM> sa[SLOTS_OF(NCH)], sb[SLOTS_OF(NCH)];
         constexpr int SLOTS = SLOTS_OF(NCH);
+        // (after the 4 warm-up pairs, pair m + k of the loops below is in slot k)
+        static_assert(4 % SLOTS == 0, "the slot count must divide the 4 warm-up pairs");
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) reload(sa[k], sb[k], k);
 #pragma unroll
@@ -623,9 +625,7 @@ struct IStripRaw {
     float ll[NCH], lh[NCH], hl[NCH], hh[NCH];
 };
 
-// (deeper than the forward DWT's: the inverse has 4 planes per channel to load per pair
-// row and little arithmetic to hide their latency behind; measured faster at 8 / 4)
-#define ISLOTS_OF(NCH) ((NCH) == 1 ? 8 : 4)
+#define ISLOTS_OF(NCH) ((NCH) == 1 ? 4 : 2)
 
 // OUT: 0 = one float plane (J.out), 1 = colour egress (RGB24 and/or float RGB).
 template <int NCH, int OUT, bool FAST>
@@ -764,6 +764,8 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
     }
     const int M = min(SP, sh - py0) + 4;
     constexpr int SLOTS = ISLOTS_OF(NCH);
+    // (after the 4 warm-up pairs, pair m + k of the loops below is in slot k)
+    static_assert(4 % SLOTS == 0, "the slot count must divide the 4 warm-up pairs");
     IStripRaw<NCH> s[SLOTS];
 #pragma unroll
     for (int k = 0; k < SLOTS; ++k) s[k] = load(py0 - 2 + k);
