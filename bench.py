@@ -66,7 +66,7 @@ def read_traffic(kernel, workload):
         return None
 
 
-def run_gpu(cfg, name, steps, warmup, batch, tile_rows=0):
+def run_gpu(cfg, name, steps, warmup, batch, tile_rows=()):
     import torch
     import wema_amd
 
@@ -82,7 +82,8 @@ def run_gpu(cfg, name, steps, warmup, batch, tile_rows=0):
                             use_iir=cfg["use_iir"], color=cfg["color"],
                             bilateral=cfg["bilateral"], max_batch=batch)
     if tile_rows:
-        sess.set_tile_rows(tile_rows)
+        sess.set_tile_rows(tile_rows[0], 1)
+        sess.set_tile_rows(tile_rows[-1], 2)
     torch.cuda.synchronize()
     for k in range(warmup):
         sess.process_device(frames[k * batch:(k + 1) * batch], d_out_u8=out[k * batch:(k + 1) * batch], n=batch)
@@ -308,9 +309,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the multi-stream and serving measurements")
     ap.add_argument("--serve-duration", type=float, default=4.0, help="seconds per serving run")
-    ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16, 32],
-                    help="guided-filter tile height (0: the autotuner's pick; a fixed height "
-                         "skips the tuner's timing launches, e.g. under a profiler)")
+    ap.add_argument("--tile-rows", default="0",
+                    help="guided-filter tile height, R or R1,R2 for the two passes, each 8, 16 or "
+                         "32 (0: the autotuner's pick; fixed heights skip the tuner's timing "
+                         "launches, e.g. under a profiler)")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the pinned-host pipeline measurement (e.g. under a profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -335,6 +337,11 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
 
+    tr = [int(x) for x in str(args.tile_rows).split(",")]
+    if len(tr) > 2 or any(x not in (0, 8, 16, 32) for x in tr) or (0 in tr and tr != [0]):
+        print("--tile-rows: R or R1,R2 with each 8, 16 or 32", file=sys.stderr)
+        sys.exit(2)
+    args.tile_rows = [] if tr == [0] else tr
     if args.tile_rows:
         import wema_amd
         wema_amd.autotune(False)

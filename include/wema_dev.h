@@ -109,13 +109,18 @@ int wema_dev_wait_ticket(wema_dev_t *s, long ticket);
 
 /* Tile shapes (no counterpart in the reference). The guided-filter kernels come in tile
  * heights of 8, 16 and 32 rows (64 columns). A session whose output stage is the guided
- * filter takes the height an autotuner picked for its frame size: the first session of a
- * size on a device times the three on frames of that size and keeps the fastest for the
- * process. Results never depend on the tile shape. wema_dev_autotune(0) turns the timing
- * off (16 rows); wema_dev_set_tile_rows forces a height for the session's next calls. */
+ * filter takes, for each of the filter's two passes, the height an autotuner picked for
+ * its frame size: the first session of a size on a device times the three on frames of
+ * that size and keeps the fastest of each pass for the process. Results never depend on
+ * the tile shape. wema_dev_autotune(0) turns the timing off (16 rows).
+ * wema_dev_tile_rows: the first pass's height; wema_dev_tile_rows_pass: pass 1 or 2
+ * (-1 for another pass). wema_dev_set_tile_rows forces one height for both passes of the
+ * session's next calls, wema_dev_set_tile_rows_pass for pass 1, 2 or both (0). */
 void wema_dev_autotune(int enable);
 int  wema_dev_tile_rows(const wema_dev_t *s);
+int  wema_dev_tile_rows_pass(const wema_dev_t *s, int pass);
 int  wema_dev_set_tile_rows(wema_dev_t *s, int rows);
+int  wema_dev_set_tile_rows_pass(wema_dev_t *s, int pass, int rows);
 
 /* Pinned host memory for frame slots (hipHostMalloc). */
 void *wema_dev_host_alloc(size_t bytes);

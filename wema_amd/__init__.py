@@ -97,7 +97,9 @@ _SIGS = {
     "wema_dev_iir_coeffs": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     "wema_dev_frames": (C.c_long, [_vp]),
     "wema_dev_tile_rows": (C.c_int, [_vp]),
+    "wema_dev_tile_rows_pass": (C.c_int, [_vp, C.c_int]),
     "wema_dev_set_tile_rows": (C.c_int, [_vp, C.c_int]),
+    "wema_dev_set_tile_rows_pass": (C.c_int, [_vp, C.c_int, C.c_int]),
     "wema_dev_autotune": (None, [C.c_int]),
     "wema_dev_process_u8": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "wema_dev_process_f32": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
@@ -259,11 +261,12 @@ class Session:
 
     @property
     def tile_rows(self):
-        """Guided-filter tile height the session uses (picked by the autotuner)."""
-        return lib.wema_dev_tile_rows(self._h)
+        """Guided-filter tile heights of the two passes (picked by the autotuner)."""
+        return (lib.wema_dev_tile_rows_pass(self._h, 1), lib.wema_dev_tile_rows_pass(self._h, 2))
 
-    def set_tile_rows(self, rows):
-        _check(lib.wema_dev_set_tile_rows(self._h, int(rows)), "wema_dev_set_tile_rows")
+    def set_tile_rows(self, rows, pass_=0):
+        """Force the tile height of pass 1, pass 2 or both (0)."""
+        _check(lib.wema_dev_set_tile_rows_pass(self._h, int(pass_), int(rows)), "wema_dev_set_tile_rows_pass")
 
     def iir_coeffs(self):
         c = (C.c_float * 10)()

@@ -261,7 +261,8 @@ int wema_dev_open_ex(wema_dev_t **out, const wema_dev_params *p, bool pooled) {
         wema_dev_close(s);
         return -1;
     }
-    if (!p->color && (p->edge_aware || p->batch_path)) s->gf_rows = wema_gf_rows_for(p->device, p->width, p->height);
+    if (!p->color && (p->edge_aware || p->batch_path))
+        wema_gf_rows_for(p->device, p->width, p->height, &s->gf_rows, &s->gf_rows2);
     *out = s;
     return 0;
 }
@@ -530,7 +531,8 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
 
     // output stage
     const int gf_tx = wema_ceil_div(g.W, WEMA_GF_TX);
-    const int gf_tiles = gf_tx * wema_ceil_div(g.H, s->gf_rows);      // guided filter
+    const int gf_tiles1 = gf_tx * wema_ceil_div(g.H, s->gf_rows);     // guided filter, pass 1
+    const int gf_tiles2 = gf_tx * wema_ceil_div(g.H, s->gf_rows2);    // and pass 2
     const int gs_tiles = gf_tx * wema_ceil_div(g.H, WEMA_GF_TY);      // 5x5 path
     const int byte_tiles = (int)((3 * P + 255) / 256);
     for (int f = 0; f < n; f++) {
@@ -552,9 +554,8 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         } else if (p.color) {
             continue;   // written by the colour output job of the inverse stage
         } else if (p.edge_aware || p.batch_path) {
-            const int ri = wema_rows_index(s->gf_rows);
-            jb.add_egress(EG_GF1 + ri, j, gf_tiles, 4.0 * P + src_b + 8.0 * P);
-            jb.add_egress(EG_GF2 + ri, j, gf_tiles, 8.0 * P + src_b + out_b);
+            jb.add_egress(EG_GF1 + wema_rows_index(s->gf_rows), j, gf_tiles1, 4.0 * P + src_b + 8.0 * P);
+            jb.add_egress(EG_GF2 + wema_rows_index(s->gf_rows2), j, gf_tiles2, 8.0 * P + src_b + out_b);
         } else {
             j.tiles_x = gf_tx;
             jb.add_egress(EG_GAUSS, j, gs_tiles, 4.0 * P + src_b + out_b);

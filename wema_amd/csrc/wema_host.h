@@ -57,7 +57,8 @@ struct wema_dev {
     float c10[10];
     int warmup = 0;
     int ring_slots = 0;
-    int gf_rows = 16;       // guided-filter tile height (autotuned at open)
+    int gf_rows = 16;       // guided-filter tile height of the first pass (autotuned at open)
+    int gf_rows2 = 16;      // and of the second pass
     long frames = 0;        // frames processed since open
     int last_n = 0;         // frames of the last call
     hipStream_t stream = nullptr;       // the session's own stream (host-buffer calls)
@@ -206,7 +207,7 @@ void wema_set_error(const char *fmt, ...);
 int wema_design_bandpass(float f_low, float f_high, float fs, float *c10, int *warmup);
 float wema_fade_amp(float amp, long global_frame);
 int wema_auto_levels(int width, int height);
-int wema_gf_rows_for(int device, int w, int h);
+void wema_gf_rows_for(int device, int w, int h, int *rows1, int *rows2);
 void wema_geometry_init(WemaGeom *g, const wema_dev_params *p);
 void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,
                      float *out_f32, int n);

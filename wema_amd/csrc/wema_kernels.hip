@@ -1160,6 +1160,9 @@ __device__ __forceinline__ void count_at(int x, int n, const float *sRcp, float 
 template <bool B> using bool_c = std::integral_constant<bool, B>;
 
 constexpr int GF_TX = WEMA_GF_TX;        // output columns of a tile
+// Threads per block of the guided-filter kernels: the 32-row tiles take 8 waves, so that
+// their larger LDS tile (2 blocks per CU) still gives every SIMD 4 waves.
+#define GF_NT(TY) ((TY) == 32 ? 512 : 256)
 constexpr int GF_EX = GF_TX + 8;         // with the 4-column halo on each side
 constexpr int GF_PITCH = GF_EX + 4;      // multiple of 4: rows start 16-byte aligned
 
@@ -1200,14 +1203,15 @@ struct SrcPix {
 template <bool U8, int TY>
 __device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, float (*sI)[GF_PITCH],
                                          float (*sP)[GF_PITCH]) {
-    constexpr int EY = TY + 8, NLD = (EY * GF_EX + 255) / 256;
+    constexpr int NT = GF_NT(TY);
+    constexpr int EY = TY + 8, NLD = (EY * GF_EX + NT - 1) / NT;
     const int w = J.w, h = J.h;
     const GLOBAL_AS float *gray = gp(J.gray);
     SrcPix<U8> px[NLD];
     float g[NLD];
 #pragma unroll
     for (int j = 0; j < NLD; ++j) {
-        const int i = threadIdx.x + 256 * j;
+        const int i = threadIdx.x + NT * j;
         const int ey = i / GF_EX, ex = i - ey * GF_EX;
         const int x = min(max(x0 - 4 + ex, 0), w - 1), y = min(max(y0 - 4 + ey, 0), h - 1);
         const size_t idx = (size_t)y * w + x;
@@ -1216,7 +1220,7 @@ __device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, flo
     }
 #pragma unroll
     for (int j = 0; j < NLD; ++j) {
-        const int i = threadIdx.x + 256 * j;
+        const int i = threadIdx.x + NT * j;
         if (i < EY * GF_EX) {
             const int ey = i / GF_EX, ex = i - ey * GF_EX;
             const int x = x0 - 4 + ex, y = y0 - 4 + ey;
@@ -1234,13 +1238,14 @@ __device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, flo
 template <bool U8, int TY>
 __device__ __forceinline__ void gf1_load_vec(const EgressJob &J, int x0, int y0, float (*sI)[GF_PITCH],
                                              float (*sP)[GF_PITCH]) {
-    constexpr int GX = GF_EX / 4, NG = (TY + 8) * GX, NIT = (NG + 255) / 256;
+    constexpr int NT = GF_NT(TY);
+    constexpr int GX = GF_EX / 4, NG = (TY + 8) * GX, NIT = (NG + NT - 1) / NT;
     const int w = J.w, h = J.h;
     unsigned raw[NIT][12];
     v4f g[NIT];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-        const int i = threadIdx.x + 256 * it;
+        const int i = threadIdx.x + NT * it;
         const int ey = i / GX, gx = i - ey * GX;
         const int x = min(max(x0 - 4 + 4 * gx, 0), w - 4), y = min(max(y0 - 4 + ey, 0), h - 1);
         const size_t idx = (size_t)y * w + x;
@@ -1260,7 +1265,7 @@ __device__ __forceinline__ void gf1_load_vec(const EgressJob &J, int x0, int y0,
     }
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-        const int i = threadIdx.x + 256 * it;
+        const int i = threadIdx.x + NT * it;
         if (i >= NG) break;
         const int ey = i / GX, gx = i - ey * GX;
         const int x = x0 - 4 + 4 * gx, y = y0 - 4 + ey;
@@ -1300,6 +1305,7 @@ template <int TY>
 __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int njobs, float (*sI)[GF_PITCH],
                                         float (*sP)[GF_PITCH], float (*hI)[GF_TX], float (*hP)[GF_TX],
                                         float (*hII)[GF_TX], float (*hIP)[GF_TX], float *sRcp) {
+    constexpr int NT = GF_NT(TY);
     constexpr int EY = TY + 8;
     fill_rcp(sRcp);
     int x0, y0;
@@ -1316,7 +1322,7 @@ __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int 
     }
     __syncthreads();
     // horizontal: 4 outputs per item, taps read as three 16-byte vectors per quantity; a
-    // thread's items share their 4 columns (256 is a multiple of GF_TX / 4), so the window
+    // thread's items share their 4 columns (NT is a multiple of GF_TX / 4), so the window
     // counts and reciprocals are taken once
     const int hx4 = 4 * ((int)threadIdx.x % (GF_TX / 4));
     float hcnt[4], hk[4];
@@ -1326,7 +1332,7 @@ __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int 
         hcnt[j] = (float)n;
         hk[j] = sRcp[n];
     }
-    for (int i = threadIdx.x; i < EY * (GF_TX / 4); i += 256) {
+    for (int i = threadIdx.x; i < EY * (GF_TX / 4); i += NT) {
         const int ey = i / (GF_TX / 4), x4 = hx4;
         float vi[12], vp[12], vii[12], vip[12];
 #pragma unroll
@@ -1356,7 +1362,7 @@ __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int 
     }
     __syncthreads();
     // vertical: 4 consecutive rows of one column per item
-    for (int i = threadIdx.x; i < (TY / 4) * GF_TX; i += 256) {
+    for (int i = threadIdx.x; i < (TY / 4) * GF_TX; i += NT) {
         const int y4 = 4 * (i / GF_TX), x = i % GF_TX;
         const int gx = x0 + x;
         if (gx >= w) continue;
@@ -1413,7 +1419,8 @@ template <bool U8, int TY, bool VOUT>
 __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, float (*sA)[GF_PITCH],
                                          float (*sB)[GF_PITCH], float (*hA)[GF_TX], float (*hB)[GF_TX],
                                          const float *sRcp) {
-    constexpr int EY = TY + 8, NV = (TY / 4) * GF_TX, VI = (NV + 255) / 256;
+    constexpr int NT = GF_NT(TY);
+    constexpr int EY = TY + 8, NV = (TY / 4) * GF_TX, VI = (NV + NT - 1) / NT;
     const int w = J.w, h = J.h;
     // the source pixels of this thread's outputs (VI columns of 4 rows) and the a, b tile,
     // all loaded first from coordinates clamped into the frame
@@ -1421,7 +1428,7 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
     if (!VOUT) {
 #pragma unroll
         for (int v = 0; v < VI; ++v) {
-            const int i = threadIdx.x + 256 * v;
+            const int i = threadIdx.x + NT * v;
             const int y4 = 4 * (i / GF_TX), gx = x0 + i % GF_TX;
 #pragma unroll
             for (int j = 0; j < 4; ++j)
@@ -1431,11 +1438,11 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
     const GLOBAL_AS float2 *ab = gp(J.ab);
     if ((w & 3) == 0 && ((uintptr_t)J.ab & 15) == 0) {
         // groups of 4 pixels wholly inside or outside the frame, two 16-byte loads each
-        constexpr int GX = GF_EX / 4, NG = EY * GX, NIT = (NG + 255) / 256;
+        constexpr int GX = GF_EX / 4, NG = EY * GX, NIT = (NG + NT - 1) / NT;
         v4f q[NIT][2];
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
-            const int i = threadIdx.x + 256 * it;
+            const int i = threadIdx.x + NT * it;
             const int ey = i / GX, gx = i - ey * GX;
             const int x = min(max(x0 - 4 + 4 * gx, 0), w - 4), y = min(max(y0 - 4 + ey, 0), h - 1);
             const GLOBAL_AS v4f *p = (const GLOBAL_AS v4f *)(ab + (size_t)y * w + x);
@@ -1444,7 +1451,7 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
         }
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
-            const int i = threadIdx.x + 256 * it;
+            const int i = threadIdx.x + NT * it;
             if (i >= NG) break;
             const int ey = i / GX, gx = i - ey * GX;
             const int x = x0 - 4 + 4 * gx, y = y0 - 4 + ey;
@@ -1455,11 +1462,11 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
             *(v4f *)&sB[ey][4 * gx] = in ? b : v4f{0.0f, 0.0f, 0.0f, 0.0f};
         }
     } else {
-        constexpr int NLD = (EY * GF_EX + 255) / 256;
+        constexpr int NLD = (EY * GF_EX + NT - 1) / NT;
         float2 lab[NLD];
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
-            const int i = threadIdx.x + 256 * j;
+            const int i = threadIdx.x + NT * j;
             const int ey = i / GF_EX, ex = i - ey * GF_EX;
             const int x = min(max(x0 - 4 + ex, 0), w - 1), y = min(max(y0 - 4 + ey, 0), h - 1);
             const GLOBAL_AS float *q = (const GLOBAL_AS float *)(ab + (size_t)y * w + x);
@@ -1468,7 +1475,7 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
         }
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
-            const int i = threadIdx.x + 256 * j;
+            const int i = threadIdx.x + NT * j;
             if (i < EY * GF_EX) {
                 const int ey = i / GF_EX, ex = i - ey * GF_EX;
                 const int x = x0 - 4 + ex, y = y0 - 4 + ey;
@@ -1483,7 +1490,7 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
     const bool in_x = x0 >= 4 && x0 + GF_TX + 4 <= w, in_y = y0 >= 4 && y0 + TY + 4 <= h;
     auto hpass = [&](auto inx) {
     constexpr bool INX = decltype(inx)::value;
-    for (int i = threadIdx.x; i < EY * (GF_TX / 4); i += 256) {
+    for (int i = threadIdx.x; i < EY * (GF_TX / 4); i += NT) {
         const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
         float va[12], vb[12];
 #pragma unroll
@@ -1512,7 +1519,7 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
     constexpr bool INY = decltype(iny)::value;
 #pragma unroll
     for (int v = 0; v < VI; ++v) {
-        const int i = threadIdx.x + 256 * v;
+        const int i = threadIdx.x + NT * v;
         const int y4 = 4 * (i / GF_TX), xo = i % GF_TX;
         const int gx = x0 + xo;
         if (i >= NV || gx >= w) break;
@@ -1547,7 +1554,7 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
     if (VOUT) {
         __syncthreads();
         // 4 pixels of a row per thread: q = mean(a) I + mean(b), out = clamp(src + q)
-        for (int i = threadIdx.x; i < TY * (GF_TX / 4); i += 256) {
+        for (int i = threadIdx.x; i < TY * (GF_TX / 4); i += NT) {
             const int r = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
             const int gy = y0 + r, gx = x0 + x4;
             if (gy >= h || gx >= w) continue;
@@ -1592,7 +1599,7 @@ __device__ __forceinline__ void gf2_run(const EgressJob *__restrict__ jobs, int 
 
 // One kernel per tile height (GF_TX x TY outputs per block); the session's tuner picks one.
 #define WEMA_GF_KERNELS(TY)                                                                    \
-    __global__ void __launch_bounds__(256) k_gf1_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
+    __global__ void __launch_bounds__(GF_NT(TY)) k_gf1_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
         __shared__ __attribute__((aligned(16))) float sI[TY + 8][GF_PITCH];                    \
         __shared__ __attribute__((aligned(16))) float sP[TY + 8][GF_PITCH];                    \
         __shared__ float hI[TY + 8][GF_TX], hP[TY + 8][GF_TX];                                 \
@@ -1600,7 +1607,7 @@ __device__ __forceinline__ void gf2_run(const EgressJob *__restrict__ jobs, int 
         __shared__ float sRcp[16];                                                             \
         gf1_run<TY>(jobs, njobs, sI, sP, hI, hP, hII, hIP, sRcp);                              \
     }                                                                                          \
-    __global__ void __launch_bounds__(256) k_gf2_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
+    __global__ void __launch_bounds__(GF_NT(TY)) k_gf2_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
         __shared__ __attribute__((aligned(16))) float sA[TY + 8][GF_PITCH];                    \
         __shared__ __attribute__((aligned(16))) float sB[TY + 8][GF_PITCH];                    \
         __shared__ float hA[TY + 8][GF_TX], hB[TY + 8][GF_TX];                                 \
@@ -1761,12 +1768,12 @@ hipError_t wema_launch_egress(int kind, int rows, const EgressJob *jobs, int njo
     switch (kind) {
     case EG_GF1:
         if (rows == 8) hipLaunchKernelGGL(k_gf1_r8, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
-        else if (rows == 32) hipLaunchKernelGGL(k_gf1_r32, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+        else if (rows == 32) hipLaunchKernelGGL(k_gf1_r32, dim3(nblocks), dim3(GF_NT(32)), 0, s, jobs, njobs);
         else hipLaunchKernelGGL(k_gf1_r16, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
         break;
     case EG_GF2:
         if (rows == 8) hipLaunchKernelGGL(k_gf2_r8, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
-        else if (rows == 32) hipLaunchKernelGGL(k_gf2_r32, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+        else if (rows == 32) hipLaunchKernelGGL(k_gf2_r32, dim3(nblocks), dim3(GF_NT(32)), 0, s, jobs, njobs);
         else hipLaunchKernelGGL(k_gf2_r16, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
         break;
     case EG_GAUSS: hipLaunchKernelGGL(k_gauss, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;

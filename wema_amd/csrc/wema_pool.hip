@@ -207,7 +207,10 @@ extern "C" int wema_pool_timing_report(wema_pool_t *p, int kind, char *name, int
 static int egress_kind(const wema_dev *s) {
     if (!s->p.use_iir && s->frames + 1 < s->p.window) return EG_PASSTHRU;
     if (s->p.color) return EG_KINDS;
-    return (s->p.edge_aware || s->p.batch_path) ? EG_GF1 + wema_rows_index(s->gf_rows) : EG_GAUSS;
+    // (the guided filter's two tile heights, as one code)
+    return (s->p.edge_aware || s->p.batch_path)
+               ? EG_KINDS + 1 + wema_rows_index(s->gf_rows) + 3 * wema_rows_index(s->gf_rows2)
+               : EG_GAUSS;
 }
 
 extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8_t *const *d_in,
