@@ -1301,10 +1301,15 @@ __device__ __forceinline__ const EgressJob &gf_tile(const EgressJob *__restrict_
 }
 
 // First pass: box means of I, p, I^2, I p over the tile, then a and b (src/wema.c:173-219).
+// The horizontal means of I and p are written over the tile of I and p they come from
+// (hI, hP alias sI, sP): every thread computes all of its items first, then a barrier
+// separates the last read of sI, sP from the first write.
 template <int TY>
 __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int njobs, float (*sI)[GF_PITCH],
-                                        float (*sP)[GF_PITCH], float (*hI)[GF_TX], float (*hP)[GF_TX],
-                                        float (*hII)[GF_TX], float (*hIP)[GF_TX], float *sRcp) {
+                                        float (*sP)[GF_PITCH], float (*hII)[GF_TX], float (*hIP)[GF_TX],
+                                        float *sRcp) {
+    float (*hI)[GF_PITCH] = sI;
+    float (*hP)[GF_PITCH] = sP;
     constexpr int NT = GF_NT(TY);
     constexpr int EY = TY + 8;
     fill_rcp(sRcp);
@@ -1332,7 +1337,12 @@ __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int 
         hcnt[j] = (float)n;
         hk[j] = sRcp[n];
     }
-    for (int i = threadIdx.x; i < EY * (GF_TX / 4); i += NT) {
+    constexpr int NI = (EY * (GF_TX / 4) + NT - 1) / NT;   // items per thread
+    float hv[NI][4][4];
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+        const int i = threadIdx.x + NT * it;
+        if (i >= EY * (GF_TX / 4)) break;
         const int ey = i / (GF_TX / 4), x4 = hx4;
         float vi[12], vp[12], vii[12], vip[12];
 #pragma unroll
@@ -1354,11 +1364,22 @@ __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int 
         sums4(vip, sip);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            hI[ey][x4 + j] = div_count(si[j], hcnt[j], hk[j]);
-            hP[ey][x4 + j] = div_count(spp[j], hcnt[j], hk[j]);
-            hII[ey][x4 + j] = div_count(sii[j], hcnt[j], hk[j]);
-            hIP[ey][x4 + j] = div_count(sip[j], hcnt[j], hk[j]);
+            hv[it][0][j] = div_count(si[j], hcnt[j], hk[j]);
+            hv[it][1][j] = div_count(spp[j], hcnt[j], hk[j]);
+            hv[it][2][j] = div_count(sii[j], hcnt[j], hk[j]);
+            hv[it][3][j] = div_count(sip[j], hcnt[j], hk[j]);
         }
+    }
+    __syncthreads();   // (every read of sI, sP is done)
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+        const int i = threadIdx.x + NT * it;
+        if (i >= EY * (GF_TX / 4)) break;
+        const int ey = i / (GF_TX / 4), x4 = hx4;
+        *reinterpret_cast<float4 *>(&hI[ey][x4]) = make_float4(hv[it][0][0], hv[it][0][1], hv[it][0][2], hv[it][0][3]);
+        *reinterpret_cast<float4 *>(&hP[ey][x4]) = make_float4(hv[it][1][0], hv[it][1][1], hv[it][1][2], hv[it][1][3]);
+        *reinterpret_cast<float4 *>(&hII[ey][x4]) = make_float4(hv[it][2][0], hv[it][2][1], hv[it][2][2], hv[it][2][3]);
+        *reinterpret_cast<float4 *>(&hIP[ey][x4]) = make_float4(hv[it][3][0], hv[it][3][1], hv[it][3][2], hv[it][3][3]);
     }
     __syncthreads();
     // vertical: 4 consecutive rows of one column per item
@@ -1602,10 +1623,10 @@ __device__ __forceinline__ void gf2_run(const EgressJob *__restrict__ jobs, int 
     __global__ void __launch_bounds__(GF_NT(TY)) k_gf1_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
         __shared__ __attribute__((aligned(16))) float sI[TY + 8][GF_PITCH];                    \
         __shared__ __attribute__((aligned(16))) float sP[TY + 8][GF_PITCH];                    \
-        __shared__ float hI[TY + 8][GF_TX], hP[TY + 8][GF_TX];                                 \
-        __shared__ float hII[TY + 8][GF_TX], hIP[TY + 8][GF_TX];                               \
+        __shared__ __attribute__((aligned(16))) float hII[TY + 8][GF_TX];                      \
+        __shared__ __attribute__((aligned(16))) float hIP[TY + 8][GF_TX];                      \
         __shared__ float sRcp[16];                                                             \
-        gf1_run<TY>(jobs, njobs, sI, sP, hI, hP, hII, hIP, sRcp);                              \
+        gf1_run<TY>(jobs, njobs, sI, sP, hII, hIP, sRcp);                                      \
     }                                                                                          \
     __global__ void __launch_bounds__(GF_NT(TY)) k_gf2_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
         __shared__ __attribute__((aligned(16))) float sA[TY + 8][GF_PITCH];                    \
