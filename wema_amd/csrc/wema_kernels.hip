@@ -1436,10 +1436,14 @@ __device__ __forceinline__ void write_rgb_px(const EgressJob &J, size_t idx, con
 // leaves mean(a) and mean(b) in LDS, and the output stage takes 4 consecutive pixels of a
 // row per thread, reading and writing their RGB24 as 3 dwords; otherwise each thread
 // finishes its own column of 4 rows pixel by pixel.
+// As in the first pass, the horizontal means of a and b overwrite the a, b tile once every
+// thread has read it, and (VOUT) mean(a), mean(b) overwrite those once every thread has
+// read them.
 template <bool U8, int TY, bool VOUT>
 __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, float (*sA)[GF_PITCH],
-                                         float (*sB)[GF_PITCH], float (*hA)[GF_TX], float (*hB)[GF_TX],
-                                         const float *sRcp) {
+                                         float (*sB)[GF_PITCH], const float *sRcp) {
+    float (*hA)[GF_PITCH] = sA;
+    float (*hB)[GF_PITCH] = sB;
     constexpr int NT = GF_NT(TY);
     constexpr int EY = TY + 8, NV = (TY / 4) * GF_TX, VI = (NV + NT - 1) / NT;
     const int w = J.w, h = J.h;
@@ -1509,9 +1513,14 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
     __syncthreads();
     // windows wholly inside the frame across the tile (x) and down it (y)
     const bool in_x = x0 >= 4 && x0 + GF_TX + 4 <= w, in_y = y0 >= 4 && y0 + TY + 4 <= h;
+    constexpr int NI = (EY * (GF_TX / 4) + NT - 1) / NT;   // horizontal items per thread
+    float hv[NI][2][4];
     auto hpass = [&](auto inx) {
     constexpr bool INX = decltype(inx)::value;
-    for (int i = threadIdx.x; i < EY * (GF_TX / 4); i += NT) {
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+        const int i = threadIdx.x + NT * it;
+        if (i >= EY * (GF_TX / 4)) break;
         const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
         float va[12], vb[12];
 #pragma unroll
@@ -1528,14 +1537,24 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
         for (int j = 0; j < 4; ++j) {
             float cnt, k;
             count_at<INX>(x0 + x4 + j, w, sRcp, cnt, k);
-            hA[ey][x4 + j] = div_count(sa[j], cnt, k);
-            hB[ey][x4 + j] = div_count(sb[j], cnt, k);
+            hv[it][0][j] = div_count(sa[j], cnt, k);
+            hv[it][1][j] = div_count(sb[j], cnt, k);
         }
     }
     };
     if (in_x) hpass(bool_c<true>{});
     else hpass(bool_c<false>{});
+    __syncthreads();   // (every read of the a, b tile is done)
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+        const int i = threadIdx.x + NT * it;
+        if (i >= EY * (GF_TX / 4)) break;
+        const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
+        *reinterpret_cast<float4 *>(&hA[ey][x4]) = make_float4(hv[it][0][0], hv[it][0][1], hv[it][0][2], hv[it][0][3]);
+        *reinterpret_cast<float4 *>(&hB[ey][x4]) = make_float4(hv[it][1][0], hv[it][1][1], hv[it][1][2], hv[it][1][3]);
+    }
     __syncthreads();
+    float mv[VOUT ? VI : 1][2][4];
     auto vpass = [&](auto iny) {
     constexpr bool INY = decltype(iny)::value;
 #pragma unroll
@@ -1560,9 +1579,8 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
             float cnt, k;
             count_at<INY>(gy, h, sRcp, cnt, k);
             if (VOUT) {
-                // (sA, sB were last read by the horizontal pass, before the barrier)
-                sA[y4 + j][xo] = div_count(sa[j], cnt, k);
-                sB[y4 + j][xo] = div_count(sb[j], cnt, k);
+                mv[v][0][j] = div_count(sa[j], cnt, k);
+                mv[v][1][j] = div_count(sb[j], cnt, k);
             } else {
                 const float q = fmaf(div_count(sa[j], cnt, k), src[v][j].luma(), div_count(sb[j], cnt, k));
                 write_rgb_px<U8>(J, (size_t)gy * w + gx, src[v][j], q);
@@ -1573,6 +1591,19 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
     if (in_y) vpass(bool_c<true>{});
     else vpass(bool_c<false>{});
     if (VOUT) {
+        __syncthreads();   // (every read of the horizontal means is done)
+#pragma unroll
+        for (int v = 0; v < VI; ++v) {
+            const int i = threadIdx.x + NT * v;
+            const int y4 = 4 * (i / GF_TX), xo = i % GF_TX;
+            if (i >= NV || x0 + xo >= w) break;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (y0 + y4 + j >= h) break;
+                sA[y4 + j][xo] = mv[v][0][j];
+                sB[y4 + j][xo] = mv[v][1][j];
+            }
+        }
         __syncthreads();
         // 4 pixels of a row per thread: q = mean(a) I + mean(b), out = clamp(src + q)
         for (int i = threadIdx.x; i < TY * (GF_TX / 4); i += NT) {
@@ -1606,16 +1637,15 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
 
 template <int TY>
 __device__ __forceinline__ void gf2_run(const EgressJob *__restrict__ jobs, int njobs, float (*sA)[GF_PITCH],
-                                        float (*sB)[GF_PITCH], float (*hA)[GF_TX], float (*hB)[GF_TX],
-                                        float *sRcp) {
+                                        float (*sB)[GF_PITCH], float *sRcp) {
     fill_rcp(sRcp);
     int x0, y0;
     const EgressJob &J = gf_tile<TY>(jobs, njobs, x0, y0);
     const bool vout = J.src_u8 && J.out_u8 && !J.out_f32 && (J.w & 3) == 0 &&
                       (((uintptr_t)J.src_u8 | (uintptr_t)J.out_u8) & 3) == 0;
-    if (vout) gf2_body<true, TY, true>(J, x0, y0, sA, sB, hA, hB, sRcp);
-    else if (J.src_u8) gf2_body<true, TY, false>(J, x0, y0, sA, sB, hA, hB, sRcp);
-    else gf2_body<false, TY, false>(J, x0, y0, sA, sB, hA, hB, sRcp);
+    if (vout) gf2_body<true, TY, true>(J, x0, y0, sA, sB, sRcp);
+    else if (J.src_u8) gf2_body<true, TY, false>(J, x0, y0, sA, sB, sRcp);
+    else gf2_body<false, TY, false>(J, x0, y0, sA, sB, sRcp);
 }
 
 // One kernel per tile height (GF_TX x TY outputs per block); the session's tuner picks one.
@@ -1631,9 +1661,8 @@ __device__ __forceinline__ void gf2_run(const EgressJob *__restrict__ jobs, int 
     __global__ void __launch_bounds__(GF_NT(TY)) k_gf2_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
         __shared__ __attribute__((aligned(16))) float sA[TY + 8][GF_PITCH];                    \
         __shared__ __attribute__((aligned(16))) float sB[TY + 8][GF_PITCH];                    \
-        __shared__ float hA[TY + 8][GF_TX], hB[TY + 8][GF_TX];                                 \
         __shared__ float sRcp[16];                                                             \
-        gf2_run<TY>(jobs, njobs, sA, sB, hA, hB, sRcp);                                        \
+        gf2_run<TY>(jobs, njobs, sA, sB, sRcp);                                                \
     }
 WEMA_GF_KERNELS(8)
 WEMA_GF_KERNELS(16)
