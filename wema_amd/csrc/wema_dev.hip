@@ -2,6 +2,8 @@
 // stage launches, IIR design, synthetic frames.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -300,17 +302,18 @@ int wema_session_alloc(wema_dev *s, int with_filt) {
     s->thr_ovr_d = (float *)(a + L.thr_ovr);
     s->tw_d = (float *)(a + L.tw);
     s->arena_bytes = L.total;
-    // time weights of the bilateral window (src/temporal_filter.c:210-218)
+    // time weights of the bilateral window (src/temporal_filter.c:210-218), as their base-2
+    // logarithms: the kernels fold them into the exponent of the value weight
     if (!s->p.use_iir) {
         const int ws = s->p.window;
-        std::vector<float> tw(256, 0.0f);
+        std::vector<float> ltw(256, 0.0f);
         const float ts = (float)ws * 0.15f;
         const float inv = 1.0f / (2.0f * ts * ts);
         for (int t = 0; t < ws; t++) {
             const float d = (float)(t - ws / 2);
-            tw[t] = expf(-(d * d) * inv);
+            ltw[t] = (float)std::log2((double)expf(-(d * d) * inv));
         }
-        HIP_OK(hipMemcpy(s->tw_d, tw.data(), 256 * sizeof(float), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->tw_d, ltw.data(), 256 * sizeof(float), hipMemcpyHostToDevice));
     }
     s->with_filt = with_filt;
     for (int k = 0; k < 2 && !s->pooled; k++) {
@@ -472,7 +475,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
             if (first < 0) first += s->ring_slots;
             j.ring_first = (int)first;
             j.bilateral = p.bilateral;
-            j.tw = s->tw_d;
+            j.ltw = s->tw_d;
         }
         // IIR: coefficients read + written per frame, state read + written once; window:
         // the ws - 1 + n ring samples the call's windows cover, read once, and one amplified

@@ -76,7 +76,7 @@ struct wema_dev {
     float *amp_d = nullptr;     // [nch][B] effective amplification
     float *thr_d = nullptr;     // [B][3] thresholds used
     float *thr_ovr_d = nullptr; // [B][3] overrides (NaN = none)
-    float *tw_d = nullptr;      // bilateral time weights [256]
+    float *tw_d = nullptr;      // log2 of the bilateral time weights [256]
     int thr_ovr_active = 0;
     int with_filt = 0;
     void *stage_h[2] = {nullptr, nullptr};

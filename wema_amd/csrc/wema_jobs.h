@@ -112,7 +112,7 @@ struct TemporalJob {
     const float *thr_override;      // [nframes] stride 3, NaN = compute (or null)
     float *thr_out;                 // [nframes] stride 3 (or null)
     const float *amp;               // [nframes] effective amplification of each frame
-    const float *tw;                // window mode: bilateral time weights [ws]
+    const float *ltw;               // window mode: log2 of the bilateral time weights [ws]
     long long out_stride;           // frame stride of amp_out / filt_out
     int amp_index;                  // host-side index of amp in the uploaded amp array
     float c[10];                    // HP b0 b1 b2 a1 a2, LP b0 b1 b2 a1 a2

@@ -948,13 +948,17 @@ __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ job
 // read twice, oldest first: once for the mean (and, bilateral, the mean absolute deviation
 // from the centre sample), once for the bilateral weights and the weighted sum (the
 // reference's two weight loops read the same values with the same weights, so one pass
-// with one exponential per sample gives the same sums). The weights use the hardware
-// exp2 (relative error of a few 1e-7 where it matters, against about 6e-8 for libm's expf).
+// with one exponential per sample gives the same sums). Each weight is one hardware exp2
+// of the summed base-2 exponents of its value and time weights (relative error of a few
+// 1e-7 where it matters, against about 6e-8 for libm's expf and the product).
 
 // Temporal output of one frame from its window, given in order by win.each(fn(t, v)).
+// ltw: base-2 logarithms of the time weights; the weight of a sample is
+// exp2(fma(d^2, -log2(e) / (2 sigma^2), log2(tw))), the product of the value and time
+// weights in one exponential.
 template <class Win>
 __device__ __forceinline__ float win_delta(const Win &win, int ws, float inv_ws, float cur, bool bilateral,
-                                           const float *tw, float &mean_out) {
+                                           const float *ltw, float &mean_out) {
     float mean = 0.0f, mad = 0.0f;
     win.each([&](int, float v) {
         mean += v;
@@ -969,10 +973,11 @@ __device__ __forceinline__ float win_delta(const Win &win, int ws, float inv_ws,
     if (bilateral) {
         const float sigma = fmaxf(mad * inv_ws * 0.5f, 0.001f);
         const float inv2 = 1.0f / (2.0f * sigma * sigma);
+        const float c = inv2 * -1.4426950408889634f;   // -log2(e) / (2 sigma^2)
         float wsum = 0.0f, acc = 0.0f;
         win.each([&](int t, float v) {
             const float d = v - cur;
-            const float wt = __expf(-(d * d) * inv2) * tw[t];
+            const float wt = __builtin_amdgcn_exp2f(fmaf(d * d, c, ltw[t]));
             wsum += wt;
             acc = fmaf(wt, v - mean, acc);
         });
@@ -1036,7 +1041,7 @@ __global__ void __launch_bounds__(256) k_win(const TemporalJob *__restrict__ job
         const float cur = gp(J.ring)[(size_t)((first + center) % J.ring_slots) * plane + p];
         const float orig = gp(J.ring)[(size_t)((first + ws - 1) % J.ring_slots) * plane + p];
         float mean;
-        const float delta = win_delta(win, ws, inv_ws, cur, J.bilateral != 0, J.tw, mean);
+        const float delta = win_delta(win, ws, inv_ws, cur, J.bilateral != 0, J.ltw, mean);
         win_store(J, f, p, orig, delta, sT, sA, lscale);
     }
 }
@@ -1052,7 +1057,7 @@ __device__ __forceinline__ void win_lds_body(const TemporalJob *__restrict__ job
     frame_params(J, sT, sA);
     const int nframes = J.nframes, ws = J.window, center = ws / 2;
     if (J.bilateral)
-        for (int t = threadIdx.x; t < ws; t += blockDim.x) sTw[t] = J.tw[t];
+        for (int t = threadIdx.x; t < ws; t += blockDim.x) sTw[t] = J.ltw[t];
     __syncthreads();
     const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
     if (p >= J.n) return;
