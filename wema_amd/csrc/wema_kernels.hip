@@ -53,6 +53,14 @@ __device__ __forceinline__ GLOBAL_AS T *gp(T *p) {
     return (GLOBAL_AS T *)p;
 }
 
+// The element at a 32-bit byte offset from base: with base the same for the whole wave,
+// the access takes it from scalar registers and the offset from one vector register (no
+// 64-bit address arithmetic per lane).
+template <typename T>
+__device__ __forceinline__ GLOBAL_AS T &at_b(GLOBAL_AS T *base, unsigned byte_off) {
+    return *(GLOBAL_AS T *)((GLOBAL_AS char *)base + byte_off);
+}
+
 // Workgroups are dealt to the 8 XCDs of the chip round-robin (block b runs on XCD b % 8),
 // and each XCD has its own L2. Tile kernels re-read a halo around their tile, so blocks
 // that run on one XCD are given consecutive tiles: neighbours then share that XCD's L2.
@@ -378,10 +386,11 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
         GLOBAL_AS float *LLc = gp(J.ll) + c * J.ll_cstride + (size_t)py * sw;
         GLOBAL_AS float *LHc = gp(J.planes) + c * J.planes_cstride + (size_t)py * sw;
         if (col_ok) {
-            LLc[px] = ll;
-            LHc[px] = lh;
-            LHc[n + px] = hh;
-            LHc[2 * n + px] = hl;
+            const unsigned xb = 4u * (unsigned)px;
+            at_b(LLc, xb) = ll;
+            at_b(LHc, xb) = lh;
+            at_b(LHc + n, xb) = hh;
+            at_b(LHc + 2 * n, xb) = hl;
         }
         bool small = true;
         if (FAM != FAM_U8_RGB)   // (RGB24 samples give coefficients below 64)
@@ -658,15 +667,15 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
         // (FAST strips: h >= 8, so one reflection reaches every row a strip reads)
         ye2 = FAST ? (ye2 < 0 ? -ye2 : ye2 >= h ? 2 * h - 2 - ye2 : ye2) : reflect_idx(ye2, h);
         yo2 = FAST ? (yo2 < 0 ? -yo2 : yo2 >= h ? 2 * h - 2 - yo2 : yo2) : reflect_idx(yo2, h);
-        const unsigned re = (unsigned)(ye2 >> 1) * (unsigned)sw, ro = (unsigned)(yo2 >> 1) * (unsigned)sw;
+        const size_t re = (size_t)(ye2 >> 1) * sw, ro = (size_t)(yo2 >> 1) * sw;
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const GLOBAL_AS float *LL = gp(J.ll) + c * J.ll_cstride;
             const GLOBAL_AS float *LH = gp(J.planes) + c * J.planes_cstride;
-            r.ll[c] = LL[re + colL];
-            r.lh[c] = LH[ro + colL];
-            r.hl[c] = LH[2 * n + re + colH];
-            r.hh[c] = LH[n + ro + colH];
+            r.ll[c] = at_b(LL + re, 4u * colL);
+            r.lh[c] = at_b(LH + ro, 4u * colL);
+            r.hl[c] = at_b(LH + 2 * n + re, 4u * colH);
+            r.hh[c] = at_b(LH + n + ro, 4u * colH);
         }
         return r;
     };
@@ -687,13 +696,14 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
         if (!lane_ok || y >= h) return;
         if constexpr (OUT == 0) {
             GLOBAL_AS float *row = gp(J.out) + (size_t)y * w;
+            const unsigned xb = 4u * (unsigned)x;
             if (even_w && two) {
                 // (even width: the pair starts on 8 bytes)
                 typedef float v2f __attribute__((ext_vector_type(2)));
-                *(GLOBAL_AS v2f *)(row + x) = v2f{ev[0], ov[0]};
+                at_b((GLOBAL_AS v2f *)row, xb) = v2f{ev[0], ov[0]};
             } else {
-                row[x] = ev[0];
-                if (two) row[x + 1] = ov[0];
+                at_b(row, xb) = ev[0];
+                if (two) at_b(row, xb + 4u) = ov[0];
             }
         } else {
             float v0[3], v1[3];
@@ -703,7 +713,9 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
             if (J.out_u8) {
                 GLOBAL_AS unsigned char *o = gp(J.out_u8) + idx;
                 const unsigned b0 = egress_u8(v0[0]), b1 = egress_u8(v0[1]), b2 = egress_u8(v0[2]);
-                const size_t a = (size_t)o;
+                // row base (the same for the wave) and the pair's byte offset in the row
+                GLOBAL_AS unsigned char *rb = gp(J.out_u8) + (size_t)y * w * 3;
+                const unsigned xo = 3u * (unsigned)x, a = (unsigned)(size_t)rb + xo;
                 if (two && (a & 1) == 0) {
                     // the 6 bytes as one dword and one short, both aligned: bytes 0-3 and
                     // 4-5 when the pair starts on a dword, else bytes 0-1 and 2-5
@@ -712,8 +724,8 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
                     const bool d0 = (a & 2) == 0;
                     const unsigned dw = d0 ? lo : (lo >> 16) | (hi << 16);
                     const unsigned short sv = (unsigned short)(d0 ? hi : lo);
-                    *(GLOBAL_AS unsigned *)(a + (d0 ? 0 : 2)) = dw;
-                    *(GLOBAL_AS unsigned short *)(a + (d0 ? 4 : 0)) = sv;
+                    at_b((GLOBAL_AS unsigned *)rb, xo + (d0 ? 0u : 2u)) = dw;
+                    at_b((GLOBAL_AS unsigned short *)rb, xo + (d0 ? 4u : 0u)) = sv;
                 } else {
                     o[0] = b0;
                     o[1] = b1;
