@@ -82,6 +82,39 @@ def test_dwt_odd_level0_differs_only_at_unwritten_entries():
     assert np.array_equal(planes[:n0][mask[:n0]], ref_planes[:n0][mask[:n0]])
 
 
+def _fma32(a, b, c):
+    """fmaf on float32 operands: the product is exact in float64 and, for operands in
+    [0, 1], so is the sum, so one rounding to float32 remains."""
+    return (np.float64(a) * np.asarray(b, np.float64) + np.asarray(c, np.float64)).astype(np.float32)
+
+
+@pytest.mark.parametrize("npx", [800, 801, 805, 807, 808, 813, 3263])
+def test_luma_loop_epilogue_contraction(npx):
+    """frame_rgb_to_gray (src/wema.c:42-51) as compiled (-Ofast, AVX-512) runs 16- and
+    8-wide vector bodies, whose contraction fma(0.114 b, fma(0.299 r, 0.587 g)) the oracle
+    and the kernels use at every pixel, and a scalar epilogue for the last npx mod 8
+    pixels, which evaluates fma(0.299 r, fma(0.587 g, 0.114 b)). Frame sizes whose pixel
+    count is not a multiple of 8 therefore differ from the reference by an ulp of Y at up
+    to 7 pixels of the last row."""
+    f = np.float32
+    rng = np.random.default_rng(npx)
+    tail = npx - npx % 8
+    seen_diff = False
+    for _ in range(20):
+        u8 = rng.integers(0, 256, npx * 3).astype(np.uint8)
+        rgb = u8.astype(f) * f(1.0 / 255.0)
+        ref = np.zeros(npx, f)
+        O.ref().refh_rgb_to_gray(ptr(rgb), npx, 1, ptr(ref))
+        orc = np.zeros(npx, f)
+        O.orc().orc_ingest_f32(ptr(u8), npx, 0, ptr(orc))
+        assert np.array_equal(orc[:tail], ref[:tail])
+        r, g, b = rgb[0::3][tail:], rgb[1::3][tail:], rgb[2::3][tail:]
+        scalar = _fma32(f(0.299), r, _fma32(f(0.587), g, (f(0.114) * b).astype(f)))
+        assert np.array_equal(scalar, ref[tail:])
+        seen_diff |= not np.array_equal(orc[tail:], ref[tail:])
+    assert seen_diff == (npx % 8 != 0)
+
+
 def test_iir_design_bitexact_sweep():
     bad = 0
     for fs in (24.0, 25.0, 29.97, 30.0, 50.0, 59.94, 60.0, 120.0):
