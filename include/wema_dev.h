@@ -29,8 +29,13 @@ typedef struct wema_dev wema_dev_t;
 
 /* Options of one session. Mirrors WemaConfig (src/wema.h:192-212) and the fields main.c
  * writes after wema_init (src/main.c:288-290). */
+/* Largest frame, in pixels (16384 x 16384). Frame offsets stay below 2^31 bytes per plane
+ * and the exact Q32 threshold sum cannot overflow its 64 bits. The reference indexes a
+ * frame's floats with an int (src/wema.c:44-50), which limits it to 715,827,882 pixels. */
+#define WEMA_DEV_MAX_PIXELS 268435456L
+
 typedef struct {
-    int   width, height;
+    int   width, height;  /* 1 <= width * height <= WEMA_DEV_MAX_PIXELS */
     float fps;
     float amp;            /* amplification factor (-a, main.c:96-105) */
     float f_low, f_high;  /* band edges in Hz (--fl/--fh); IIR mode */

@@ -82,6 +82,13 @@ def test_query_bytes_and_rejections():
         wema_amd.Session.query_bytes(640, 480, 30.0, use_iir=0, window=2)
     with pytest.raises(wema_amd.WemaError, match="max_batch"):
         wema_amd.Session.query_bytes(640, 480, 30.0, max_batch=0)
+    # the largest frame is accepted, one more row is not
+    big = wema_amd.Session.query_bytes(16384, 16384, 30.0, f_high=3.0)
+    assert big > 16384 * 16384 * 16
+    with pytest.raises(wema_amd.WemaError, match="exceeds"):
+        wema_amd.Session.query_bytes(16384, 16385, 30.0, f_high=3.0)
+    with pytest.raises(wema_amd.WemaError, match="exceeds"):
+        wema_amd.Session.query_bytes(65536, 65536, 30.0, f_high=3.0)
 
 
 def test_design_matches_oracle():

@@ -213,6 +213,52 @@ def test_tiny_and_degenerate_shapes_vs_oracle(w, h, levels, mode):
         assert np.array_equal(g_out[t], o_out), (t, np.abs(g_out[t].astype(int) - o_out.astype(int)).max())
 
 
+@pytest.mark.parametrize("w,h,levels,mode", [(16384, 16, 1, "iir"), (16, 16384, 1, "iir_color"),
+                                             (8191, 9, 2, "iir_color"), (4097, 64, 4, "iir")])
+def test_extreme_aspect_shapes_vs_oracle(w, h, levels, mode):
+    """Frames of the widest and tallest shapes: 137 strip columns in one strip row, one
+    strip column of 8192 pair rows, and wide odd sizes whose last strip holds a few pair
+    columns. Thresholds and output bytes equal to the oracle's."""
+    fps, n = 30.0, 4
+    kw = dict(fps=fps, amp=90.0, f_low=0.5, f_high=3.0, levels=levels, color=mode == "iir_color")
+    rng = np.random.default_rng(w * 7 + h)
+    frames = rng.integers(0, 256, size=(n, h, w, 3), dtype=np.uint8)
+    _, g_out, g_thr = run_gpu(frames, batch=2, **kw)
+    orc = O.OraclePipe(w, h, **kw)
+    nch = 3 if kw["color"] else 1
+    for t in range(n):
+        o_out, o_thr = orc.frame(frames[t])
+        assert np.array_equal(g_thr[t][:nch], o_thr[:nch]), (t, g_thr[t], o_thr)
+        assert np.array_equal(g_out[t], o_out), (t, int(np.abs(g_out[t].astype(int) - o_out.astype(int)).max()))
+
+
+def test_largest_frame_batch_equals_streaming():
+    """Two 16384 x 16384 frames (the largest size a session accepts): processed as one batch,
+    whose second frame starts 768 MiB into the RGB24 buffers and 3 GiB into the coefficient
+    planes, they give the bytes and thresholds of the same frames processed one call
+    each."""
+    w = h = 16384
+    kw = dict(fps=30.0, amp=100.0, f_low=0.5, f_high=3.0)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    d_in = torch.randint(0, 256, (2, h, w, 3), dtype=torch.uint8, device="cuda", generator=g)
+    outs, thrs = [], []
+    for batch in (2, 1):
+        sess = wema_amd.Session(w, h, max_batch=batch, **kw)
+        d_out = torch.empty_like(d_in)
+        thr = []
+        for t0 in range(0, 2, batch):
+            sess.process_device(d_in[t0:t0 + batch], d_out_u8=d_out[t0:t0 + batch], n=batch)
+            sess.sync()
+            thr.append(sess.thresholds(batch))
+        outs.append(d_out)
+        thrs.append(np.concatenate(thr))
+        sess.close()
+    assert np.array_equal(thrs[0], thrs[1], equal_nan=True)
+    assert torch.equal(outs[0], outs[1])
+    assert not torch.equal(outs[0], d_in)
+
+
 @pytest.mark.parametrize("preset,w,h,fps,kw", [
     (2, 1920, 1080, 30.0, dict(levels=5, amp=120.0, f_low=0.5, f_high=3.0, color=True)),
     (3, 3840, 2160, 60.0, dict(levels=6, amp=50.0, f_low=0.8, f_high=2.0)),

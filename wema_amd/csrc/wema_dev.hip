@@ -189,6 +189,10 @@ static void plan_layout(const WemaGeom &g, const wema_dev_params &p, int with_fi
 
 static int validate(const wema_dev_params *p) {
     if (p->width < 1 || p->height < 1) { wema_set_error("invalid frame size %dx%d", p->width, p->height); return -1; }
+    if ((long long)p->width * p->height > WEMA_DEV_MAX_PIXELS) {
+        wema_set_error("frame size %dx%d exceeds %ld pixels", p->width, p->height, WEMA_DEV_MAX_PIXELS);
+        return -1;
+    }
     if (!(p->fps > 0)) { wema_set_error("fps must be positive"); return -1; }
     if (!(p->amp > 0)) { wema_set_error("amplification factor must be positive"); return -1; }
     if (p->max_batch < 1 || p->max_batch > WEMA_MAX_BATCH) {

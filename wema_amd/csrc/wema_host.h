@@ -180,8 +180,11 @@ struct JobBuilder {
         jl.blocks = blocks;
     }
     void assign_strips() {
-        for (int l = 0; l < WEMA_MAX_LEVELS; l++) strips_of(dwtf_v[l], dwtf[l], WEMA_STRIP_TARGET);
-        for (int l = 0; l < WEMA_MAX_LEVELS; l++) strips_of(dwti_v[l], dwti[l], WEMA_STRIP_TARGET_INV);
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++)
+            strips_of(dwtf_v[l], dwtf[l], l ? WEMA_STRIP_TARGET_COARSE : WEMA_STRIP_TARGET);
+        for (int st = 0; st < WEMA_MAX_LEVELS; st++)   // inverse stage st holds level MAX - 1 - st
+            strips_of(dwti_v[st], dwti[st],
+                      st < WEMA_MAX_LEVELS - 1 ? WEMA_STRIP_TARGET_INV_COARSE : WEMA_STRIP_TARGET_INV);
         strips_of(dwti_ycc_v, dwti_ycc, WEMA_STRIP_TARGET_INV);
     }
     int pack(void *dst_v, size_t cap, size_t *used) {

@@ -35,6 +35,13 @@
 #ifndef WEMA_STRIP_TARGET_INV
 #define WEMA_STRIP_TARGET_INV 262144
 #endif
+// The same for the levels below full resolution (float LL in, one channel per wave).
+#ifndef WEMA_STRIP_TARGET_COARSE
+#define WEMA_STRIP_TARGET_COARSE 65536
+#endif
+#ifndef WEMA_STRIP_TARGET_INV_COARSE
+#define WEMA_STRIP_TARGET_INV_COARSE 262144
+#endif
 
 // Guided filter tiles (outputs per block).
 #define WEMA_GF_TX 64
