@@ -11,7 +11,8 @@ Workloads (the default is the single-stream drop-in case, 1920x1080 colour):
        bilateral option is inert in IIR mode), 300 frames = (warmup + steps) x 60
   c3   3840x2160 RGB24 @60, 6-level DWT, band 0.8-2.0 Hz, amp 50, guided filter,
        240 frames = (warmup + steps) x 48  (reported under "also" by default)
-  c1   320x240 RGB24 @30, 3-level DWT, band 0.5-3 Hz, amp 120, guided filter
+  c1   320x240 RGB24 @30, 3-level DWT, band 0.5-3 Hz, amp 120, guided filter, 128 frames
+       (the clip) per step; also reported
   c2w  as c2 in sliding-window mode with bilateral weights (window 32); also reported
   multistream  24 concurrent streams of mixed resolution, batched ticks vs one at a time
   serving      Poisson sessions at real-time pace: p50/p99 latency, capacity at p99 <= 33 ms
@@ -40,7 +41,7 @@ METRIC = "Mpixel/s & fps (1080p/4K) at % HBM peak; p50/p99 frame latency, mixed 
 
 CONFIGS = {
     "c1": dict(preset=1, w=320, h=240, fps=30.0, levels=3, amp=120.0, f_low=0.5, f_high=3.0,
-               window=32, color=False, use_iir=True, bilateral=False, batch=32,
+               window=32, color=False, use_iir=True, bilateral=False, batch=128,
                desc="320x240 RGB24@30 disc, L3, IIR 0.5-3 Hz, amp 120, guided filter"),
     "c2": dict(preset=2, w=1920, h=1080, fps=30.0, levels=5, amp=120.0, f_low=0.5, f_high=3.0,
                window=32, color=True, use_iir=True, bilateral=False, batch=60,
@@ -124,8 +125,8 @@ def run_gpu(cfg, name, steps, warmup, batch, tile_rows=()):
 
 def run_host_pipeline(cfg, steps, batch):
     """The same workload from pinned host memory: each step's frames are uploaded, processed
-    and downloaded with the asynchronous pipeline (two batches in flight, copies on their
-    own streams), so the rate includes the PCIe transfers both ways."""
+    and downloaded with the asynchronous pipeline (two device slots, copies on their own
+    streams), so the rate includes the PCIe transfers both ways."""
     import wema_amd
     w, h = cfg["w"], cfg["h"]
     synth = wema_amd.Synth(cfg["preset"], w, h, cfg["fps"])
@@ -133,19 +134,20 @@ def run_host_pipeline(cfg, steps, batch):
                             f_high=cfg["f_high"], window=cfg["window"], levels=cfg["levels"],
                             use_iir=cfg["use_iir"], color=cfg["color"],
                             bilateral=cfg["bilateral"], max_batch=batch)
-    pin_in = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(2)]
-    pin_out = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(2)]
-    for k in range(2):
+    nbuf = 3   # host buffers: a third batch is queued while two are on the device
+    pin_in = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(nbuf)]
+    pin_out = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(nbuf)]
+    for k in range(nbuf):
         for i in range(batch):
             pin_in[k].array[i] = synth.frame(k * batch + i)
-    # warm-up batch, then `steps` timed batches, alternating the two slots
+    # warm-up batch, then `steps` timed batches, rotating through the host buffers
     sess.wait(sess.submit_host(pin_in[0].array, pin_out[0].array, batch))
     t0 = time.perf_counter()
     tickets = []
     for k in range(steps):
-        if len(tickets) == 2:
+        if len(tickets) == nbuf:
             sess.wait(tickets.pop(0))
-        tickets.append(sess.submit_host(pin_in[k % 2].array, pin_out[k % 2].array, batch))
+        tickets.append(sess.submit_host(pin_in[k % nbuf].array, pin_out[k % nbuf].array, batch))
     for t in tickets:
         sess.wait(t)
     dt = time.perf_counter() - t0
@@ -303,7 +305,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--batch", type=int, default=0, help="frames per step (default per config)")
-    ap.add_argument("--also", default="c3,c2w", help="comma list of extra workloads ('' = none)")
+    ap.add_argument("--also", default="c1,c3,c2w", help="comma list of extra workloads ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true",
@@ -372,7 +374,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_host:
-        line["host_pipeline"] = run_host_pipeline(cfg, max(args.steps, 4), batch)
+        line["host_pipeline"] = run_host_pipeline(cfg, max(args.steps, 16), batch)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
     also = {}

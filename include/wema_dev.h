@@ -105,8 +105,9 @@ int wema_dev_sync(wema_dev_t *s);
 /* Asynchronous host-buffer pipeline, the GPU counterpart of the reader/writer slots of
  * async_io.c (async_io.c:183-272): n RGB24 frames (n <= max_batch) are uploaded on a copy
  * stream, processed on the session's stream and downloaded on a second copy stream; the
- * call returns at once with a ticket. Two batches can be in flight, so the uploads and
- * downloads of one overlap the processing of the other. h_in and h_out must stay valid
+ * call returns at once with a ticket. The session has two device slots, so the upload of
+ * one batch overlaps the processing and download of the other; further submits queue on
+ * the device behind the slot they reuse. h_in and h_out must stay valid
  * until wema_dev_wait_ticket returns for the ticket; with pinned memory
  * (wema_dev_host_alloc) the copies run at full link speed. */
 int wema_dev_submit_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out, int n, long *ticket);

@@ -159,22 +159,24 @@ def test_tile_rows_do_not_change_results(w, h):
         assert np.array_equal(outs[1][t], o_out), t
 
 
-def test_async_host_pipeline_equals_device_path():
-    """Submit/wait over pinned host frames (two batches in flight) gives the bytes of the
-    device-buffer path, for every frame and batch boundary."""
-    w, h, fps, n, batch = 96, 64, 30.0, 23, 5
+@pytest.mark.parametrize("inflight", [2, 3, 5])
+def test_async_host_pipeline_equals_device_path(inflight):
+    """Submit/wait over pinned host frames, with 2 (the library's device slots), 3 or 5
+    batches in flight (slots reused while earlier batches are still queued), gives the bytes
+    of the device-buffer path, for every frame and batch boundary."""
+    w, h, fps, n, batch = 96, 64, 30.0, 23, 3
     _, frames = clip(4, w, h, fps, n, seed=9)
     kw = dict(fps=fps, amp=120.0, f_low=0.5, f_high=3.0, levels=3)
     _, ref, _ = run_gpu(frames, batch=batch, **kw)
     sess = wema_amd.Session(w, h, max_batch=batch, **kw)
-    pin_in = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(2)]
-    pin_out = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(2)]
+    pin_in = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(inflight)]
+    pin_out = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(inflight)]
     got = np.zeros_like(frames)
     pending = []
     for b, t0 in enumerate(range(0, n, batch)):
         k = min(batch, n - t0)
-        slot = b % 2
-        if len(pending) == 2:   # the slot's previous batch must be back before reuse
+        slot = b % inflight
+        if len(pending) == inflight:   # the host buffers' previous batch must be back
             tk, s0, kk, sl = pending.pop(0)
             sess.wait(tk)
             got[s0:s0 + kk] = pin_out[sl].array[:kk]

@@ -764,7 +764,7 @@ extern "C" int wema_dev_submit_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8
         }
     }
     const long t = s->next_ticket++;
-    auto &a = s->aslot[t & 1];
+    auto &a = s->aslot[t % WEMA_ASYNC_SLOTS];
     // the slot's previous batch has left the device before its buffers are overwritten
     if (a.ticket >= 0) HIP_OK(hipStreamWaitEvent(s->s_up, a.ev_out, 0));
     HIP_OK(hipMemcpyAsync(a.d_in, h_in, bytes, hipMemcpyHostToDevice, s->s_up));
@@ -784,7 +784,7 @@ extern "C" int wema_dev_wait_ticket(wema_dev_t *s, long ticket) {
     if (!s || ticket < 0 || ticket >= s->next_ticket) { wema_set_error("bad ticket"); return -1; }
     // the slot's event marks this batch's download or a later one, and downloads run in
     // order on one stream
-    HIP_OK(hipEventSynchronize(s->aslot[ticket & 1].ev_out));
+    HIP_OK(hipEventSynchronize(s->aslot[ticket % WEMA_ASYNC_SLOTS].ev_out));
     return 0;
 }
 

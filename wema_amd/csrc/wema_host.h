@@ -9,6 +9,13 @@
 #include "../../include/wema_dev.h"
 #include "wema_jobs.h"
 
+// Batches of the asynchronous host pipeline held on the device: the upload of one overlaps
+// the processing and download of the other. (Three or four slots, which let uploads run
+// back to back, measured 3-5% slower at 1080p: the link, not the gaps, is the limit.)
+#ifndef WEMA_ASYNC_SLOTS
+#define WEMA_ASYNC_SLOTS 2
+#endif
+
 #define WEMA_MAX_LEVELS 12
 #define WEMA_STAGE_BYTES (8 << 20)
 
@@ -87,13 +94,14 @@ struct wema_dev {
     bool pooled = false;        // owned by a wema_pool (no staging buffers of its own)
     void *io_in = nullptr, *io_out = nullptr;           // host-buffer staging (RGB24)
     float *io_in_f32 = nullptr, *io_out_f32 = nullptr;  // host-buffer staging (float RGB)
-    // asynchronous host pipeline: two device slots of max_batch RGB24 frames in and out,
-    // uploads and downloads on their own streams
+    // asynchronous host pipeline: WEMA_ASYNC_SLOTS device slots of max_batch RGB24 frames in
+    // and out (batch t uses slot t % WEMA_ASYNC_SLOTS), uploads and downloads on their own
+    // streams
     struct AsyncSlot {
         uint8_t *d_in = nullptr, *d_out = nullptr;
         hipEvent_t ev_in = nullptr, ev_comp = nullptr, ev_out = nullptr;
         long ticket = -1;
-    } aslot[2];
+    } aslot[WEMA_ASYNC_SLOTS];
     hipStream_t s_up = nullptr, s_down = nullptr;
     long next_ticket = 0;
 };
