@@ -98,6 +98,26 @@ def test_modes_vs_oracle(mode):
     assert worst <= (1 if mode == "window_bilateral" else 0), worst
 
 
+@pytest.mark.parametrize("ws,bilateral,color", [(32, False, False), (32, True, True), (16, True, False),
+                                                (64, False, True), (64, True, False), (24, True, False)])
+def test_window_sizes_vs_oracle(ws, bilateral, color):
+    """Window sizes with kernels of their own (16, 32 and 64 samples held in registers) and
+    one without (24, through LDS): output bytes equal to the oracle's (within 1 LSB with
+    bilateral weights), for frames after the window has filled and across batch
+    boundaries."""
+    w, h, fps, n = 64, 48, 30.0, ws + 9
+    kw = dict(fps=fps, amp=80.0, f_low=0.5, f_high=3.0, levels=3, window=ws, use_iir=False,
+              bilateral=bilateral, color=color)
+    _, frames = clip(2 if color else 4, w, h, fps, n, seed=ws)
+    _, g_out, _ = run_gpu(frames, batch=4, **kw)
+    orc = O.OraclePipe(w, h, **kw)
+    worst = 0
+    for t in range(n):
+        o_out, _ = orc.frame(frames[t])
+        worst = max(worst, int(np.abs(g_out[t].astype(int) - o_out.astype(int)).max()))
+    assert worst <= (1 if bilateral else 0), worst
+
+
 def test_batch_equals_streaming():
     w, h, fps, n = 96, 64, 30.0, 20
     _, frames = clip(4, w, h, fps, n, seed=2)

@@ -628,7 +628,7 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
         HIP_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
         if (timer_mark(timer, TK_IIR, tl.bytes, st) < 0) return -1;
     }
-    for (int v = 0; v < 3; v++) {
+    for (int v = 0; v < WEMA_WIN_KINDS; v++) {
         const JobList &wl = jb.window[v];
         if (!wl.count) continue;
         HIP_OK(wema_launch_win(v, (const TemporalJob *)(base + wl.offset), wl.count, wl.blocks, st));

@@ -109,11 +109,11 @@ struct wema_dev {
 struct JobBuilder {
     std::vector<DwtfJob> dwtf_v[WEMA_MAX_LEVELS];
     std::vector<TemporalJob> temporal_v;      // IIR jobs
-    std::vector<TemporalJob> window_v[3];     // sliding-window jobs: ws <= 32, <= 64, any
+    std::vector<TemporalJob> window_v[WEMA_WIN_KINDS];   // sliding-window jobs by kernel (wema_win_kind)
     std::vector<DwtiJob> dwti_v[WEMA_MAX_LEVELS];
     std::vector<DwtiJob> dwti_ycc_v;          // colour output jobs (level 0 + YCbCr -> RGB)
     std::vector<EgressJob> egress_v[EG_KINDS];
-    JobList dwtf[WEMA_MAX_LEVELS], temporal, window[3], dwti[WEMA_MAX_LEVELS], dwti_ycc, egress[EG_KINDS];
+    JobList dwtf[WEMA_MAX_LEVELS], temporal, window[WEMA_WIN_KINDS], dwti[WEMA_MAX_LEVELS], dwti_ycc, egress[EG_KINDS];
     std::vector<float> host_amp;
     size_t amp_offset = 0;
 
@@ -124,7 +124,7 @@ struct JobBuilder {
         dwtf_v[l].push_back(j);
     }
     void add_temporal(TemporalJob j, int tiles, double bytes, bool window_mode) {
-        const int wv = !window_mode ? 0 : (j.window <= 32 ? 0 : (j.window <= 64 ? 1 : 2));
+        const int wv = !window_mode ? 0 : wema_win_kind(j.window);
         JobList &jl = window_mode ? window[wv] : temporal;
         j.tile_begin = jl.blocks;
         j.tiles = tiles;
@@ -202,7 +202,7 @@ struct JobBuilder {
         bool ok = true;
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwtf_v[l], dwtf[l]);
         ok = ok && put(dst, cap, off, temporal_v, temporal);
-        for (int k = 0; k < 3; k++) ok = ok && put(dst, cap, off, window_v[k], window[k]);
+        for (int k = 0; k < WEMA_WIN_KINDS; k++) ok = ok && put(dst, cap, off, window_v[k], window[k]);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_v[l], dwti[l]);
         ok = ok && put(dst, cap, off, dwti_ycc_v, dwti_ycc);
         for (int k = 0; k < EG_KINDS; k++) ok = ok && put(dst, cap, off, egress_v[k], egress[k]);

@@ -108,6 +108,19 @@ enum EgressKind { EG_GF1 = 0, EG_GF2 = 3, EG_GAUSS = 6, EG_PASSTHRU = 7, EG_KIND
 __host__ __device__ static inline int wema_rows_index(int rows) { return rows == 8 ? 0 : (rows == 32 ? 2 : 1); }
 __host__ __device__ static inline int wema_index_rows(int i) { return i == 0 ? 8 : (i == 2 ? 32 : 16); }
 
+// Sliding-window kernels: 0 = windows of <= 32 samples through LDS, 1 = <= 64 through LDS,
+// 2 = any size from the ring; exactly 32, 16 or 64 samples in registers: 3, 4, 5.
+#define WEMA_WIN_KINDS 6
+#ifndef WEMA_WIN_REG_MASK
+#define WEMA_WIN_REG_MASK 7   // register kernels in use: 1 = 32, 2 = 16, 4 = 64
+#endif
+__host__ __device__ static inline int wema_win_kind(int ws) {
+    if (ws == 32 && (WEMA_WIN_REG_MASK & 1)) return 3;
+    if (ws == 16 && (WEMA_WIN_REG_MASK & 2)) return 4;
+    if (ws == 64 && (WEMA_WIN_REG_MASK & 4)) return 5;
+    return ws <= 32 ? 0 : (ws <= 64 ? 1 : 2);
+}
+
 // Temporal stage (IIR or sliding window) over one (stream, channel) and its frames.
 struct TemporalJob {
     float *coef;                    // [nframes][coef_stride] coefficients (IIR: in place)

@@ -980,16 +980,13 @@ __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ job
 // ltw: base-2 logarithms of the time weights; the weight of a sample is
 // exp2(fma(d^2, -log2(e) / (2 sigma^2), log2(tw))), the product of the value and time
 // weights in one exponential.
-template <class Win>
+template <class Win, class LTW>
 __device__ __forceinline__ float win_delta(const Win &win, int ws, float inv_ws, float cur, bool bilateral,
-                                           const float *ltw, float &mean_out) {
+                                           LTW ltw, float &mean_out) {
     float mean = 0.0f, mad = 0.0f;
     win.each([&](int, float v) {
         mean += v;
-        if (bilateral) {
-            const float d = v - cur;
-            mad += d > 0.0f ? d : -d;
-        }
+        if (bilateral) mad += fabsf(v - cur);   // d > 0 ? d : -d, the same sum from +0
     });
     mean *= inv_ws;
     mean_out = mean;
@@ -1113,6 +1110,64 @@ __device__ __forceinline__ void win_lds_body(const TemporalJob *__restrict__ job
         if (++head == ws) head = 0;
     }
 }
+
+// Windows of exactly WS samples held in registers: the frame loop shifts the window down
+// one register and appends the newest sample (loaded one frame ahead), so every access is
+// static: no LDS column, no wrap-around index, and the time weights are uniform loads.
+template <int WS>
+struct WinReg {
+    const float (&x)[WS];
+    template <class F>
+    __device__ __forceinline__ void each(F &&fn) const {
+#pragma unroll
+        for (int t = 0; t < WS; ++t) fn(t, x[t]);
+    }
+};
+
+template <int WS>
+__device__ __forceinline__ void win_reg_body(const TemporalJob *__restrict__ jobs, int njobs, float *sT,
+                                             float *sA) {
+    const int jid = find_job(jobs, njobs, (int)blockIdx.x);
+    const TemporalJob J = jobs[jid];
+    frame_params(J, sT, sA);
+    const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
+    if (p >= J.n) return;
+    const int nframes = J.nframes;
+    const float inv_ws = 1.0f / (float)WS;
+    const float lscale = p < J.level0_end ? 0.5f : 1.0f;
+    const size_t plane = (size_t)J.coef_stride;
+    const GLOBAL_AS float *ring = gp(J.ring);
+    const GLOBAL_AS float *ltw = gp(J.ltw);
+    float x[WS];
+    // the first WS - 1 samples of frame 0's window, then its newest
+    int slot = J.ring_first;
+#pragma unroll
+    for (int t = 1; t < WS; ++t) {
+        x[t] = ring[(size_t)slot * plane + p];
+        if (++slot == J.ring_slots) slot = 0;
+    }
+    float next = ring[(size_t)slot * plane + p];
+    for (int f = 0; f < nframes; ++f) {
+#pragma unroll
+        for (int t = 0; t < WS - 1; ++t) x[t] = x[t + 1];
+        x[WS - 1] = next;
+        if (++slot == J.ring_slots) slot = 0;
+        if (f + 1 < nframes) next = ring[(size_t)slot * plane + p];
+        const WinReg<WS> win{x};
+        float mean;
+        const float delta = win_delta(win, WS, inv_ws, x[WS / 2], J.bilateral != 0, ltw, mean);
+        win_store(J, f, p, x[WS - 1], delta, sT, sA, lscale);
+    }
+}
+
+#define WEMA_WIN_REG_KERNEL(WS)                                                                 \
+    __global__ void __launch_bounds__(256) k_win_r##WS(const TemporalJob *__restrict__ jobs, int njobs) { \
+        __shared__ float sT[WEMA_MAX_BATCH], sA[WEMA_MAX_BATCH];                                \
+        win_reg_body<WS>(jobs, njobs, sT, sA);                                                  \
+    }
+WEMA_WIN_REG_KERNEL(16)
+WEMA_WIN_REG_KERNEL(32)
+WEMA_WIN_REG_KERNEL(64)
 
 #define WEMA_WIN_LDS_KERNEL(WSMAX)                                                              \
     __global__ void __launch_bounds__(256) k_win_l##WSMAX(const TemporalJob *__restrict__ jobs, int njobs) { \
@@ -1839,6 +1894,9 @@ hipError_t wema_launch_win(int variant, const TemporalJob *jobs, int njobs, int 
     if (nblocks <= 0) return hipSuccess;
     if (variant == 0) hipLaunchKernelGGL(k_win_l32, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
     else if (variant == 1) hipLaunchKernelGGL(k_win_l64, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    else if (variant == 3) hipLaunchKernelGGL(k_win_r32, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    else if (variant == 4) hipLaunchKernelGGL(k_win_r16, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    else if (variant == 5) hipLaunchKernelGGL(k_win_r64, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
     else hipLaunchKernelGGL(k_win, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError();
 }
