@@ -1040,7 +1040,8 @@ __device__ __forceinline__ void win_store(const TemporalJob &J, int f, long long
                                           const float *sT, const float *sA, float lscale) {
     const float T = sT[f];
     const float a = sA[f] * lscale;
-    J.amp_out[(size_t)f * J.out_stride + p] = (fabsf(orig) <= T) ? orig : fmaf(a, delta, orig);
+    const float o = (fabsf(orig) <= T) ? orig : fmaf(a, delta, orig);
+    __builtin_nontemporal_store(o, gp(J.amp_out) + ((size_t)f * J.out_stride + p));   // written once
     if (J.filt_out) J.filt_out[(size_t)f * J.out_stride + p] = delta;
 }
 
@@ -1152,7 +1153,7 @@ __device__ __forceinline__ void win_reg_body(const TemporalJob *__restrict__ job
         for (int t = 0; t < WS - 1; ++t) x[t] = x[t + 1];
         x[WS - 1] = next;
         if (++slot == J.ring_slots) slot = 0;
-        if (f + 1 < nframes) next = ring[(size_t)slot * plane + p];
+        if (f + 1 < nframes) next = __builtin_nontemporal_load(&ring[(size_t)slot * plane + p]);
         const WinReg<WS> win{x};
         float mean;
         const float delta = win_delta(win, WS, inv_ws, x[WS / 2], J.bilateral != 0, ltw, mean);
