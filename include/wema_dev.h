@@ -113,6 +113,19 @@ int wema_dev_sync(wema_dev_t *s);
 int wema_dev_submit_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out, int n, long *ticket);
 int wema_dev_wait_ticket(wema_dev_t *s, long ticket);
 
+/* Temporal state of a session (no counterpart in the reference, whose IIR state lives in
+ * IIRTemporalFilter, iir_filter.h:45-61, and window ring in TemporalBuffer, wema.h:67-81):
+ * the IIR sections' state, or the last window - 1 frames of the window ring, with the
+ * number of frames processed. Exported from one session and imported into another of the
+ * same size, levels, colour mode and temporal filter (IIR band and frame rate, or window
+ * size; max_batch, device and amplification may differ), the second continues with the
+ * outputs the first would have given. Both calls wait for the session's work; import
+ * rejects the state of another geometry or filter. The buffer is plain host memory:
+ * a versioned header, then the state, in the host's byte order. */
+int wema_dev_state_bytes(const wema_dev_t *s, size_t *bytes);
+int wema_dev_export_state(wema_dev_t *s, void *host, size_t bytes);
+int wema_dev_import_state(wema_dev_t *s, const void *host, size_t bytes);
+
 /* Tile shapes (no counterpart in the reference). The guided-filter kernels come in tile
  * heights of 8, 16 and 32 rows (64 columns). A session whose output stage is the guided
  * filter takes, for each of the filter's two passes, the height an autotuner picked for

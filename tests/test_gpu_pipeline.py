@@ -211,6 +211,58 @@ def test_async_host_pipeline_equals_device_path(inflight):
         a.close()
 
 
+@pytest.mark.parametrize("mode,cut", [("iir_color", 7), ("iir", 0), ("window_bilateral", 5),
+                                      ("window32", 12), ("window32", 40)])
+def test_state_export_import_continues_the_stream(mode, cut):
+    """A session that imports another's exported temporal state (IIR sections, or the last
+    window - 1 frames of the ring, and the frame count) continues with the bytes the first
+    session gives, with a different max_batch; including exports before the first frame,
+    while the window fills, and after it has filled."""
+    w, h, fps, n = 64, 48, 30.0, 50
+    kw = dict(fps=fps, amp=100.0, f_low=0.5, f_high=3.0, levels=3)
+    if mode == "iir_color":
+        kw.update(color=True)
+    elif mode == "window_bilateral":
+        kw.update(use_iir=False, bilateral=True, window=8)
+    elif mode == "window32":
+        kw.update(use_iir=False, window=32, color=True)
+    _, frames = clip(2 if kw.get("color") else 4, w, h, fps, n, seed=cut + 3)
+    d_in = torch.from_numpy(frames).cuda()
+    a = wema_amd.Session(w, h, max_batch=4, **kw)
+    out_a = torch.empty_like(d_in)
+    for t0 in range(0, n, 4):
+        k = min(4, n - t0)
+        a.process_device(d_in[t0:t0 + k], d_out_u8=out_a[t0:t0 + k], n=k)
+    a.sync()
+    first = wema_amd.Session(w, h, max_batch=3, **kw)
+    tmp = torch.empty_like(d_in)
+    for t in range(cut):
+        first.process_device(d_in[t:t + 1], d_out_u8=tmp[t:t + 1], n=1)
+    state = first.export_state()
+    first.close()
+    b = wema_amd.Session(w, h, max_batch=6, **kw)
+    b.import_state(state)
+    out_b = torch.empty_like(d_in)
+    for t0 in range(cut, n, 6):
+        k = min(6, n - t0)
+        b.process_device(d_in[t0:t0 + k], d_out_u8=out_b[t0:t0 + k], n=k)
+    b.sync()
+    assert torch.equal(out_a[cut:], out_b[cut:])
+    if cut:
+        assert torch.equal(tmp[:cut], out_a[:cut])
+    # the state of another filter or size is refused
+    other = dict(kw, f_high=2.5) if kw.get("use_iir", True) else dict(kw, window=kw["window"] + 2)
+    for ww, hh, kk in [(w, h, other), (w + 2, h, kw)]:
+        c = wema_amd.Session(ww, hh, **kk)
+        with pytest.raises(wema_amd.WemaError, match="state of"):
+            c.import_state(state)
+        c.close()
+    with pytest.raises(wema_amd.WemaError, match="not a WEMA"):
+        b.import_state(b"\0" * len(state))
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("w,h,levels,mode", [(1, 1, 1, "iir"), (3, 2, 1, "iir"), (7, 5, 2, "iir_color"),
                                              (17, 1, 3, "iir"), (16, 9, 3, "window"),
                                              (5, 19, 2, "gauss")])

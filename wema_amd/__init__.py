@@ -109,6 +109,9 @@ _SIGS = {
     "wema_dev_submit_host_u8": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_long)]),
     "wema_dev_wait_ticket": (C.c_int, [_vp, C.c_long]),
     "wema_dev_host_free": (None, [_vp]),
+    "wema_dev_state_bytes": (C.c_int, [_vp, C.POINTER(C.c_size_t)]),
+    "wema_dev_export_state": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "wema_dev_import_state": (C.c_int, [_vp, _vp, C.c_size_t]),
     "wema_dev_debug_capture": (C.c_int, [_vp, C.c_int]),
     "wema_dev_timing": (C.c_int, [_vp, C.c_int]),
     "wema_dev_timing_report": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_long),
@@ -324,6 +327,20 @@ class Session:
                 rows.append(dict(name=name.value.decode(), launches=la.value, ms=ms.value, bytes=by.value))
             k += 1
         return rows
+
+    # ---- temporal state (move a session to another process or GPU)
+    def export_state(self):
+        """The session's temporal state as bytes (wema_dev_export_state)."""
+        n = C.c_size_t()
+        _check(lib.wema_dev_state_bytes(self._h, C.byref(n)), "state_bytes")
+        buf = np.zeros(n.value, np.uint8)
+        _check(lib.wema_dev_export_state(self._h, _ptr(buf), n.value), "export_state")
+        return buf.tobytes()
+
+    def import_state(self, state):
+        """Continue from a state exported by a session of the same geometry and filter."""
+        buf = np.frombuffer(state, np.uint8).copy()
+        _check(lib.wema_dev_import_state(self._h, _ptr(buf), buf.size), "import_state")
 
     # ---- test hooks
     def thresholds(self, n):

@@ -704,6 +704,135 @@ extern "C" int wema_dev_sync(wema_dev_t *s) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// temporal state export / import (the IIR sections' state, or the window ring's last
+// window - 1 frames, with the frame count): a session continues elsewhere, in another
+// process or on another GPU, with the outputs it would have given in place
+
+namespace {
+struct StateHeader {
+    char magic[8];              // "WEMASTT1"
+    int32_t width, height, levels, nch, use_iir, window;
+    float c10[10];              // IIR sections (must match for the state to mean the same)
+    uint64_t n_unique;
+    int64_t frames;             // frames processed
+    int32_t ring_frames;        // window mode: frames of coefficients that follow
+    int32_t pad_;
+    uint64_t payload;           // bytes after the header
+};
+const char kStateMagic[8] = {'W', 'E', 'M', 'A', 'S', 'T', 'T', '1'};
+
+int ring_frames_of(const wema_dev *s) {
+    if (s->p.use_iir) return 0;
+    const long k = s->p.window - 1;
+    return (int)(s->frames < k ? s->frames : k);
+}
+
+size_t payload_bytes(const wema_dev *s, int ring_frames) {
+    const size_t nu = s->g.nu, nch = s->g.nch;
+    return s->p.use_iir ? nch * nu * sizeof(float4) : nch * (size_t)ring_frames * nu * sizeof(float);
+}
+}  // namespace
+
+extern "C" int wema_dev_state_bytes(const wema_dev_t *s, size_t *bytes) {
+    if (!s || !bytes) { wema_set_error("null argument"); return -1; }
+    *bytes = sizeof(StateHeader) + payload_bytes(s, ring_frames_of(s));
+    return 0;
+}
+
+extern "C" int wema_dev_export_state(wema_dev_t *s, void *host, size_t bytes) {
+    if (!s || !host) { wema_set_error("null argument"); return -1; }
+    const int rf = ring_frames_of(s);
+    const size_t need = sizeof(StateHeader) + payload_bytes(s, rf);
+    if (bytes < need) { wema_set_error("state needs %zu bytes, buffer has %zu", need, bytes); return -1; }
+    HIP_OK(hipSetDevice(s->p.device));
+    if (s->last_stream) HIP_OK(hipStreamSynchronize(s->last_stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    StateHeader hd;
+    memset(&hd, 0, sizeof(hd));
+    memcpy(hd.magic, kStateMagic, 8);
+    hd.width = s->p.width;
+    hd.height = s->p.height;
+    hd.levels = s->g.L;
+    hd.nch = s->g.nch;
+    hd.use_iir = s->p.use_iir;
+    hd.window = s->p.use_iir ? 0 : s->p.window;
+    memcpy(hd.c10, s->c10, sizeof(hd.c10));
+    hd.n_unique = s->g.nu;
+    hd.frames = s->frames;
+    hd.ring_frames = rf;
+    hd.payload = need - sizeof(StateHeader);
+    memcpy(host, &hd, sizeof(hd));
+    char *dst = (char *)host + sizeof(hd);
+    const size_t nu = s->g.nu, nus = s->g.nus;
+    for (int c = 0; c < s->g.nch; c++) {
+        if (s->p.use_iir) {
+            HIP_OK(hipMemcpy(dst, s->state + (size_t)c * nus, nu * sizeof(float4), hipMemcpyDeviceToHost));
+            dst += nu * sizeof(float4);
+        } else {
+            // oldest first: frames frames - rf .. frames - 1, ring slot g % ring_slots
+            for (long g = s->frames - rf; g < s->frames; g++) {
+                const float *src = s->coef + ((size_t)c * s->ring_slots + (size_t)(g % s->ring_slots)) * nus;
+                HIP_OK(hipMemcpy(dst, src, nu * sizeof(float), hipMemcpyDeviceToHost));
+                dst += nu * sizeof(float);
+            }
+        }
+    }
+    return 0;
+}
+
+extern "C" int wema_dev_import_state(wema_dev_t *s, const void *host, size_t bytes) {
+    if (!s || !host) { wema_set_error("null argument"); return -1; }
+    StateHeader hd;
+    if (bytes < sizeof(hd)) { wema_set_error("state buffer too short"); return -1; }
+    memcpy(&hd, host, sizeof(hd));
+    if (memcmp(hd.magic, kStateMagic, 8) != 0) { wema_set_error("not a WEMA session state"); return -1; }
+    if (hd.width != s->p.width || hd.height != s->p.height || hd.levels != s->g.L || hd.nch != s->g.nch ||
+        hd.n_unique != s->g.nu) {
+        wema_set_error("state of a %dx%d, %d-level, %d-channel session", hd.width, hd.height, hd.levels, hd.nch);
+        return -1;
+    }
+    if (hd.use_iir != s->p.use_iir || (!hd.use_iir && hd.window != s->p.window)) {
+        wema_set_error("state of another temporal filter (%s, window %d)", hd.use_iir ? "IIR" : "window", hd.window);
+        return -1;
+    }
+    if (hd.use_iir && memcmp(hd.c10, s->c10, sizeof(hd.c10)) != 0) {
+        wema_set_error("state of another band-pass (frame rate or band edges differ)");
+        return -1;
+    }
+    const long want_rf = hd.use_iir ? 0 : (hd.frames < s->p.window - 1 ? hd.frames : s->p.window - 1);
+    if (hd.frames < 0 || hd.ring_frames != want_rf) {
+        wema_set_error("inconsistent state header");
+        return -1;
+    }
+    const int rf = hd.ring_frames;
+    const size_t payload = payload_bytes(s, rf);
+    if (hd.payload != payload || bytes < sizeof(hd) + payload) {
+        wema_set_error("state payload of %llu bytes, expected %zu", (unsigned long long)hd.payload, payload);
+        return -1;
+    }
+    HIP_OK(hipSetDevice(s->p.device));
+    if (s->last_stream) HIP_OK(hipStreamSynchronize(s->last_stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    const char *src = (const char *)host + sizeof(hd);
+    const size_t nu = s->g.nu, nus = s->g.nus;
+    for (int c = 0; c < s->g.nch; c++) {
+        if (s->p.use_iir) {
+            HIP_OK(hipMemcpy(s->state + (size_t)c * nus, src, nu * sizeof(float4), hipMemcpyHostToDevice));
+            src += nu * sizeof(float4);
+        } else {
+            for (long g = hd.frames - rf; g < hd.frames; g++) {
+                float *dst = s->coef + ((size_t)c * s->ring_slots + (size_t)(g % s->ring_slots)) * nus;
+                HIP_OK(hipMemcpy(dst, src, nu * sizeof(float), hipMemcpyHostToDevice));
+                src += nu * sizeof(float);
+            }
+        }
+    }
+    s->frames = (long)hd.frames;
+    s->last_n = 0;
+    return 0;
+}
+
 extern "C" int wema_dev_process_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out) {
     if (!s || !h_in || !h_out) { wema_set_error("null argument"); return -1; }
     HIP_OK(hipSetDevice(s->p.device));
