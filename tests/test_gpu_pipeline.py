@@ -98,18 +98,21 @@ def test_modes_vs_oracle(mode):
     assert worst <= (1 if mode == "window_bilateral" else 0), worst
 
 
-@pytest.mark.parametrize("ws,bilateral,color", [(32, False, False), (32, True, True), (16, True, False),
-                                                (64, False, True), (64, True, False), (24, True, False)])
-def test_window_sizes_vs_oracle(ws, bilateral, color):
+@pytest.mark.parametrize("ws,bilateral,color,batch", [(32, False, False, 4), (32, True, True, 4),
+                                                      (16, True, False, 4), (64, False, True, 4),
+                                                      (64, True, False, 4), (24, True, False, 4),
+                                                      (32, True, False, 1), (16, False, False, 7)])
+def test_window_sizes_vs_oracle(ws, bilateral, color, batch):
     """Window sizes with kernels of their own (16, 32 and 64 samples held in registers) and
     one without (24, through LDS): output bytes equal to the oracle's (within 1 LSB with
     bilateral weights), for frames after the window has filled and across batch
-    boundaries."""
+    boundaries, with one frame per call (a ring of exactly the window) and with batches
+    that do not divide the clip."""
     w, h, fps, n = 64, 48, 30.0, ws + 9
     kw = dict(fps=fps, amp=80.0, f_low=0.5, f_high=3.0, levels=3, window=ws, use_iir=False,
               bilateral=bilateral, color=color)
     _, frames = clip(2 if color else 4, w, h, fps, n, seed=ws)
-    _, g_out, _ = run_gpu(frames, batch=4, **kw)
+    _, g_out, _ = run_gpu(frames, batch=batch, **kw)
     orc = O.OraclePipe(w, h, **kw)
     worst = 0
     for t in range(n):
