@@ -292,6 +292,9 @@ int wema_session_alloc(wema_dev *s, int with_filt) {
     }
     HIP_OK(hipMalloc(&s->arena, L.total));
     HIP_OK(hipMemsetAsync(s->arena, 0, L.total, s->stream));
+    // the clear runs on the session's non-blocking stream, which the synchronous copies
+    // below (null stream) do not wait for: let it finish first, or it may zero them again
+    HIP_OK(hipStreamSynchronize(s->stream));
     char *a = (char *)s->arena;
     s->coef = (float *)(a + L.coef);
     s->ampc = (s->p.use_iir && !with_filt) ? nullptr : (float *)(a + L.ampc);
