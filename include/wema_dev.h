@@ -113,6 +113,17 @@ int wema_dev_sync(wema_dev_t *s);
 int wema_dev_submit_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out, int n, long *ticket);
 int wema_dev_wait_ticket(wema_dev_t *s, long ticket);
 
+/* The same pipeline one frame at a time, with presentation timestamps (the shape of
+ * async_io.c's slot queue, src/async_io.c:183-272, where frames travel with their pts):
+ * wema_dev_submit queues one RGB24 frame and the buffer its output goes to; a full batch
+ * (max_batch frames) is dispatched at once. wema_dev_collect waits for the oldest queued
+ * frame (dispatching a partial batch if that frame is still in one) and returns its pts and
+ * output buffer: frames come back in submission order. Both buffers must stay valid until
+ * the frame is collected; pinned memory (wema_dev_host_alloc) copies at full link speed.
+ * Collecting with nothing queued is an error. */
+int wema_dev_submit(wema_dev_t *s, const uint8_t *rgb24_in, uint8_t *rgb24_out, int64_t pts);
+int wema_dev_collect(wema_dev_t *s, int64_t *pts, uint8_t **rgb24_out);
+
 /* Temporal state of a session (no counterpart in the reference, whose IIR state lives in
  * IIRTemporalFilter, iir_filter.h:45-61, and window ring in TemporalBuffer, wema.h:67-81):
  * the IIR sections' state, or the last window - 1 frames of the window ring, with the

@@ -214,6 +214,42 @@ def test_async_host_pipeline_equals_device_path(inflight):
         a.close()
 
 
+@pytest.mark.parametrize("max_batch", [1, 4])
+def test_submit_collect_per_frame_with_timestamps(max_batch):
+    """Frames submitted one at a time from pinned memory, with timestamps, come back in
+    order with their own timestamps and output buffers, and with the device path's bytes;
+    collects interleave with submits (partial batches dispatched on demand)."""
+    w, h, fps, n = 96, 64, 30.0, 23
+    _, frames = clip(4, w, h, fps, n, seed=12)
+    kw = dict(fps=fps, amp=120.0, f_low=0.5, f_high=3.0, levels=3)
+    _, ref, _ = run_gpu(frames, batch=1, **kw)
+    sess = wema_amd.Session(w, h, max_batch=max_batch, **kw)
+    pin_in = wema_amd.PinnedArray((n, h, w, 3))
+    pin_out = wema_amd.PinnedArray((n, h, w, 3))
+    pin_in.array[:] = frames
+    base = pin_out.array.ctypes.data
+    frame_bytes = h * w * 3
+    got, queued = [], 0
+    for t in range(n):
+        sess.submit(pin_in.array[t], pin_out.array[t], 1000 + 33 * t)
+        queued += 1
+        if queued > 6 or t % 5 == 4:
+            pts, addr = sess.collect()
+            got.append((pts, addr))
+            queued -= 1
+    while queued:
+        got.append(sess.collect())
+        queued -= 1
+    with pytest.raises(wema_amd.WemaError, match="no frame"):
+        sess.collect()
+    assert [p for p, _ in got] == [1000 + 33 * t for t in range(n)]
+    assert [(a - base) // frame_bytes for _, a in got] == list(range(n))
+    assert np.array_equal(pin_out.array, ref)
+    sess.close()
+    pin_in.close()
+    pin_out.close()
+
+
 @pytest.mark.parametrize("mode,cut", [("iir_color", 7), ("iir", 0), ("window_bilateral", 5),
                                       ("window32", 12), ("window32", 40)])
 def test_state_export_import_continues_the_stream(mode, cut):

@@ -108,6 +108,8 @@ _SIGS = {
     "wema_dev_host_alloc": (_vp, [C.c_size_t]),
     "wema_dev_submit_host_u8": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_long)]),
     "wema_dev_wait_ticket": (C.c_int, [_vp, C.c_long]),
+    "wema_dev_submit": (C.c_int, [_vp, _vp, _vp, C.c_int64]),
+    "wema_dev_collect": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(_vp)]),
     "wema_dev_host_free": (None, [_vp]),
     "wema_dev_state_bytes": (C.c_int, [_vp, C.POINTER(C.c_size_t)]),
     "wema_dev_export_state": (C.c_int, [_vp, _vp, C.c_size_t]),
@@ -307,6 +309,17 @@ class Session:
 
     def wait(self, ticket):
         _check(lib.wema_dev_wait_ticket(self._h, int(ticket)), "wait_ticket")
+
+    def submit(self, frame_in, frame_out, pts):
+        """Queue one RGB24 frame (host array) with its timestamp; its output goes to frame_out
+        (both stay alive until collected)."""
+        _check(lib.wema_dev_submit(self._h, _ptr(frame_in), _ptr(frame_out), int(pts)), "submit")
+
+    def collect(self):
+        """Wait for the oldest queued frame: (pts, address of its output buffer)."""
+        pts, out = C.c_int64(), _vp()
+        _check(lib.wema_dev_collect(self._h, C.byref(pts), C.byref(out)), "collect")
+        return pts.value, out.value
 
     # ---- measurement
     def timing(self, enable):

@@ -876,15 +876,14 @@ extern "C" int wema_dev_process_host_f32(wema_dev_t *s, const float *const *h_in
     return 0;
 }
 
-extern "C" int wema_dev_submit_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out, int n,
-                                       long *ticket) {
-    if (!s || !h_in || !h_out || n < 1) { wema_set_error("bad argument"); return -1; }
-    if (s->pooled) { wema_set_error("a pooled session is processed by wema_pool_tick"); return -1; }
-    if (n > s->p.max_batch) { wema_set_error("%d frames exceed max_batch %d", n, s->p.max_batch); return -1; }
+// One batch of frames from host memory through a device slot: upload on s_up, process on
+// the session's stream, download on s_down. Runs of consecutive host frames go in one copy.
+static int submit_frames(wema_dev *s, const uint8_t *const *h_in, uint8_t *const *h_out, int n,
+                         long *ticket) {
     HIP_OK(hipSetDevice(s->p.device));
-    const size_t bytes = s->g.P * 3 * (size_t)n;
+    const size_t fb = s->g.P * 3;
     if (!s->s_up) {
-        const size_t cap = s->g.P * 3 * (size_t)s->p.max_batch;
+        const size_t cap = fb * (size_t)s->p.max_batch;
         HIP_OK(hipStreamCreateWithFlags(&s->s_up, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&s->s_down, hipStreamNonBlocking));
         for (auto &a : s->aslot) {
@@ -899,16 +898,76 @@ extern "C" int wema_dev_submit_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8
     auto &a = s->aslot[t % WEMA_ASYNC_SLOTS];
     // the slot's previous batch has left the device before its buffers are overwritten
     if (a.ticket >= 0) HIP_OK(hipStreamWaitEvent(s->s_up, a.ev_out, 0));
-    HIP_OK(hipMemcpyAsync(a.d_in, h_in, bytes, hipMemcpyHostToDevice, s->s_up));
+    for (int i = 0, k; i < n; i = k) {
+        for (k = i + 1; k < n && h_in[k] == h_in[k - 1] + fb; k++) {}
+        HIP_OK(hipMemcpyAsync(a.d_in + fb * i, h_in[i], fb * (k - i), hipMemcpyHostToDevice, s->s_up));
+    }
     HIP_OK(hipEventRecord(a.ev_in, s->s_up));
     HIP_OK(hipStreamWaitEvent(s->stream, a.ev_in, 0));
     if (process(s, a.d_in, 0, a.d_out, nullptr, n, s->stream) < 0) return -1;
     HIP_OK(hipEventRecord(a.ev_comp, s->stream));
     HIP_OK(hipStreamWaitEvent(s->s_down, a.ev_comp, 0));
-    HIP_OK(hipMemcpyAsync(h_out, a.d_out, bytes, hipMemcpyDeviceToHost, s->s_down));
+    for (int i = 0, k; i < n; i = k) {
+        for (k = i + 1; k < n && h_out[k] == h_out[k - 1] + fb; k++) {}
+        HIP_OK(hipMemcpyAsync(h_out[i], a.d_out + fb * i, fb * (k - i), hipMemcpyDeviceToHost, s->s_down));
+    }
     HIP_OK(hipEventRecord(a.ev_out, s->s_down));
     a.ticket = t;
     if (ticket) *ticket = t;
+    return 0;
+}
+
+extern "C" int wema_dev_submit_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8_t *h_out, int n,
+                                       long *ticket) {
+    if (!s || !h_in || !h_out || n < 1) { wema_set_error("bad argument"); return -1; }
+    if (s->pooled) { wema_set_error("a pooled session is processed by wema_pool_tick"); return -1; }
+    if (n > s->p.max_batch) { wema_set_error("%d frames exceed max_batch %d", n, s->p.max_batch); return -1; }
+    if (s->fq_pending) { wema_set_error("frames submitted one by one are still pending"); return -1; }
+    const size_t fb = s->g.P * 3;
+    std::vector<const uint8_t *> in((size_t)n);
+    std::vector<uint8_t *> out((size_t)n);
+    for (int i = 0; i < n; i++) {
+        in[(size_t)i] = h_in + fb * i;
+        out[(size_t)i] = h_out + fb * i;
+    }
+    return submit_frames(s, in.data(), out.data(), n, ticket);
+}
+
+// The undispatched tail of the per-frame queue as one batch.
+static int dispatch_pending(wema_dev *s) {
+    const int n = s->fq_pending;
+    if (!n) return 0;
+    std::vector<const uint8_t *> in((size_t)n);
+    std::vector<uint8_t *> out((size_t)n);
+    const size_t first = s->fq.size() - (size_t)n;
+    for (int i = 0; i < n; i++) {
+        in[(size_t)i] = s->fq[first + i].in;
+        out[(size_t)i] = s->fq[first + i].out;
+    }
+    long t;
+    if (submit_frames(s, in.data(), out.data(), n, &t) < 0) return -1;
+    for (int i = 0; i < n; i++) s->fq[first + i].ticket = t;
+    s->fq_pending = 0;
+    return 0;
+}
+
+extern "C" int wema_dev_submit(wema_dev_t *s, const uint8_t *rgb24_in, uint8_t *rgb24_out, int64_t pts) {
+    if (!s || !rgb24_in || !rgb24_out) { wema_set_error("bad argument"); return -1; }
+    if (s->pooled) { wema_set_error("a pooled session is processed by wema_pool_tick"); return -1; }
+    s->fq.push_back({rgb24_in, rgb24_out, pts, -1});
+    if (++s->fq_pending == s->p.max_batch) return dispatch_pending(s);
+    return 0;
+}
+
+extern "C" int wema_dev_collect(wema_dev_t *s, int64_t *pts, uint8_t **rgb24_out) {
+    if (!s) { wema_set_error("null session"); return -1; }
+    if (s->fq.empty()) { wema_set_error("no frame submitted"); return -1; }
+    if (s->fq.front().ticket < 0 && dispatch_pending(s) < 0) return -1;
+    const auto f = s->fq.front();
+    HIP_OK(hipEventSynchronize(s->aslot[f.ticket % WEMA_ASYNC_SLOTS].ev_out));
+    s->fq.pop_front();
+    if (pts) *pts = f.pts;
+    if (rgb24_out) *rgb24_out = f.out;
     return 0;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <deque>
 #include <vector>
 
 #include "../../include/wema_dev.h"
@@ -104,6 +105,16 @@ struct wema_dev {
     } aslot[WEMA_ASYNC_SLOTS];
     hipStream_t s_up = nullptr, s_down = nullptr;
     long next_ticket = 0;
+    // per-frame submit/collect: frames in submission order; the trailing fq_pending ones
+    // wait to be dispatched as one batch (ticket -1)
+    struct QueuedFrame {
+        const uint8_t *in;
+        uint8_t *out;
+        int64_t pts;
+        long ticket;
+    };
+    std::deque<QueuedFrame> fq;
+    int fq_pending = 0;
 };
 
 struct JobBuilder {
