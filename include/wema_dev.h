@@ -120,7 +120,8 @@ int wema_dev_wait_ticket(wema_dev_t *s, long ticket);
  * frame (dispatching a partial batch if that frame is still in one) and returns its pts and
  * output buffer: frames come back in submission order. Both buffers must stay valid until
  * the frame is collected; pinned memory (wema_dev_host_alloc) copies at full link speed.
- * Collecting with nothing queued is an error. */
+ * Collecting with nothing queued is an error, and so is wema_dev_submit_host_u8 while
+ * frames submitted here wait to be dispatched. */
 int wema_dev_submit(wema_dev_t *s, const uint8_t *rgb24_in, uint8_t *rgb24_out, int64_t pts);
 int wema_dev_collect(wema_dev_t *s, int64_t *pts, uint8_t **rgb24_out);
 
