@@ -124,6 +124,12 @@ int wema_dev_wait_ticket(wema_dev_t *s, long ticket);
  * frames submitted here wait to be dispatched. */
 int wema_dev_submit(wema_dev_t *s, const uint8_t *rgb24_in, uint8_t *rgb24_out, int64_t pts);
 int wema_dev_collect(wema_dev_t *s, int64_t *pts, uint8_t **rgb24_out);
+/* The same on the reference's Frame data (float RGB in [0,1], HWC; clamped float RGB out),
+ * through float device slots allocated on first use. RGB24 and float frames may be queued
+ * in any order (a change of kind starts a new batch); each is collected with the call of
+ * its kind, in submission order. */
+int wema_dev_submit_f32(wema_dev_t *s, const float *rgb_in, float *rgb_out, int64_t pts);
+int wema_dev_collect_f32(wema_dev_t *s, int64_t *pts, float **rgb_out);
 
 /* Temporal state of a session (no counterpart in the reference, whose IIR state lives in
  * IIRTemporalFilter, iir_filter.h:45-61, and window ring in TemporalBuffer, wema.h:67-81):

@@ -250,6 +250,60 @@ def test_submit_collect_per_frame_with_timestamps(max_batch):
     pin_out.close()
 
 
+def test_submit_collect_float_and_rgb24_frames():
+    """Float RGB frames (the reference's Frame data) and RGB24 frames queued one at a time in
+    one stream, in runs of either kind, come back in order with the outputs of the device
+    path (float RGB bit for bit, bytes exactly); collecting with the other kind's call is
+    refused."""
+    w, h, fps, n = 80, 60, 30.0, 17
+    _, frames = clip(2, w, h, fps, n, seed=5)
+    kw = dict(fps=fps, amp=110.0, f_low=0.5, f_high=3.0, levels=3, color=True)
+    kinds = [t % 7 < 4 for t in range(n)]          # True: float frame
+    frames_f = frames.astype(np.float32) * np.float32(1.0 / 255.0)
+    ref = wema_amd.Session(w, h, **kw)
+    ref_f = np.zeros(frames.shape, np.float32)
+    ref_u = np.zeros_like(frames)
+    for t in range(n):
+        if kinds[t]:
+            d_in = torch.from_numpy(frames_f[t:t + 1]).cuda()
+            d_out = torch.empty_like(d_in)
+            ref.process_device(d_in, d_out_f32=d_out, n=1)
+            ref.sync()
+            ref_f[t] = d_out[0].cpu().numpy()
+        else:
+            d_in = torch.from_numpy(frames[t:t + 1]).cuda()
+            d_out = torch.empty_like(d_in)
+            ref.process_device(d_in, d_out_u8=d_out, n=1)
+            ref.sync()
+            ref_u[t] = d_out[0].cpu().numpy()
+    ref.close()
+    sess = wema_amd.Session(w, h, max_batch=3, **kw)
+    in_f = wema_amd.PinnedArray(frames.shape, np.float32)
+    out_f = wema_amd.PinnedArray(frames.shape, np.float32)
+    in_u = wema_amd.PinnedArray(frames.shape)
+    out_u = wema_amd.PinnedArray(frames.shape)
+    in_f.array[:] = frames_f
+    in_u.array[:] = frames
+    for t in range(n):
+        if kinds[t]:
+            sess.submit(in_f.array[t], out_f.array[t], t)
+        else:
+            sess.submit(in_u.array[t], out_u.array[t], t)
+    with pytest.raises(wema_amd.WemaError, match="collect it with"):
+        sess.collect(f32=False)
+    for t in range(n):
+        pts, _ = sess.collect(f32=kinds[t])
+        assert pts == t
+    for t in range(n):
+        if kinds[t]:
+            assert np.array_equal(out_f.array[t], ref_f[t]), t
+        else:
+            assert np.array_equal(out_u.array[t], ref_u[t]), t
+    sess.close()
+    for a in (in_f, out_f, in_u, out_u):
+        a.close()
+
+
 @pytest.mark.parametrize("mode,cut", [("iir_color", 7), ("iir", 0), ("window_bilateral", 5),
                                       ("window32", 12), ("window32", 40)])
 def test_state_export_import_continues_the_stream(mode, cut):

@@ -110,6 +110,8 @@ _SIGS = {
     "wema_dev_wait_ticket": (C.c_int, [_vp, C.c_long]),
     "wema_dev_submit": (C.c_int, [_vp, _vp, _vp, C.c_int64]),
     "wema_dev_collect": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(_vp)]),
+    "wema_dev_submit_f32": (C.c_int, [_vp, _vp, _vp, C.c_int64]),
+    "wema_dev_collect_f32": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(_vp)]),
     "wema_dev_host_free": (None, [_vp]),
     "wema_dev_state_bytes": (C.c_int, [_vp, C.POINTER(C.c_size_t)]),
     "wema_dev_export_state": (C.c_int, [_vp, _vp, C.c_size_t]),
@@ -311,14 +313,17 @@ class Session:
         _check(lib.wema_dev_wait_ticket(self._h, int(ticket)), "wait_ticket")
 
     def submit(self, frame_in, frame_out, pts):
-        """Queue one RGB24 frame (host array) with its timestamp; its output goes to frame_out
-        (both stay alive until collected)."""
-        _check(lib.wema_dev_submit(self._h, _ptr(frame_in), _ptr(frame_out), int(pts)), "submit")
+        """Queue one frame (host array: RGB24 uint8, or float32 RGB in [0, 1]) with its
+        timestamp; its output goes to frame_out (both stay alive until collected)."""
+        f32 = np.asarray(frame_in).dtype == np.float32
+        fn = lib.wema_dev_submit_f32 if f32 else lib.wema_dev_submit
+        _check(fn(self._h, _ptr(frame_in), _ptr(frame_out), int(pts)), "submit")
 
-    def collect(self):
+    def collect(self, f32=False):
         """Wait for the oldest queued frame: (pts, address of its output buffer)."""
         pts, out = C.c_int64(), _vp()
-        _check(lib.wema_dev_collect(self._h, C.byref(pts), C.byref(out)), "collect")
+        fn = lib.wema_dev_collect_f32 if f32 else lib.wema_dev_collect
+        _check(fn(self._h, C.byref(pts), C.byref(out)), "collect")
         return pts.value, out.value
 
     # ---- measurement

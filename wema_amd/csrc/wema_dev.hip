@@ -335,29 +335,31 @@ int wema_session_alloc(wema_dev *s, int with_filt) {
 
 extern "C" void wema_dev_close(wema_dev_t *s) {
     if (!s) return;
-    hipSetDevice(s->p.device);
-    if (s->stream) hipStreamSynchronize(s->stream);
+    (void)hipSetDevice(s->p.device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int k = 0; k < 2; k++) {
-        if (s->stage_h[k]) hipHostFree(s->stage_h[k]);
-        if (s->stage_d[k]) hipFree(s->stage_d[k]);
-        if (s->stage_ev[k]) hipEventDestroy(s->stage_ev[k]);
+        if (s->stage_h[k]) (void)hipHostFree(s->stage_h[k]);
+        if (s->stage_d[k]) (void)hipFree(s->stage_d[k]);
+        if (s->stage_ev[k]) (void)hipEventDestroy(s->stage_ev[k]);
     }
-    if (s->arena) hipFree(s->arena);
-    if (s->io_in) hipFree(s->io_in);
-    if (s->io_out) hipFree(s->io_out);
-    if (s->io_in_f32) hipFree(s->io_in_f32);
-    if (s->io_out_f32) hipFree(s->io_out_f32);
+    if (s->arena) (void)hipFree(s->arena);
+    if (s->io_in) (void)hipFree(s->io_in);
+    if (s->io_out) (void)hipFree(s->io_out);
+    if (s->io_in_f32) (void)hipFree(s->io_in_f32);
+    if (s->io_out_f32) (void)hipFree(s->io_out_f32);
     for (auto &a : s->aslot) {
-        if (a.ev_out) hipEventSynchronize(a.ev_out);
-        if (a.d_in) hipFree(a.d_in);
-        if (a.d_out) hipFree(a.d_out);
-        if (a.ev_in) hipEventDestroy(a.ev_in);
-        if (a.ev_comp) hipEventDestroy(a.ev_comp);
-        if (a.ev_out) hipEventDestroy(a.ev_out);
+        if (a.ev_out) (void)hipEventSynchronize(a.ev_out);
+        if (a.d_in) (void)hipFree(a.d_in);
+        if (a.d_out) (void)hipFree(a.d_out);
+        if (a.d_in_f) (void)hipFree(a.d_in_f);
+        if (a.d_out_f) (void)hipFree(a.d_out_f);
+        if (a.ev_in) (void)hipEventDestroy(a.ev_in);
+        if (a.ev_comp) (void)hipEventDestroy(a.ev_comp);
+        if (a.ev_out) (void)hipEventDestroy(a.ev_out);
     }
-    if (s->s_up) hipStreamDestroy(s->s_up);
-    if (s->s_down) hipStreamDestroy(s->s_down);
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->s_up) (void)hipStreamDestroy(s->s_up);
+    if (s->s_down) (void)hipStreamDestroy(s->s_down);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
@@ -878,12 +880,13 @@ extern "C" int wema_dev_process_host_f32(wema_dev_t *s, const float *const *h_in
 
 // One batch of frames from host memory through a device slot: upload on s_up, process on
 // the session's stream, download on s_down. Runs of consecutive host frames go in one copy.
-static int submit_frames(wema_dev *s, const uint8_t *const *h_in, uint8_t *const *h_out, int n,
+// Frames are RGB24, or float RGB (f32) through float slots allocated on first use.
+static int submit_frames(wema_dev *s, const void *const *h_in, void *const *h_out, int n, bool f32,
                          long *ticket) {
     HIP_OK(hipSetDevice(s->p.device));
-    const size_t fb = s->g.P * 3;
+    const size_t fb = s->g.P * 3 * (f32 ? sizeof(float) : 1);
     if (!s->s_up) {
-        const size_t cap = fb * (size_t)s->p.max_batch;
+        const size_t cap = s->g.P * 3 * (size_t)s->p.max_batch;
         HIP_OK(hipStreamCreateWithFlags(&s->s_up, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&s->s_down, hipStreamNonBlocking));
         for (auto &a : s->aslot) {
@@ -894,22 +897,33 @@ static int submit_frames(wema_dev *s, const uint8_t *const *h_in, uint8_t *const
             HIP_OK(hipEventCreateWithFlags(&a.ev_out, hipEventDisableTiming));
         }
     }
+    if (f32 && !s->aslot[0].d_in_f) {
+        const size_t cap = fb * (size_t)s->p.max_batch;
+        for (auto &a : s->aslot) {
+            HIP_OK(hipMalloc(&a.d_in_f, cap));
+            HIP_OK(hipMalloc(&a.d_out_f, cap));
+        }
+    }
     const long t = s->next_ticket++;
     auto &a = s->aslot[t % WEMA_ASYNC_SLOTS];
+    char *d_in = f32 ? (char *)a.d_in_f : (char *)a.d_in;
+    char *d_out = f32 ? (char *)a.d_out_f : (char *)a.d_out;
     // the slot's previous batch has left the device before its buffers are overwritten
     if (a.ticket >= 0) HIP_OK(hipStreamWaitEvent(s->s_up, a.ev_out, 0));
     for (int i = 0, k; i < n; i = k) {
-        for (k = i + 1; k < n && h_in[k] == h_in[k - 1] + fb; k++) {}
-        HIP_OK(hipMemcpyAsync(a.d_in + fb * i, h_in[i], fb * (k - i), hipMemcpyHostToDevice, s->s_up));
+        for (k = i + 1; k < n && (const char *)h_in[k] == (const char *)h_in[k - 1] + fb; k++) {}
+        HIP_OK(hipMemcpyAsync(d_in + fb * i, h_in[i], fb * (k - i), hipMemcpyHostToDevice, s->s_up));
     }
     HIP_OK(hipEventRecord(a.ev_in, s->s_up));
     HIP_OK(hipStreamWaitEvent(s->stream, a.ev_in, 0));
-    if (process(s, a.d_in, 0, a.d_out, nullptr, n, s->stream) < 0) return -1;
+    if (process(s, d_in, f32 ? 1 : 0, f32 ? nullptr : (uint8_t *)d_out, f32 ? (float *)d_out : nullptr, n,
+                s->stream) < 0)
+        return -1;
     HIP_OK(hipEventRecord(a.ev_comp, s->stream));
     HIP_OK(hipStreamWaitEvent(s->s_down, a.ev_comp, 0));
     for (int i = 0, k; i < n; i = k) {
-        for (k = i + 1; k < n && h_out[k] == h_out[k - 1] + fb; k++) {}
-        HIP_OK(hipMemcpyAsync(h_out[i], a.d_out + fb * i, fb * (k - i), hipMemcpyDeviceToHost, s->s_down));
+        for (k = i + 1; k < n && (char *)h_out[k] == (char *)h_out[k - 1] + fb; k++) {}
+        HIP_OK(hipMemcpyAsync(h_out[i], d_out + fb * i, fb * (k - i), hipMemcpyDeviceToHost, s->s_down));
     }
     HIP_OK(hipEventRecord(a.ev_out, s->s_down));
     a.ticket = t;
@@ -924,51 +938,80 @@ extern "C" int wema_dev_submit_host_u8(wema_dev_t *s, const uint8_t *h_in, uint8
     if (n > s->p.max_batch) { wema_set_error("%d frames exceed max_batch %d", n, s->p.max_batch); return -1; }
     if (s->fq_pending) { wema_set_error("frames submitted one by one are still pending"); return -1; }
     const size_t fb = s->g.P * 3;
-    std::vector<const uint8_t *> in((size_t)n);
-    std::vector<uint8_t *> out((size_t)n);
+    std::vector<const void *> in((size_t)n);
+    std::vector<void *> out((size_t)n);
     for (int i = 0; i < n; i++) {
         in[(size_t)i] = h_in + fb * i;
         out[(size_t)i] = h_out + fb * i;
     }
-    return submit_frames(s, in.data(), out.data(), n, ticket);
+    return submit_frames(s, in.data(), out.data(), n, false, ticket);
 }
 
 // The undispatched tail of the per-frame queue as one batch.
 static int dispatch_pending(wema_dev *s) {
     const int n = s->fq_pending;
     if (!n) return 0;
-    std::vector<const uint8_t *> in((size_t)n);
-    std::vector<uint8_t *> out((size_t)n);
+    std::vector<const void *> in((size_t)n);
+    std::vector<void *> out((size_t)n);
     const size_t first = s->fq.size() - (size_t)n;
     for (int i = 0; i < n; i++) {
         in[(size_t)i] = s->fq[first + i].in;
         out[(size_t)i] = s->fq[first + i].out;
     }
     long t;
-    if (submit_frames(s, in.data(), out.data(), n, &t) < 0) return -1;
+    if (submit_frames(s, in.data(), out.data(), n, s->fq[first].f32, &t) < 0) return -1;
     for (int i = 0; i < n; i++) s->fq[first + i].ticket = t;
     s->fq_pending = 0;
     return 0;
 }
 
-extern "C" int wema_dev_submit(wema_dev_t *s, const uint8_t *rgb24_in, uint8_t *rgb24_out, int64_t pts) {
-    if (!s || !rgb24_in || !rgb24_out) { wema_set_error("bad argument"); return -1; }
+static int submit_one(wema_dev *s, const void *in, void *out, int64_t pts, bool f32) {
+    if (!s || !in || !out) { wema_set_error("bad argument"); return -1; }
     if (s->pooled) { wema_set_error("a pooled session is processed by wema_pool_tick"); return -1; }
-    s->fq.push_back({rgb24_in, rgb24_out, pts, -1});
+    // a batch holds frames of one kind: a change of kind dispatches the frames before it
+    if (s->fq_pending && s->fq.back().f32 != f32 && dispatch_pending(s) < 0) return -1;
+    s->fq.push_back({in, out, pts, -1, f32});
     if (++s->fq_pending == s->p.max_batch) return dispatch_pending(s);
     return 0;
 }
 
-extern "C" int wema_dev_collect(wema_dev_t *s, int64_t *pts, uint8_t **rgb24_out) {
+static int collect_one(wema_dev *s, int64_t *pts, void **out, bool f32) {
     if (!s) { wema_set_error("null session"); return -1; }
     if (s->fq.empty()) { wema_set_error("no frame submitted"); return -1; }
+    if (s->fq.front().f32 != f32) {
+        wema_set_error("the oldest frame is %s: collect it with wema_dev_collect%s",
+                       f32 ? "RGB24" : "float RGB", f32 ? "" : "_f32");
+        return -1;
+    }
     if (s->fq.front().ticket < 0 && dispatch_pending(s) < 0) return -1;
     const auto f = s->fq.front();
     HIP_OK(hipEventSynchronize(s->aslot[f.ticket % WEMA_ASYNC_SLOTS].ev_out));
     s->fq.pop_front();
     if (pts) *pts = f.pts;
-    if (rgb24_out) *rgb24_out = f.out;
+    if (out) *out = f.out;
     return 0;
+}
+
+extern "C" int wema_dev_submit(wema_dev_t *s, const uint8_t *rgb24_in, uint8_t *rgb24_out, int64_t pts) {
+    return submit_one(s, rgb24_in, rgb24_out, pts, false);
+}
+
+extern "C" int wema_dev_collect(wema_dev_t *s, int64_t *pts, uint8_t **rgb24_out) {
+    void *o = nullptr;
+    const int rc = collect_one(s, pts, &o, false);
+    if (rc == 0 && rgb24_out) *rgb24_out = (uint8_t *)o;
+    return rc;
+}
+
+extern "C" int wema_dev_submit_f32(wema_dev_t *s, const float *rgb_in, float *rgb_out, int64_t pts) {
+    return submit_one(s, rgb_in, rgb_out, pts, true);
+}
+
+extern "C" int wema_dev_collect_f32(wema_dev_t *s, int64_t *pts, float **rgb_out) {
+    void *o = nullptr;
+    const int rc = collect_one(s, pts, &o, true);
+    if (rc == 0 && rgb_out) *rgb_out = (float *)o;
+    return rc;
 }
 
 extern "C" int wema_dev_wait_ticket(wema_dev_t *s, long ticket) {

@@ -100,6 +100,7 @@ struct wema_dev {
     // streams
     struct AsyncSlot {
         uint8_t *d_in = nullptr, *d_out = nullptr;
+        float *d_in_f = nullptr, *d_out_f = nullptr;   // float RGB frames (allocated on first use)
         hipEvent_t ev_in = nullptr, ev_comp = nullptr, ev_out = nullptr;
         long ticket = -1;
     } aslot[WEMA_ASYNC_SLOTS];
@@ -108,10 +109,11 @@ struct wema_dev {
     // per-frame submit/collect: frames in submission order; the trailing fq_pending ones
     // wait to be dispatched as one batch (ticket -1)
     struct QueuedFrame {
-        const uint8_t *in;
-        uint8_t *out;
+        const void *in;
+        void *out;
         int64_t pts;
         long ticket;
+        bool f32;       // float RGB frame (wema_dev_submit_f32)
     };
     std::deque<QueuedFrame> fq;
     int fq_pending = 0;
