@@ -121,7 +121,8 @@ int wema_dev_wait_ticket(wema_dev_t *s, long ticket);
  * output buffer: frames come back in submission order. Both buffers must stay valid until
  * the frame is collected; pinned memory (wema_dev_host_alloc) copies at full link speed.
  * Collecting with nothing queued is an error, and so is wema_dev_submit_host_u8 while
- * frames submitted here wait to be dispatched. */
+ * frames submitted here wait to be dispatched. wema_dev_close drops frames not collected
+ * (those already dispatched finish first). */
 int wema_dev_submit(wema_dev_t *s, const uint8_t *rgb24_in, uint8_t *rgb24_out, int64_t pts);
 int wema_dev_collect(wema_dev_t *s, int64_t *pts, uint8_t **rgb24_out);
 /* The same on the reference's Frame data (float RGB in [0,1], HWC; clamped float RGB out),
