@@ -39,6 +39,7 @@ CASES = {
     "window": (40, 30, 10, 24, dict(use_iir=False)),
     "window_bilateral_color": (40, 30, 10, 25, dict(use_iir=False, bilateral=True, color=True)),
     "gauss": (40, 30, 8, 26, dict(use_iir=True, edge_aware=False)),
+    "window32_bilateral": (32, 24, 36, 28, dict(use_iir=False, bilateral=True, window=32)),
 }
 COMMON = dict(fps=30.0, amp=120.0, f_low=0.5, f_high=3.0, window=6, levels=3)
 
