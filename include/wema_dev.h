@@ -88,6 +88,20 @@ long wema_dev_frames(const wema_dev_t *s);
  * called n times, or wema_batch_process (wema_batch.c:352) when batch_path is set. */
 int wema_dev_process_u8(wema_dev_t *s, const uint8_t *d_in, uint8_t *d_out, int n,
                         void *hip_stream);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * wema_dev_process_u8 with the kernels' per-call buffers (coefficient planes, LL pyramid,
+ * reconstructions, guided-filter a and b, threshold accumulators) in one static device
+ * array of WEMA_DEV_SCRATCH_BYTES per GPU instead of the session's own memory; the session
+ * keeps its temporal state. It calls no hipMalloc or hipFree. Calls that need more scratch
+ * than that (about one 1080p colour frame, or three 1080p luma frames) return -1 with an
+ * error that says so. Every call on one GPU shares that GPU's array: calls that overlap in
+ * time on one GPU (on different streams, or from different threads) race and must be
+ * ordered by the caller. The array's address is looked up when the first session is opened
+ * on a device. */
+#define WEMA_DEV_SCRATCH_BYTES (64L << 20)
+int wema_dev_process_u8_noalloc(wema_dev_t *s, const uint8_t *d_in, uint8_t *d_out, int n,
+                                void *hip_stream);
 /* Same on float RGB in [0,1] (the reference's Frame data, wema.h:28-33). d_out_u8 and
  * d_out_f32 may each be NULL. */
 int wema_dev_process_f32(wema_dev_t *s, const float *d_in, uint8_t *d_out_u8,

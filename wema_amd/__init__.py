@@ -102,6 +102,7 @@ _SIGS = {
     "wema_dev_set_tile_rows_pass": (C.c_int, [_vp, C.c_int, C.c_int]),
     "wema_dev_autotune": (None, [C.c_int]),
     "wema_dev_process_u8": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "wema_dev_process_u8_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "wema_dev_process_f32": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "wema_dev_process_host_u8": (C.c_int, [_vp, _vp, _vp]),
     "wema_dev_sync": (C.c_int, [_vp]),
@@ -290,6 +291,13 @@ class Session:
         else:
             rc = lib.wema_dev_process_u8(self._h, a(d_in), a(d_out_u8), n, st)
         _check(rc, "wema_dev_process")
+
+    def process_device_noalloc(self, d_in, d_out_u8, n=1, stream=None):
+        """wema_dev_process_u8 with the kernels' per-call buffers in the library's static
+        device scratch (no device allocation)."""
+        st = stream if stream is not None else _stream_handle()
+        _check(lib.wema_dev_process_u8_noalloc(self._h, d_in.data_ptr(), d_out_u8.data_ptr(), n, st),
+               "wema_dev_process_u8_noalloc")
 
     def sync(self):
         _check(lib.wema_dev_sync(self._h), "wema_dev_sync")

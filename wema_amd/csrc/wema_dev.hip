@@ -28,6 +28,7 @@ hipError_t wema_launch_win(int, const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_egress(int, int, const EgressJob *, int, int, hipStream_t);
 hipError_t wema_launch_synth(const SynthFrame *, const unsigned short *, unsigned char *, int,
                              int, int, hipStream_t);
+hipError_t wema_launch_zero_u64(unsigned long long *, long long, hipStream_t);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -249,6 +250,54 @@ extern "C" int wema_dev_query_bytes(const wema_dev_params *p, size_t *bytes) {
 }
 
 // ---------------------------------------------------------------------------------------
+// static scratch of wema_dev_process_u8_noalloc: the per-call buffers of the kernels
+// (coefficient planes, LL pyramid, reconstructions, guided-filter a and b, threshold
+// accumulators) for callers that cannot allocate device memory. Each GPU the library runs
+// on holds its own copy until the process ends. In host code `scratch` names a host-side
+// stand-in: its device address on each GPU comes from hipGetSymbolAddress, taken once per
+// device when a session is opened there, and only that address is ever handed to kernels
+// or copies.
+
+static __device__ __align__(256) unsigned char scratch[WEMA_DEV_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_done[kMaxDevices];
+void *g_scratch_ptr[kMaxDevices];
+char g_scratch_err[kMaxDevices][160];
+
+// With `device` current: looks up the device address of `scratch` the first time.
+void scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    g_scratch_done[device] = true;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        snprintf(g_scratch_err[device], sizeof(g_scratch_err[device]),
+                 "hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+        (void)hipGetLastError();
+        return;
+    }
+    g_scratch_ptr[device] = ptr;
+}
+
+// The scratch's device address on `device` (null, with the error set, if there is none).
+void *scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        wema_set_error("device %d out of range for the static scratch", device);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device])
+        wema_set_error("%s", g_scratch_done[device] ? g_scratch_err[device] : "no session opened on this device");
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
 // session
 
 int wema_session_alloc(wema_dev *s, int with_filt);
@@ -267,6 +316,7 @@ int wema_dev_open_ex(wema_dev_t **out, const wema_dev_params *p, bool pooled) {
         wema_dev_close(s);
         return -1;
     }
+    scratch_setup(p->device);   // (the session's device is current)
     if (!p->color && (p->edge_aware || p->batch_path))
         wema_gf_rows_for(p->device, p->width, p->height, &s->gf_rows, &s->gf_rows2);
     *out = s;
@@ -384,9 +434,10 @@ extern "C" long wema_dev_frames(const wema_dev_t *s) { return s ? s->frames : -1
 // job tables for n frames of one session, appended to a JobBuilder (also used by the pool)
 
 void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,
-                     float *out_f32, int n) {
+                     float *out_f32, int n, const CallBufs *bufs) {
     const WemaGeom &g = s->g;
     const wema_dev_params &p = s->p;
+    const CallBufs b = bufs ? *bufs : CallBufs{s->coef, s->ampc, s->ll, s->rec, s->ab, s->acc};
     const int nch = g.nch, L = g.L;
     const size_t P = g.P, nu = g.nu, nus = g.nus;
     const size_t in_stride = P * 3 * (in_f32 ? sizeof(float) : 1);
@@ -401,14 +452,14 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         }
 
     auto coef_ptr = [&](int f, int c) -> float * {
-        if (p.use_iir) return s->coef + ((size_t)f * nch + c) * nus;
+        if (p.use_iir) return b.coef + ((size_t)f * nch + c) * nus;
         const long gidx = frame0 + f;
         return s->coef + ((size_t)c * s->ring_slots + (size_t)(gidx % s->ring_slots)) * nus;
     };
     auto ll_ptr = [&](int f, int c, int l) -> float * {
-        return s->ll + ((size_t)f * nch + c) * g.ll_total + g.ll_off[l];
+        return b.ll + ((size_t)f * nch + c) * g.ll_total + g.ll_off[l];
     };
-    auto rec_ptr = [&](int f, int c) -> float * { return s->rec + ((size_t)f * nch + c) * P; };
+    auto rec_ptr = [&](int f, int c) -> float * { return b.rec + ((size_t)f * nch + c) * P; };
 
     // forward DWT levels
     for (int l = 0; l < L; l++) {
@@ -432,7 +483,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 }
                 j.ll = ll_ptr(f, c, l);
                 j.planes = coef_ptr(f, c) + g.plane_off[l];
-                j.acc = s->acc + (size_t)f * nch + c;
+                j.acc = b.acc + (size_t)f * nch + c;
                 j.nch = jch;
                 j.ll_cstride = jch > 1 ? (long long)(ll_ptr(f, 1, l) - ll_ptr(f, 0, l)) : 0;
                 j.planes_cstride = jch > 1 ? (long long)(coef_ptr(f, 1) - coef_ptr(f, 0)) : 0;
@@ -453,7 +504,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         TemporalJob j{};
         j.n = (long long)nu;
         j.level0_end = (long long)g.plane_off[1];
-        j.acc = s->acc + c;
+        j.acc = b.acc + c;
         j.acc_stride = nch;
         j.nframes = n;
         j.n6f = (float)(2 * nu);
@@ -464,7 +515,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
             j.coef = coef_ptr(0, c);
             j.coef_stride = (long long)(nch * nus);
             j.out_stride = (long long)(nch * nus);
-            j.amp_out = s->ampc ? s->ampc + (size_t)c * nus : nullptr;
+            j.amp_out = b.ampc ? b.ampc + (size_t)c * nus : nullptr;
             j.filt_out = s->filt ? s->filt + (size_t)c * nus : nullptr;
             j.state = s->state + (size_t)c * nus;
             memcpy(j.c, s->c10, sizeof(j.c));
@@ -473,7 +524,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         } else {
             j.coef = nullptr;
             j.ring = s->coef + (size_t)c * s->ring_slots * nus;
-            j.amp_out = s->ampc + (size_t)c * nus;
+            j.amp_out = b.ampc + (size_t)c * nus;
             j.coef_stride = (long long)nus;          // ring plane stride
             j.out_stride = (long long)(nch * nus);   // amp_out / filt_out frame stride
             j.filt_out = s->filt ? s->filt + (size_t)c * nus : nullptr;
@@ -497,8 +548,8 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
     // frames whose output is the input itself (window mode while the window fills)
     auto passthrough = [&](int f) { return !p.use_iir && frame0 + f + 1 < p.window; };
     auto planes_ptr = [&](int f, int c, int l) -> const float * {
-        if (!s->ampc) return coef_ptr(f, c) + g.plane_off[l];
-        return s->ampc + ((size_t)f * nch + c) * nus + g.plane_off[l];
+        if (!b.ampc) return coef_ptr(f, c) + g.plane_off[l];
+        return b.ampc + ((size_t)f * nch + c) * nus + g.plane_off[l];
     };
     const size_t out_b = (out_u8 ? 3 * P : 0) + (out_f32 ? 12 * P : 0);
 
@@ -519,7 +570,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.nch = nch;
                 j.out_u8 = out_u8 ? out_u8 + (size_t)f * P * 3 : nullptr;
                 j.out_f32 = out_f32 ? out_f32 + (size_t)f * P * 3 : nullptr;
-                j.acc_clear = L == 1 ? s->acc + (size_t)f * nch : nullptr;
+                j.acc_clear = L == 1 ? b.acc + (size_t)f * nch : nullptr;
                 j.w = w;
                 j.h = h;
                 j.tiles_x = tiles_x;
@@ -531,7 +582,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.ll = ll_ptr(f, c, l);
                 j.planes = planes_ptr(f, c, l);
                 j.out = l == 0 ? rec_ptr(f, c) : ll_ptr(f, c, l - 1);
-                j.acc_clear = l == L - 1 ? s->acc + (size_t)f * nch + c : nullptr;
+                j.acc_clear = l == L - 1 ? b.acc + (size_t)f * nch + c : nullptr;
                 j.nch = 1;
                 j.w = w;
                 j.h = h;
@@ -554,7 +605,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         j.cr = p.color ? rec_ptr(f, 2) : nullptr;
         if (in_f32) j.src_f32 = (const float *)((const char *)in + (size_t)f * in_stride);
         else j.src_u8 = (const uint8_t *)in + (size_t)f * in_stride;
-        j.ab = s->ab ? s->ab + (size_t)f * P : nullptr;
+        j.ab = b.ab ? b.ab + (size_t)f * P : nullptr;
         j.out_u8 = out_u8 ? out_u8 + (size_t)f * P * 3 : nullptr;
         j.out_f32 = out_f32 ? out_f32 + (size_t)f * P * 3 : nullptr;
         j.w = g.W;
@@ -666,7 +717,7 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
 }
 
 static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, float *out_f32,
-                   int n, void *stream) {
+                   int n, void *stream, const CallBufs *bufs = nullptr) {
     if (!s) { wema_set_error("null session"); return -1; }
     if (s->pooled) { wema_set_error("a pooled session is processed by wema_pool_tick"); return -1; }
     if (n < 1 || n > s->p.max_batch) {
@@ -680,8 +731,10 @@ static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, flo
     const int k = s->stage_cur;
     s->stage_cur ^= 1;
     HIP_OK(hipEventSynchronize(s->stage_ev[k]));   // staging buffer k free again
+    if (bufs)   // the call's accumulators start at zero (the scratch keeps the last call's contents)
+        HIP_OK(wema_launch_zero_u64(bufs->acc, (long long)n * s->g.nch, st));
     JobBuilder jb;
-    wema_build_jobs(s, jb, in, in_f32, out_u8, out_f32, n);
+    wema_build_jobs(s, jb, in, in_f32, out_u8, out_f32, n, bufs);
     jb.fix_amp(s->amp_d);
     if (wema_run_jobs(jb, s->stage_h[k], s->stage_d[k], s->stage_ev[k], s->amp_d, st, &s->timer) < 0)
         return -1;
@@ -694,6 +747,47 @@ static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, flo
 extern "C" int wema_dev_process_u8(wema_dev_t *s, const uint8_t *d_in, uint8_t *d_out, int n,
                                    void *hip_stream) {
     return process(s, d_in, 0, d_out, nullptr, n, hip_stream);
+}
+
+// Bytes of static scratch a call of n frames needs, and the buffers carved from base.
+static size_t scratch_plan(const wema_dev *s, int n, char *base, CallBufs *b) {
+    const WemaGeom &g = s->g;
+    const size_t nf = (size_t)n, nch = (size_t)g.nch;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base ? base + off : nullptr;
+        off = align_up(off + bytes, 256);
+        return p;
+    };
+    // IIR: the coefficient planes are per call (window mode keeps them in the session's ring);
+    // window mode: the amplified planes; IIR with the test capture keeps its arena copies
+    b->coef = s->p.use_iir ? (float *)take(sizeof(float) * g.nus * nch * nf) : s->coef;
+    b->ampc = s->p.use_iir ? s->ampc : (float *)take(sizeof(float) * g.nus * nch * nf);
+    b->ll = (float *)take(sizeof(float) * g.ll_total * nch * nf);
+    b->rec = (float *)take(sizeof(float) * g.P * nch * nf);
+    b->ab = s->ab ? (float2 *)take(sizeof(float2) * g.P * nf) : nullptr;
+    b->acc = (unsigned long long *)take(sizeof(unsigned long long) * nch * nf);
+    return off;
+}
+
+extern "C" int wema_dev_process_u8_noalloc(wema_dev_t *s, const uint8_t *d_in, uint8_t *d_out, int n,
+                                           void *hip_stream) {
+    if (!s) { wema_set_error("null session"); return -1; }
+    if (n < 1 || n > s->p.max_batch) {
+        wema_set_error("frame count %d outside 1..%d", n, s->p.max_batch);
+        return -1;
+    }
+    CallBufs b;
+    const size_t need = scratch_plan(s, n, nullptr, &b);
+    if (need > (size_t)WEMA_DEV_SCRATCH_BYTES) {
+        wema_set_error("%d frame(s) of %dx%d need %zu bytes of scratch; the static scratch holds %zu",
+                       n, s->p.width, s->p.height, need, (size_t)WEMA_DEV_SCRATCH_BYTES);
+        return -1;
+    }
+    char *base = (char *)scratch_ptr(s->p.device);
+    if (!base) return -1;
+    scratch_plan(s, n, base, &b);
+    return process(s, d_in, 0, d_out, nullptr, n, hip_stream, &b);
 }
 
 extern "C" int wema_dev_process_f32(wema_dev_t *s, const float *d_in, uint8_t *d_out_u8,

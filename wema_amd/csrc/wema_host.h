@@ -34,6 +34,18 @@ struct WemaGeom {
 
 
 
+// Device buffers that a call's kernels write and read within the call: by default the
+// session's arena; wema_dev_process_u8_noalloc points them into the static scratch.
+struct CallBufs {
+    float *coef;                // IIR: coefficient planes [B][nch][nus] (window mode: the
+                                // session's ring, never replaced)
+    float *ampc;                // amplified coefficients when kept apart (or null)
+    float *ll;                  // [B][nch][ll_total]
+    float *rec;                 // [B][nch][P]
+    float2 *ab;                 // [B][P] (or null)
+    unsigned long long *acc;    // [B][nch], zero at the start of the call
+};
+
 struct JobList {
     size_t offset = 0;   // byte offset in the packed table
     int count = 0;
@@ -234,7 +246,7 @@ int wema_auto_levels(int width, int height);
 void wema_gf_rows_for(int device, int w, int h, int *rows1, int *rows2);
 void wema_geometry_init(WemaGeom *g, const wema_dev_params *p);
 void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,
-                     float *out_f32, int n);
+                     float *out_f32, int n, const CallBufs *bufs = nullptr);
 int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage_ev,
                   float *amp_d, hipStream_t st, LaunchTimer *timer);
 int wema_timer_collect(LaunchTimer *t);

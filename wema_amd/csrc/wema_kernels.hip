@@ -1847,6 +1847,13 @@ __global__ void __launch_bounds__(256) k_passthru(const EgressJob *__restrict__ 
     if (J.out_f32) J.out_f32[i] = v;
 }
 
+// Clears n 64-bit words (threshold accumulators) with vector stores: a kernel node, which
+// graph capture records as it records the pipeline's own launches.
+__global__ void __launch_bounds__(256) k_zero_u64(unsigned long long *__restrict__ p, long long n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
 // ---------------------------------------------------------------------------------------
 // Synthetic frames: one thread per pixel.
 __global__ void __launch_bounds__(256) k_synth(const SynthFrame *__restrict__ frames,
@@ -1918,6 +1925,11 @@ hipError_t wema_launch_egress(int kind, int rows, const EgressJob *jobs, int njo
     case EG_GAUSS: hipLaunchKernelGGL(k_gauss, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     default: hipLaunchKernelGGL(k_passthru, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     }
+    return hipGetLastError();
+}
+hipError_t wema_launch_zero_u64(unsigned long long *p, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 hipError_t wema_launch_synth(const SynthFrame *frames, const unsigned short *tex,
