@@ -12,7 +12,12 @@
  *   - num_levels may be changed between wema_init and the first frame (the reference
  *     always uses its automatic rule, src/wema.c:384); up to 12 levels;
  *   - frame_alloc allocates pinned host memory, so frames copy to and from the GPU at full
- *     link speed (frame_free releases it);
+ *     link speed, while the process's pinned frame bytes stay within a cap
+ *     (WEMA_PINNED_MAX_BYTES, default a quarter of physical memory up to 16 GiB); past it,
+ *     or where pinning fails, frames are 64-byte-aligned pageable memory as in the
+ *     reference; frame_free releases either (wema_frame_pinned tells which);
+ *   - wema_batch_process streams its frames through the GPU in sub-batches, so the copies
+ *     of one sub-batch overlap the processing of the next;
  *   - the device session is opened at the first wema_process_frame / wema_batch_process
  *     call, when the fields the caller writes after wema_init (src/main.c:288-290) are
  *     final; option errors (an invalid band or window) are reported there as -1;
@@ -105,7 +110,8 @@ int  wema_flush(WemaContext *ctx, Frame *out);
 bool wema_ready(const WemaContext *ctx);
 
 /* src/wema_batch.c:138-217, 219-234, 352-457: frame_count frames of the stream in, the same
- * number out (guided filter always on, src/wema_batch.c:452); returns frame_count or -1. */
+ * number out (guided filter always on, src/wema_batch.c:452); frame_count is clamped to the
+ * batch size (src/wema_batch.c:355-357); returns that count or -1. */
 int  wema_batch_init(WemaBatchContext *batch, const WemaContext *wema,
                      int batch_size, int num_threads);
 void wema_batch_free(WemaBatchContext *batch);
@@ -113,9 +119,12 @@ int  wema_batch_process(WemaContext *wema, WemaBatchContext *batch,
                         const Frame *frames_in, Frame *frames_out,
                         int frame_count);
 
-/* src/wema.c:29-40 (pinned host memory here) */
+/* src/wema.c:29-40 (pinned host memory here, within the cap above) */
 int  frame_alloc(Frame *frame, int width, int height, int channels);
 void frame_free(Frame *frame);
+/* 1 if frame_alloc pinned the frame, 0 if it is pageable, -1 if frame_alloc did not
+ * allocate it (no counterpart in the reference). */
+int  wema_frame_pinned(const Frame *frame);
 
 /* Types of the CLI and its FFmpeg pipes, which live in the same header in the reference
  * (src/wema.h:166-212); carried with the same fields so that its main.c, ffmpeg_io.c and

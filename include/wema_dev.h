@@ -266,6 +266,11 @@ int wema_dev_debug_set_thresholds(wema_dev_t *s, const float *thr, int n);
  *       1: temporal filter output (IIR y / window delta), 2: amplified coefficients,
  *       3: IIR state (4 * n_unique floats, per position HP w1, HP w2, LP w1, LP w2). */
 int wema_dev_debug_read(wema_dev_t *s, int which, int frame, int ch, float *host, size_t count);
+/* The guided filter's covariance division q[i] = a[i] / b[i] as the kernels compute it, and
+ * the hardware reciprocal estimate rcp[i] of b[i] it starts from; n elements, device
+ * buffers, enqueued on the stream. */
+int wema_dev_debug_div_mid(const float *d_a, const float *d_b, float *d_q, float *d_rcp, long n,
+                           void *hip_stream);
 
 /* ---- synthetic video (deterministic, integer-only; identical on host and device) ---- */
 typedef struct {

@@ -21,6 +21,23 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <xmmintrin.h>
+
+/* Floating-point mode. The reference runs with denormals flushed (FTZ) and treated as zero
+ * (DAZ): its -Ofast build links crtfastmath, which sets both at start-up; the kernels flush
+ * denormals too. Every FP32 entry point below (and the DWT, whose FP32 and FP64 forms share
+ * one body; FP64 values here never come near its denormal range) sets both for its own
+ * duration and restores
+ * the caller's mode on return, so results do not depend on the mode of the calling thread
+ * (for instance on whether the compiled reference was loaded into the process first). */
+#define MXCSR_FTZ_DAZ 0x8040u
+static unsigned fp_enter(void) {
+    const unsigned m = _mm_getcsr();
+    _mm_setcsr(m | MXCSR_FTZ_DAZ);
+    return m;
+}
+static void fp_restore(unsigned *m) { _mm_setcsr(*m); }
+#define FP_REFERENCE_MODE() unsigned fpm_ __attribute__((cleanup(fp_restore))) = fp_enter()
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -88,6 +105,7 @@ size_t orc_num_unique(int w, int h, int levels) {
 #define FMA_D(a, b, c) ((a) * (b) + (c))
 
 void orc_fwd1d_f32(float *x, int n) {
+    FP_REFERENCE_MODE();
     if (n < 2) return;
     const int half = (n + 1) / 2, odd_n = n / 2;
     float *e = malloc(sizeof(float) * (size_t)n), *o = e + half;
@@ -103,6 +121,7 @@ void orc_fwd1d_f32(float *x, int n) {
 }
 
 void orc_inv1d_f32(float *x, int n) {
+    FP_REFERENCE_MODE();
     if (n < 2) return;
     const int half = (n + 1) / 2, odd_n = n / 2;
     float *e = malloc(sizeof(float) * (size_t)n), *o = e + half;
@@ -176,6 +195,7 @@ static void inv2d_##SUF(T *img, int w, int h, T *col) {                         
     for (int y = 0; y < h; y++) INV1(img + (size_t)y * w, w);                              \
 }                                                                                          \
 void orc_dwt_fwd_##SUF(const T *image, int W, int H, int levels, T *planes, T *ll_out) {   \
+    FP_REFERENCE_MODE();                                                                   \
     size_t P = (size_t)W * H;                                                              \
     T *work = malloc(P * sizeof(T)), *col = malloc((size_t)(W > H ? W : H) * sizeof(T));   \
     T *sb[3];                                                                              \
@@ -208,6 +228,7 @@ void orc_dwt_fwd_##SUF(const T *image, int W, int H, int levels, T *planes, T *l
     free(col);                                                                             \
 }                                                                                          \
 void orc_dwt_inv_##SUF(const T *planes, const T *ll, int W, int H, int levels, T *img) {   \
+    FP_REFERENCE_MODE();                                                                   \
     int lw[32], lh[32];                                                                    \
     orc_level_dims(W, H, levels, lw, lh);                                                  \
     size_t P = (size_t)W * H, off[33];                                                     \
@@ -277,6 +298,7 @@ static inline float cr_f32(float r, float g, float b) {
 }
 
 void orc_ingest_f32(const uint8_t *rgb, size_t npx, int channel, float *out) {
+    FP_REFERENCE_MODE();
     for (size_t i = 0; i < npx; i++) {
         float r = u8f(rgb[3 * i]), g = u8f(rgb[3 * i + 1]), b = u8f(rgb[3 * i + 2]);
         out[i] = channel == 0 ? luma_f32(r, g, b) : channel == 1 ? cb_f32(r, g, b) : cr_f32(r, g, b);
@@ -322,6 +344,7 @@ static void design_lp(float *c, float fc, float fs) {
 }
 
 int orc_iir_design(float f_low, float f_high, float fs, float *c10, int *warmup) {
+    FP_REFERENCE_MODE();
     if (f_low >= f_high || f_low <= 0 || f_high >= fs * 0.5f) return -1;
     design_hp(c10, f_low, fs);
     design_lp(c10 + 5, f_high, fs);
@@ -377,6 +400,7 @@ static inline double biquad2_f64(const double *c, double *st, double x) {
 }
 
 void orc_iir_preseed_f32(const float *c10, const float *x, size_t n, int warmup, float *st) {
+    FP_REFERENCE_MODE();
     for (size_t i = 0; i < n; i++) {
         float *s = st + 4 * i;
         s[0] = s[1] = s[2] = s[3] = 0.0f;
@@ -385,6 +409,7 @@ void orc_iir_preseed_f32(const float *c10, const float *x, size_t n, int warmup,
 }
 
 void orc_iir_step_f32(const float *c10, const float *x, size_t n, float *st, float *y) {
+    FP_REFERENCE_MODE();
     for (size_t i = 0; i < n; i++) y[i] = biquad2_f32(c10, st + 4 * i, x[i]);
 }
 
@@ -410,6 +435,7 @@ void orc_iir_step_f64(const double *c10, const double *x, size_t n, double *st, 
  *===========================================================================*/
 
 uint64_t orc_abs_fixed(float v) {
+    FP_REFERENCE_MODE();
     union { float f; uint32_t u; } b = {v};
     uint32_t e = (b.u >> 23) & 0xFF;
     if (e == 0) return 0;                     /* zero / denormal */
@@ -421,6 +447,7 @@ uint64_t orc_abs_fixed(float v) {
 }
 
 float orc_threshold_from_fixed(uint64_t sum_unique, size_t n_unique) {
+    FP_REFERENCE_MODE();
     double s = (double)sum_unique * (1.0 / 4294967296.0);
     float mag_sum = (float)(2.0 * s);
     float n6 = (float)(2 * n_unique);
@@ -430,6 +457,7 @@ float orc_threshold_from_fixed(uint64_t sum_unique, size_t n_unique) {
 
 /* Fade-in after the preseed (src/wema.c:830-839, src/wema_batch.c:426-435). */
 float orc_fade_amp(float amp, int global_frame) {
+    FP_REFERENCE_MODE();
     const int fade_in_frames = 15;
     if (global_frame < 0) return 0.0f;
     if (global_frame < fade_in_frames) {
@@ -473,6 +501,7 @@ static void box_line_f32(const float *v, size_t st, int n, int r, int prefix, fl
 }
 
 void orc_box_f32(const float *in, float *out, int w, int h, int r, int prefix) {
+    FP_REFERENCE_MODE();
     size_t P = (size_t)w * h;
     float *tmp = malloc(P * sizeof(float));
     float *pre = malloc((size_t)((w > h ? w : h) + 1) * sizeof(float));
@@ -500,6 +529,7 @@ void orc_box_f64(const double *in, double *out, int w, int h, int r) {
 }
 
 void orc_guided_f32(const float *I, const float *p, float *q, int w, int h, int prefix) {
+    FP_REFERENCE_MODE();
     const int r = 4;
     const float eps = 0.01f;
     size_t n = (size_t)w * h;
@@ -549,6 +579,7 @@ void orc_guided_f64(const double *I, const double *p, double *q, int w, int h) {
 }
 
 void orc_gauss5_f32(const float *in, float *out, int w, int h) {
+    FP_REFERENCE_MODE();
     static const float k[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.0f, 1.0f / 16.0f};
     float *tmp = malloc((size_t)w * h * sizeof(float));
     for (int y = 0; y < h; y++)
@@ -587,6 +618,7 @@ typedef struct {
 } OrcPipe;
 
 void *orc_pipe_open(const OrcParams *prm) {
+    FP_REFERENCE_MODE();
     OrcPipe *o = calloc(1, sizeof(OrcPipe));
     o->p = *prm;
     int L = prm->levels;
@@ -681,6 +713,7 @@ static float window_delta(const float *s, int ws, int bilateral, const float *tw
 
 int orc_pipe_frame(void *h, const uint8_t *rgb, uint8_t *out_u8, float *out_f32,
                    const float *thr_override, float *thr_used) {
+    FP_REFERENCE_MODE();
     OrcPipe *o = h;
     const OrcParams *p = &o->p;
     const int W = p->width, H = p->height, L = o->L;

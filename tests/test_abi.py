@@ -119,3 +119,70 @@ def test_synth_disc_moves():
     # the disc edge moves by <= 0.2 px; frames differ near its boundary and by noise elsewhere
     f0, f5 = s.frame(0).astype(int), s.frame(5).astype(int)
     assert np.abs(f0 - f5).max() > 0
+
+
+class Frame(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_float)), ("width", C.c_int), ("height", C.c_int),
+                ("channels", C.c_int)]
+
+
+@pytest.mark.parametrize("cap", ["0", None])
+def test_frame_alloc_falls_back_to_pageable_memory(cap, monkeypatch):
+    """frame_alloc past its pinned-byte cap (0 here), or where pinning fails (no GPU here):
+    64-byte-aligned, zeroed pageable frames that frame_free releases."""
+    if cap is None:
+        monkeypatch.delenv("WEMA_PINNED_MAX_BYTES", raising=False)
+    else:
+        monkeypatch.setenv("WEMA_PINNED_MAX_BYTES", cap)
+    host = C.CDLL(LIBS["wema.h"])
+    host.frame_alloc.argtypes = [C.POINTER(Frame), C.c_int, C.c_int, C.c_int]
+    host.frame_free.argtypes = [C.POINTER(Frame)]
+    host.wema_frame_pinned.argtypes = [C.POINTER(Frame)]
+    probe = wema_amd.lib.wema_dev_host_alloc(64)   # can this process pin memory at all?
+    if probe:
+        wema_amd.lib.wema_dev_host_free(probe)
+    frames = [Frame() for _ in range(3)]
+    for i, f in enumerate(frames):
+        assert host.frame_alloc(C.byref(f), 33 + i, 17, 3) == 0
+        addr = C.cast(f.data, C.c_void_p).value
+        assert addr and addr % 64 == 0
+        assert (f.width, f.height, f.channels) == (33 + i, 17, 3)
+        a = np.ctypeslib.as_array(f.data, ((33 + i) * 17 * 3,))
+        assert not a.any()
+        a[:] = 0.5
+        assert host.wema_frame_pinned(C.byref(f)) == (0 if cap == "0" or not probe else 1)
+    for f in frames:
+        host.frame_free(C.byref(f))
+        assert not f.data
+    assert host.frame_alloc(C.byref(frames[0]), 0, 17, 3) == -1
+
+
+def _closed_form_levels(w, h):
+    """dwt_compute_levels (src/dwt.c:588-596) with the shipped cap of 4: the smaller side is
+    halved (integer division) while it is >= 16, at most 4 times, and at least one level is
+    used: clamp(floor(log2(min(w, h))) - 3, 1, 4)."""
+    m = min(w, h)
+    return min(max(m.bit_length() - 1 - 3, 1), 4)
+
+
+def test_auto_levels_closed_form_every_size():
+    f = wema_amd.lib.wema_dev_auto_levels
+    for m in range(1, 4097):
+        for other in (m, m + 1, 4096, 16384):
+            for w, h in ((m, other), (other, m)):
+                want = _closed_form_levels(w, h)
+                assert f(w, h) == want, (w, h, f(w, h), want)
+                assert wema_amd.auto_levels(w, h) == want
+
+
+@pytest.mark.ref
+def test_auto_levels_equal_the_compiled_reference():
+    """The reference's own dwt_compute_levels, compiled from its sources with the level cap
+    raised to 8 (oracle/Makefile): capped at the shipped 4, it gives our count at every
+    smaller side from 1 to 4096."""
+    from oracle import oracle as O
+    ref = O.ref()
+    ref.dwt_compute_levels.argtypes = [C.c_int, C.c_int]
+    for m in range(1, 4097):
+        assert min(ref.dwt_compute_levels(m, 4096), 4) == wema_amd.lib.wema_dev_auto_levels(m, 4096), m
+        assert min(ref.dwt_compute_levels(4096, m), 4) == wema_amd.lib.wema_dev_auto_levels(4096, m), m

@@ -1,6 +1,9 @@
 """The C11 host API (include/wema.h, wema_amd/libwema.so), driven from C the way the
 reference's CLI drives its wema.h (tests/c/wema_c_driver.c): per-frame and batch paths,
-luma/colour, IIR/window. The bar: output bytes identical to the oracle run with the same
+luma/colour, IIR/window; pinned and pageable frames (frame_alloc past its pinned-byte
+cap); batches larger than the host's sub-batch (several sub-batches in flight); calls
+passing more frames than the batch size, which are clamped to it as the reference does
+(src/wema_batch.c:355-357). The bar: output bytes identical to the oracle run with the same
 options (the bilateral mode within 1 LSB: expf)."""
 import os
 import subprocess
@@ -26,6 +29,9 @@ CASES = {
     "window_frame": dict(use_iir=0, color=0, edge=1, bil=0, batch=1),
     "window_bilateral_color": dict(use_iir=0, color=1, edge=1, bil=1, batch=1),
     "gauss_frame": dict(use_iir=1, color=0, edge=0, bil=0, batch=1),
+    "iir_batch_pageable": dict(use_iir=1, color=0, edge=1, bil=0, batch=5, pinned_cap=0),
+    "iir_batch_over": dict(use_iir=1, color=0, edge=1, bil=0, batch=5, over=3),
+    "iir_batch_sub_batches": dict(use_iir=1, color=0, edge=1, bil=0, batch=20),
 }
 
 
@@ -48,8 +54,11 @@ def test_c_host_equals_oracle(case, driver, tmp_path):
     frames.tofile(fin)
     args = [driver, str(fin), str(fout), str(w), str(h), str(n), str(fps), "90", "0.5", "3.0",
             str(window), str(levels), str(c["use_iir"]), str(c["color"]), str(c["edge"]), str(c["bil"]),
-            str(c["batch"])]
-    res = subprocess.run(args, capture_output=True, text=True, timeout=120)
+            str(c["batch"]), str(c.get("over", 0))]
+    env = dict(os.environ)
+    if "pinned_cap" in c:
+        env["WEMA_PINNED_MAX_BYTES"] = str(c["pinned_cap"])
+    res = subprocess.run(args, capture_output=True, text=True, timeout=120, env=env)
     assert res.returncode == 0, res.stderr
     got = np.fromfile(fout, np.uint8).reshape(n, h, w, 3)
     kw = dict(fps=fps, amp=90.0, f_low=0.5, f_high=3.0, window=window, levels=levels,
@@ -61,5 +70,6 @@ def test_c_host_equals_oracle(case, driver, tmp_path):
         o_out, _ = orc.frame(frames[t])
         worst = max(worst, int(np.abs(got[t].astype(int) - o_out.astype(int)).max()))
     assert worst <= (1 if c["bil"] else 0), worst
-    ready = int(res.stdout.split()[-1])
-    assert ready == 1
+    words = res.stdout.split()
+    assert int(words[words.index("ready") + 1]) == 1
+    assert int(words[words.index("pinned") + 1]) == (0 if "pinned_cap" in c else 1)

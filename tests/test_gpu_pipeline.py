@@ -9,6 +9,10 @@ Bars (stated per test):
     thresholds are injected (the reference's FP32 threshold sum is compiler-vectorised and
     cannot be reproduced by a parallel reduction; mask flips are counted separately).
 """
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -39,6 +43,21 @@ def run_gpu(frames, batch=1, capture=False, **kw):
         sess.sync()
         thr.append(sess.thresholds(k))
     return sess, d_out.cpu().numpy(), np.concatenate(thr)
+
+
+def oracle_clip(frames, want_f32=False, **kw):
+    """The oracle over a whole clip (u8 frames; float frames in the device path are these
+    bytes times fl(1/255), which is what the oracle's ingest computes)."""
+    n, h, w, _ = frames.shape
+    orc = O.OraclePipe(w, h, **kw)
+    res = [orc.frame(frames[t], want_f32=want_f32) for t in range(n)]
+    out = np.stack([r[0] for r in res])
+    orc.close()
+    return (out, np.stack([r[2] for r in res])) if want_f32 else out
+
+
+def max_diff(a, b):
+    return int(np.abs(a.astype(int) - b.astype(int)).max()) if a.size else 0
 
 
 def test_synth_device_matches_host():
@@ -185,12 +204,12 @@ def test_tile_rows_do_not_change_results(w, h):
 @pytest.mark.parametrize("inflight", [2, 3, 5])
 def test_async_host_pipeline_equals_device_path(inflight):
     """Submit/wait over pinned host frames, with 2 (the library's device slots), 3 or 5
-    batches in flight (slots reused while earlier batches are still queued), gives the bytes
-    of the device-buffer path, for every frame and batch boundary."""
+    batches in flight (slots reused while earlier batches are still queued), gives the
+    oracle's bytes, for every frame and batch boundary."""
     w, h, fps, n, batch = 96, 64, 30.0, 23, 3
     _, frames = clip(4, w, h, fps, n, seed=9)
     kw = dict(fps=fps, amp=120.0, f_low=0.5, f_high=3.0, levels=3)
-    _, ref, _ = run_gpu(frames, batch=batch, **kw)
+    ref = oracle_clip(frames, **kw)
     sess = wema_amd.Session(w, h, max_batch=batch, **kw)
     pin_in = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(inflight)]
     pin_out = [wema_amd.PinnedArray((batch, h, w, 3)) for _ in range(inflight)]
@@ -217,12 +236,12 @@ def test_async_host_pipeline_equals_device_path(inflight):
 @pytest.mark.parametrize("max_batch", [1, 4])
 def test_submit_collect_per_frame_with_timestamps(max_batch):
     """Frames submitted one at a time from pinned memory, with timestamps, come back in
-    order with their own timestamps and output buffers, and with the device path's bytes;
+    order with their own timestamps and output buffers, and with the oracle's bytes;
     collects interleave with submits (partial batches dispatched on demand)."""
     w, h, fps, n = 96, 64, 30.0, 23
     _, frames = clip(4, w, h, fps, n, seed=12)
     kw = dict(fps=fps, amp=120.0, f_low=0.5, f_high=3.0, levels=3)
-    _, ref, _ = run_gpu(frames, batch=1, **kw)
+    ref = oracle_clip(frames, **kw)
     sess = wema_amd.Session(w, h, max_batch=max_batch, **kw)
     pin_in = wema_amd.PinnedArray((n, h, w, 3))
     pin_out = wema_amd.PinnedArray((n, h, w, 3))
@@ -252,9 +271,9 @@ def test_submit_collect_per_frame_with_timestamps(max_batch):
 
 def test_submit_collect_float_and_rgb24_frames():
     """Float RGB frames (the reference's Frame data) and RGB24 frames queued one at a time in
-    one stream, in runs of either kind, come back in order with the outputs of the device
-    path (float RGB bit for bit, bytes exactly); collecting with the other kind's call is
-    refused."""
+    one stream, in runs of either kind, come back in order with the oracle's outputs (float
+    RGB bit for bit, bytes exactly) and the device path's; collecting with the other kind's
+    call is refused."""
     w, h, fps, n = 80, 60, 30.0, 17
     _, frames = clip(2, w, h, fps, n, seed=5)
     kw = dict(fps=fps, amp=110.0, f_low=0.5, f_high=3.0, levels=3, color=True)
@@ -294,11 +313,14 @@ def test_submit_collect_float_and_rgb24_frames():
     for t in range(n):
         pts, _ = sess.collect(f32=kinds[t])
         assert pts == t
+    orc_u, orc_f = oracle_clip(frames, want_f32=True, **kw)
     for t in range(n):
         if kinds[t]:
             assert np.array_equal(out_f.array[t], ref_f[t]), t
+            assert np.array_equal(out_f.array[t], orc_f[t]), t
         else:
             assert np.array_equal(out_u.array[t], ref_u[t]), t
+            assert np.array_equal(out_u.array[t], orc_u[t]), t
     sess.close()
     for a in (in_f, out_f, in_u, out_u):
         a.close()
@@ -310,7 +332,8 @@ def test_state_export_import_continues_the_stream(mode, cut):
     """A session that imports another's exported temporal state (IIR sections, or the last
     window - 1 frames of the ring, and the frame count) continues with the bytes the first
     session gives, with a different max_batch; including exports before the first frame,
-    while the window fills, and after it has filled."""
+    while the window fills, and after it has filled. Both equal the oracle's bytes over the
+    whole clip (within 1 LSB with bilateral weights)."""
     w, h, fps, n = 64, 48, 30.0, 50
     kw = dict(fps=fps, amp=100.0, f_low=0.5, f_high=3.0, levels=3)
     if mode == "iir_color":
@@ -343,6 +366,10 @@ def test_state_export_import_continues_the_stream(mode, cut):
     assert torch.equal(out_a[cut:], out_b[cut:])
     if cut:
         assert torch.equal(tmp[:cut], out_a[:cut])
+    orc = oracle_clip(frames, **kw)
+    tol = 1 if kw.get("bilateral") else 0
+    assert max_diff(out_b[cut:].cpu().numpy(), orc[cut:]) <= tol
+    assert max_diff(out_a.cpu().numpy(), orc) <= tol
     # the state of another filter or size is refused
     other = dict(kw, f_high=2.5) if kw.get("use_iir", True) else dict(kw, window=kw["window"] + 2)
     for ww, hh, kk in [(w, h, other), (w + 2, h, kw)]:
@@ -426,25 +453,56 @@ def test_largest_frame_batch_equals_streaming():
     assert not torch.equal(outs[0], d_in)
 
 
-@pytest.mark.parametrize("preset,w,h,fps,kw", [
-    (2, 1920, 1080, 30.0, dict(levels=5, amp=120.0, f_low=0.5, f_high=3.0, color=True)),
-    (3, 3840, 2160, 60.0, dict(levels=6, amp=50.0, f_low=0.8, f_high=2.0)),
-])
-def test_full_size_workloads_vs_oracle(preset, w, h, fps, kw):
-    """The 1080p colour and 4K guided-filter workloads at their full sizes (including the
-    odd-parent level 135 -> 68 where stale entries enter the threshold): thresholds and
-    output bytes equal to the oracle's for the first frames, batched as the benchmark runs
-    them."""
-    n = 3 if w < 3000 else 2   # the oracle's 4K frame 0 runs the IIR warm-up: ~10 s on a CPU
-    syn = wema_amd.Synth(preset, w, h, fps)
-    frames = np.stack([syn.frame(t) for t in range(n)])
-    sess, g_out, g_thr = run_gpu(frames, batch=n, fps=fps, **kw)
-    orc = O.OraclePipe(w, h, fps=fps, **kw)
+DIGESTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_digests.json")
+
+
+def digest_case(name):
+    with open(DIGESTS) as f:
+        return json.load(f)["cases"][name]
+
+
+@pytest.mark.parametrize("name,batch", [("c2", 10), ("c3", 8)])
+def test_full_size_full_gain_vs_oracle_digests(name, batch):
+    """The 1080p colour and 4K guided-filter workloads at their full sizes for 20 frames,
+    past the 15-frame fade-in (src/wema.c:830-839), so frames 15-19 run at full gain; in
+    uneven batches (10 + 10, 8 + 8 + 4). Every frame's output bytes and thresholds equal the
+    oracle's: SHA-256 digests and threshold bits written by tests/golden/make_digests.py
+    (including the odd-parent level 135 -> 68, where stale entries enter the threshold)."""
+    case = digest_case(name)
+    c = case["params"]
+    w, h, n = c["w"], c["h"], case["frames"]
+    syn = wema_amd.Synth(c["preset"], w, h, c["fps"], c["seed_offset"])
+    d_in = syn.frames_device(0, n)
+    sess = wema_amd.Session(w, h, c["fps"], max_batch=batch, **c["kw"])
+    d_out = torch.empty_like(d_in)
+    thr = []
+    for t0 in range(0, n, batch):
+        k = min(batch, n - t0)
+        sess.process_device(d_in[t0:t0 + k], d_out_u8=d_out[t0:t0 + k], n=k)
+        sess.sync()
+        thr.append(sess.thresholds(k))
+    thr = np.concatenate(thr)
+    out = d_out.cpu().numpy()
+    nch = 3 if c["kw"].get("color") else 1
+    for t in range(n):
+        assert list(thr[t][:nch].view(np.uint32)) == case["thr_bits"][t][:nch], (name, t)
+        assert hashlib.sha256(out[t].tobytes()).hexdigest() == case["out_sha256"][t], (name, t)
+    sess.close()
+
+
+def test_c1_clip_vs_oracle():
+    """The 320x240 workload as the benchmark runs it: its whole 128-frame clip in one call
+    (the IIR state in registers over all frames), bytes and thresholds equal to the oracle's
+    for every frame."""
+    w, h, fps, n = 320, 240, 30.0, 128
+    _, frames = clip(1, w, h, fps, n)
+    kw = dict(fps=fps, amp=120.0, f_low=0.5, f_high=3.0, levels=3)
+    _, g_out, g_thr = run_gpu(frames, batch=n, **kw)
+    orc = O.OraclePipe(w, h, **kw)
     for t in range(n):
         o_out, o_thr = orc.frame(frames[t])
-        nch = 3 if kw.get("color") else 1
-        assert np.array_equal(g_thr[t][:nch], o_thr[:nch]), (t, g_thr[t], o_thr)
-        assert np.array_equal(g_out[t], o_out), (t, int(np.abs(g_out[t].astype(int) - o_out.astype(int)).max()))
+        assert np.array_equal(g_thr[t][:1], o_thr[:1]), t
+        assert np.array_equal(g_out[t], o_out), (t, max_diff(g_out[t], o_out))
 
 
 @pytest.mark.parametrize("w,h,levels,color,batch", [(642, 481, 4, True, 3), (1000, 701, 5, False, 2),
@@ -465,3 +523,36 @@ def test_medium_shapes_batched_vs_oracle(w, h, levels, color, batch):
         o_out, o_thr = orc.frame(frames[t])
         assert np.array_equal(g_thr[t][:nch], o_thr[:nch]), (t, g_thr[t], o_thr)
         assert np.array_equal(g_out[t], o_out), (t, int(np.abs(g_out[t].astype(int) - o_out.astype(int)).max()))
+
+
+def test_covariance_division_equals_ieee_on_the_gpu():
+    """The guided filter divides the covariance by var + eps (in [2^-7, 2)) with div_mid, the
+    IEEE sequence without its range scaling, started from the hardware reciprocal estimate.
+    tests/div_mid_check.c shows on the CPU that this equals IEEE division from the correctly
+    rounded 1/b everywhere, and from its neighbours except at the 8 values b = 2^k (2 - 2^-23).
+    Here, on the GPU, over every b in [2^-7, 2): the quotient equals numpy's IEEE quotient for
+    a = 1 (the tie case at those b) and for random a of both signs over 2^-66..2^64, and the
+    estimate at those 8 values of b is the correctly rounded one."""
+    rng = np.random.default_rng(11)
+    m = np.arange(1 << 23, dtype=np.uint32)
+    st = wema_amd._stream_handle()
+    for e in range(127 - 7, 127 + 1):
+        b = ((np.uint32(e) << np.uint32(23)) | m).view(np.float32)
+        ex = rng.integers(127 - 66, 127 + 64, size=m.size, dtype=np.uint32)
+        ra = ((ex << np.uint32(23)) | rng.integers(0, 1 << 23, size=m.size, dtype=np.uint32)).view(np.float32)
+        ra = np.where(rng.random(m.size) < 0.5, -ra, ra).astype(np.float32)
+        d_b = torch.from_numpy(b).cuda()
+        for a in (np.ones_like(b), ra):
+            d_a = torch.from_numpy(a).cuda()
+            d_q = torch.empty_like(d_a)
+            d_r = torch.empty_like(d_a)
+            assert wema_amd.lib.wema_dev_debug_div_mid(d_a.data_ptr(), d_b.data_ptr(), d_q.data_ptr(),
+                                                       d_r.data_ptr(), m.size, st) == 0
+            torch.cuda.synchronize()
+            q = d_q.cpu().numpy()
+            want = a / b
+            bad = np.flatnonzero(q.view(np.uint32) != want.view(np.uint32))
+            assert bad.size == 0, (e, bad[:4], a[bad[:4]], b[bad[:4]], q[bad[:4]], want[bad[:4]])
+        r = d_r.cpu().numpy()
+        top = b[-1]
+        assert r[-1] == np.float32(1.0) / top, (top, r[-1])

@@ -2,8 +2,13 @@
 
 The bar: every stream's output in a mixed tick (and replayed from a captured hipGraph) is
 byte-identical to the same stream processed alone (no computation crosses streams, and
-the threshold sums are order-independent integers); admission never exceeds the budget.
+the threshold sums are order-independent integers), and equal to the oracle's (within
+1 LSB with bilateral weights); admission never exceeds the budget.
 """
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -14,6 +19,9 @@ if not torch.cuda.is_available():
     pytest.skip("no HIP device", allow_module_level=True)
 
 import wema_amd  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+DIGESTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_digests.json")
 
 STREAMS = [
     dict(w=64, h=48, fps=30.0, kw=dict(amp=120.0, f_low=0.5, f_high=3.0, levels=3), preset=4),
@@ -68,7 +76,59 @@ def test_mixed_ticks_equal_alone(graphs):
     pool.sync()
     for i in range(len(STREAMS)):
         assert np.array_equal(outs[i].cpu().numpy(), ref[i]), STREAMS[i]
+        s = STREAMS[i]
+        orc = O.OraclePipe(s["w"], s["h"], fps=s["fps"], **s["kw"])
+        fr = frames[i].cpu().numpy()
+        want = np.stack([orc.frame(fr[t])[0] for t in range(n)])
+        tol = 1 if s["kw"].get("bilateral") else 0
+        assert int(np.abs(want.astype(int) - ref[i].astype(int)).max()) <= tol, s
     pool.close()
+
+
+def test_c4_mix_full_size_vs_alone_and_oracle():
+    """The 24-stream mix at its full sizes (8 x 640x480@30, 8 x 1280x720@30, 6 x 1920x1080@60,
+    2 x 3840x2160@30; per-stream band, amplification and window; automatic levels), 20 ticks
+    of all 24 streams, once with direct launches and once with hipGraph replay (captured at
+    the second tick): every stream's bytes equal the stream run alone, and the 640x480 and
+    1280x720 streams' bytes and thresholds equal the oracle's for every frame (digests
+    written by tests/golden/make_digests.py; frames 15-19 are past the fade-in)."""
+    with open(DIGESTS) as f:
+        doc = json.load(f)
+    streams, cases = doc["c4_streams"], doc["cases"]
+    n = 20
+    inputs = [wema_amd.Synth(s["preset"], s["w"], s["h"], s["fps"], s["seed_offset"]).frames_device(0, n)
+              for s in streams]
+    alone = []
+    for s, x in zip(streams, inputs):
+        sess = wema_amd.Session(s["w"], s["h"], s["fps"], **s["kw"])
+        out = torch.empty_like(x)
+        for t in range(n):
+            sess.process_device(x[t:t + 1], d_out_u8=out[t:t + 1], n=1)
+        sess.sync()
+        sess.close()
+        alone.append(out)
+    for graphs in (False, True):
+        print("c4 mix, graphs=%s" % graphs, flush=True)
+        pool = wema_amd.Pool(40e9, graphs=graphs)
+        ids = [pool.add(s["w"], s["h"], s["fps"], **s["kw"]) for s in streams]
+        outs = [torch.empty_like(x) for x in inputs]
+        thr = {i: [] for i in range(len(streams))}
+        for t in range(n):
+            pool.tick(ids, [x[t] for x in inputs], [o[t] for o in outs])
+            for i in range(len(streams)):
+                if "c4_%02d" % i in cases:
+                    thr[i].append(pool.thresholds(ids[i]))
+        pool.sync()
+        for i, s in enumerate(streams):
+            assert torch.equal(outs[i], alone[i]), (graphs, i, s)
+            name = "c4_%02d" % i
+            if name not in cases:
+                continue
+            got = outs[i].cpu().numpy()
+            for t in range(n):
+                assert list(thr[i][t][:1].view(np.uint32)) == cases[name]["thr_bits"][t][:1], (graphs, name, t)
+                assert hashlib.sha256(got[t].tobytes()).hexdigest() == cases[name]["out_sha256"][t], (graphs, name, t)
+        pool.close()
 
 
 def test_admission_respects_budget():

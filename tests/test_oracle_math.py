@@ -243,3 +243,19 @@ def test_box_and_guided_vs_brute_force(prefix):
     b = mp - a * mI
     q = box_brute(a, w, h) * I64 + box_brute(b, w, h)
     np.testing.assert_allclose(q32, q, atol=2e-6)
+
+
+def test_fp32_entry_points_flush_denormals():
+    """The reference runs with FTZ/DAZ (its -Ofast build), the kernels flush denormals: the
+    oracle's FP32 entry points do too, whatever the calling thread's mode. A denormal input
+    to the IIR step is read as zero, so the output and state stay exactly zero."""
+    c, _ = np.zeros(10, np.float32), None
+    O.orc().orc_iir_design(C.c_float(0.5), C.c_float(3.0), C.c_float(30.0), O.ptr(c), C.byref(C.c_int()))
+    x = np.full(64, 1e-39, np.float32)
+    assert x[0] != 0.0 and abs(float(x[0])) < np.finfo(np.float32).tiny   # denormal
+    st = np.zeros(4 * 64, np.float32)
+    y = np.ones(64, np.float32)
+    O.orc().orc_iir_step_f32(O.ptr(c), O.ptr(x), C.c_size_t(64), O.ptr(st), O.ptr(y))
+    assert not y.any() and not st.any()
+    # and the caller's mode is restored: numpy still produces denormals afterwards
+    assert np.float32(1e-38) * np.float32(0.01) != 0.0

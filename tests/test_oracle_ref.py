@@ -9,11 +9,14 @@ slot 0 (a vectorised sum). The threshold sum is compared separately: the referen
 |c| in FP32 in a compiler-vectorised order, the oracle (like the GPU) exactly.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
 
 from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.ref
 
@@ -213,3 +216,29 @@ def test_threshold_sum_relative_error():
     exact = float(np.abs(f6.astype(np.float64)).sum()) / f6.size * 0.2
     assert abs(t_ref - exact) / exact < 1e-4
     assert s > 0
+
+
+def test_oracle_independent_of_loading_the_reference():
+    """The compiled reference's start-up code sets FTZ/DAZ in whatever process loads it. The
+    oracle sets that mode itself per call: in a fresh process, one oracle case gives the same
+    bytes and thresholds before and after the reference library is loaded."""
+    import subprocess
+    import sys
+    code = r'''
+import sys, numpy as np
+sys.path.insert(0, %r)
+from oracle import oracle as O
+rng = np.random.default_rng(5)
+frames = rng.integers(0, 256, size=(6, 29, 37, 3), dtype=np.uint8)
+def run():
+    o = O.OraclePipe(37, 29, fps=30.0, amp=120.0, f_low=0.5, f_high=3.0, levels=3)
+    outs = [o.frame(f) for f in frames]
+    return np.stack([a for a, _ in outs]), np.stack([t for _, t in outs])
+a0, t0 = run()
+O.ref()   # loads libwema_ref.so (FTZ/DAZ set by its start-up code)
+a1, t1 = run()
+assert np.array_equal(a0, a1) and np.array_equal(t0, t1, equal_nan=True)
+print("same")
+''' % ROOT
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and "same" in out.stdout, out.stderr

@@ -124,6 +124,7 @@ _SIGS = {
     "wema_dev_debug_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
     "wema_dev_debug_set_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
     "wema_dev_debug_read": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_size_t]),
+    "wema_dev_debug_div_mid": (C.c_int, [_vp, _vp, _vp, _vp, C.c_long, _vp]),
     "wema_pool_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_size_t]),
     "wema_pool_destroy": (None, [_vp]),
     "wema_pool_add": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(C.c_int)]),

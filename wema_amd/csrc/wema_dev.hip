@@ -29,6 +29,7 @@ hipError_t wema_launch_egress(int, int, const EgressJob *, int, int, hipStream_t
 hipError_t wema_launch_synth(const SynthFrame *, const unsigned short *, unsigned char *, int,
                              int, int, hipStream_t);
 hipError_t wema_launch_zero_u64(unsigned long long *, long long, hipStream_t);
+hipError_t wema_launch_debug_div(const float *, const float *, float *, float *, long long, hipStream_t);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1188,6 +1189,16 @@ extern "C" int wema_dev_debug_set_thresholds(wema_dev_t *s, const float *thr, in
     HIP_OK(hipStreamSynchronize(s->last_stream));
     HIP_OK(hipMemcpy(s->thr_ovr_d, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
     s->thr_ovr_active = 1;
+    return 0;
+}
+
+extern "C" int wema_dev_debug_div_mid(const float *d_a, const float *d_b, float *d_q, float *d_rcp, long n,
+                                      void *hip_stream) {
+    if (!d_a || !d_b || !d_q || !d_rcp || n < 0 || (long long)n > (1ll << 31)) {
+        wema_set_error("bad argument");
+        return -1;
+    }
+    HIP_OK(wema_launch_debug_div(d_a, d_b, d_q, d_rcp, (long long)n, (hipStream_t)hip_stream));
     return 0;
 }
 

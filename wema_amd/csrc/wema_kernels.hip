@@ -1210,11 +1210,14 @@ __device__ __forceinline__ float div_count(float a, float c, float k) {
     return fmaf(fmaf(-q, c, a), k, q);
 }
 
-// a / b for b in [2^-7, 2) and 2^-66 <= |a| < 2^64 (or a = 0): the steps of the IEEE
+// a / b for b in [2^-7, 2) and 2^-66 <= |a| < 2^64 (or a = +0): the steps of the IEEE
 // division sequence (reciprocal estimate, one refinement, two corrections with exact
 // residuals) without its range scaling and special-case fix-up, which that range never
-// needs; bit-identical to a / b there (and within an ulp of it for smaller nonzero |a|,
-// whose residuals flush to zero).
+// needs; bit-identical to a / b there when the estimate is the correctly rounded 1/b, and
+// from the neighbouring estimates too except at b = 2^k (2 - 2^-23), where 1/b lies just
+// above a rounding tie (tests/div_mid_check.c on the CPU; the hardware estimate at those b
+// is checked on the GPU); within an ulp of a / b for smaller nonzero |a|, whose residuals
+// flush to zero, and +0 for a = -0 (a zero that only ever meets other terms).
 __device__ __forceinline__ float div_mid(float a, float b) {
     float y = __builtin_amdgcn_rcpf(b);
     y = fmaf(fmaf(-b, y, 1.0f), y, y);
@@ -1847,6 +1850,16 @@ __global__ void __launch_bounds__(256) k_passthru(const EgressJob *__restrict__ 
     if (J.out_f32) J.out_f32[i] = v;
 }
 
+// Test hook: the guided filter's covariance division and the hardware reciprocal estimate
+// it starts from, element by element.
+__global__ void __launch_bounds__(256) k_debug_div(const float *__restrict__ a, const float *__restrict__ b,
+                                                   float *__restrict__ q, float *__restrict__ rcp, long long n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    q[i] = div_mid(a[i], b[i]);
+    rcp[i] = __builtin_amdgcn_rcpf(b[i]);
+}
+
 // Clears n 64-bit words (threshold accumulators) with vector stores: a kernel node, which
 // graph capture records as it records the pipeline's own launches.
 __global__ void __launch_bounds__(256) k_zero_u64(unsigned long long *__restrict__ p, long long n) {
@@ -1925,6 +1938,12 @@ hipError_t wema_launch_egress(int kind, int rows, const EgressJob *jobs, int njo
     case EG_GAUSS: hipLaunchKernelGGL(k_gauss, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     default: hipLaunchKernelGGL(k_passthru, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     }
+    return hipGetLastError();
+}
+hipError_t wema_launch_debug_div(const float *a, const float *b, float *q, float *rcp, long long n,
+                                 hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_debug_div, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, q, rcp, n);
     return hipGetLastError();
 }
 hipError_t wema_launch_zero_u64(unsigned long long *p, long long n, hipStream_t s) {

@@ -7,8 +7,12 @@
 #include "wema.h"
 
 #include <stdarg.h>
+#include <stdatomic.h>
+#include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "wema_dev.h"
 
@@ -23,6 +27,37 @@ static void set_err(const char *fmt, ...) {
 
 const char *wema_last_error(void) { return g_err; }
 
+/* Frames are pinned (wema_dev_host_alloc) while the process's pinned frame bytes stay
+ * within a cap, so they copy to and from the GPU at full link speed; past the cap, or where
+ * pinning fails (no GPU, a locked-memory limit), they are 64-byte-aligned pageable memory,
+ * as the reference's frames are (mem_alloc, src/alloc.c:9-19). A 64-byte header before the
+ * pixels records which, for frame_free. The cap is WEMA_PINNED_MAX_BYTES if set, else a
+ * quarter of physical memory up to 16 GiB (the reference's CLI allocates 2 x batch x 2
+ * frames, src/async_io.c:215-233: 25 GB at 4K with its default batch of 64). */
+#define FRAME_HDR 64
+#define FRAME_MAGIC_PINNED 0x57454D4150494E4EULL   /* "WEMAPINN" */
+#define FRAME_MAGIC_PAGED 0x57454D4150414745ULL    /* "WEMAPAGE" */
+
+typedef struct {
+    uint64_t magic;
+    uint64_t bytes;     /* whole allocation, header included */
+} FrameHeader;
+
+static _Atomic size_t g_pinned_bytes;
+
+static size_t pinned_cap(void) {
+    const char *env = getenv("WEMA_PINNED_MAX_BYTES");
+    if (env && *env) return (size_t)strtoull(env, NULL, 10);
+    const long pages = sysconf(_SC_PHYS_PAGES), psz = sysconf(_SC_PAGESIZE);
+    size_t cap = pages > 0 && psz > 0 ? (size_t)pages * (size_t)psz / 4 : ((size_t)1 << 30);
+    if (cap > ((size_t)16 << 30)) cap = (size_t)16 << 30;
+    return cap;
+}
+
+static FrameHeader *frame_header(const Frame *frame) {
+    return (FrameHeader *)((char *)frame->data - FRAME_HDR);
+}
+
 int frame_alloc(Frame *frame, int width, int height, int channels) {
     frame->width = width;
     frame->height = height;
@@ -32,19 +67,47 @@ int frame_alloc(Frame *frame, int width, int height, int channels) {
         set_err("invalid frame size %dx%dx%d", width, height, channels);
         return -1;
     }
-    const size_t bytes = (size_t)width * height * channels * sizeof(float);
-    frame->data = (float *)wema_dev_host_alloc(bytes);
-    if (!frame->data) {
-        set_err("%s", wema_dev_last_error());
-        return -1;
+    const size_t bytes = (size_t)width * height * channels * sizeof(float) + FRAME_HDR;
+    char *base = NULL;
+    uint64_t magic = FRAME_MAGIC_PINNED;
+    const size_t before = atomic_fetch_add(&g_pinned_bytes, bytes);
+    if (before + bytes <= pinned_cap()) base = (char *)wema_dev_host_alloc(bytes);
+    if (!base) {
+        atomic_fetch_sub(&g_pinned_bytes, bytes);
+        magic = FRAME_MAGIC_PAGED;
+        void *p = NULL;
+        if (posix_memalign(&p, FRAME_HDR, bytes) != 0) {
+            set_err("out of host memory for a %dx%dx%d frame", width, height, channels);
+            return -1;
+        }
+        base = (char *)p;
     }
-    memset(frame->data, 0, bytes);
+    memset(base, 0, bytes);
+    FrameHeader *h = (FrameHeader *)base;
+    h->magic = magic;
+    h->bytes = bytes;
+    frame->data = (float *)(base + FRAME_HDR);
     return 0;
 }
 
 void frame_free(Frame *frame) {
-    if (frame->data) wema_dev_host_free(frame->data);
+    if (!frame->data) return;
+    FrameHeader *h = frame_header(frame);
+    if (h->magic == FRAME_MAGIC_PINNED) {
+        atomic_fetch_sub(&g_pinned_bytes, (size_t)h->bytes);
+        h->magic = 0;
+        wema_dev_host_free(h);
+    } else if (h->magic == FRAME_MAGIC_PAGED) {
+        h->magic = 0;
+        free(h);
+    }
     frame->data = NULL;
+}
+
+int wema_frame_pinned(const Frame *frame) {
+    if (!frame || !frame->data) return -1;
+    const uint64_t m = frame_header(frame)->magic;
+    return m == FRAME_MAGIC_PINNED ? 1 : (m == FRAME_MAGIC_PAGED ? 0 : -1);
 }
 
 int wema_init(WemaContext *ctx, int width, int height, float fps, float amp_factor, float f_low,
@@ -215,22 +278,35 @@ int wema_batch_init(WemaBatchContext *batch, const WemaContext *wema, int batch_
 
 void wema_batch_free(WemaBatchContext *batch) { memset(batch, 0, sizeof(*batch)); }
 
+/* Frames of a batch call go through the device in sub-batches of this many: each is
+ * uploaded, processed and downloaded on its own streams (wema_dev_submit_f32), two on the
+ * device at a time, so the copies of one overlap the processing of the next. */
+#ifndef WEMA_HOST_SUB_BATCH
+#define WEMA_HOST_SUB_BATCH 8
+#endif
+
 int wema_batch_process(WemaContext *wema, WemaBatchContext *batch, const Frame *frames_in,
                        Frame *frames_out, int frame_count) {
     if (frame_count < 1) return 0;
-    if (frame_count > batch->batch_size) {
-        set_err("%d frames exceed the batch size %d", frame_count, batch->batch_size);
-        return -1;
-    }
-    if (open_session(wema, batch->batch_size > 1 ? batch->batch_size : 2) < 0) return -1;
-    const float *src[WEMA_BATCH_SIZE_MAX];
-    float *dst[WEMA_BATCH_SIZE_MAX];
-    for (int i = 0; i < frame_count; i++) {
+    /* src/wema_batch.c:355-357: at most batch_size frames, and that count is returned */
+    if (frame_count > batch->batch_size) frame_count = batch->batch_size;
+    const int sub = batch->batch_size < WEMA_HOST_SUB_BATCH ? (batch->batch_size > 1 ? batch->batch_size : 2)
+                                                            : WEMA_HOST_SUB_BATCH;
+    if (open_session(wema, sub) < 0) return -1;
+    for (int i = 0; i < frame_count; i++)
         if (check_frame(wema, &frames_in[i]) < 0 || check_frame(wema, &frames_out[i]) < 0) return -1;
-        src[i] = frames_in[i].data;
-        dst[i] = frames_out[i].data;
+    int queued = 0, rc = 0;
+    for (int i = 0; i < frame_count && rc == 0; i++) {
+        if (wema_dev_submit_f32(wema->dev, frames_in[i].data, frames_out[i].data, (int64_t)i) < 0) rc = -1;
+        else queued++;
     }
-    if (wema_dev_process_host_f32(wema->dev, src, dst, frame_count) < 0) {
+    /* every queued frame is collected (in order), also after an error */
+    for (int i = 0; i < queued; i++) {
+        int64_t pts = -1;
+        float *o = NULL;
+        if (wema_dev_collect_f32(wema->dev, &pts, &o) < 0 || pts != i || o != frames_out[i].data) rc = -1;
+    }
+    if (rc < 0) {
         set_err("%s", wema_dev_last_error());
         return -1;
     }
