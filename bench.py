@@ -28,6 +28,7 @@ host's cores, over a bounded sample of the same frames.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -53,6 +54,17 @@ CONFIGS = {
                window=64, color=False, use_iir=True, bilateral=False, batch=48,
                desc="3840x2160 RGB24@60 checkerboard shear, L6, IIR 0.8-2 Hz, amp 50, guided filter"),
 }
+# The same workloads one frame per call (the path of wema_process_frame, src/wema.c:746-952,
+# and of the serving loop): the IIR state is read and written for every frame.
+CONFIGS["c2_n1"] = dict(CONFIGS["c2"], batch=1, desc=CONFIGS["c2"]["desc"] + ", one frame per call")
+CONFIGS["c3_n1"] = dict(CONFIGS["c3"], batch=1, desc=CONFIGS["c3"]["desc"] + ", one frame per call")
+# steps x frames of the per-frame legs (the c3 clip: 240 frames; c2: 300)
+PER_FRAME_STEPS = {"c2_n1": 300, "c3_n1": 240}
+
+
+def log(msg):
+    """A progress line on stderr as each measurement ends (stdout carries only the result)."""
+    print("bench: %s" % msg, file=sys.stderr, flush=True)
 
 
 def read_traffic(kernel, workload):
@@ -162,37 +174,71 @@ def run_host_pipeline(cfg, steps, batch):
             "link_GBs": round(2 * frames * w * h * 3 / dt / 1e9, 1)}
 
 
-def cpu_baseline(cfg, budget_s=20.0, first_frame=0, gpu_sample=None):
+def cpu_info():
+    """lscpu facts of this host, and the cores this process may use."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    def num(k):
+        try:
+            return int(info.get(k, ""))
+        except ValueError:
+            return None
+    sockets, cps, tpc = num("Socket(s)"), num("Core(s) per socket"), num("Thread(s) per core")
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    physical = sockets * cps if sockets and cps else None
+    return {"model": info.get("Model name"), "sockets": sockets, "physical_cores": physical,
+            "threads_per_core": tpc, "smt": (tpc or 1) > 1, "cpus_available": avail}
+
+
+def cpu_threads(ci):
+    """All physical cores this process may use (the box gives one GPU's process 16 CPUs)."""
+    n = min(x for x in (ci["physical_cores"], ci["cpus_available"], 16) if x)
+    if ci["threads_per_core"] and ci["threads_per_core"] > 1:
+        n = min(n, max(1, ci["cpus_available"] // ci["threads_per_core"]))
+    return n
+
+
+def cpu_baseline(cfg, budget_s=20.0, batch_size=64):
     """The compiled reference on this host: colour and window modes run the per-frame path
-    (src/main.c:355: colour never batches; single-threaded), luma IIR the OpenMP batch path."""
+    (src/main.c:355: colour never batches; single-threaded), luma IIR the OpenMP batch path
+    (src/wema_batch.c:352-457) with its default batch of 64 frames (src/wema.h:218) on all
+    physical cores available, OMP_PROC_BIND=close, OMP_PLACES=cores. The first batch, which
+    runs the serial preseed (src/iir_filter.c:376-471), is reported apart from the steady
+    state."""
     from oracle import oracle as O
     import wema_amd
 
     if not O.ref_available():
         return None
+    ci = cpu_info()
     w, h = cfg["w"], cfg["h"]
     synth = wema_amd.Synth(cfg["preset"], w, h, cfg["fps"])
     ref = O.RefPipe(w, h, cfg["fps"], cfg["amp"], cfg["f_low"], cfg["f_high"], cfg["window"],
                     cfg["levels"], cfg["use_iir"], cfg["color"], True, cfg["bilateral"])
     batch_path = cfg["use_iir"] and not cfg["color"]
-    threads = os.cpu_count() if batch_path else 1
+    threads = cpu_threads(ci) if batch_path else 1
     if batch_path:
-        threads = min(threads, 16)
-        bsz = 8
+        bsz = batch_size
         first = np.stack([synth.frame(t) for t in range(bsz)])
         t0 = time.perf_counter()
         ref.batch(first, bsz, threads)                      # includes the serial preseed
         t_first = time.perf_counter() - t0
         done, t_sum, t = 0, 0.0, bsz
-        while t_sum < budget_s and done < 64:
+        while (t_sum < budget_s and done < 256) or done == 0:
             fr = np.stack([synth.frame(t + i) for i in range(bsz)])
             t0 = time.perf_counter()
             ref.batch(fr, bsz, threads)
             t_sum += time.perf_counter() - t0
             done += bsz
             t += bsz
-        sample = "%d frames (after a first batch of %d incl. preseed, %.1f s), batch path, " \
-                 "%d OpenMP threads" % (done, bsz, t_first, threads)
+        sample = "%d frames in batches of %d (after a first batch incl. the preseed, %.1f s), batch " \
+                 "path, %d OpenMP threads" % (done, bsz, t_first, threads)
     else:
         f0 = synth.frame(0)
         t0 = time.perf_counter()
@@ -211,7 +257,32 @@ def cpu_baseline(cfg, budget_s=20.0, first_frame=0, gpu_sample=None):
     ref.close()
     return dict(value=round(mpix / t_sum, 3), unit="Mpixel/s", cores=threads, kind="reference",
                 sample=sample + "; u8<->float conversion included",
-                fps=round(done / t_sum, 3))
+                fps=round(done / t_sum, 3), first_batch_s=round(t_first, 2), host=ci)
+
+
+def run_dropin():
+    """The drop-in wema.h path (wema_amd/libwema.so) driven from C as the reference's CLI
+    drives it (src/main.c:343-470), with frames from frame_alloc allocated as async_io does
+    (2 slots x batch x in/out): c3 through wema_batch_process with the reference's default
+    batch of 64 (luma IIR batches, src/main.c:355), c2 through wema_process_frame (colour
+    never batches). Host float frames cross PCIe both ways (24 bytes per pixel)."""
+    exe = os.path.join(ROOT, "wema_amd", "wema_h_bench")
+    if not os.path.exists(exe):
+        return None
+    res = {}
+    for name, batch, batches in (("c3", 64, 3), ("c2", 30, 4)):
+        c = CONFIGS[name]
+        args = [exe, str(c["w"]), str(c["h"]), str(c["fps"]), str(c["levels"]), str(c["amp"]),
+                str(c["f_low"]), str(c["f_high"]), str(int(c["color"])), str(c["preset"]), str(batch),
+                str(batches)]
+        out = subprocess.run(args, capture_output=True, text=True, timeout=600)
+        if out.returncode != 0:
+            res[name] = {"error": (out.stderr or out.stdout)[-300:]}
+            continue
+        res[name] = json.loads(out.stdout.strip().splitlines()[-1])
+        log("drop-in %s: %.1f Mpixel/s" % (name, res[name]["mpix_per_s"]))
+    return {"desc": "include/wema.h through libwema.so from C: host float Frames (frame_alloc, as "
+                    "async_io allocates them), the reference CLI's call pattern", "runs": res}
 
 
 REALTIME_MIX_MPXS = (8 * 640 * 480 * 30 + 8 * 1280 * 720 * 30 + 6 * 1920 * 1080 * 60
@@ -221,7 +292,8 @@ REALTIME_MIX_MPXS = (8 * 640 * 480 * 30 + 8 * 1280 * 720 * 30 + 6 * 1920 * 1080 
 def _serve_summary(st):
     keys = ("frames", "mpix_per_s", "wall_s", "lat_p50_ms", "lat_p99_ms", "lat_max_ms",
             "sessions_arrived", "sessions_admitted", "admission_waits", "peak_sessions",
-            "peak_bytes", "ticks", "mean_frames_per_tick")
+            "peak_bytes", "ticks", "mean_frames_per_tick", "host_ms_per_tick", "graph_tick_frac",
+            "gpu_busy_frac", "link_gbs")
     return {k: (round(st[k], 3) if isinstance(st[k], float) else st[k]) for k in keys}
 
 
@@ -238,6 +310,7 @@ def run_multistream(frames_per_session=180):
                      ("batched", dict(batched=True, graphs=False)),
                      ("plain", dict(batched=False, graphs=False))):
         res[name] = _serve_summary(wema_amd.serve(**kw, **common))
+        log("multistream %s: %.1f Mpixel/s" % (name, res[name]["mpix_per_s"]))
     b, p = res["batched_graphs"]["mpix_per_s"], res["plain"]["mpix_per_s"]
     return {"desc": "24 streams x %d frames, mix 8x640x480@30 8x1280x720@30 6x1920x1080@60 "
                     "2x3840x2160@30, per-stream band/amp/window" % frames_per_session,
@@ -250,24 +323,36 @@ def run_multistream(frames_per_session=180):
 def run_serving(duration_s=4.0, max_mult=64.0):
     """Poisson sessions at 4/s (life 2-20 s, the 8:8:6:2 resolution mix, 30 or 60 fps),
     frames delivered at real-time pace, admission under a 200 GB HBM budget, started in
-    steady state. paced: that load, batched (continuous batching) and plain (one session at
+    steady state; frames resident in HBM ("device"), or uploaded from and downloaded to
+    pinned host memory on copy streams overlapping the ticks ("host_io", the path a real
+    ingest takes). paced: that load, batched (continuous batching) and plain (one session at
     a time). capacity: the arrival rate is raised in steps of sqrt(2) until p99 latency
     exceeds 33 ms, then bisected twice; the aggregate Mpixel/s of the highest load that met
-    the bound, for both modes."""
+    the bound, for each mode."""
     import wema_amd
     base = dict(arrival_rate=4.0, life=(2.0, 20.0), hbm_budget=200e9, warm_start=True,
                 duration_s=duration_s, fps_from_mix=False, seed=5)
     out = {"desc": "Poisson 4/s, life 2-20 s, mix 8:8:6:2 of 640x480..3840x2160 at 30/60 fps, "
-                   "real-time pace, 200 GB HBM budget, %.0f s runs from steady state" % duration_s}
+                   "real-time pace, 200 GB HBM budget, %.0f s runs from steady state; frames in "
+                   "HBM (device) or through pinned host memory (host_io)" % duration_s}
     out["paced"] = {"batched": _serve_summary(wema_amd.serve(batched=True, graphs=True, **base)),
-                    "plain": _serve_summary(wema_amd.serve(batched=False, graphs=False, **base))}
+                    "plain": _serve_summary(wema_amd.serve(batched=False, graphs=False, **base)),
+                    "batched_host_io": _serve_summary(wema_amd.serve(batched=True, graphs=True,
+                                                                     host_io=True, **base)),
+                    "plain_host_io": _serve_summary(wema_amd.serve(batched=False, graphs=False,
+                                                                   host_io=True, **base))}
+    log("serving at the paced load done")
     cap = {}
     for mode, kw in (("batched", dict(batched=True, graphs=True)),
-                     ("plain", dict(batched=False, graphs=False))):
+                     ("plain", dict(batched=False, graphs=False)),
+                     ("batched_host_io", dict(batched=True, graphs=True, host_io=True)),
+                     ("plain_host_io", dict(batched=False, graphs=False, host_io=True))):
         best, tried = None, []
 
         def run(m):
             st = wema_amd.serve(**kw, **dict(base, arrival_rate=4.0 * m))
+            log("serving %s at %.1f/s: %.1f Mpixel/s, p99 %.2f ms" % (mode, 4.0 * m, st["mpix_per_s"],
+                                                                      st["lat_p99_ms"]))
             tried.append({"arrival_rate": round(4.0 * m, 2), "mpix_per_s": round(st["mpix_per_s"], 1),
                           "lat_p99_ms": round(st["lat_p99_ms"], 2), "peak_sessions": st["peak_sessions"]})
             return st, st["lat_p99_ms"] <= 33.0
@@ -291,11 +376,24 @@ def run_serving(duration_s=4.0, max_mult=64.0):
                 m_bad = mid
         cap[mode] = {"best": best, "sweep": tried}
     out["capacity"] = cap
-    b = (cap["batched"]["best"] or {}).get("mpix_per_s")
-    p = (cap["plain"]["best"] or {}).get("mpix_per_s")
-    out["capacity_batched_over_plain"] = round(b / p, 2) if b and p else None
+    for suffix in ("", "_host_io"):
+        b = (cap["batched" + suffix]["best"] or {}).get("mpix_per_s")
+        p = (cap["plain" + suffix]["best"] or {}).get("mpix_per_s")
+        out["capacity_batched_over_plain" + suffix] = round(b / p, 2) if b and p else None
     out["p99_target_ms"] = 33.0
     return out
+
+
+def state_model(cfg, batch):
+    """How the temporal state's traffic enters the algorithmic bytes of a launch."""
+    if not cfg["use_iir"]:
+        return ("window: the ring samples a launch's windows cover (window - 1 + frames) read once "
+                "per launch, one amplified coefficient written per frame")
+    if batch == 1:
+        return "per frame: IIR state read and written for every frame (40 B per unique position per frame)"
+    return ("per launch: the %d frames of a launch run with the IIR state in registers, so the state "
+            "is read and written once per launch (8 B per position per frame + 32 B per position per "
+            "launch); the one-frame-per-call path is reported separately" % batch)
 
 
 def main():
@@ -305,7 +403,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--batch", type=int, default=0, help="frames per step (default per config)")
-    ap.add_argument("--also", default="c1,c3,c2w", help="comma list of extra workloads ('' = none)")
+    ap.add_argument("--also", default="c1,c3,c2w,c2_n1,c3_n1",
+                    help="comma list of extra workloads ('' = none)")
+    ap.add_argument("--cpu-also", default="c1,c3",
+                    help="workloads whose reference CPU baseline (OpenMP batch path) is also timed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true",
@@ -316,7 +417,8 @@ def main():
                          "32 (0: the autotuner's pick; fixed heights skip the tuner's timing "
                          "launches, e.g. under a profiler)")
     ap.add_argument("--no-host", action="store_true",
-                    help="skip the pinned-host pipeline measurement (e.g. under a profiler)")
+                    help="skip the pinned-host pipeline and drop-in wema.h measurements "
+                         "(e.g. under a profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (for rocprofv3 counter scaling)")
     args = ap.parse_args()
@@ -349,15 +451,18 @@ def main():
         wema_amd.autotune(False)
     cfg = CONFIGS[args.config]
     batch = args.batch or cfg["batch"]
-    r = run_gpu(cfg, args.config, args.steps, args.warmup, batch, args.tile_rows)
+    steps = PER_FRAME_STEPS.get(args.config, args.steps) if batch == 1 and not args.batch else args.steps
+    r = run_gpu(cfg, args.config, steps, args.warmup, batch, args.tile_rows)
+    r["roofline"]["state_model"] = state_model(cfg, batch)
+    log("%s: %.1f Mpixel/s" % (args.config, r["mpix_s"]))
     line = {
         "metric": METRIC,
         "value": round(r["mpix_s"], 1),
         "unit": "Mpixel/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r["ms"] / args.steps, 3),
+        "ms_per_step": round(r["ms"] / steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -374,20 +479,32 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_host:
-        line["host_pipeline"] = run_host_pipeline(cfg, max(args.steps, 16), batch)
+        line["host_pipeline"] = run_host_pipeline(cfg, max(args.steps, 16), max(batch, 8))
+        log("host pipeline done")
+        line["dropin_wema_h"] = run_dropin()
+        log("drop-in wema.h done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
+        log("cpu baseline done")
     also = {}
     for name in [a for a in args.also.split(",") if a and a != args.config]:
         c = CONFIGS[name]
-        rr = run_gpu(c, name, args.steps, args.warmup, c["batch"], args.tile_rows)
+        st_n = PER_FRAME_STEPS.get(name, args.steps)
+        rr = run_gpu(c, name, st_n, args.warmup * (8 if c["batch"] == 1 else 1), c["batch"], args.tile_rows)
+        rr["roofline"]["state_model"] = state_model(c, c["batch"])
+        log("%s: %.1f Mpixel/s" % (name, rr["mpix_s"]))
         also[name] = {"desc": c["desc"], "value": round(rr["mpix_s"], 1), "unit": "Mpixel/s",
-                      "fps": round(rr["fps"], 1), "ms_per_step": round(rr["ms"] / args.steps, 3),
+                      "fps": round(rr["fps"], 1), "steps": st_n, "ms_per_step": round(rr["ms"] / st_n, 3),
                       "frames_per_step": c["batch"], "pipeline_algorithmic_GBs": rr["pipeline_gbs"],
                       "gf_tile_rows": rr["tile_rows"] if not c["color"] else None,
                       "roofline": rr["roofline"],
                       "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
                                   for d in rr["kernels"]]}
+    if not args.no_cpu:
+        for name in [a for a in args.cpu_also.split(",") if a]:
+            if name in also and name != args.config:
+                also[name]["cpu_baseline"] = cpu_baseline(CONFIGS[name], args.cpu_budget)
+                log("%s cpu baseline done" % name)
     if also:
         line["also"] = also
     if not args.no_serve:

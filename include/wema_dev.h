@@ -198,6 +198,8 @@ int    wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8_t *cons
                       uint8_t *const *d_out, void *hip_stream);
 int    wema_pool_sync(wema_pool_t *p);
 void  *wema_pool_stream(wema_pool_t *p);
+/* Ticks run so far, and how many of them were replayed from a captured hipGraph. */
+int    wema_pool_stats(const wema_pool_t *p, long *ticks, long *graph_ticks);
 int    wema_pool_timing(wema_pool_t *p, int enable);
 int    wema_pool_timing_report(wema_pool_t *p, int kind, char *name, int name_len, long *launches,
                                double *total_ms, double *bytes);
@@ -216,7 +218,20 @@ int    wema_pool_timing_report(wema_pool_t *p, int kind, char *name, int name_le
  * the clock starts, each at a random frame phase). frames_per_session > 0: a session leaves
  * after that many measured frames (a saturate run then ends when all have left).
  * warmup_rounds: the first rounds (iterations that ran frames) are run but not measured.
- * Latency = frame due time to output complete. */
+ * host_io: every frame is uploaded from pinned host memory and its output downloaded into
+ * pinned host memory, on copy streams that overlap the ticks (the path of async_io.c's
+ * slots, src/async_io.c:183-272); otherwise frames stay in HBM. Latency = frame due time to
+ * output complete (in host memory with host_io).
+ * Input frames: session k of the plan (wema_serve_plan, in arrival order) of resolution class
+ * c (0..3 = 640x480, 1280x720, 1920x1080, 3840x2160) gets as its frame t the synthetic frame
+ * t % WEMA_SERVE_CLASS_FRAMES of preset 4 at that class's size and mix frame rate (30, 30, 60,
+ * 30) with seed offset c (wema_synth_preset). on_frame, if set, is called from the serving
+ * thread for every output frame once it is complete, with the session's plan index, the
+ * frame index and the output (host memory with host_io, else device memory; valid during
+ * the call). */
+#define WEMA_SERVE_CLASS_FRAMES 4
+typedef void (*wema_serve_frame_cb)(void *user, int session, long frame, const uint8_t *rgb24,
+                                    int width, int height);
 typedef struct {
     int      device;
     double   duration_s, drain_s;
@@ -231,6 +246,9 @@ typedef struct {
     int      warm_start;
     long     frames_per_session;
     int      warmup_rounds;
+    int      host_io;
+    wema_serve_frame_cb on_frame;
+    void    *cb_user;
 } wema_serve_config;
 typedef struct {
     long   frames;
@@ -240,8 +258,17 @@ typedef struct {
     size_t peak_bytes;
     long   ticks;
     double mean_frames_per_tick;
+    double host_ms_per_tick;    /* host time to build and launch one tick (mean, measured ticks) */
+    double graph_tick_frac;     /* fraction of measured ticks replayed from a hipGraph */
+    double gpu_busy_frac;       /* GPU time of the measured ticks / measured wall time */
+    double link_gbs;            /* host_io: bytes copied over the link, both ways, per second */
 } wema_serve_stats;
 int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *out);
+/* The sessions a run with this configuration admits, in arrival order (deterministic for a
+ * seed): their parameters, arrival times (s) and resolution classes, up to max of them;
+ * returns the number of sessions planned, or -1. */
+int wema_serve_plan(const wema_serve_config *cfg, int max, wema_dev_params *params, double *arrive_s,
+                    int *cls);
 
 /* ---- measurement (no counterpart in the reference) --------------------------------
  * While enabled, HIP events are recorded on the call's stream between the launches of

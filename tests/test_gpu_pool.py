@@ -154,3 +154,59 @@ def test_serving_loop_runs():
     assert st["sessions_admitted"] > 0 and st["frames"] > 0
     assert st["peak_bytes"] <= 40e9
     assert 0 < st["lat_p50_ms"] <= st["lat_p99_ms"] <= st["lat_max_ms"]
+
+
+def test_serving_c5_outputs_equal_alone_and_oracle():
+    """The serving loop at the c5 load (Poisson 4 sessions/s, lifetime 2-20 s, 30/60 fps,
+    resolutions from the 8:8:6:2 mix, 200 GB budget), from steady state for 3 s, with frames
+    uploaded from pinned host memory and downloaded back (copies overlapping the ticks):
+    the outputs of two sessions of each resolution class, recorded as they are served (up to
+    24 frames each, in whichever ticks they shared with the others), equal the same streams
+    run alone, frame by frame; one 640x480 session also equals the oracle."""
+    cfg = dict(duration_s=3.0, arrival_rate=4.0, life=(2.0, 20.0), hbm_budget=200e9,
+               warm_start=True, seed=21)
+    plan = wema_amd.serve_plan(**{k: v for k, v in cfg.items() if k != "hbm_budget"})
+    pick, per_cls = [], {}
+    for i, (p, arrive, c) in enumerate(plan):
+        if arrive == 0.0 and per_cls.get(c, 0) < 2:
+            per_cls[c] = per_cls.get(c, 0) + 1
+            pick.append(i)
+    assert len(per_cls) == 4, per_cls
+    want_frames = 24
+    got = {i: {} for i in pick}
+
+    def on_frame(session, frame, arr):
+        if session in got and frame < want_frames:
+            got[session][frame] = hashlib.sha256(arr.tobytes()).hexdigest()
+
+    st = wema_amd.serve(batched=True, graphs=True, host_io=True, on_frame=on_frame, **cfg)
+    assert st["frames"] > 0 and st["peak_bytes"] <= 200e9
+    assert 0 < st["lat_p50_ms"] <= st["lat_p99_ms"] and st["link_gbs"] > 0
+    inputs = {}
+    for c, (w, h, fps) in enumerate(wema_amd.SERVE_MIX):
+        syn = wema_amd.Synth(4, w, h, fps, c)
+        inputs[c] = np.stack([syn.frame(k) for k in range(wema_amd.SERVE_CLASS_FRAMES)])
+    checked_oracle = False
+    for i in pick:
+        p, _, c = plan[i]
+        n = len(got[i])
+        assert n >= 10 and sorted(got[i]) == list(range(n)), (i, n)
+        sess = wema_amd.Session(p.width, p.height, p.fps, amp=p.amp, f_low=p.f_low, f_high=p.f_high,
+                                window=p.window)
+        seq = np.stack([inputs[c][t % wema_amd.SERVE_CLASS_FRAMES] for t in range(n)])
+        d_in = torch.from_numpy(seq).cuda()
+        d_out = torch.empty_like(d_in)
+        for t in range(n):
+            sess.process_device(d_in[t:t + 1], d_out_u8=d_out[t:t + 1], n=1)
+        sess.sync()
+        sess.close()
+        alone = d_out.cpu().numpy()
+        for t in range(n):
+            assert hashlib.sha256(alone[t].tobytes()).hexdigest() == got[i][t], (i, c, t)
+        if c == 0 and not checked_oracle:
+            orc = O.OraclePipe(p.width, p.height, fps=p.fps, amp=p.amp, f_low=p.f_low, f_high=p.f_high,
+                               window=p.window)
+            for t in range(n):
+                assert np.array_equal(orc.frame(seq[t])[0], alone[t]), (i, t)
+            checked_oracle = True
+    assert checked_oracle

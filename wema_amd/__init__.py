@@ -62,6 +62,9 @@ class SynthParams(C.Structure):
     ]
 
 
+FRAME_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_long, C.POINTER(C.c_uint8), C.c_int, C.c_int)
+
+
 class ServeConfig(C.Structure):
     """wema_serve_config (include/wema_dev.h)."""
     _fields_ = [
@@ -70,7 +73,8 @@ class ServeConfig(C.Structure):
         ("hbm_budget", C.c_size_t), ("seed", C.c_uint64), ("saturate", C.c_int),
         ("sessions", C.c_int), ("batched", C.c_int), ("use_graphs", C.c_int),
         ("max_sessions", C.c_int), ("fps_from_mix", C.c_int), ("warm_start", C.c_int),
-        ("frames_per_session", C.c_long), ("warmup_rounds", C.c_int),
+        ("frames_per_session", C.c_long), ("warmup_rounds", C.c_int), ("host_io", C.c_int),
+        ("on_frame", FRAME_CB), ("cb_user", C.c_void_p),
     ]
 
 
@@ -82,6 +86,8 @@ class ServeStats(C.Structure):
         ("lat_max_ms", C.c_double), ("sessions_arrived", C.c_int), ("sessions_admitted", C.c_int),
         ("admission_waits", C.c_int), ("peak_sessions", C.c_int), ("peak_bytes", C.c_size_t),
         ("ticks", C.c_long), ("mean_frames_per_tick", C.c_double),
+        ("host_ms_per_tick", C.c_double), ("graph_tick_frac", C.c_double),
+        ("gpu_busy_frac", C.c_double), ("link_gbs", C.c_double),
     ]
 
 
@@ -139,6 +145,9 @@ _SIGS = {
     "wema_pool_timing_report": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_long),
                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "wema_serve_run": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(ServeStats)]),
+    "wema_serve_plan": (C.c_int, [C.POINTER(ServeConfig), C.c_int, C.POINTER(Params),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "wema_pool_stats": (C.c_int, [_vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "wema_synth_preset": (C.c_int, [C.POINTER(SynthParams), C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint32]),
     "wema_synth_texture_host": (C.c_int, [C.POINTER(SynthParams), _vp]),
     "wema_synth_frame_host": (C.c_int, [C.POINTER(SynthParams), _vp, C.c_int, _vp]),
@@ -466,17 +475,58 @@ class Pool:
             pass
 
 
+SERVE_MIX = [(640, 480, 30.0), (1280, 720, 30.0), (1920, 1080, 60.0), (3840, 2160, 30.0)]
+SERVE_CLASS_FRAMES = 4
+
+
+def _serve_config(duration_s, arrival_rate, life, hbm_budget, batched, graphs, saturate, sessions,
+                  drain_s, max_sessions, fps_from_mix, seed, device, warm_start, frames_per_session,
+                  warmup_rounds, host_io, on_frame=None):
+    return ServeConfig(device, duration_s, drain_s, arrival_rate, life[0], life[1], int(hbm_budget),
+                       seed, int(saturate), sessions, int(batched), int(graphs), max_sessions,
+                       int(fps_from_mix), int(warm_start), int(frames_per_session), int(warmup_rounds),
+                       int(host_io), on_frame if on_frame is not None else FRAME_CB(), None)
+
+
 def serve(duration_s=10.0, arrival_rate=4.0, life=(2.0, 20.0), hbm_budget=200e9, batched=True,
           graphs=True, saturate=False, sessions=0, drain_s=0.0, max_sessions=0,
           fps_from_mix=False, seed=0, device=0, warm_start=False, frames_per_session=0,
-          warmup_rounds=0):
-    """Runs the serving loop (wema_serve_run) and returns its statistics as a dict."""
-    c = ServeConfig(device, duration_s, drain_s, arrival_rate, life[0], life[1], int(hbm_budget),
-                    seed, int(saturate), sessions, int(batched), int(graphs), max_sessions,
-                    int(fps_from_mix), int(warm_start), int(frames_per_session), int(warmup_rounds))
+          warmup_rounds=0, host_io=False, on_frame=None):
+    """Runs the serving loop (wema_serve_run) and returns its statistics as a dict.
+    on_frame(session, frame, array) is called for every output frame (a (h, w, 3) uint8 view
+    of host memory with host_io, valid during the call; without host_io, the device address
+    as an int)."""
+    cb = FRAME_CB()
+    if on_frame is not None:
+        def _cb(_user, session, frame, rgb, w, h):
+            if host_io:
+                arr = np.ctypeslib.as_array(rgb, (h, w, 3))
+            else:
+                arr = C.cast(rgb, C.c_void_p).value
+            on_frame(session, frame, arr)
+        cb = FRAME_CB(_cb)
+    c = _serve_config(duration_s, arrival_rate, life, hbm_budget, batched, graphs, saturate, sessions,
+                      drain_s, max_sessions, fps_from_mix, seed, device, warm_start, frames_per_session,
+                      warmup_rounds, host_io, cb)
     st = ServeStats()
     _check(lib.wema_serve_run(C.byref(c), C.byref(st)), "wema_serve_run")
     return {k: getattr(st, k) for k, _ in ServeStats._fields_}
+
+
+def serve_plan(duration_s=10.0, arrival_rate=4.0, life=(2.0, 20.0), saturate=False, sessions=0,
+               fps_from_mix=False, seed=0, device=0, warm_start=False):
+    """The sessions wema_serve_run admits for these settings, in arrival order: a list of
+    (Params, arrival time in s, resolution class index into SERVE_MIX)."""
+    c = _serve_config(duration_s, arrival_rate, life, 0, True, True, saturate, sessions, 0.0, 0,
+                      fps_from_mix, seed, device, warm_start, 0, 0, False)
+    n = lib.wema_serve_plan(C.byref(c), 0, None, None, None)
+    if n < 0:
+        raise WemaError(last_error())
+    prm = (Params * max(n, 1))()
+    arr = (C.c_double * max(n, 1))()
+    cls = (C.c_int * max(n, 1))()
+    lib.wema_serve_plan(C.byref(c), n, prm, arr, cls)
+    return [(prm[i], arr[i], cls[i]) for i in range(n)]
 
 
 # ---------------------------------------------------------------------------------------

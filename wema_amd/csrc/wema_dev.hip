@@ -634,7 +634,12 @@ const char *const wema_timer_names[TK_COUNT] = {
 static int timer_mark(LaunchTimer *t, int kind, double bytes, hipStream_t st) {
     if (!t || !t->enabled) return 0;
     hipEvent_t ev;
-    HIP_OK(hipEventCreate(&ev));
+    if (!t->spare.empty()) {
+        ev = t->spare.back();
+        t->spare.pop_back();
+    } else {
+        HIP_OK(hipEventCreate(&ev));
+    }
     HIP_OK(hipEventRecord(ev, st));
     t->recs.push_back({ev, kind, bytes});
     return 0;
@@ -654,7 +659,7 @@ int wema_timer_collect(LaunchTimer *t) {
             t->launches[k] += 1;
         }
     }
-    for (auto &r : t->recs) hipEventDestroy(r.ev);
+    for (auto &r : t->recs) t->spare.push_back(r.ev);
     t->recs.clear();
     return 0;
 }

@@ -63,9 +63,17 @@ struct LaunchTimer {
         double bytes;
     };
     std::vector<Rec> recs;
+    std::vector<hipEvent_t> spare;   // events of collected records, reused by the next marks
     double ms[16] = {0};
     double bytes[16] = {0};
     long launches[16] = {0};
+    LaunchTimer() = default;
+    LaunchTimer(const LaunchTimer &) = delete;
+    LaunchTimer &operator=(const LaunchTimer &) = delete;
+    ~LaunchTimer() {
+        for (auto &r : recs) (void)hipEventDestroy(r.ev);
+        for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+    }
 };
 enum TimerKind { TK_DWT_FWD0 = 0, TK_DWT_FWD = 1, TK_IIR = 2, TK_WIN = 3, TK_DWT_INV = 4,
                  TK_DWT_INV0 = 5, TK_DWT_INV0_YCC = 6, TK_GF1 = 7, TK_GF2 = 8, TK_GAUSS = 9,

@@ -55,6 +55,7 @@ struct wema_pool {
     std::map<std::string, int> seen;   // tick shapes met so far (captured from the 2nd time)
     LaunchTimer timer;
     long ticks = 0;
+    long graph_ticks = 0;   // ticks replayed from a captured hipGraph
 };
 
 static int stage_alloc(StageBuf &b) {
@@ -283,6 +284,7 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
         }
         HIP_OK(hipGraphLaunch(it->second.exec, st));
         HIP_OK(hipEventRecord(it->second.buf.ev, st));
+        p->graph_ticks++;
     }
     for (wema_dev *s : ss) {
         s->frames += 1;
@@ -302,3 +304,10 @@ extern "C" int wema_pool_sync(wema_pool_t *p) {
 }
 
 extern "C" void *wema_pool_stream(wema_pool_t *p) { return p ? (void *)p->stream : nullptr; }
+
+extern "C" int wema_pool_stats(const wema_pool_t *p, long *ticks, long *graph_ticks) {
+    if (!p) { wema_set_error("null pool"); return -1; }
+    if (ticks) *ticks = p->ticks;
+    if (graph_ticks) *graph_ticks = p->graph_ticks;
+    return 0;
+}

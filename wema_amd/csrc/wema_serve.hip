@@ -3,7 +3,9 @@
 // their lifetime. Every iteration gathers the frames that are due and runs them as one
 // pool tick (continuous batching: sessions that arrive while others are on the GPU join
 // the next tick); the plain mode processes the same frames one session at a time.
-// Reports per-frame latency (due time -> output complete) and aggregate Mpixel/s.
+// Frames stay in HBM, or (host_io) travel from pinned host memory and back on copy streams
+// that overlap the ticks. Reports per-frame latency (due time -> output complete) and
+// aggregate Mpixel/s, with a per-tick breakdown (host time, graph replays, GPU busy time).
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -41,6 +43,7 @@ struct ResClass {
 
 // The multi-stream mix: 8 x 640x480@30, 8 x 1280x720@30, 6 x 1920x1080@60, 2 x 3840x2160@30.
 const ResClass kMix[4] = {{640, 480, 30.f, 8}, {1280, 720, 30.f, 8}, {1920, 1080, 60.f, 6}, {3840, 2160, 30.f, 2}};
+constexpr int kFrames = WEMA_SERVE_CLASS_FRAMES;
 
 struct Planned {
     double arrive, life, phase;
@@ -48,29 +51,9 @@ struct Planned {
     int cls;
 };
 
-struct Live {
-    int id;
-    int cls;
-    double start, end;      // seconds since run start
-    double fps;
-    long next;              // index of the next frame to deliver
-    uint8_t *out;           // device output slot
-    size_t out_bytes;
-};
-
-double now_s(std::chrono::steady_clock::time_point t0) {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-}
-
-}  // namespace
-
-extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *out) {
-    if (!cfg || !out) { wema_set_error("null argument"); return -1; }
-    memset(out, 0, sizeof(*out));
-    if (hipSetDevice(cfg->device) != hipSuccess) { wema_set_error("hipSetDevice failed"); return -1; }
+// The sessions of a run, in arrival order (deterministic for a seed).
+std::vector<Planned> make_plan(const wema_serve_config *cfg) {
     Rng rng{cfg->seed ? cfg->seed : 0x57454D41ull};
-
-    // ---- arrival plan (deterministic for a seed)
     std::vector<Planned> plan;
     int wsum = 0;
     for (const ResClass &c : kMix) wsum += c.weight;
@@ -117,10 +100,104 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         }
         plan.push_back(pl);
     }
-    out->sessions_arrived = (int)plan.size();
+    return plan;
+}
 
-    // ---- input frames: a few per resolution class, resident in HBM, cycled
-    const int kFrames = 4;
+// Per-session frame buffers: two device inputs (host_io), two device outputs and two
+// pinned host outputs (host_io), alternating by frame parity, so that the next frame's
+// upload overlaps the running tick. Buffers of departed sessions are kept per resolution
+// class and handed to the next session of that class.
+struct Slots {
+    uint8_t *din[2] = {nullptr, nullptr};
+    uint8_t *dout[2] = {nullptr, nullptr};
+    uint8_t *hout[2] = {nullptr, nullptr};
+};
+
+struct SlotPool {
+    bool host_io = false;
+    std::vector<Slots> free[4];
+    std::vector<Slots> all;
+    int take(int c, Slots &out) {
+        if (!free[c].empty()) {
+            out = free[c].back();
+            free[c].pop_back();
+            return 0;
+        }
+        const size_t bytes = (size_t)kMix[c].w * kMix[c].h * 3;
+        Slots s;
+        for (int k = 0; k < 2; k++) {
+            if (hipMalloc(&s.dout[k], bytes) != hipSuccess ||
+                (host_io && (hipMalloc(&s.din[k], bytes) != hipSuccess ||
+                             hipHostMalloc(&s.hout[k], bytes, hipHostMallocDefault) != hipSuccess))) {
+                wema_set_error("frame slot allocation failed");
+                all.push_back(s);
+                return -1;
+            }
+        }
+        all.push_back(s);
+        out = s;
+        return 0;
+    }
+    void give(int c, const Slots &s) { free[c].push_back(s); }
+    void release() {
+        for (Slots &s : all)
+            for (int k = 0; k < 2; k++) {
+                if (s.din[k]) hipFree(s.din[k]);
+                if (s.dout[k]) hipFree(s.dout[k]);
+                if (s.hout[k]) hipHostFree(s.hout[k]);
+            }
+        all.clear();
+        for (auto &f : free) f.clear();
+    }
+};
+
+struct Live {
+    int id;
+    int cls;
+    int plan_idx;
+    double start, end;      // seconds since run start
+    double fps;
+    long next;              // index of the next frame to deliver
+    Slots sl;
+};
+
+struct Done {                // a frame whose output the callback sees
+    int plan_idx, cls;
+    long frame;
+    const uint8_t *out;
+};
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+extern "C" int wema_serve_plan(const wema_serve_config *cfg, int max, wema_dev_params *params,
+                               double *arrive_s, int *cls) {
+    if (!cfg) { wema_set_error("null argument"); return -1; }
+    if (!cfg->saturate && !(cfg->arrival_rate > 0)) { wema_set_error("arrival rate must be positive"); return -1; }
+    const std::vector<Planned> plan = make_plan(cfg);
+    for (int i = 0; i < max && i < (int)plan.size(); i++) {
+        if (params) params[i] = plan[(size_t)i].p;
+        if (arrive_s) arrive_s[i] = plan[(size_t)i].arrive;
+        if (cls) cls[i] = plan[(size_t)i].cls;
+    }
+    return (int)plan.size();
+}
+
+extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *out) {
+    if (!cfg || !out) { wema_set_error("null argument"); return -1; }
+    memset(out, 0, sizeof(*out));
+    if (!cfg->saturate && !(cfg->arrival_rate > 0)) { wema_set_error("arrival rate must be positive"); return -1; }
+    if (hipSetDevice(cfg->device) != hipSuccess) { wema_set_error("hipSetDevice failed"); return -1; }
+    const std::vector<Planned> plan = make_plan(cfg);
+    out->sessions_arrived = (int)plan.size();
+    const bool fixed = cfg->saturate != 0;
+    const bool warm = !fixed && cfg->warm_start;
+    const bool host_io = cfg->host_io != 0;
+
+    // ---- input frames: a few per resolution class, cycled; in HBM, or pinned host memory
     std::vector<uint8_t *> in_frames[4];
     for (int c = 0; c < 4; c++) {
         wema_synth_params sp;
@@ -128,35 +205,69 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         const size_t P = (size_t)kMix[c].w * kMix[c].h;
         std::vector<uint16_t> tex(P);
         wema_synth_texture_host(&sp, tex.data());
-        uint16_t *dtex = nullptr;
         uint8_t *buf = nullptr;
-        if (hipMalloc(&dtex, P * 2) != hipSuccess || hipMalloc(&buf, P * 3 * kFrames) != hipSuccess) {
-            wema_set_error("frame allocation failed");
-            return -1;
+        if (host_io) {
+            if (hipHostMalloc(&buf, P * 3 * kFrames, hipHostMallocDefault) != hipSuccess) {
+                wema_set_error("pinned frame allocation failed");
+                return -1;
+            }
+            for (int k = 0; k < kFrames; k++) wema_synth_frame_host(&sp, tex.data(), k, buf + P * 3 * k);
+        } else {
+            uint16_t *dtex = nullptr;
+            if (hipMalloc(&dtex, P * 2) != hipSuccess || hipMalloc(&buf, P * 3 * kFrames) != hipSuccess) {
+                wema_set_error("frame allocation failed");
+                return -1;
+            }
+            hipMemcpy(dtex, tex.data(), P * 2, hipMemcpyHostToDevice);
+            if (wema_synth_frames_device(&sp, dtex, 0, kFrames, buf, nullptr) < 0) return -1;
+            hipDeviceSynchronize();
+            hipFree(dtex);
         }
-        hipMemcpy(dtex, tex.data(), P * 2, hipMemcpyHostToDevice);
-        if (wema_synth_frames_device(&sp, dtex, 0, kFrames, buf, nullptr) < 0) return -1;
-        hipFree(dtex);
         for (int k = 0; k < kFrames; k++) in_frames[c].push_back(buf + P * 3 * k);
+    }
+    auto free_frames = [&]() {
+        for (int c = 0; c < 4; c++)
+            if (!in_frames[c].empty()) {
+                if (host_io) hipHostFree(in_frames[c][0]);
+                else hipFree(in_frames[c][0]);
+                in_frames[c].clear();
+            }
+    };
+
+    // guided-filter tile heights of the mix's sizes, tuned before the clock starts (the
+    // first session of a size would otherwise time the candidate tiles while ticks run)
+    for (const ResClass &c : kMix) {
+        int r1, r2;
+        wema_gf_rows_for(cfg->device, c.w, c.h, &r1, &r2);
     }
 
     wema_pool_t *pool = nullptr;
-    if (wema_pool_create(&pool, cfg->device, cfg->hbm_budget) < 0) return -1;
+    if (wema_pool_create(&pool, cfg->device, cfg->hbm_budget) < 0) { free_frames(); return -1; }
     wema_pool_graphs(pool, cfg->use_graphs);
     hipStream_t st = (hipStream_t)wema_pool_stream(pool);
+    hipStream_t s_up = nullptr, s_down = nullptr;
+    if (host_io && (hipStreamCreateWithFlags(&s_up, hipStreamNonBlocking) != hipSuccess ||
+                    hipStreamCreateWithFlags(&s_down, hipStreamNonBlocking) != hipSuccess)) {
+        wema_set_error("copy stream creation failed");
+        wema_pool_destroy(pool);
+        free_frames();
+        return -1;
+    }
+    SlotPool slots_pool;
+    slots_pool.host_io = host_io;
 
     std::vector<double> lat;
     lat.reserve(1 << 20);
     std::deque<int> waiting;      // planned sessions not yet admitted
     std::vector<Live> live;
     size_t next_plan = 0;
-    long frames = 0, ticks = 0, tick_frames = 0, rounds = 0;
-    double mpix = 0.0;
+    long frames = 0, ticks = 0, tick_frames = 0, rounds = 0, graph_ticks = 0;
+    double mpix = 0.0, host_ms = 0.0, gpu_ms = 0.0, link_bytes = 0.0;
     int rc = 0;
     auto t0 = std::chrono::steady_clock::now();
     double t_meas = 0.0;          // start of the measured part (after the warm-up rounds)
     double t_last = 0.0;          // end of the last round
-    const double horizon = cfg->saturate ? cfg->duration_s : cfg->duration_s + cfg->drain_s;
+    const double horizon = fixed ? cfg->duration_s : cfg->duration_s + cfg->drain_s;
     const long quota = cfg->frames_per_session > 0 ? cfg->frames_per_session + cfg->warmup_rounds : 0;
     std::vector<int> ids;
     std::vector<const uint8_t *> ins;
@@ -164,30 +275,44 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
     std::vector<double> due;
     std::vector<size_t> who;
 
-    // batched ticks are pipelined: a tick is launched before the previous one is waited
-    // for, so the host gathers and builds the next tick while the GPU runs this one
+    // Ticks are pipelined: a tick is launched before the previous one is waited for, so the
+    // host gathers the next tick (and, with host_io, uploads its frames) while the GPU runs
+    // this one. ev_begin / ev_end bracket the tick's kernels on the pool stream (GPU busy
+    // time); ev_done marks its outputs complete (downloaded, with host_io).
     struct Pending {
-        hipEvent_t ev = nullptr;
+        hipEvent_t ev_up = nullptr, ev_begin = nullptr, ev_end = nullptr, ev_done = nullptr;
         std::vector<double> due;
+        std::vector<Done> done;
         bool measured = false;
     };
     Pending slots[2];
     for (Pending &q : slots)
-        if (hipEventCreateWithFlags(&q.ev, hipEventDisableTiming) != hipSuccess) {
+        if (hipEventCreateWithFlags(&q.ev_up, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreate(&q.ev_begin) != hipSuccess || hipEventCreate(&q.ev_end) != hipSuccess ||
+            hipEventCreateWithFlags(&q.ev_done, hipEventDisableTiming) != hipSuccess) {
             wema_set_error("event creation failed");
-            return -1;
+            rc = -1;
         }
     std::deque<Pending *> inflight;
     int slot = 0;
+    auto complete = [&](Pending *q) {
+        if (hipEventSynchronize(q->ev_done) != hipSuccess) { wema_set_error("tick failed"); return -1; }
+        const double done = now_s(t0);
+        if (q->measured) {
+            for (double d : q->due) lat.push_back(done - d);
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, q->ev_begin, q->ev_end) == hipSuccess) gpu_ms += ms;
+        }
+        if (cfg->on_frame)
+            for (const Done &d : q->done)
+                cfg->on_frame(cfg->cb_user, d.plan_idx, d.frame, d.out, kMix[d.cls].w, kMix[d.cls].h);
+        t_last = done;
+        return 0;
+    };
     auto drain_one = [&]() {
         Pending *q = inflight.front();
         inflight.pop_front();
-        if (hipEventSynchronize(q->ev) != hipSuccess) { wema_set_error("tick failed"); return -1; }
-        const double done = now_s(t0);
-        if (q->measured)
-            for (double d : q->due) lat.push_back(done - d);
-        t_last = done;
-        return 0;
+        return complete(q);
     };
     auto drain_all = [&]() {
         while (!inflight.empty())
@@ -198,7 +323,8 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
     // admission (first come, first served; the head waits until it fits)
     auto admit = [&](double now) {
         while (!waiting.empty()) {
-            const Planned &pl = plan[(size_t)waiting.front()];
+            const int pi = waiting.front();
+            const Planned &pl = plan[(size_t)pi];
             if (cfg->max_sessions > 0 && (int)live.size() >= cfg->max_sessions) break;
             int id = -1;
             if (wema_pool_add(pool, &pl.p, &id) < 0) {
@@ -208,15 +334,14 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
             Live L;
             L.id = id;
             L.cls = pl.cls;
+            L.plan_idx = pi;
             // streams that join in steady state run at a random phase of their frame period
             L.start = now + (warm && pl.arrive == 0.0 ? pl.phase / pl.p.fps : 0.0);
             L.end = now + pl.life;
             L.fps = pl.p.fps;
             L.next = 0;
-            L.out_bytes = (size_t)pl.p.width * pl.p.height * 3;
-            if (hipMalloc(&L.out, L.out_bytes) != hipSuccess) {
+            if (slots_pool.take(pl.cls, L.sl) < 0) {
                 wema_pool_remove(pool, id);
-                wema_set_error("output slot allocation failed");
                 return -1;
             }
             live.push_back(L);
@@ -226,28 +351,64 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         return 0;
     };
 
-    if (warm) {
+    if (warm && rc == 0) {
         // the steady-state population joins before the clock starts
         while (next_plan < plan.size() && plan[next_plan].arrive <= 0.0) waiting.push_back((int)next_plan++);
         if (admit(0.0) < 0) rc = -1;
         t0 = std::chrono::steady_clock::now();
     }
 
+    // uploads of this round's due frames (host_io), then a wait of the pool stream on them
+    auto upload = [&](Pending &q, size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; k++) {
+            const Live &L = live[who[k]];
+            const size_t bytes = (size_t)kMix[L.cls].w * kMix[L.cls].h * 3;
+            if (hipMemcpyAsync((void *)ins[k], in_frames[L.cls][(size_t)(L.next % kFrames)], bytes,
+                               hipMemcpyHostToDevice, s_up) != hipSuccess)
+                return -1;
+        }
+        if (hipEventRecord(q.ev_up, s_up) != hipSuccess || hipStreamWaitEvent(st, q.ev_up, 0) != hipSuccess) return -1;
+        return 0;
+    };
+    auto download = [&](Pending &q, size_t k0, size_t k1) {
+        if (hipStreamWaitEvent(s_down, q.ev_end, 0) != hipSuccess) return -1;
+        for (size_t k = k0; k < k1; k++) {
+            const Live &L = live[who[k]];
+            const size_t bytes = (size_t)kMix[L.cls].w * kMix[L.cls].h * 3;
+            const int b = (int)(L.next & 1);
+            if (hipMemcpyAsync(L.sl.hout[b], outs[k], bytes, hipMemcpyDeviceToHost, s_down) != hipSuccess) return -1;
+        }
+        return hipEventRecord(q.ev_done, s_down) == hipSuccess ? 0 : -1;
+    };
+    auto record_done = [&](Pending &q, size_t k0, size_t k1) {
+        q.done.clear();
+        if (!cfg->on_frame) return;
+        for (size_t k = k0; k < k1; k++) {
+            const Live &L = live[who[k]];
+            const int b = (int)(L.next & 1);
+            q.done.push_back({L.plan_idx, L.cls, L.next, host_io ? L.sl.hout[b] : L.sl.dout[b]});
+        }
+    };
+
     while (rc == 0) {
         const double now = now_s(t0);
         if (now >= horizon) break;
         // arrivals
         while (next_plan < plan.size() && plan[next_plan].arrive <= now) waiting.push_back((int)next_plan++);
-        // departures
-        for (size_t i = 0; i < live.size();) {
-            if (live[i].end <= now || (quota && live[i].next >= quota)) {
-                if (drain_all() < 0) { rc = -1; break; }
-                hipStreamSynchronize(st);
-                wema_pool_remove(pool, live[i].id);
-                hipFree(live[i].out);
-                live.erase(live.begin() + (long)i);
-            } else {
-                ++i;
+        // departures: every session leaving now, after one wait for the ticks in flight
+        bool leaving = false;
+        for (const Live &L : live) leaving = leaving || L.end <= now || (quota && L.next >= quota);
+        if (leaving) {
+            if (drain_all() < 0) { rc = -1; break; }
+            for (size_t i = 0; i < live.size();) {
+                Live &L = live[i];
+                if (L.end <= now || (quota && L.next >= quota)) {
+                    wema_pool_remove(pool, L.id);
+                    slots_pool.give(L.cls, L.sl);
+                    live.erase(live.begin() + (long)i);
+                } else {
+                    ++i;
+                }
             }
         }
         if (admit(now) < 0) { rc = -1; break; }
@@ -259,11 +420,12 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         double next_due = 1e30;
         for (size_t i = 0; i < live.size(); i++) {
             Live &L = live[i];
-            const double d = cfg->saturate ? now : L.start + (double)L.next / L.fps;
+            const double d = fixed ? now : L.start + (double)L.next / L.fps;
             if (d <= now) {
+                const int b = (int)(L.next & 1);
                 ids.push_back(L.id);
-                ins.push_back(in_frames[L.cls][(size_t)(L.next % kFrames)]);
-                outs.push_back(L.out);
+                ins.push_back(host_io ? L.sl.din[b] : in_frames[L.cls][(size_t)(L.next % kFrames)]);
+                outs.push_back(L.sl.dout[b]);
                 due.push_back(d);
                 who.push_back(i);
             } else {
@@ -283,35 +445,75 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
             continue;
         }
         const bool measured = rounds >= cfg->warmup_rounds;
+        long g0 = 0, g1 = 0;
+        wema_pool_stats(pool, nullptr, &g0);
         if (cfg->batched) {
             Pending &q = slots[slot];
             slot ^= 1;
             if (!inflight.empty() && inflight.front() == &q && drain_one() < 0) { rc = -1; break; }
+            const auto h0 = std::chrono::steady_clock::now();
+            if (host_io && upload(q, 0, ids.size()) < 0) { wema_set_error("frame upload failed"); rc = -1; break; }
+            if (hipEventRecord(q.ev_begin, st) != hipSuccess) { wema_set_error("event record failed"); rc = -1; break; }
             if (wema_pool_tick(pool, (int)ids.size(), ids.data(), ins.data(), outs.data(), nullptr) < 0) { rc = -1; break; }
-            if (hipEventRecord(q.ev, st) != hipSuccess) { wema_set_error("event record failed"); rc = -1; break; }
+            if (hipEventRecord(q.ev_end, st) != hipSuccess) { wema_set_error("event record failed"); rc = -1; break; }
+            if (host_io) {
+                if (download(q, 0, ids.size()) < 0) { wema_set_error("frame download failed"); rc = -1; break; }
+            } else if (hipEventRecord(q.ev_done, st) != hipSuccess) {
+                wema_set_error("event record failed");
+                rc = -1;
+                break;
+            }
+            const double hms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
             q.due = due;
             q.measured = measured;
+            record_done(q, 0, ids.size());
             inflight.push_back(&q);
             if (inflight.size() > 1 && drain_one() < 0) { rc = -1; break; }
-            if (measured) ticks++;
+            if (measured) {
+                ticks++;
+                host_ms += hms;
+            }
         } else {
+            // the plain path: each frame through the device alone, waited for
+            Pending &q = slots[0];
             for (size_t k = 0; k < ids.size(); k++) {
-                if (wema_pool_tick(pool, 1, &ids[k], &ins[k], &outs[k], nullptr) < 0) { rc = -1; break; }
-                if (hipStreamSynchronize(st) != hipSuccess) { wema_set_error("tick failed"); rc = -1; break; }
+                const auto h0 = std::chrono::steady_clock::now();
+                if (host_io && upload(q, k, k + 1) < 0) { wema_set_error("frame upload failed"); rc = -1; break; }
+                if (hipEventRecord(q.ev_begin, st) != hipSuccess ||
+                    wema_pool_tick(pool, 1, &ids[k], &ins[k], &outs[k], nullptr) < 0 ||
+                    hipEventRecord(q.ev_end, st) != hipSuccess) {
+                    rc = -1;
+                    break;
+                }
+                if (host_io) {
+                    if (download(q, k, k + 1) < 0) { wema_set_error("frame download failed"); rc = -1; break; }
+                } else if (hipEventRecord(q.ev_done, st) != hipSuccess) {
+                    rc = -1;
+                    break;
+                }
+                const double hms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+                q.due.assign(1, due[k]);
+                q.measured = measured;
+                record_done(q, k, k + 1);
+                if (complete(&q) < 0) { rc = -1; break; }
                 if (measured) {
-                    lat.push_back(now_s(t0) - due[k]);
                     ticks++;
+                    host_ms += hms;
                 }
             }
             if (rc < 0) break;
         }
+        wema_pool_stats(pool, nullptr, &g1);
+        if (measured) graph_ticks += g1 - g0;
         for (size_t k = 0; k < who.size(); k++) {
             Live &L = live[who[k]];
-            L.next++;
             if (measured) {
                 frames++;
-                mpix += (double)kMix[L.cls].w * kMix[L.cls].h / 1e6;
+                const double px = (double)kMix[L.cls].w * kMix[L.cls].h;
+                mpix += px / 1e6;
+                if (host_io) link_bytes += 6.0 * px;
             }
+            L.next++;
         }
         if (measured) tick_frames += (long)ids.size();
         rounds++;
@@ -320,16 +522,22 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
     }
     if (drain_all() < 0) rc = -1;
     // a saturate run is measured up to its last round, a paced one over its whole span
-    const double wall = (cfg->saturate ? t_last : now_s(t0)) - t_meas;
+    const double wall = (fixed ? t_last : now_s(t0)) - t_meas;
     hipStreamSynchronize(st);
-    for (Pending &q : slots) hipEventDestroy(q.ev);
-    for (Live &L : live) {
-        wema_pool_remove(pool, L.id);
-        hipFree(L.out);
+    if (s_up) hipStreamSynchronize(s_up);
+    if (s_down) hipStreamSynchronize(s_down);
+    for (Pending &q : slots) {
+        if (q.ev_up) hipEventDestroy(q.ev_up);
+        if (q.ev_begin) hipEventDestroy(q.ev_begin);
+        if (q.ev_end) hipEventDestroy(q.ev_end);
+        if (q.ev_done) hipEventDestroy(q.ev_done);
     }
+    for (Live &L : live) wema_pool_remove(pool, L.id);
     wema_pool_destroy(pool);
-    for (int c = 0; c < 4; c++)
-        if (!in_frames[c].empty()) hipFree(in_frames[c][0]);
+    slots_pool.release();
+    if (s_up) hipStreamDestroy(s_up);
+    if (s_down) hipStreamDestroy(s_down);
+    free_frames();
     if (rc < 0) return -1;
 
     std::sort(lat.begin(), lat.end());
@@ -341,11 +549,15 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
     out->frames = frames;
     out->mpix = mpix;
     out->wall_s = wall;
-    out->mpix_per_s = mpix / wall;
+    out->mpix_per_s = wall > 0 ? mpix / wall : 0.0;
     out->lat_p50_ms = pct(0.50);
     out->lat_p99_ms = pct(0.99);
     out->lat_max_ms = lat.empty() ? 0.0 : lat.back() * 1e3;
     out->ticks = ticks;
     out->mean_frames_per_tick = ticks ? (double)tick_frames / (double)ticks : 0.0;
+    out->host_ms_per_tick = ticks ? host_ms / (double)ticks : 0.0;
+    out->graph_tick_frac = ticks ? (double)graph_ticks / (double)ticks : 0.0;
+    out->gpu_busy_frac = wall > 0 ? gpu_ms / 1e3 / wall : 0.0;
+    out->link_gbs = wall > 0 ? link_bytes / wall / 1e9 : 0.0;
     return 0;
 }
