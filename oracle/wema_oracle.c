@@ -472,42 +472,67 @@ float orc_fade_amp(float amp, int global_frame) {
  * Box / guided / Gaussian (src/wema.c:114-283; src/wema_batch.c:26-132)
  *===========================================================================*/
 
-/* Window sums of one line of n samples (stride st), radius r, samples outside [0, n) taken
- * as 0. prefix: the reference's FP32 prefix sums (src/wema.c:132-148). Otherwise the
- * order the GPU kernels use: the line is cut into blocks of 4 outputs starting at multiples
- * of 4; the first sum of a block adds its window left to right from 0, each next one is
- * (previous - leaving sample) + entering sample. */
-static void box_line_f32(const float *v, size_t st, int n, int r, int prefix, float *pre,
-                         float *out, size_t ost) {
-    if (prefix) {
-        pre[0] = 0.0f;
-        for (int i = 0; i < n; i++) pre[i + 1] = pre[i] + v[(size_t)i * st];
-    }
-    float s = 0.0f;
+/* The reference's box mean (src/wema.c:114-171): per line, FP32 prefix sums, the window's
+ * sum as a difference of two of them, divided by the clipped count; rows, then columns of
+ * the row means (a mean of means). */
+static void box_prefix_line_f32(const float *v, size_t st, int n, int r, float *pre, float *out,
+                                size_t ost) {
+    pre[0] = 0.0f;
+    for (int i = 0; i < n; i++) pre[i + 1] = pre[i] + v[(size_t)i * st];
     for (int x = 0; x < n; x++) {
         const int lo = x - r > 0 ? x - r : 0, hi = x + r + 1 < n ? x + r + 1 : n;
-        if (prefix) {
-            s = pre[hi] - pre[lo];
-        } else if (x % 4 == 0) {
-            s = 0.0f;
-            for (int k = lo; k < hi; k++) s += v[(size_t)k * st];
-        } else {
-            const int leave = x - r - 1, enter = x + r;
-            s = s - (leave >= 0 ? v[(size_t)leave * st] : 0.0f);
-            s = s + (enter < n ? v[(size_t)enter * st] : 0.0f);
-        }
-        out[(size_t)x * ost] = s / (float)(hi - lo);
+        out[(size_t)x * ost] = (pre[hi] - pre[lo]) / (float)(hi - lo);
     }
 }
 
+/* Window sums of one line of n samples (stride st), radius r, samples outside [0, n) taken
+ * as 0, in the order the GPU kernels use: the line is cut into blocks of 4 outputs starting
+ * at multiples of 4; the first sum of a block adds its window's 2r + 1 samples left to
+ * right, starting from the first, each next one is (previous - leaving sample) + entering
+ * sample. */
+static void box_sum_line_f32(const float *v, size_t st, int n, int r, float *out, size_t ost) {
+#define VAL(i) ((i) >= 0 && (i) < n ? v[(size_t)(i) * st] : 0.0f)
+    float s = 0.0f;
+    for (int x = 0; x < n; x++) {
+        if (x % 4 == 0) {
+            s = VAL(x - r);
+            for (int k = -r + 1; k <= r; k++) s += VAL(x + k);
+        } else {
+            s = (s - VAL(x - r - 1)) + VAL(x + r);
+        }
+        out[(size_t)x * ost] = s;
+    }
+#undef VAL
+}
+
+/* Clipped window count at x of n (src/wema.c:144-147). */
+static int clip_count(int x, int n, int r) {
+    const int lo = x - r > 0 ? x - r : 0, hi = x + r + 1 < n ? x + r + 1 : n;
+    return hi > lo ? hi - lo : 1;
+}
+
+/* Box mean over the (2r + 1)^2 window clipped to the frame. prefix: the reference's order
+ * (above). Otherwise the kernels': the window sum as the vertical window sum of horizontal
+ * window sums (box_sum_line_f32), times the correctly rounded reciprocal of the clipped
+ * window's area; in real arithmetic the same mean as the reference's mean of means. */
 void orc_box_f32(const float *in, float *out, int w, int h, int r, int prefix) {
     FP_REFERENCE_MODE();
     size_t P = (size_t)w * h;
     float *tmp = malloc(P * sizeof(float));
     float *pre = malloc((size_t)((w > h ? w : h) + 1) * sizeof(float));
-    for (int y = 0; y < h; y++)
-        box_line_f32(in + (size_t)y * w, 1, w, r, prefix, pre, tmp + (size_t)y * w, 1);
-    for (int x = 0; x < w; x++) box_line_f32(tmp + x, (size_t)w, h, r, prefix, pre, out + x, (size_t)w);
+    if (prefix) {
+        for (int y = 0; y < h; y++)
+            box_prefix_line_f32(in + (size_t)y * w, 1, w, r, pre, tmp + (size_t)y * w, 1);
+        for (int x = 0; x < w; x++) box_prefix_line_f32(tmp + x, (size_t)w, h, r, pre, out + x, (size_t)w);
+    } else {
+        for (int y = 0; y < h; y++) box_sum_line_f32(in + (size_t)y * w, 1, w, r, tmp + (size_t)y * w, 1);
+        for (int x = 0; x < w; x++) box_sum_line_f32(tmp + x, (size_t)w, h, r, out + x, (size_t)w);
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                const float k = 1.0f / (float)(clip_count(x, w, r) * clip_count(y, h, r));
+                out[(size_t)y * w + x] *= k;
+            }
+    }
     free(tmp);
     free(pre);
 }

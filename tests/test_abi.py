@@ -22,6 +22,7 @@ LIBS = {"wema_dev.h": wema_amd.LIB_PATH,
 def declared_functions(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"^\s*typedef[^;]*;", "", text, flags=re.M | re.S)   # (function pointer types)
     return {m.group(2) for m in re.finditer(r"(^|[\s*])([a-z_][a-z0-9_]*)\s*\(", text, flags=re.M)
             if not m.group(2).startswith(("if", "sizeof"))}
 

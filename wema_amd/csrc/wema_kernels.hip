@@ -1181,8 +1181,13 @@ WEMA_WIN_LDS_KERNEL(64)
 
 // ---------------------------------------------------------------------------------------
 // Guided filter (src/wema.c:114-230, 289-335; src/wema_batch.c:26-132, 311-346).
-// Box means: horizontal 9-tap sum divided by the clipped count, then vertical likewise
-// (src/wema.c:137-167); the sums are formed in blocks of 4 outputs (sums4).
+// Box means over the 9x9 window clipped to the frame. The reference divides each row's
+// window sums by their count and then the column sums of those by theirs (a mean of means
+// over FP32 prefix sums, src/wema.c:114-171). Here the window sum is the vertical 9-tap sum
+// of horizontal 9-tap sums, each formed in blocks of 4 outputs (sums4), and the mean is that
+// sum times the correctly rounded reciprocal of the clipped window's area: the same mean in
+// real arithmetic, with one multiply where the reference divides twice (the oracle restates
+// this order exactly).
 
 __device__ __forceinline__ float src_luma(const EgressJob &J, size_t idx) {
     if (J.src_u8) {
@@ -1198,16 +1203,6 @@ __device__ __forceinline__ int clip_count(int x, int n) {
     const int lo = x - 4 > 0 ? x - 4 : 0;
     const int hi = x + 5 < n ? x + 5 : n;
     return hi > lo ? hi - lo : 1;
-}
-
-// a / c for a window count c in 1..9, given k = the correctly rounded 1/c: q = a * k is
-// within an ulp of a / c, and one correction with the exact residual fma(-q, c, a) rounds
-// to the correctly rounded quotient (Markstein). Equal to IEEE a / c for every normal a
-// (checked exhaustively for c = 1..9 over whole binades of a by tests/div_check.c), and
-// free of the mode switches and range scaling of the general division.
-__device__ __forceinline__ float div_count(float a, float c, float k) {
-    const float q = a * k;
-    return fmaf(fmaf(-q, c, a), k, q);
 }
 
 // a / b for b in [2^-7, 2) and 2^-66 <= |a| < 2^64 (or a = +0): the steps of the IEEE
@@ -1226,24 +1221,19 @@ __device__ __forceinline__ float div_mid(float a, float b) {
     return fmaf(fmaf(-b, q, a), y, q);
 }
 
-// Correctly rounded 1/c for c = 0..15 in LDS (entry 0 unused), filled by the first threads.
+// Correctly rounded 1/c for c = 0..81 (the areas of clipped 9x9 windows) in LDS (entry 0
+// unused), filled by the first threads.
+constexpr int GF_RCP = 82;
 __device__ __forceinline__ void fill_rcp(float *t) {
-    if (threadIdx.x < 16) t[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 0.0f;
+    if (threadIdx.x < GF_RCP) t[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 0.0f;
 }
 
-// Window count at x of n and its reciprocal: inside (IN: the whole tile's windows lie in
-// the frame) 9 and the constant 1/9 (the correctly rounded value, as in the table), else
-// the clipped count and the table entry.
+// 1 / (area of the clipped window) at column count cx and row y of h: inside (IN: every
+// window of the tile lies in the frame) the constant 1/81, correctly rounded as in the table.
 template <bool IN>
-__device__ __forceinline__ void count_at(int x, int n, const float *sRcp, float &cnt, float &k) {
-    if (IN) {
-        cnt = 9.0f;
-        k = 1.0f / 9.0f;
-    } else {
-        const int c = clip_count(x, n);
-        cnt = (float)c;
-        k = sRcp[c];
-    }
+__device__ __forceinline__ float inv_area(int cx, int y, int h, const float *sRcp) {
+    if (IN) return 1.0f / 81.0f;
+    return sRcp[cx * clip_count(y, h)];
 }
 template <bool B> using bool_c = std::integral_constant<bool, B>;
 
@@ -1255,14 +1245,14 @@ constexpr int GF_EX = GF_TX + 8;         // with the 4-column halo on each side
 constexpr int GF_PITCH = GF_EX + 4;      // multiple of 4: rows start 16-byte aligned
 
 // 9-tap sums of a block of 4 consecutive outputs (starting at a multiple of 4 of the image)
-// from 12 samples held in registers: the first adds its window left to right from +0, each
-// next one is (previous - leaving sample) + entering sample. Samples outside the image are
-// 0 (exact in both steps). The blocks are anchored to the image, so results do not depend on
-// the tile shape.
+// from 12 samples held in registers: the first adds its window left to right, starting from
+// its first sample, each next one is (previous - leaving sample) + entering sample. Samples
+// outside the image are 0 (exact in both steps). The blocks are anchored to the image, so
+// results do not depend on the tile shape.
 __device__ __forceinline__ void sums4(const float (&v)[12], float (&out)[4]) {
-    float a = 0.0f;
+    float a = v[0];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) a += v[k];
+    for (int k = 1; k < 9; ++k) a += v[k];
     out[0] = a;
 #pragma unroll
     for (int j = 1; j < 4; ++j) out[j] = (out[j - 1] - v[j - 1]) + v[j + 8];
@@ -1285,12 +1275,14 @@ struct SrcPix {
     __device__ __forceinline__ float luma() const { return luma_of(ch(r), ch(g), ch(b)); }
 };
 
-// Guide (source luma) and input (reconstruction - guide) of the extended tile (TY + 8 rows)
-// into LDS, zero outside the frame. Coordinates are clamped into the frame so that every
-// load is valid and all of them are issued before any is used (one memory round trip).
+// Guide I (source luma), input p (reconstruction - guide) and the products I^2 and I p of
+// the extended tile (TY + 8 rows) into LDS, zero outside the frame. Coordinates are clamped
+// into the frame so that every load is valid and all of them are issued before any is used
+// (one memory round trip).
 template <bool U8, int TY>
 __device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, float (*sI)[GF_PITCH],
-                                         float (*sP)[GF_PITCH]) {
+                                         float (*sP)[GF_PITCH], float (*sII)[GF_PITCH],
+                                         float (*sIP)[GF_PITCH]) {
     constexpr int NT = GF_NT(TY);
     constexpr int EY = TY + 8, NLD = (EY * GF_EX + NT - 1) / NT;
     const int w = J.w, h = J.h;
@@ -1313,9 +1305,12 @@ __device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, flo
             const int ey = i / GF_EX, ex = i - ey * GF_EX;
             const int x = x0 - 4 + ex, y = y0 - 4 + ey;
             const bool in = x >= 0 && x < w && y >= 0 && y < h;
-            const float I = px[j].luma();
-            sI[ey][ex] = in ? I : 0.0f;
-            sP[ey][ex] = in ? g[j] - I : 0.0f;
+            const float l = px[j].luma();
+            const float I = in ? l : 0.0f, pv = in ? g[j] - l : 0.0f;
+            sI[ey][ex] = I;
+            sP[ey][ex] = pv;
+            sII[ey][ex] = I * I;
+            sIP[ey][ex] = I * pv;
         }
     }
 }
@@ -1325,7 +1320,8 @@ __device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, flo
 // RGB24 is 3 dwords (float RGB: 3 x 16 bytes) and its guide a float4.
 template <bool U8, int TY>
 __device__ __forceinline__ void gf1_load_vec(const EgressJob &J, int x0, int y0, float (*sI)[GF_PITCH],
-                                             float (*sP)[GF_PITCH]) {
+                                             float (*sP)[GF_PITCH], float (*sII)[GF_PITCH],
+                                             float (*sIP)[GF_PITCH]) {
     constexpr int NT = GF_NT(TY);
     constexpr int GX = GF_EX / 4, NG = (TY + 8) * GX, NIT = (NG + NT - 1) / NT;
     const int w = J.w, h = J.h;
@@ -1372,6 +1368,8 @@ __device__ __forceinline__ void gf1_load_vec(const EgressJob &J, int x0, int y0,
         }
         *(v4f *)&sI[ey][4 * gx] = v4f{I[0], I[1], I[2], I[3]};
         *(v4f *)&sP[ey][4 * gx] = v4f{pv[0], pv[1], pv[2], pv[3]};
+        *(v4f *)&sII[ey][4 * gx] = v4f{I[0] * I[0], I[1] * I[1], I[2] * I[2], I[3] * I[3]};
+        *(v4f *)&sIP[ey][4 * gx] = v4f{I[0] * pv[0], I[1] * pv[1], I[2] * pv[2], I[3] * pv[3]};
     }
 }
 
@@ -1388,18 +1386,87 @@ __device__ __forceinline__ const EgressJob &gf_tile(const EgressJob *__restrict_
     return J;
 }
 
+// Horizontal window sums of NQ quantities over the extended tile, 4 outputs per item (taps
+// read as three 16-byte vectors per quantity), written over the samples they come from: every
+// thread computes all of its items first, then a barrier separates the last read of the tile
+// from the first write.
+template <int TY, int NQ>
+__device__ __forceinline__ void gf_hsums(float (*const (&S)[NQ])[GF_PITCH]) {
+    constexpr int NT = GF_NT(TY);
+    constexpr int EY = TY + 8, NI = (EY * (GF_TX / 4) + NT - 1) / NT;   // items per thread
+    float hv[NI][NQ][4];
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+        const int i = threadIdx.x + NT * it;
+        if (i >= EY * (GF_TX / 4)) break;
+        const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            float v[12];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float4 a = *reinterpret_cast<const float4 *>(&S[q][ey][x4 + 4 * k]);
+                v[4 * k] = a.x; v[4 * k + 1] = a.y; v[4 * k + 2] = a.z; v[4 * k + 3] = a.w;
+            }
+            sums4(v, hv[it][q]);
+        }
+    }
+    __syncthreads();   // (every read of the tile is done)
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+        const int i = threadIdx.x + NT * it;
+        if (i >= EY * (GF_TX / 4)) break;
+        const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            *reinterpret_cast<float4 *>(&S[q][ey][x4]) = make_float4(hv[it][q][0], hv[it][q][1], hv[it][q][2], hv[it][q][3]);
+    }
+}
+
 // First pass: box means of I, p, I^2, I p over the tile, then a and b (src/wema.c:173-219).
-// The horizontal means of I and p are written over the tile of I and p they come from
-// (hI, hP alias sI, sP): every thread computes all of its items first, then a barrier
-// separates the last read of sI, sP from the first write.
+// IN: every window of the tile lies in the frame (area 81 everywhere).
+template <int TY, bool IN>
+__device__ __forceinline__ void gf1_vert(const EgressJob &J, int x0, int y0, float (*const (&S)[4])[GF_PITCH],
+                                         const float *sRcp) {
+    constexpr int NT = GF_NT(TY);
+    const int w = J.w, h = J.h;
+    GLOBAL_AS v2f *ab = (GLOBAL_AS v2f *)gp((float *)J.ab);
+    // 4 consecutive rows of one column per item
+    for (int i = threadIdx.x; i < (TY / 4) * GF_TX; i += NT) {
+        const int y4 = 4 * (i / GF_TX), x = i % GF_TX;
+        const int gx = x0 + x;
+        if (!IN && gx >= w) continue;
+        float sm[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float v[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) v[k] = S[q][y4 + k][x];
+            sums4(v, sm[q]);
+        }
+        const int cx = IN ? 9 : clip_count(gx, w);
+        const unsigned row0 = (unsigned)(y0 + y4) * (unsigned)w + (unsigned)gx;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gy = y0 + y4 + j;
+            if (!IN && gy >= h) break;
+            const float k = inv_area<IN>(cx, gy, h, sRcp);
+            const float mI = sm[0][j] * k, mp = sm[1][j] * k, cI = sm[2][j] * k, cIp = sm[3][j] * k;
+            const float var = fmaf(-mI, mI, cI);
+            const float cov = fmaf(-mI, mp, cIp);
+            // var + eps lies in [0.0099, 1.02): I is in [0, 1]
+            const float av = div_mid(cov, var + 0.01f);
+            const float bv = fmaf(-av, mI, mp);
+            // written once, read by the second pass's next launch: nontemporal
+            __builtin_nontemporal_store(v2f{av, bv}, ab + (row0 + (unsigned)(j * w)));
+        }
+    }
+}
+
 template <int TY>
 __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int njobs, float (*sI)[GF_PITCH],
-                                        float (*sP)[GF_PITCH], float (*hII)[GF_TX], float (*hIP)[GF_TX],
+                                        float (*sP)[GF_PITCH], float (*sII)[GF_PITCH], float (*sIP)[GF_PITCH],
                                         float *sRcp) {
-    float (*hI)[GF_PITCH] = sI;
-    float (*hP)[GF_PITCH] = sP;
-    constexpr int NT = GF_NT(TY);
-    constexpr int EY = TY + 8;
     fill_rcp(sRcp);
     int x0, y0;
     const EgressJob &J = gf_tile<TY>(jobs, njobs, x0, y0);
@@ -1407,104 +1474,19 @@ __device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int 
     const uintptr_t al = (uintptr_t)J.gray | (J.src_u8 ? 0 : (uintptr_t)J.src_f32);
     const bool vec = (w & 3) == 0 && (al & 15) == 0 && (!J.src_u8 || ((uintptr_t)J.src_u8 & 3) == 0);
     if (vec) {
-        if (J.src_u8) gf1_load_vec<true, TY>(J, x0, y0, sI, sP);
-        else gf1_load_vec<false, TY>(J, x0, y0, sI, sP);
+        if (J.src_u8) gf1_load_vec<true, TY>(J, x0, y0, sI, sP, sII, sIP);
+        else gf1_load_vec<false, TY>(J, x0, y0, sI, sP, sII, sIP);
     } else {
-        if (J.src_u8) gf1_load<true, TY>(J, x0, y0, sI, sP);
-        else gf1_load<false, TY>(J, x0, y0, sI, sP);
+        if (J.src_u8) gf1_load<true, TY>(J, x0, y0, sI, sP, sII, sIP);
+        else gf1_load<false, TY>(J, x0, y0, sI, sP, sII, sIP);
     }
     __syncthreads();
-    // horizontal: 4 outputs per item, taps read as three 16-byte vectors per quantity; a
-    // thread's items share their 4 columns (NT is a multiple of GF_TX / 4), so the window
-    // counts and reciprocals are taken once
-    const int hx4 = 4 * ((int)threadIdx.x % (GF_TX / 4));
-    float hcnt[4], hk[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int n = clip_count(x0 + hx4 + j, w);
-        hcnt[j] = (float)n;
-        hk[j] = sRcp[n];
-    }
-    constexpr int NI = (EY * (GF_TX / 4) + NT - 1) / NT;   // items per thread
-    float hv[NI][4][4];
-#pragma unroll
-    for (int it = 0; it < NI; ++it) {
-        const int i = threadIdx.x + NT * it;
-        if (i >= EY * (GF_TX / 4)) break;
-        const int ey = i / (GF_TX / 4), x4 = hx4;
-        float vi[12], vp[12], vii[12], vip[12];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const float4 a = *reinterpret_cast<const float4 *>(&sI[ey][x4 + 4 * q]);
-            const float4 b = *reinterpret_cast<const float4 *>(&sP[ey][x4 + 4 * q]);
-            vi[4 * q] = a.x; vi[4 * q + 1] = a.y; vi[4 * q + 2] = a.z; vi[4 * q + 3] = a.w;
-            vp[4 * q] = b.x; vp[4 * q + 1] = b.y; vp[4 * q + 2] = b.z; vp[4 * q + 3] = b.w;
-        }
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            vii[k] = vi[k] * vi[k];
-            vip[k] = vi[k] * vp[k];
-        }
-        float si[4], spp[4], sii[4], sip[4];
-        sums4(vi, si);
-        sums4(vp, spp);
-        sums4(vii, sii);
-        sums4(vip, sip);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            hv[it][0][j] = div_count(si[j], hcnt[j], hk[j]);
-            hv[it][1][j] = div_count(spp[j], hcnt[j], hk[j]);
-            hv[it][2][j] = div_count(sii[j], hcnt[j], hk[j]);
-            hv[it][3][j] = div_count(sip[j], hcnt[j], hk[j]);
-        }
-    }
-    __syncthreads();   // (every read of sI, sP is done)
-#pragma unroll
-    for (int it = 0; it < NI; ++it) {
-        const int i = threadIdx.x + NT * it;
-        if (i >= EY * (GF_TX / 4)) break;
-        const int ey = i / (GF_TX / 4), x4 = hx4;
-        *reinterpret_cast<float4 *>(&hI[ey][x4]) = make_float4(hv[it][0][0], hv[it][0][1], hv[it][0][2], hv[it][0][3]);
-        *reinterpret_cast<float4 *>(&hP[ey][x4]) = make_float4(hv[it][1][0], hv[it][1][1], hv[it][1][2], hv[it][1][3]);
-        *reinterpret_cast<float4 *>(&hII[ey][x4]) = make_float4(hv[it][2][0], hv[it][2][1], hv[it][2][2], hv[it][2][3]);
-        *reinterpret_cast<float4 *>(&hIP[ey][x4]) = make_float4(hv[it][3][0], hv[it][3][1], hv[it][3][2], hv[it][3][3]);
-    }
+    float (*const S[4])[GF_PITCH] = {sI, sP, sII, sIP};
+    gf_hsums<TY, 4>(S);
     __syncthreads();
-    // vertical: 4 consecutive rows of one column per item
-    for (int i = threadIdx.x; i < (TY / 4) * GF_TX; i += NT) {
-        const int y4 = 4 * (i / GF_TX), x = i % GF_TX;
-        const int gx = x0 + x;
-        if (gx >= w) continue;
-        float vi[12], vp[12], vii[12], vip[12];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            vi[k] = hI[y4 + k][x];
-            vp[k] = hP[y4 + k][x];
-            vii[k] = hII[y4 + k][x];
-            vip[k] = hIP[y4 + k][x];
-        }
-        float si[4], spp[4], sii[4], sip[4];
-        sums4(vi, si);
-        sums4(vp, spp);
-        sums4(vii, sii);
-        sums4(vip, sip);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int gy = y0 + y4 + j;
-            if (gy >= h) break;
-            const int n = clip_count(gy, h);
-            const float cnt = (float)n, k = sRcp[n];
-            const float mI = div_count(si[j], cnt, k), mp = div_count(spp[j], cnt, k);
-            const float cI = div_count(sii[j], cnt, k), cIp = div_count(sip[j], cnt, k);
-            const float var = fmaf(-mI, mI, cI);
-            const float cov = fmaf(-mI, mp, cIp);
-            // var + eps lies in [0.0099, 1.02): I is in [0, 1]
-            const float av = div_mid(cov, var + 0.01f);
-            const float bv = fmaf(-av, mI, mp);
-            // written once, read by the second pass's next launch: nontemporal (4K: 1880 -> 1812 us)
-            __builtin_nontemporal_store(v2f{av, bv}, (GLOBAL_AS v2f *)gp((float *)J.ab) + ((size_t)gy * w + gx));
-        }
-    }
+    const bool in = x0 >= 4 && x0 + GF_TX + 4 <= w && y0 >= 4 && y0 + TY + 4 <= h;
+    if (in) gf1_vert<TY, true>(J, x0, y0, S, sRcp);
+    else gf1_vert<TY, false>(J, x0, y0, S, sRcp);
 }
 
 // RGB out = clamp(source + q) (src/wema.c:329-334).
@@ -1525,9 +1507,9 @@ __device__ __forceinline__ void write_rgb_px(const EgressJob &J, size_t idx, con
 // leaves mean(a) and mean(b) in LDS, and the output stage takes 4 consecutive pixels of a
 // row per thread, reading and writing their RGB24 as 3 dwords; otherwise each thread
 // finishes its own column of 4 rows pixel by pixel.
-// As in the first pass, the horizontal means of a and b overwrite the a, b tile once every
-// thread has read it, and (VOUT) mean(a), mean(b) overwrite those once every thread has
-// read them.
+// As in the first pass, the horizontal window sums of a and b overwrite the a, b tile once
+// every thread has read it, and (VOUT) mean(a), mean(b) overwrite those once every thread
+// has read them.
 template <bool U8, int TY, bool VOUT>
 __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, float (*sA)[GF_PITCH],
                                          float (*sB)[GF_PITCH], const float *sRcp) {
@@ -1600,58 +1582,20 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
         }
     }
     __syncthreads();
-    // windows wholly inside the frame across the tile (x) and down it (y)
-    const bool in_x = x0 >= 4 && x0 + GF_TX + 4 <= w, in_y = y0 >= 4 && y0 + TY + 4 <= h;
-    constexpr int NI = (EY * (GF_TX / 4) + NT - 1) / NT;   // horizontal items per thread
-    float hv[NI][2][4];
-    auto hpass = [&](auto inx) {
-    constexpr bool INX = decltype(inx)::value;
-#pragma unroll
-    for (int it = 0; it < NI; ++it) {
-        const int i = threadIdx.x + NT * it;
-        if (i >= EY * (GF_TX / 4)) break;
-        const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
-        float va[12], vb[12];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const float4 a = *reinterpret_cast<const float4 *>(&sA[ey][x4 + 4 * q]);
-            const float4 b = *reinterpret_cast<const float4 *>(&sB[ey][x4 + 4 * q]);
-            va[4 * q] = a.x; va[4 * q + 1] = a.y; va[4 * q + 2] = a.z; va[4 * q + 3] = a.w;
-            vb[4 * q] = b.x; vb[4 * q + 1] = b.y; vb[4 * q + 2] = b.z; vb[4 * q + 3] = b.w;
-        }
-        float sa[4], sb[4];
-        sums4(va, sa);
-        sums4(vb, sb);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            float cnt, k;
-            count_at<INX>(x0 + x4 + j, w, sRcp, cnt, k);
-            hv[it][0][j] = div_count(sa[j], cnt, k);
-            hv[it][1][j] = div_count(sb[j], cnt, k);
-        }
-    }
-    };
-    if (in_x) hpass(bool_c<true>{});
-    else hpass(bool_c<false>{});
-    __syncthreads();   // (every read of the a, b tile is done)
-#pragma unroll
-    for (int it = 0; it < NI; ++it) {
-        const int i = threadIdx.x + NT * it;
-        if (i >= EY * (GF_TX / 4)) break;
-        const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
-        *reinterpret_cast<float4 *>(&hA[ey][x4]) = make_float4(hv[it][0][0], hv[it][0][1], hv[it][0][2], hv[it][0][3]);
-        *reinterpret_cast<float4 *>(&hB[ey][x4]) = make_float4(hv[it][1][0], hv[it][1][1], hv[it][1][2], hv[it][1][3]);
-    }
+    float (*const S[2])[GF_PITCH] = {sA, sB};
+    gf_hsums<TY, 2>(S);
     __syncthreads();
+    // every window of the tile inside the frame (area 81 everywhere)
+    const bool in = x0 >= 4 && x0 + GF_TX + 4 <= w && y0 >= 4 && y0 + TY + 4 <= h;
     float mv[VOUT ? VI : 1][2][4];
-    auto vpass = [&](auto iny) {
-    constexpr bool INY = decltype(iny)::value;
+    auto vpass = [&](auto in_c) {
+    constexpr bool IN = decltype(in_c)::value;
 #pragma unroll
     for (int v = 0; v < VI; ++v) {
         const int i = threadIdx.x + NT * v;
         const int y4 = 4 * (i / GF_TX), xo = i % GF_TX;
         const int gx = x0 + xo;
-        if (i >= NV || gx >= w) break;
+        if (i >= NV || (!IN && gx >= w)) break;
         float va[12], vb[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
@@ -1661,23 +1605,23 @@ __device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, flo
         float sa[4], sb[4];
         sums4(va, sa);
         sums4(vb, sb);
+        const int cx = IN ? 9 : clip_count(gx, w);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int gy = y0 + y4 + j;
-            if (!INY && gy >= h) break;
-            float cnt, k;
-            count_at<INY>(gy, h, sRcp, cnt, k);
+            if (!IN && gy >= h) break;
+            const float k = inv_area<IN>(cx, gy, h, sRcp);
             if (VOUT) {
-                mv[v][0][j] = div_count(sa[j], cnt, k);
-                mv[v][1][j] = div_count(sb[j], cnt, k);
+                mv[v][0][j] = sa[j] * k;
+                mv[v][1][j] = sb[j] * k;
             } else {
-                const float q = fmaf(div_count(sa[j], cnt, k), src[v][j].luma(), div_count(sb[j], cnt, k));
+                const float q = fmaf(sa[j] * k, src[v][j].luma(), sb[j] * k);
                 write_rgb_px<U8>(J, (size_t)gy * w + gx, src[v][j], q);
             }
         }
     }
     };
-    if (in_y) vpass(bool_c<true>{});
+    if (in) vpass(bool_c<true>{});
     else vpass(bool_c<false>{});
     if (VOUT) {
         __syncthreads();   // (every read of the horizontal means is done)
@@ -1742,15 +1686,15 @@ __device__ __forceinline__ void gf2_run(const EgressJob *__restrict__ jobs, int 
     __global__ void __launch_bounds__(GF_NT(TY)) k_gf1_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
         __shared__ __attribute__((aligned(16))) float sI[TY + 8][GF_PITCH];                    \
         __shared__ __attribute__((aligned(16))) float sP[TY + 8][GF_PITCH];                    \
-        __shared__ __attribute__((aligned(16))) float hII[TY + 8][GF_TX];                      \
-        __shared__ __attribute__((aligned(16))) float hIP[TY + 8][GF_TX];                      \
-        __shared__ float sRcp[16];                                                             \
-        gf1_run<TY>(jobs, njobs, sI, sP, hII, hIP, sRcp);                                      \
+        __shared__ __attribute__((aligned(16))) float sII[TY + 8][GF_PITCH];                   \
+        __shared__ __attribute__((aligned(16))) float sIP[TY + 8][GF_PITCH];                   \
+        __shared__ float sRcp[GF_RCP];                                                         \
+        gf1_run<TY>(jobs, njobs, sI, sP, sII, sIP, sRcp);                                      \
     }                                                                                          \
     __global__ void __launch_bounds__(GF_NT(TY)) k_gf2_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
         __shared__ __attribute__((aligned(16))) float sA[TY + 8][GF_PITCH];                    \
         __shared__ __attribute__((aligned(16))) float sB[TY + 8][GF_PITCH];                    \
-        __shared__ float sRcp[16];                                                             \
+        __shared__ float sRcp[GF_RCP];                                                         \
         gf2_run<TY>(jobs, njobs, sA, sB, sRcp);                                                \
     }
 WEMA_GF_KERNELS(8)

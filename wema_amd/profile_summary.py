@@ -6,7 +6,11 @@ Inputs: one `--kernel-trace --stats` directory and one `--pmc FETCH_SIZE` and on
 `--pmc WRITE_SIZE` directory of the same workload, each run with bench.py
 --copy-calibration (the PMC runs), which adds a 1 GiB device copy: its counter readings
 give the scale from counter units to bytes (FETCH_SIZE and WRITE_SIZE are in KiB; the copy
-fetches and writes exactly 1 GiB).
+fetches and writes exactly 1 GiB). Optionally (--valu) a `--pmc SQ_INSTS_VALU SQ_INSTS_LDS
+SQ_INSTS_SALU SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES` directory: instructions per wave and
+the VALU issue fraction, SQ_INSTS_VALU x 4 cycles (a wave64 instruction occupies a 16-lane
+SIMD for 4 cycles) / (1024 SIMDs x 2.4 GHz x the kernel's mean duration in the trace),
+written to valu_<workload>.json, which bench.py reports beside the HBM fraction.
 
     python -m wema_amd.profile_summary --workload c2 --title "..." \\
         --trace DIR --fetch DIR --write DIR --cmd "..." --out profiles
@@ -59,6 +63,33 @@ def counters(pmc_dir, counter):
     return per
 
 
+SIMDS = 1024          # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9      # peak engine clock
+VALU_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_WAVES", "SQ_BUSY_CYCLES",
+                 "SQ_WAVE_CYCLES")
+
+
+def valu_stats(pmc_dir, stats):
+    """Per kernel: mean counter values per launch, instructions per wave, VALU issue fraction."""
+    per = {c: counters(pmc_dir, c) for c in VALU_COUNTERS}
+    dur = collections.defaultdict(list)
+    for n, g, calls, avg, mn, mx, pct in stats:
+        dur[n] += [avg] * calls
+    out = {}
+    for name in sorted(per["SQ_INSTS_VALU"]):
+        if not name.startswith("k_") or name == "k_synth" or not dur.get(name):
+            continue
+        mean = {c: sum(per[c].get(name, [0])) / max(len(per[c].get(name, [])), 1) for c in VALU_COUNTERS}
+        waves = max(mean["SQ_WAVES"], 1.0)
+        us = sum(dur[name]) / len(dur[name])
+        out[name] = {"launches": len(per["SQ_INSTS_VALU"][name]), "mean_us": round(us, 2),
+                     "waves": round(waves), "valu_per_wave": round(mean["SQ_INSTS_VALU"] / waves, 1),
+                     "lds_per_wave": round(mean["SQ_INSTS_LDS"] / waves, 1),
+                     "salu_per_wave": round(mean["SQ_INSTS_SALU"] / waves, 1),
+                     "valu_frac": round(mean["SQ_INSTS_VALU"] * 4.0 / (SIMDS * CLOCK_HZ * us * 1e-6), 4)}
+    return out
+
+
 def calibration(per, counter):
     """Bytes per counter unit, from the 1 GiB copy: the copy dispatches are the device
     copies whose reading is the largest (the other copies are small table uploads)."""
@@ -79,6 +110,9 @@ def main():
     ap.add_argument("--cmd", action="append", default=[], help="command lines that made the data")
     ap.add_argument("--out", default="profiles")
     ap.add_argument("--md-name", required=True)
+    ap.add_argument("--valu", help="--pmc run of the SQ instruction counters (optional)")
+    ap.add_argument("--pixels-per-launch", type=float, default=0.0,
+                    help="output pixels per launch, for VALU instructions per pixel")
     a = ap.parse_args()
 
     stats = kernel_stats(a.trace)
@@ -116,6 +150,24 @@ def main():
         lines.append("| %s | %d | %.1f | %.1f | %.1f |" % (
             n, t["launches"], t["fetch_bytes_per_launch"] / 1e6, t["write_bytes_per_launch"] / 1e6,
             t["hbm_bytes_per_launch"] / 1e6))
+    if a.valu:
+        vs = valu_stats(a.valu, stats)
+        with open(os.path.join(a.out, "valu_%s.json" % a.workload), "w") as f:
+            json.dump({"workload": a.workload, "simds": SIMDS, "clock_hz": CLOCK_HZ, "kernels": vs}, f, indent=1)
+            f.write("\n")
+        lines += ["", "## Instructions (rocprofv3 --pmc %s)" % " ".join(VALU_COUNTERS), "",
+                  "VALU issue fraction = SQ_INSTS_VALU x 4 cycles / (%d SIMDs x %.1f GHz x mean duration)."
+                  % (SIMDS, CLOCK_HZ / 1e9), "",
+                  "| kernel | launches | waves per launch | VALU / wave | LDS / wave | SALU / wave |%s VALU issue fraction |"
+                  % (" VALU / pixel |" if a.pixels_per_launch else ""),
+                  "|---|---:|---:|---:|---:|---:|%s---:|" % ("---:|" if a.pixels_per_launch else "")]
+        for n, v in sorted(vs.items(), key=lambda kv: -kv[1]["mean_us"]):
+            ppx = ""
+            if a.pixels_per_launch:
+                ppx = " %.1f |" % (v["valu_per_wave"] * v["waves"] / a.pixels_per_launch)
+            lines.append("| %s | %d | %d | %.1f | %.1f | %.1f |%s %.3f |" % (
+                n, v["launches"], v["waves"], v["valu_per_wave"], v["lds_per_wave"], v["salu_per_wave"],
+                ppx, v["valu_frac"]))
     lines.append("")
     with open(os.path.join(a.out, a.md_name), "w") as f:
         f.write("\n".join(lines))
