@@ -79,6 +79,17 @@ def read_traffic(kernel, workload):
         return None
 
 
+def read_valu(kernel, workload):
+    """VALU issue fraction of a kernel from profiles/valu_<workload>.json (rocprofv3 SQ
+    counters of the same workload, wema_amd/profile_summary.py), or None."""
+    path = os.path.join(ROOT, "profiles", "valu_%s.json" % workload)
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["valu_frac"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def run_gpu(cfg, name, steps, warmup, batch, tile_rows=()):
     import torch
     import wema_amd
@@ -126,6 +137,7 @@ def run_gpu(cfg, name, steps, warmup, batch, tile_rows=()):
     res["roofline"] = dict(bound="hbm", kernel=dom["name"], achieved=round(achieved, 1),
                            peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                            traffic=read_traffic(dom["name"], name),
+                           valu_frac=read_valu(dom["name"], name),
                            algorithmic_bytes_per_launch=bytes_per_launch,
                            mean_launch_us=round(avg_ms * 1e3, 2))
     tot_bytes = sum(k["bytes"] for k in kernels)
