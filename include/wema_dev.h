@@ -293,6 +293,15 @@ int wema_dev_debug_set_thresholds(wema_dev_t *s, const float *thr, int n);
  *       1: temporal filter output (IIR y / window delta), 2: amplified coefficients,
  *       3: IIR state (4 * n_unique floats, per position HP w1, HP w2, LP w1, LP w2). */
 int wema_dev_debug_read(wema_dev_t *s, int which, int frame, int ch, float *host, size_t count);
+/* Injected failures: the next `after`-th occurrence of a site fails as if its HIP call had
+ * returned an error, and the call returns -1 with the error text. site 1: the upload of a
+ * call's job tables (before anything of the call is enqueued: no frame is consumed and the
+ * session stays usable); 2: a stage launch (part of the call already enqueued: the session
+ * is marked failed, every later call on it returns -1 and it can only be closed, or removed
+ * from its pool); 3: the upload of a batch of host frames (wema_dev_submit*, before anything
+ * of the batch is enqueued). A real HIP error at the same points behaves the same way.
+ * site 0 disarms. */
+int wema_dev_debug_inject_error(int site, int after);
 /* The guided filter's covariance division q[i] = a[i] / b[i] as the kernels compute it, and
  * the hardware reciprocal estimate rcp[i] of b[i] it starts from; n elements, device
  * buffers, enqueued on the stream. */

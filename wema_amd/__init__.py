@@ -131,6 +131,7 @@ _SIGS = {
     "wema_dev_debug_set_thresholds": (C.c_int, [_vp, _vp, C.c_int]),
     "wema_dev_debug_read": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_size_t]),
     "wema_dev_debug_div_mid": (C.c_int, [_vp, _vp, _vp, _vp, C.c_long, _vp]),
+    "wema_dev_debug_inject_error": (C.c_int, [C.c_int, C.c_int]),
     "wema_pool_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_size_t]),
     "wema_pool_destroy": (None, [_vp]),
     "wema_pool_add": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(C.c_int)]),
@@ -199,6 +200,12 @@ class PinnedArray:
 
     def __del__(self):
         self.close()
+
+
+def inject_error(site, after=1):
+    """Arms an injected HIP failure (wema_dev_debug_inject_error): 1 job-table upload, 2 stage
+    launch, 3 host-frame upload; 0 disarms."""
+    _check(lib.wema_dev_debug_inject_error(int(site), int(after)), "inject_error")
 
 
 def autotune(enable):

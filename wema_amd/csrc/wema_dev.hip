@@ -46,6 +46,31 @@ void wema_set_error(const char *fmt, ...) {
 
 extern "C" const char *wema_dev_last_error(void) { return g_err; }
 
+// ---------------------------------------------------------------------------------------
+// injected failures (test hook): the armed site fails at its `after`-th occurrence
+
+namespace {
+std::mutex g_inj_mu;
+int g_inj_site = INJ_NONE;
+int g_inj_left = 0;
+}  // namespace
+
+bool wema_inject(int site) {
+    std::lock_guard<std::mutex> lk(g_inj_mu);
+    if (g_inj_site != site || g_inj_site == INJ_NONE) return false;
+    if (--g_inj_left > 0) return false;
+    g_inj_site = INJ_NONE;
+    return true;
+}
+
+extern "C" int wema_dev_debug_inject_error(int site, int after) {
+    if (site < INJ_NONE || site > INJ_FRAME_UPLOAD || after < 1) { wema_set_error("bad injection site"); return -1; }
+    std::lock_guard<std::mutex> lk(g_inj_mu);
+    g_inj_site = site;
+    g_inj_left = after;
+    return 0;
+}
+
 #define HIP_OK(call)                                                                       \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \
@@ -672,29 +697,43 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
         wema_set_error("job tables exceed %d bytes", WEMA_STAGE_BYTES);
         return -1;
     }
+    if (wema_inject(INJ_TABLE_UPLOAD)) {
+        wema_set_error("hipMemcpyAsync of the job tables failed: injected failure");
+        return -1;
+    }
     HIP_OK(hipMemcpyAsync(stage_d, stage_h, bytes, hipMemcpyHostToDevice, st));
     if (!jb.host_amp.empty())
         HIP_OK(hipMemcpyAsync(amp_d, (char *)stage_h + jb.amp_offset,
                               jb.host_amp.size() * sizeof(float), hipMemcpyHostToDevice, st));
     if (stage_ev) HIP_OK(hipEventRecord(stage_ev, st));
     char *base = (char *)stage_d;
-    if (timer_mark(timer, -1, 0, st) < 0) return -1;
+    // from here on part of the call is enqueued: a failure returns -2
+#define LAUNCH_OK(call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = wema_inject(INJ_LAUNCH) ? hipErrorLaunchFailure : (call);           \
+        if (e_ != hipSuccess) {                                                            \
+            wema_set_error("%s failed: %s%s", #call, hipGetErrorString(e_),                \
+                           " (the session's state is now unknown)");                     \
+            return -2;                                                                     \
+        }                                                                                  \
+    } while (0)
+    if (timer_mark(timer, -1, 0, st) < 0) return -2;
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
         const JobList &jl = jb.dwtf[l];
         if (!jl.count) continue;
-        HIP_OK(wema_launch_dwt_fwd((const DwtfJob *)(base + jl.offset), jl.count, jl.blocks, l == 0, st));
-        if (timer_mark(timer, l == 0 ? TK_DWT_FWD0 : TK_DWT_FWD, jl.bytes, st) < 0) return -1;
+        LAUNCH_OK(wema_launch_dwt_fwd((const DwtfJob *)(base + jl.offset), jl.count, jl.blocks, l == 0, st));
+        if (timer_mark(timer, l == 0 ? TK_DWT_FWD0 : TK_DWT_FWD, jl.bytes, st) < 0) return -2;
     }
     const JobList &tl = jb.temporal;
     if (tl.count) {
-        HIP_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
-        if (timer_mark(timer, TK_IIR, tl.bytes, st) < 0) return -1;
+        LAUNCH_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
+        if (timer_mark(timer, TK_IIR, tl.bytes, st) < 0) return -2;
     }
     for (int v = 0; v < WEMA_WIN_KINDS; v++) {
         const JobList &wl = jb.window[v];
         if (!wl.count) continue;
-        HIP_OK(wema_launch_win(v, (const TemporalJob *)(base + wl.offset), wl.count, wl.blocks, st));
-        if (timer_mark(timer, TK_WIN, wl.bytes, st) < 0) return -1;
+        LAUNCH_OK(wema_launch_win(v, (const TemporalJob *)(base + wl.offset), wl.count, wl.blocks, st));
+        if (timer_mark(timer, TK_WIN, wl.bytes, st) < 0) return -2;
     }
     // inverse stages, coarsest first: level l of every stream is stage WEMA_MAX_LEVELS-1-l,
     // so the last stage holds exactly the level-0 (full-resolution) jobs
@@ -702,30 +741,32 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
         const JobList &jl = jb.dwti[l];
         if (!jl.count) continue;
-        HIP_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, l == last_inv, st));
-        if (timer_mark(timer, l == last_inv ? TK_DWT_INV0 : TK_DWT_INV, jl.bytes, st) < 0) return -1;
+        LAUNCH_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, l == last_inv, st));
+        if (timer_mark(timer, l == last_inv ? TK_DWT_INV0 : TK_DWT_INV, jl.bytes, st) < 0) return -2;
     }
     if (jb.dwti_ycc.count) {
         const JobList &jl = jb.dwti_ycc;
-        HIP_OK(wema_launch_dwt_inv_ycc((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, st));
-        if (timer_mark(timer, TK_DWT_INV0_YCC, jl.bytes, st) < 0) return -1;
+        LAUNCH_OK(wema_launch_dwt_inv_ycc((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, st));
+        if (timer_mark(timer, TK_DWT_INV0_YCC, jl.bytes, st) < 0) return -2;
     }
     static const int eg_kind[EG_KINDS] = {TK_GF1, TK_GF1, TK_GF1, TK_GF2, TK_GF2, TK_GF2, TK_GAUSS, TK_PASSTHRU};
     for (int k = 0; k < EG_KINDS; k++) {
         const JobList &jl = jb.egress[k];
         if (!jl.count) continue;
         const int kind = k < EG_GAUSS ? (k < EG_GF2 ? EG_GF1 : EG_GF2) : k;
-        HIP_OK(wema_launch_egress(kind, wema_index_rows((k - kind) % 3), (const EgressJob *)(base + jl.offset),
+        LAUNCH_OK(wema_launch_egress(kind, wema_index_rows((k - kind) % 3), (const EgressJob *)(base + jl.offset),
                                   jl.count, jl.blocks, st));
-        if (timer_mark(timer, eg_kind[k], jl.bytes, st) < 0) return -1;
+        if (timer_mark(timer, eg_kind[k], jl.bytes, st) < 0) return -2;
     }
     return 0;
+#undef LAUNCH_OK
 }
 
 static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, float *out_f32,
                    int n, void *stream, const CallBufs *bufs = nullptr) {
     if (!s) { wema_set_error("null session"); return -1; }
     if (s->pooled) { wema_set_error("a pooled session is processed by wema_pool_tick"); return -1; }
+    if (s->failed) { wema_set_error("the session failed in an earlier call; close it"); return -1; }
     if (n < 1 || n > s->p.max_batch) {
         wema_set_error("frame count %d outside 1..%d", n, s->p.max_batch);
         return -1;
@@ -742,8 +783,11 @@ static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, flo
     JobBuilder jb;
     wema_build_jobs(s, jb, in, in_f32, out_u8, out_f32, n, bufs);
     jb.fix_amp(s->amp_d);
-    if (wema_run_jobs(jb, s->stage_h[k], s->stage_d[k], s->stage_ev[k], s->amp_d, st, &s->timer) < 0)
+    const int rc = wema_run_jobs(jb, s->stage_h[k], s->stage_d[k], s->stage_ev[k], s->amp_d, st, &s->timer);
+    if (rc < 0) {
+        s->failed = rc == -2;
         return -1;
+    }
     s->frames += n;
     s->last_n = n;
     s->thr_ovr_active = 0;
@@ -1003,6 +1047,11 @@ static int submit_frames(wema_dev *s, const void *const *h_in, void *const *h_ou
             HIP_OK(hipMalloc(&a.d_in_f, cap));
             HIP_OK(hipMalloc(&a.d_out_f, cap));
         }
+    }
+    if (s->failed) { wema_set_error("the session failed in an earlier call; close it"); return -1; }
+    if (wema_inject(INJ_FRAME_UPLOAD)) {
+        wema_set_error("hipMemcpyAsync of the frames failed: injected failure");
+        return -1;
     }
     const long t = s->next_ticket++;
     auto &a = s->aslot[t % WEMA_ASYNC_SLOTS];

@@ -113,6 +113,8 @@ struct wema_dev {
     int stage_cur = 0;
     LaunchTimer timer;
     bool pooled = false;        // owned by a wema_pool (no staging buffers of its own)
+    bool failed = false;        // a call failed after enqueuing part of its stages: the
+                                // session's state is unknown and every later call fails
     void *io_in = nullptr, *io_out = nullptr;           // host-buffer staging (RGB24)
     float *io_in_f32 = nullptr, *io_out_f32 = nullptr;  // host-buffer staging (float RGB)
     // asynchronous host pipeline: WEMA_ASYNC_SLOTS device slots of max_batch RGB24 frames in
@@ -255,6 +257,12 @@ void wema_gf_rows_for(int device, int w, int h, int *rows1, int *rows2);
 void wema_geometry_init(WemaGeom *g, const wema_dev_params *p);
 void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,
                      float *out_f32, int n, const CallBufs *bufs = nullptr);
+// Injected failures (test hook, wema_dev_debug_inject_error): true when the armed site is
+// reached for the armed time.
+enum InjectSite { INJ_NONE = 0, INJ_TABLE_UPLOAD = 1, INJ_LAUNCH = 2, INJ_FRAME_UPLOAD = 3 };
+bool wema_inject(int site);
+// Launches all stages of a builder; returns 0, -1 if it failed before enqueuing anything,
+// or -2 if it failed after enqueuing some stages.
 int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage_ev,
                   float *amp_d, hipStream_t st, LaunchTimer *timer);
 int wema_timer_collect(LaunchTimer *t);

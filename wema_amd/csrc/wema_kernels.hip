@@ -210,7 +210,10 @@ template <> struct StripRaw<FAM_F32_RGB> { float ra, ga, ba, rb, gb, bb; };
 template <> struct StripRaw<FAM_F32_PLANE> { float a, b; };
 
 // Input pairs a strip keeps in flight.
-#define SLOTS_OF(NCH) ((NCH) == 1 ? 4 : 2)
+#ifndef WEMA_FSLOTS_COLOUR
+#define WEMA_FSLOTS_COLOUR 2
+#endif
+#define SLOTS_OF(NCH) ((NCH) == 1 ? 4 : WEMA_FSLOTS_COLOUR)
 
 // Per-lane constants of a strip.
 struct StripLane {
@@ -635,7 +638,10 @@ struct IStripRaw {
     float ll[NCH], lh[NCH], hl[NCH], hh[NCH];
 };
 
-#define ISLOTS_OF(NCH) ((NCH) == 1 ? 4 : 2)
+#ifndef WEMA_ISLOTS_COLOUR
+#define WEMA_ISLOTS_COLOUR 2
+#endif
+#define ISLOTS_OF(NCH) ((NCH) == 1 ? 4 : WEMA_ISLOTS_COLOUR)
 
 // OUT: 0 = one float plane (J.out), 1 = colour egress (RGB24 and/or float RGB).
 template <int NCH, int OUT, bool FAST>

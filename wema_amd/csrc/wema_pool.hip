@@ -224,6 +224,7 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
     for (int i = 0; i < n; i++) {
         ss[(size_t)i] = get(p, ids[i]);
         if (!ss[(size_t)i]) return -1;
+        if (ss[(size_t)i]->failed) { wema_set_error("session %d failed in an earlier tick; remove it", ids[i]); return -1; }
         for (int j = 0; j < i; j++)
             if (ids[j] == ids[i]) { wema_set_error("session %d twice in one tick", ids[i]); return -1; }
         key += "|" + std::to_string(ids[i]) + ":" + std::to_string(egress_kind(ss[(size_t)i]));
@@ -247,7 +248,11 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
         p->direct_cur ^= 1;
         HIP_OK(hipEventSynchronize(b.ev));
         jb.fix_amp(b.amp);
-        if (wema_run_jobs(jb, b.h, b.d, b.ev, b.amp, st, &p->timer) < 0) return -1;
+        const int rc = wema_run_jobs(jb, b.h, b.d, b.ev, b.amp, st, &p->timer);
+        if (rc < 0) {
+            for (wema_dev *s : ss) s->failed = s->failed || rc == -2;
+            return -1;
+        }
     } else {
         auto it = p->cache.find(key);
         if (it == p->cache.end()) {
