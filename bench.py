@@ -460,7 +460,7 @@ def main():
     args.tile_rows = [] if tr == [0] else tr
     if args.tile_rows:
         import wema_amd
-        wema_amd.autotune(False)
+        wema_amd.autotune(False, dwt=True)   # (fixed guided-filter tiles; the strips still tuned)
     cfg = CONFIGS[args.config]
     batch = args.batch or cfg["batch"]
     steps = PER_FRAME_STEPS.get(args.config, args.steps) if batch == 1 and not args.batch else args.steps

@@ -163,12 +163,22 @@ int wema_dev_import_state(wema_dev_t *s, const void *host, size_t bytes);
  * heights of 8, 16 and 32 rows (64 columns). A session whose output stage is the guided
  * filter takes, for each of the filter's two passes, the height an autotuner picked for
  * its frame size: the first session of a size on a device times the three on frames of
- * that size and keeps the fastest of each pass for the process. Results never depend on
- * the tile shape. wema_dev_autotune(0) turns the timing off (16 rows).
- * wema_dev_tile_rows: the first pass's height; wema_dev_tile_rows_pass: pass 1 or 2
- * (-1 for another pass). wema_dev_set_tile_rows forces one height for both passes of the
- * session's next calls, wema_dev_set_tile_rows_pass for pass 1, 2 or both (0). */
-void wema_dev_autotune(int enable);
+ * that size and keeps the fastest of each pass for the process. The DWT levels run as
+ * strips 8, 16 or 32 pair rows tall: the first session of a shape (level size, channels,
+ * frames per call) on a device times the three heights on its own buffers when it is
+ * opened and keeps the fastest for each level (wema_dev_strip_rows: a level's forward or
+ * inverse height, 0 = the fixed rule). Results never depend on either shape.
+ * wema_dev_autotune(mask) selects the tuners for sessions opened later: WEMA_TUNE_GF,
+ * WEMA_TUNE_DWT (both by default; 0: the guided filter takes 16 rows and the strips the
+ * fixed rule). wema_dev_tile_rows: the first pass's height; wema_dev_tile_rows_pass: pass 1
+ * or 2 (-1 for another pass). wema_dev_set_tile_rows forces one height for both passes of
+ * the session's next calls, wema_dev_set_tile_rows_pass for pass 1, 2 or both (0);
+ * wema_dev_set_strip_rows one strip height for every level (0 = the rule). */
+#define WEMA_TUNE_GF 1
+#define WEMA_TUNE_DWT 2
+void wema_dev_autotune(int mask);
+int  wema_dev_strip_rows(const wema_dev_t *s, int level, int inverse);
+int  wema_dev_set_strip_rows(wema_dev_t *s, int rows);
 int  wema_dev_tile_rows(const wema_dev_t *s);
 int  wema_dev_tile_rows_pass(const wema_dev_t *s, int pass);
 int  wema_dev_set_tile_rows(wema_dev_t *s, int rows);

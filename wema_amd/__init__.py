@@ -107,6 +107,8 @@ _SIGS = {
     "wema_dev_set_tile_rows": (C.c_int, [_vp, C.c_int]),
     "wema_dev_set_tile_rows_pass": (C.c_int, [_vp, C.c_int, C.c_int]),
     "wema_dev_autotune": (None, [C.c_int]),
+    "wema_dev_strip_rows": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "wema_dev_set_strip_rows": (C.c_int, [_vp, C.c_int]),
     "wema_dev_process_u8": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "wema_dev_process_u8_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "wema_dev_process_f32": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
@@ -208,9 +210,11 @@ def inject_error(site, after=1):
     _check(lib.wema_dev_debug_inject_error(int(site), int(after)), "inject_error")
 
 
-def autotune(enable):
-    """Turns the guided-filter tile-shape autotuner on or off for sessions opened later."""
-    lib.wema_dev_autotune(int(bool(enable)))
+def autotune(enable=True, dwt=None):
+    """Turns the autotuners on or off for sessions opened later: the guided filter's tile
+    heights (enable) and the DWT strip heights (dwt; the same as enable when not given)."""
+    dwt = enable if dwt is None else dwt
+    lib.wema_dev_autotune((1 if enable else 0) | (2 if dwt else 0))
 
 
 def design_bandpass(f_low, f_high, fs):
@@ -292,6 +296,14 @@ class Session:
     def set_tile_rows(self, rows, pass_=0):
         """Force the tile height of pass 1, pass 2 or both (0)."""
         _check(lib.wema_dev_set_tile_rows_pass(self._h, int(pass_), int(rows)), "wema_dev_set_tile_rows_pass")
+
+    def strip_rows(self, inverse=False):
+        """DWT strip height (pair rows) of each level, forward or inverse (0: the fixed rule)."""
+        return [lib.wema_dev_strip_rows(self._h, l, int(inverse)) for l in range(self.levels)]
+
+    def set_strip_rows(self, rows):
+        """Force one DWT strip height (8, 16 or 32 pair rows; 0: the fixed rule) for every level."""
+        _check(lib.wema_dev_set_strip_rows(self._h, int(rows)), "wema_dev_set_strip_rows")
 
     def iir_coeffs(self):
         c = (C.c_float * 10)()

@@ -345,6 +345,10 @@ int wema_dev_open_ex(wema_dev_t **out, const wema_dev_params *p, bool pooled) {
     scratch_setup(p->device);   // (the session's device is current)
     if (!p->color && (p->edge_aware || p->batch_path))
         wema_gf_rows_for(p->device, p->width, p->height, &s->gf_rows, &s->gf_rows2);
+    if (wema_dwt_tune(s) < 0) {
+        wema_dev_close(s);
+        return -1;
+    }
     *out = s;
     return 0;
 }
@@ -516,6 +520,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.w = w;
                 j.h = h;
                 j.tiles_x = tiles_x;
+                j.sp = s->sp_fwd[l];
                 // input read once (RGB24 shared by the channels at level 0) + LL and three
                 // subbands written per channel
                 const double in_b = l == 0 ? 3.0 * w * h * (in_f32 ? 4 : 1) : 4.0 * w * h;
@@ -600,6 +605,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.w = w;
                 j.h = h;
                 j.tiles_x = tiles_x;
+                j.sp = s->sp_inv[0];
                 jb.add_dwti_ycc(j, tiles, in_b * nch + (double)out_b);
                 continue;
             }
@@ -613,6 +619,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.w = w;
                 j.h = h;
                 j.tiles_x = tiles_x;
+                j.sp = s->sp_inv[l];
                 jb.add_dwti(WEMA_MAX_LEVELS - 1 - l, j, tiles, in_b + 4.0 * w * h);
             }
         }

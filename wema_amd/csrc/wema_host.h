@@ -87,6 +87,8 @@ struct wema_dev {
     int ring_slots = 0;
     int gf_rows = 16;       // guided-filter tile height of the first pass (autotuned at open)
     int gf_rows2 = 16;      // and of the second pass
+    int sp_fwd[WEMA_MAX_LEVELS] = {0};   // DWT strip height (pair rows) of each forward level
+    int sp_inv[WEMA_MAX_LEVELS] = {0};   // and inverse level (autotuned at open; 0: the rule)
     long frames = 0;        // frames processed since open
     int last_n = 0;         // frames of the last call
     hipStream_t stream = nullptr;       // the session's own stream (host-buffer calls)
@@ -201,9 +203,11 @@ struct JobBuilder {
         off += bytes;
         return true;
     }
-    // Strip height of each DWT launch, from all of its jobs: the tallest of 32, 16 and 8
-    // pair rows that still gives the target count of channel-strips (shorter strips re-read
-    // more halo rows, taller ones leave the chip under-filled); block ranges follow.
+    // Strip height of each DWT job: the height its session's autotuner picked for that level
+    // (sp set), else one height for the rest of the launch, the tallest of 32, 16 and 8 pair
+    // rows that still gives the target count of channel-strips (shorter strips re-read more
+    // halo rows, taller ones leave the chip under-filled); block ranges follow. Results
+    // never depend on the strip height.
     template <typename Job>
     static void strips_of(std::vector<Job> &v, JobList &jl, long target) {
         if (v.empty()) return;
@@ -211,14 +215,14 @@ struct JobBuilder {
         int sp = WEMA_SP_MAX;
         for (; sp > WEMA_SP_MIN; sp /= 2) {
             long work = 0;   // channel-strips: a wave of a colour job computes 3 channels
-            for (const auto &j : v) work += strips(j, sp) * (j.nch > 1 ? j.nch : 1);
+            for (const auto &j : v) work += strips(j, j.sp ? j.sp : sp) * (j.nch > 1 ? j.nch : 1);
             if (work >= target) break;
         }
         int blocks = 0;
         for (auto &j : v) {
-            j.sp = sp;
+            if (!j.sp) j.sp = sp;
             j.tile_begin = blocks;
-            blocks += wema_ceil_div((int)strips(j, sp), WEMA_STRIPS_PER_BLOCK);
+            blocks += wema_ceil_div((int)strips(j, j.sp), WEMA_STRIPS_PER_BLOCK);
         }
         jl.blocks = blocks;
     }
@@ -254,6 +258,7 @@ int wema_design_bandpass(float f_low, float f_high, float fs, float *c10, int *w
 float wema_fade_amp(float amp, long global_frame);
 int wema_auto_levels(int width, int height);
 void wema_gf_rows_for(int device, int w, int h, int *rows1, int *rows2);
+int wema_dwt_tune(wema_dev *s);
 void wema_geometry_init(WemaGeom *g, const wema_dev_params *p);
 void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,
                      float *out_f32, int n, const CallBufs *bufs = nullptr);
