@@ -1,6 +1,6 @@
 """Builds the gfx950 pipeline library wema_amd/libwema_dev.so with hipcc, and the C11 host
-library wema_amd/libwema.so (the include/wema.h API over it) and its benchmark driver
-wema_amd/wema_h_bench with gcc.
+library wema_amd/libwema.so (the include/wema.h API over it), its benchmark driver
+wema_amd/wema_h_bench and the command-line front end wema_amd/wema with gcc.
 
 Usage: python wema_amd/build.py   (a standalone script: importing the package needs the
 library it builds)
@@ -68,6 +68,17 @@ def build_host(verbose=False):
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("gcc failed building libwema.so")
     os.replace(tmp, HOST_LIB)
+    # the command-line front end (the reference CLI's option set, FFmpeg pipes)
+    cli = os.path.join(HERE, "wema")
+    cmd = ["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-I" + INCLUDE, os.path.join(HERE, "host", "wema_cli.c"),
+           "-L" + HERE, "-lwema_dev", "-Wl,-rpath,$ORIGIN", "-o", cli + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("gcc failed building the wema CLI")
+    os.replace(cli + ".tmp", cli)
     # the drop-in benchmark (bench.py runs it): the wema.h path driven from C
     exe = os.path.join(HERE, "wema_h_bench")
     cmd = ["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-I" + INCLUDE,
