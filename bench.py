@@ -90,7 +90,7 @@ def read_valu(kernel, workload):
         return None
 
 
-def run_gpu(cfg, name, steps, warmup, batch, tile_rows=()):
+def run_gpu(cfg, name, steps, warmup, batch, gf_seg=0):
     import torch
     import wema_amd
 
@@ -105,9 +105,8 @@ def run_gpu(cfg, name, steps, warmup, batch, tile_rows=()):
                             f_high=cfg["f_high"], window=cfg["window"], levels=cfg["levels"],
                             use_iir=cfg["use_iir"], color=cfg["color"],
                             bilateral=cfg["bilateral"], max_batch=batch)
-    if tile_rows:
-        sess.set_tile_rows(tile_rows[0], 1)
-        sess.set_tile_rows(tile_rows[-1], 2)
+    if gf_seg:
+        sess.set_tile_rows(gf_seg)
     torch.cuda.synchronize()
     for k in range(warmup):
         sess.process_device(frames[k * batch:(k + 1) * batch], d_out_u8=out[k * batch:(k + 1) * batch], n=batch)
@@ -125,11 +124,11 @@ def run_gpu(cfg, name, steps, warmup, batch, tile_rows=()):
     ms = e0.elapsed_time(e1)
     kernels = sess.timing_report()
     sess.timing(False)
-    tile_rows = sess.tile_rows
+    seg = sess.tile_rows
     frames_timed = steps * batch
     mpix = frames_timed * w * h / 1e6
     res = dict(ms=ms, wall_ms=t_wall * 1e3, frames=frames_timed, mpix_s=mpix / (ms / 1e3),
-               fps=frames_timed / (ms / 1e3), kernels=kernels, tile_rows=tile_rows)
+               fps=frames_timed / (ms / 1e3), kernels=kernels, gf_seg=seg)
     dom = max(kernels, key=lambda k: k["ms"])
     avg_ms = dom["ms"] / dom["launches"]
     bytes_per_launch = dom["bytes"] / dom["launches"]
@@ -424,10 +423,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the multi-stream and serving measurements")
     ap.add_argument("--serve-duration", type=float, default=4.0, help="seconds per serving run")
-    ap.add_argument("--tile-rows", default="0",
-                    help="guided-filter tile height, R or R1,R2 for the two passes, each 8, 16 or "
-                         "32 (0: the autotuner's pick; fixed heights skip the tuner's timing "
-                         "launches, e.g. under a profiler)")
+    ap.add_argument("--gf-seg", type=int, default=0,
+                    help="guided-filter segment height in rows, a multiple of 4 (0: the "
+                         "autotuner's pick; a fixed height skips the tuner's timing launches, "
+                         "e.g. under a profiler)")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the pinned-host pipeline and drop-in wema.h measurements "
                          "(e.g. under a profiler)")
@@ -453,18 +452,16 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
 
-    tr = [int(x) for x in str(args.tile_rows).split(",")]
-    if len(tr) > 2 or any(x not in (0, 8, 16, 32) for x in tr) or (0 in tr and tr != [0]):
-        print("--tile-rows: R or R1,R2 with each 8, 16 or 32", file=sys.stderr)
+    if args.gf_seg < 0 or args.gf_seg % 4:
+        print("--gf-seg: a multiple of 4 (0: the autotuner's pick)", file=sys.stderr)
         sys.exit(2)
-    args.tile_rows = [] if tr == [0] else tr
-    if args.tile_rows:
+    if args.gf_seg:
         import wema_amd
-        wema_amd.autotune(False, dwt=True)   # (fixed guided-filter tiles; the strips still tuned)
+        wema_amd.autotune(False, dwt=True)   # (a fixed guided-filter segment; the strips still tuned)
     cfg = CONFIGS[args.config]
     batch = args.batch or cfg["batch"]
     steps = PER_FRAME_STEPS.get(args.config, args.steps) if batch == 1 and not args.batch else args.steps
-    r = run_gpu(cfg, args.config, steps, args.warmup, batch, args.tile_rows)
+    r = run_gpu(cfg, args.config, steps, args.warmup, batch, args.gf_seg)
     r["roofline"]["state_model"] = state_model(cfg, batch)
     log("%s: %.1f Mpixel/s" % (args.config, r["mpix_s"]))
     line = {
@@ -482,7 +479,7 @@ def main():
         "data": "synthetic (deterministic generator, frames resident in HBM)",
         "config": {"workload": args.config, "desc": cfg["desc"], "width": cfg["w"],
                    "height": cfg["h"], "frames_per_step": batch, "levels": cfg["levels"],
-                   "gf_tile_rows": r["tile_rows"] if not cfg["color"] else None},
+                   "gf_seg_rows": r["gf_seg"] if not cfg["color"] else None},
         "fps": round(r["fps"], 1),
         "pipeline_algorithmic_GBs": r["pipeline_gbs"],
         "roofline": r["roofline"],
@@ -502,13 +499,13 @@ def main():
     for name in [a for a in args.also.split(",") if a and a != args.config]:
         c = CONFIGS[name]
         st_n = PER_FRAME_STEPS.get(name, args.steps)
-        rr = run_gpu(c, name, st_n, args.warmup * (8 if c["batch"] == 1 else 1), c["batch"], args.tile_rows)
+        rr = run_gpu(c, name, st_n, args.warmup * (8 if c["batch"] == 1 else 1), c["batch"], args.gf_seg)
         rr["roofline"]["state_model"] = state_model(c, c["batch"])
         log("%s: %.1f Mpixel/s" % (name, rr["mpix_s"]))
         also[name] = {"desc": c["desc"], "value": round(rr["mpix_s"], 1), "unit": "Mpixel/s",
                       "fps": round(rr["fps"], 1), "steps": st_n, "ms_per_step": round(rr["ms"] / st_n, 3),
                       "frames_per_step": c["batch"], "pipeline_algorithmic_GBs": rr["pipeline_gbs"],
-                      "gf_tile_rows": rr["tile_rows"] if not c["color"] else None,
+                      "gf_seg_rows": rr["gf_seg"] if not c["color"] else None,
                       "roofline": rr["roofline"],
                       "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
                                   for d in rr["kernels"]]}

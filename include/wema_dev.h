@@ -159,20 +159,22 @@ int wema_dev_state_bytes(const wema_dev_t *s, size_t *bytes);
 int wema_dev_export_state(wema_dev_t *s, void *host, size_t bytes);
 int wema_dev_import_state(wema_dev_t *s, const void *host, size_t bytes);
 
-/* Tile shapes (no counterpart in the reference). The guided-filter kernels come in tile
- * heights of 8, 16 and 32 rows (64 columns). A session whose output stage is the guided
- * filter takes, for each of the filter's two passes, the height an autotuner picked for
- * its frame size: the first session of a size on a device times the three on frames of
- * that size and keeps the fastest of each pass for the process. The DWT levels run as
- * strips 8, 16 or 32 pair rows tall: the first session of a shape (level size, channels,
- * frames per call) on a device times the three heights on its own buffers when it is
- * opened and keeps the fastest for each level (wema_dev_strip_rows: a level's forward or
- * inverse height, 0 = the fixed rule). Results never depend on either shape.
+/* Tile shapes (no counterpart in the reference). The guided filter runs as one kernel that
+ * streams strips of 240 output columns down segments of rows, each segment recomputing the
+ * 16 rows above it. A plain session whose output stage is the guided filter takes the
+ * segment height an autotuner picked for its shape: the first session of a frame size and
+ * frames per call on a device times the strips split into 1, 2, 4, .. 64 equal segments on
+ * its own buffers when it is opened and keeps the fastest for the process. Pooled sessions,
+ * and sessions opened with that tuner off, leave the split to each launch (the fewest
+ * segments that give the launch 2048 workgroups). The DWT levels run as strips 8, 16 or 32
+ * pair rows tall: the first session of a shape (level size, channels, frames per call) on a
+ * device times the three heights on its own buffers when it is opened and keeps the
+ * fastest for each level (wema_dev_strip_rows: a level's forward or inverse height, 0 = the
+ * fixed rule). Results never depend on either shape.
  * wema_dev_autotune(mask) selects the tuners for sessions opened later: WEMA_TUNE_GF,
- * WEMA_TUNE_DWT (both by default; 0: the guided filter takes 16 rows and the strips the
- * fixed rule). wema_dev_tile_rows: the first pass's height; wema_dev_tile_rows_pass: pass 1
- * or 2 (-1 for another pass). wema_dev_set_tile_rows forces one height for both passes of
- * the session's next calls, wema_dev_set_tile_rows_pass for pass 1, 2 or both (0);
+ * WEMA_TUNE_DWT (both by default; 0: the launch's split and the strips' fixed rule).
+ * wema_dev_tile_rows: the session's guided-filter segment height (0: the launch's split);
+ * wema_dev_set_tile_rows forces one (a positive multiple of 4, or 0) for its next calls;
  * wema_dev_set_strip_rows one strip height for every level (0 = the rule). */
 #define WEMA_TUNE_GF 1
 #define WEMA_TUNE_DWT 2
@@ -180,9 +182,7 @@ void wema_dev_autotune(int mask);
 int  wema_dev_strip_rows(const wema_dev_t *s, int level, int inverse);
 int  wema_dev_set_strip_rows(wema_dev_t *s, int rows);
 int  wema_dev_tile_rows(const wema_dev_t *s);
-int  wema_dev_tile_rows_pass(const wema_dev_t *s, int pass);
 int  wema_dev_set_tile_rows(wema_dev_t *s, int rows);
-int  wema_dev_set_tile_rows_pass(wema_dev_t *s, int pass, int rows);
 
 /* Pinned host memory for frame slots (hipHostMalloc). */
 void *wema_dev_host_alloc(size_t bytes);

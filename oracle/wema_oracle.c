@@ -485,13 +485,13 @@ static void box_prefix_line_f32(const float *v, size_t st, int n, int r, float *
     }
 }
 
-/* Window sums of one line of n samples (stride st), radius r, samples outside [0, n) taken
- * as 0, in the order the GPU kernels use: the line is cut into blocks of 4 outputs starting
+/* Horizontal window sums of one row of n samples, radius r, samples outside [0, n) taken
+ * as 0, in the order the GPU kernel uses: the row is cut into blocks of 4 outputs starting
  * at multiples of 4; the first sum of a block adds its window's 2r + 1 samples left to
  * right, starting from the first, each next one is (previous - leaving sample) + entering
  * sample. */
-static void box_sum_line_f32(const float *v, size_t st, int n, int r, float *out, size_t ost) {
-#define VAL(i) ((i) >= 0 && (i) < n ? v[(size_t)(i) * st] : 0.0f)
+static void box_sum_line_f32(const float *v, int n, int r, float *out) {
+#define VAL(i) ((i) >= 0 && (i) < n ? v[(i)] : 0.0f)
     float s = 0.0f;
     for (int x = 0; x < n; x++) {
         if (x % 4 == 0) {
@@ -500,9 +500,43 @@ static void box_sum_line_f32(const float *v, size_t st, int n, int r, float *out
         } else {
             s = (s - VAL(x - r - 1)) + VAL(x + r);
         }
-        out[(size_t)x * ost] = s;
+        out[x] = s;
     }
 #undef VAL
+}
+
+/* Prefix sums P[1..4] and suffix sums U[1..4] of the group of rows 4g..4g+3 of a column
+ * (stride st, n rows, rows outside [0, n) are 0): P1 = h0, P2 = P1 + h1, P3 = P2 + h2,
+ * P4 = P3 + h3 (the group's sum F); U1 = h3, U2 = h2 + h3, U3 = h1 + U2, U4 = F. */
+static void group_sums_f32(const float *v, size_t st, int n, int g, float P[5], float U[5]) {
+    float hv[4];
+    for (int i = 0; i < 4; i++) {
+        const int y = 4 * g + i;
+        hv[i] = y >= 0 && y < n ? v[(size_t)y * st] : 0.0f;
+    }
+    P[1] = hv[0];
+    P[2] = P[1] + hv[1];
+    P[3] = P[2] + hv[2];
+    P[4] = P[3] + hv[3];
+    U[1] = hv[3];
+    U[2] = hv[2] + hv[3];
+    U[3] = hv[1] + U[2];
+    U[4] = P[4];
+}
+
+/* Vertical window sums (radius 4) of one column of n rows in the order the GPU kernel
+ * streams them, one group of 4 rows at a time: the window of row 4b + j (j = 0..3) covers
+ * the last 4 - j rows of group b - 1, all of group b and the first j + 1 rows of group
+ * b + 1, and sums to (U_{b-1}(4 - j) + F_b) + P_{b+1}(j + 1). */
+static void box_vsum_groups_f32(const float *v, size_t st, int n, float *out, size_t ost) {
+    for (int b = 0; 4 * b < n; b++) {
+        float Pa[5], Ua[5], Pb[5], Ub[5], Pc[5], Uc[5];
+        group_sums_f32(v, st, n, b - 1, Pa, Ua);
+        group_sums_f32(v, st, n, b, Pb, Ub);
+        group_sums_f32(v, st, n, b + 1, Pc, Uc);
+        for (int j = 0; j < 4 && 4 * b + j < n; j++)
+            out[(size_t)(4 * b + j) * ost] = (Ua[4 - j] + Pb[4]) + Pc[j + 1];
+    }
 }
 
 /* Clipped window count at x of n (src/wema.c:144-147). */
@@ -512,11 +546,14 @@ static int clip_count(int x, int n, int r) {
 }
 
 /* Box mean over the (2r + 1)^2 window clipped to the frame. prefix: the reference's order
- * (above). Otherwise the kernels': the window sum as the vertical window sum of horizontal
- * window sums (box_sum_line_f32), times the correctly rounded reciprocal of the clipped
- * window's area; in real arithmetic the same mean as the reference's mean of means. */
-void orc_box_f32(const float *in, float *out, int w, int h, int r, int prefix) {
+ * (above), any r. Otherwise the kernel's, r = 4 (the guided filter's radius,
+ * src/wema.c:173): the window sum as the vertical window sum (box_vsum_groups_f32) of
+ * horizontal window sums (box_sum_line_f32), times the correctly rounded reciprocal of the
+ * clipped window's area; in real arithmetic the same mean as the reference's mean of
+ * means. Returns -1 for another radius in the kernel's order. */
+int orc_box_f32(const float *in, float *out, int w, int h, int r, int prefix) {
     FP_REFERENCE_MODE();
+    if (!prefix && r != 4) return -1;
     size_t P = (size_t)w * h;
     float *tmp = malloc(P * sizeof(float));
     float *pre = malloc((size_t)((w > h ? w : h) + 1) * sizeof(float));
@@ -525,8 +562,8 @@ void orc_box_f32(const float *in, float *out, int w, int h, int r, int prefix) {
             box_prefix_line_f32(in + (size_t)y * w, 1, w, r, pre, tmp + (size_t)y * w, 1);
         for (int x = 0; x < w; x++) box_prefix_line_f32(tmp + x, (size_t)w, h, r, pre, out + x, (size_t)w);
     } else {
-        for (int y = 0; y < h; y++) box_sum_line_f32(in + (size_t)y * w, 1, w, r, tmp + (size_t)y * w, 1);
-        for (int x = 0; x < w; x++) box_sum_line_f32(tmp + x, (size_t)w, h, r, out + x, (size_t)w);
+        for (int y = 0; y < h; y++) box_sum_line_f32(in + (size_t)y * w, w, r, tmp + (size_t)y * w);
+        for (int x = 0; x < w; x++) box_vsum_groups_f32(tmp + x, (size_t)w, h, out + x, (size_t)w);
         for (int y = 0; y < h; y++)
             for (int x = 0; x < w; x++) {
                 const float k = 1.0f / (float)(clip_count(x, w, r) * clip_count(y, h, r));
@@ -535,6 +572,7 @@ void orc_box_f32(const float *in, float *out, int w, int h, int r, int prefix) {
     }
     free(tmp);
     free(pre);
+    return 0;
 }
 
 void orc_box_f64(const double *in, double *out, int w, int h, int r) {

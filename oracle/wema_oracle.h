@@ -53,7 +53,7 @@ uint64_t orc_abs_fixed(float v);
 float    orc_threshold_from_fixed(uint64_t sum_unique, size_t n_unique);
 float    orc_fade_amp(float amp, int global_frame);
 
-void orc_box_f32(const float *in, float *out, int w, int h, int r, int prefix);
+int orc_box_f32(const float *in, float *out, int w, int h, int r, int prefix);
 void orc_box_f64(const double *in, double *out, int w, int h, int r);
 void orc_guided_f32(const float *I, const float *p, float *q, int w, int h, int prefix);
 void orc_guided_f64(const double *I, const double *p, double *q, int w, int h);

@@ -170,22 +170,21 @@ def test_vs_reference_with_its_thresholds(color):
     assert worst <= 1, worst
 
 
-@pytest.mark.parametrize("w,h", [(320, 240), (50, 33)])
+@pytest.mark.parametrize("w,h", [(320, 240), (50, 33), (484, 37)])
 def test_tile_rows_do_not_change_results(w, h):
-    """The guided filter's tile heights (8, 16, 32 rows, each pass its own; picked by the
-    autotuner) never change the output bytes, at widths that take the 4-pixel and the
-    1-pixel load paths."""
+    """The guided filter's segment height (rows per workgroup; picked by the autotuner, or 0:
+    each launch splits the strips itself) never changes the output bytes, at widths that
+    take the 4-pixel RGB24 kernel and the general one, with segments that do and do not
+    divide the height and strips that do not divide the width."""
     _, frames = clip(4, w, h, 30.0, 6, seed=4)
     kw = dict(fps=30.0, amp=100.0, f_low=0.5, f_high=3.0, levels=3)
     outs = []
-    for rows in (8, 16, 32, (8, 32), (32, 16)):
+    for seg in (None, 0, 4, 12, 32, 64, 1024):
         sess = wema_amd.Session(w, h, **kw)
-        assert sess.tile_rows[0] in (8, 16, 32) and sess.tile_rows[1] in (8, 16, 32)
-        if isinstance(rows, tuple):
-            sess.set_tile_rows(rows[0], 1)
-            sess.set_tile_rows(rows[1], 2)
-        else:
-            sess.set_tile_rows(rows)
+        tuned = sess.tile_rows
+        assert tuned > 0 and tuned % 4 == 0, tuned
+        if seg is not None:
+            sess.set_tile_rows(seg)
         d_in = torch.from_numpy(frames).cuda()
         d_out = torch.empty_like(d_in)
         for t in range(frames.shape[0]):

@@ -103,9 +103,7 @@ _SIGS = {
     "wema_dev_iir_coeffs": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     "wema_dev_frames": (C.c_long, [_vp]),
     "wema_dev_tile_rows": (C.c_int, [_vp]),
-    "wema_dev_tile_rows_pass": (C.c_int, [_vp, C.c_int]),
     "wema_dev_set_tile_rows": (C.c_int, [_vp, C.c_int]),
-    "wema_dev_set_tile_rows_pass": (C.c_int, [_vp, C.c_int, C.c_int]),
     "wema_dev_autotune": (None, [C.c_int]),
     "wema_dev_strip_rows": (C.c_int, [_vp, C.c_int, C.c_int]),
     "wema_dev_set_strip_rows": (C.c_int, [_vp, C.c_int]),
@@ -211,8 +209,8 @@ def inject_error(site, after=1):
 
 
 def autotune(enable=True, dwt=None):
-    """Turns the autotuners on or off for sessions opened later: the guided filter's tile
-    heights (enable) and the DWT strip heights (dwt; the same as enable when not given)."""
+    """Turns the autotuners on or off for sessions opened later: the guided filter's segment
+    height (enable) and the DWT strip heights (dwt; the same as enable when not given)."""
     dwt = enable if dwt is None else dwt
     lib.wema_dev_autotune((1 if enable else 0) | (2 if dwt else 0))
 
@@ -290,12 +288,13 @@ class Session:
 
     @property
     def tile_rows(self):
-        """Guided-filter tile heights of the two passes (picked by the autotuner)."""
-        return (lib.wema_dev_tile_rows_pass(self._h, 1), lib.wema_dev_tile_rows_pass(self._h, 2))
+        """Guided-filter segment height: rows per workgroup (picked by the autotuner; 0: each
+        launch splits its strips itself)."""
+        return lib.wema_dev_tile_rows(self._h)
 
-    def set_tile_rows(self, rows, pass_=0):
-        """Force the tile height of pass 1, pass 2 or both (0)."""
-        _check(lib.wema_dev_set_tile_rows_pass(self._h, int(pass_), int(rows)), "wema_dev_set_tile_rows_pass")
+    def set_tile_rows(self, rows):
+        """Force the guided-filter segment height (a positive multiple of 4, or 0)."""
+        _check(lib.wema_dev_set_tile_rows(self._h, int(rows)), "wema_dev_set_tile_rows")
 
     def strip_rows(self, inverse=False):
         """DWT strip height (pair rows) of each level, forward or inverse (0: the fixed rule)."""

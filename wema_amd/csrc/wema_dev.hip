@@ -25,7 +25,7 @@ hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, int, hipStream_t);
 hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *, int, int, hipStream_t);
 hipError_t wema_launch_iir(const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_win(int, const TemporalJob *, int, int, hipStream_t);
-hipError_t wema_launch_egress(int, int, const EgressJob *, int, int, hipStream_t);
+hipError_t wema_launch_egress(int, const EgressJob *, int, int, hipStream_t);
 hipError_t wema_launch_synth(const SynthFrame *, const unsigned short *, unsigned char *, int,
                              int, int, hipStream_t);
 hipError_t wema_launch_zero_u64(unsigned long long *, long long, hipStream_t);
@@ -186,7 +186,7 @@ static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Byte layout of one session's arena.
 struct Layout {
-    size_t coef, ampc, filt, state, ll, rec, ab, acc, amp, thr, thr_ovr, tw, total;
+    size_t coef, ampc, filt, state, ll, rec, acc, amp, thr, thr_ovr, tw, total;
 };
 
 static void plan_layout(const WemaGeom &g, const wema_dev_params &p, int with_filt, Layout *L) {
@@ -204,8 +204,6 @@ static void plan_layout(const WemaGeom &g, const wema_dev_params &p, int with_fi
     L->state = take(p.use_iir ? sizeof(float) * 4 * nu * nch : 0);
     L->ll = take(sizeof(float) * g.ll_total * nch * B);
     L->rec = take(sizeof(float) * g.P * nch * B);
-    const bool gf = !p.color;
-    L->ab = take(gf ? sizeof(float) * 2 * g.P * B : 0);
     L->acc = take(sizeof(unsigned long long) * nch * B);
     L->amp = take(sizeof(float) * nch * B);
     L->thr = take(sizeof(float) * 3 * B);
@@ -343,9 +341,7 @@ int wema_dev_open_ex(wema_dev_t **out, const wema_dev_params *p, bool pooled) {
         return -1;
     }
     scratch_setup(p->device);   // (the session's device is current)
-    if (!p->color && (p->edge_aware || p->batch_path))
-        wema_gf_rows_for(p->device, p->width, p->height, &s->gf_rows, &s->gf_rows2);
-    if (wema_dwt_tune(s) < 0) {
+    if (wema_dwt_tune(s) < 0 || wema_gf_tune(s) < 0) {
         wema_dev_close(s);
         return -1;
     }
@@ -382,7 +378,6 @@ int wema_session_alloc(wema_dev *s, int with_filt) {
     s->state = s->p.use_iir ? (float4 *)(a + L.state) : nullptr;
     s->ll = (float *)(a + L.ll);
     s->rec = (float *)(a + L.rec);
-    s->ab = s->p.color ? nullptr : (float2 *)(a + L.ab);
     s->acc = (unsigned long long *)(a + L.acc);
     s->amp_d = (float *)(a + L.amp);
     s->thr_d = (float *)(a + L.thr);
@@ -467,7 +462,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                      float *out_f32, int n, const CallBufs *bufs) {
     const WemaGeom &g = s->g;
     const wema_dev_params &p = s->p;
-    const CallBufs b = bufs ? *bufs : CallBufs{s->coef, s->ampc, s->ll, s->rec, s->ab, s->acc};
+    const CallBufs b = bufs ? *bufs : CallBufs{s->coef, s->ampc, s->ll, s->rec, s->acc};
     const int nch = g.nch, L = g.L;
     const size_t P = g.P, nu = g.nu, nus = g.nus;
     const size_t in_stride = P * 3 * (in_f32 ? sizeof(float) : 1);
@@ -626,10 +621,9 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
     }
 
     // output stage
-    const int gf_tx = wema_ceil_div(g.W, WEMA_GF_TX);
-    const int gf_tiles1 = gf_tx * wema_ceil_div(g.H, s->gf_rows);     // guided filter, pass 1
-    const int gf_tiles2 = gf_tx * wema_ceil_div(g.H, s->gf_rows2);    // and pass 2
-    const int gs_tiles = gf_tx * wema_ceil_div(g.H, WEMA_GF_TY);      // 5x5 path
+    const int gf_strips = wema_ceil_div(g.W, WEMA_GF_OUT);            // guided filter
+    const int gs_tx = wema_ceil_div(g.W, WEMA_GF_TX);
+    const int gs_tiles = gs_tx * wema_ceil_div(g.H, WEMA_GF_TY);      // 5x5 path
     const int byte_tiles = (int)((3 * P + 255) / 256);
     for (int f = 0; f < n; f++) {
         EgressJob j{};
@@ -638,22 +632,21 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         j.cr = p.color ? rec_ptr(f, 2) : nullptr;
         if (in_f32) j.src_f32 = (const float *)((const char *)in + (size_t)f * in_stride);
         else j.src_u8 = (const uint8_t *)in + (size_t)f * in_stride;
-        j.ab = b.ab ? b.ab + (size_t)f * P : nullptr;
         j.out_u8 = out_u8 ? out_u8 + (size_t)f * P * 3 : nullptr;
         j.out_f32 = out_f32 ? out_f32 + (size_t)f * P * 3 : nullptr;
         j.w = g.W;
         j.h = g.H;
-        j.tiles_x = gf_tx;
         const double src_b = 3.0 * P * (in_f32 ? 4 : 1);
         if (passthrough(f)) {
             jb.add_egress(EG_PASSTHRU, j, byte_tiles, src_b + out_b);
         } else if (p.color) {
             continue;   // written by the colour output job of the inverse stage
         } else if (p.edge_aware || p.batch_path) {
-            jb.add_egress(EG_GF1 + wema_rows_index(s->gf_rows), j, gf_tiles1, 4.0 * P + src_b + 8.0 * P);
-            jb.add_egress(EG_GF2 + wema_rows_index(s->gf_rows2), j, gf_tiles2, 8.0 * P + src_b + out_b);
+            j.tiles_x = gf_strips;
+            j.seg = s->gf_seg;   // 0: one height for the launch (JobBuilder::assign_segments)
+            jb.add_egress(wema_gf_fast(j) ? EG_GF : EG_GF_ANY, j, 0, 4.0 * P + src_b + out_b);
         } else {
-            j.tiles_x = gf_tx;
+            j.tiles_x = gs_tx;
             jb.add_egress(EG_GAUSS, j, gs_tiles, 4.0 * P + src_b + out_b);
         }
     }
@@ -661,7 +654,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
 
 const char *const wema_timer_names[TK_COUNT] = {
     "k_dwt_fwd0", "k_dwt_fwd", "k_iir", "k_win", "k_dwt_inv",
-    "k_dwt_inv0", "k_dwt_inv0_ycc", "k_gf1", "k_gf2", "k_gauss", "k_passthru"};
+    "k_dwt_inv0", "k_dwt_inv0_ycc", "k_gf", "k_gauss", "k_passthru"};
 
 static int timer_mark(LaunchTimer *t, int kind, double bytes, hipStream_t st) {
     if (!t || !t->enabled) return 0;
@@ -756,13 +749,11 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
         LAUNCH_OK(wema_launch_dwt_inv_ycc((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, st));
         if (timer_mark(timer, TK_DWT_INV0_YCC, jl.bytes, st) < 0) return -2;
     }
-    static const int eg_kind[EG_KINDS] = {TK_GF1, TK_GF1, TK_GF1, TK_GF2, TK_GF2, TK_GF2, TK_GAUSS, TK_PASSTHRU};
+    static const int eg_kind[EG_KINDS] = {TK_GF, TK_GF, TK_GAUSS, TK_PASSTHRU};
     for (int k = 0; k < EG_KINDS; k++) {
         const JobList &jl = jb.egress[k];
         if (!jl.count) continue;
-        const int kind = k < EG_GAUSS ? (k < EG_GF2 ? EG_GF1 : EG_GF2) : k;
-        LAUNCH_OK(wema_launch_egress(kind, wema_index_rows((k - kind) % 3), (const EgressJob *)(base + jl.offset),
-                                  jl.count, jl.blocks, st));
+        LAUNCH_OK(wema_launch_egress(k, (const EgressJob *)(base + jl.offset), jl.count, jl.blocks, st));
         if (timer_mark(timer, eg_kind[k], jl.bytes, st) < 0) return -2;
     }
     return 0;
@@ -822,7 +813,6 @@ static size_t scratch_plan(const wema_dev *s, int n, char *base, CallBufs *b) {
     b->ampc = s->p.use_iir ? s->ampc : (float *)take(sizeof(float) * g.nus * nch * nf);
     b->ll = (float *)take(sizeof(float) * g.ll_total * nch * nf);
     b->rec = (float *)take(sizeof(float) * g.P * nch * nf);
-    b->ab = s->ab ? (float2 *)take(sizeof(float2) * g.P * nf) : nullptr;
     b->acc = (unsigned long long *)take(sizeof(unsigned long long) * nch * nf);
     return off;
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <deque>
 #include <vector>
 
@@ -18,6 +19,11 @@
 #endif
 
 #define WEMA_MAX_LEVELS 12
+// Guided filter: workgroups a launch should have before its strips are split into more
+// segments (8 per CU).
+#ifndef WEMA_GF_WG_TARGET
+#define WEMA_GF_WG_TARGET 2048
+#endif
 #define WEMA_STAGE_BYTES (8 << 20)
 
 struct WemaGeom {
@@ -42,7 +48,6 @@ struct CallBufs {
     float *ampc;                // amplified coefficients when kept apart (or null)
     float *ll;                  // [B][nch][ll_total]
     float *rec;                 // [B][nch][P]
-    float2 *ab;                 // [B][P] (or null)
     unsigned long long *acc;    // [B][nch], zero at the start of the call
 };
 
@@ -76,8 +81,8 @@ struct LaunchTimer {
     }
 };
 enum TimerKind { TK_DWT_FWD0 = 0, TK_DWT_FWD = 1, TK_IIR = 2, TK_WIN = 3, TK_DWT_INV = 4,
-                 TK_DWT_INV0 = 5, TK_DWT_INV0_YCC = 6, TK_GF1 = 7, TK_GF2 = 8, TK_GAUSS = 9,
-                 TK_PASSTHRU = 10, TK_COUNT = 11 };
+                 TK_DWT_INV0 = 5, TK_DWT_INV0_YCC = 6, TK_GF = 7, TK_GAUSS = 8,
+                 TK_PASSTHRU = 9, TK_COUNT = 10 };
 
 struct wema_dev {
     wema_dev_params p;
@@ -85,8 +90,8 @@ struct wema_dev {
     float c10[10];
     int warmup = 0;
     int ring_slots = 0;
-    int gf_rows = 16;       // guided-filter tile height of the first pass (autotuned at open)
-    int gf_rows2 = 16;      // and of the second pass
+    int gf_seg = 0;         // guided-filter segment height (rows per workgroup; autotuned at
+                            // open; 0: chosen per launch by JobBuilder::assign_segments)
     int sp_fwd[WEMA_MAX_LEVELS] = {0};   // DWT strip height (pair rows) of each forward level
     int sp_inv[WEMA_MAX_LEVELS] = {0};   // and inverse level (autotuned at open; 0: the rule)
     long frames = 0;        // frames processed since open
@@ -101,7 +106,6 @@ struct wema_dev {
     float4 *state = nullptr;    // IIR state [nch][nus]
     float *ll = nullptr;        // [B][nch][ll_total]
     float *rec = nullptr;       // level-0 reconstructions [B][nch][P]
-    float2 *ab = nullptr;       // guided filter a, b [B][P]
     unsigned long long *acc = nullptr;   // [B][nch]
     float *amp_d = nullptr;     // [nch][B] effective amplification
     float *thr_d = nullptr;     // [B][3] thresholds used
@@ -226,6 +230,30 @@ struct JobBuilder {
         }
         jl.blocks = blocks;
     }
+    // Segment height of each guided-filter job: its session's tuned height (seg set), else
+    // one split for the rest of the launch: the fewest equal segments per strip (1, 2, 4,
+    // .. 64, each at least 32 rows) that give WEMA_GF_WG_TARGET workgroups. Every segment
+    // recomputes the 16 rows above it; results never depend on the split.
+    static int gf_seg_rows(int h, int nseg) {
+        const int seg = 4 * wema_ceil_div(h, 4 * nseg), whole = 4 * wema_ceil_div(h, 4);
+        return std::max(seg, std::min(32, whole));
+    }
+    static void segments_of(std::vector<EgressJob> &v, JobList &jl, long target) {
+        if (v.empty()) return;
+        int nseg = 1;
+        for (; nseg < 64; nseg *= 2) {
+            long work = 0;
+            for (const auto &j : v) work += (long)j.tiles_x * wema_ceil_div(j.h, j.seg ? j.seg : gf_seg_rows(j.h, nseg));
+            if (work >= target) break;
+        }
+        int blocks = 0;
+        for (auto &j : v) {
+            if (!j.seg) j.seg = gf_seg_rows(j.h, nseg);
+            j.tile_begin = blocks;
+            blocks += j.tiles_x * wema_ceil_div(j.h, j.seg);
+        }
+        jl.blocks = blocks;
+    }
     void assign_strips() {
         for (int l = 0; l < WEMA_MAX_LEVELS; l++)
             strips_of(dwtf_v[l], dwtf[l], l ? WEMA_STRIP_TARGET_COARSE : WEMA_STRIP_TARGET);
@@ -233,6 +261,8 @@ struct JobBuilder {
             strips_of(dwti_v[st], dwti[st],
                       st < WEMA_MAX_LEVELS - 1 ? WEMA_STRIP_TARGET_INV_COARSE : WEMA_STRIP_TARGET_INV);
         strips_of(dwti_ycc_v, dwti_ycc, WEMA_STRIP_TARGET_INV);
+        segments_of(egress_v[EG_GF], egress[EG_GF], WEMA_GF_WG_TARGET);
+        segments_of(egress_v[EG_GF_ANY], egress[EG_GF_ANY], WEMA_GF_WG_TARGET);
     }
     int pack(void *dst_v, size_t cap, size_t *used) {
         assign_strips();
@@ -257,7 +287,8 @@ void wema_set_error(const char *fmt, ...);
 int wema_design_bandpass(float f_low, float f_high, float fs, float *c10, int *warmup);
 float wema_fade_amp(float amp, long global_frame);
 int wema_auto_levels(int width, int height);
-void wema_gf_rows_for(int device, int w, int h, int *rows1, int *rows2);
+int wema_gf_tune(wema_dev *s);
+int wema_dev_open_ex(wema_dev_t **out, const wema_dev_params *p, bool pooled);
 int wema_dwt_tune(wema_dev *s);
 void wema_geometry_init(WemaGeom *g, const wema_dev_params *p);
 void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,

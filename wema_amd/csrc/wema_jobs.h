@@ -43,9 +43,15 @@
 #define WEMA_STRIP_TARGET_INV_COARSE 262144
 #endif
 
-// Guided filter tiles (outputs per block).
+// 5x5 Gaussian tiles (outputs per block).
 #define WEMA_GF_TX 64
 #define WEMA_GF_TY 16
+// Guided filter: output columns of a strip (a workgroup's 64 lanes x 4 columns less the
+// 8-column halo on each side), and the segment heights (rows per workgroup, multiples of 4)
+// the autotuner chooses from.
+#define WEMA_GF_OUT 240
+#define WEMA_GF_SEGS 6
+__host__ __device__ static inline int wema_gf_seg_rows(int i) { return 32 << i; }   // 32 .. 1024
 
 enum SrcKind {
     SRC_F32_PLANE = 0,   // float image (the previous level's LL)
@@ -102,11 +108,9 @@ struct DwtiJob {
 #endif
 
 // Output stages after the inverse DWT (colour mode has none: its inverse writes RGB). The
-// guided-filter passes have one job list per tile height (8, 16, 32 rows), all first-pass
-// lists ahead of all second-pass ones.
-enum EgressKind { EG_GF1 = 0, EG_GF2 = 3, EG_GAUSS = 6, EG_PASSTHRU = 7, EG_KINDS = 8 };
-__host__ __device__ static inline int wema_rows_index(int rows) { return rows == 8 ? 0 : (rows == 32 ? 2 : 1); }
-__host__ __device__ static inline int wema_index_rows(int i) { return i == 0 ? 8 : (i == 2 ? 32 : 16); }
+// guided filter has two kernels: k_gf for RGB24 in and out at widths that are multiples of
+// 4 with aligned buffers (wema_gf_fast), k_gf_any for the rest.
+enum EgressKind { EG_GF = 0, EG_GF_ANY = 1, EG_GAUSS = 2, EG_PASSTHRU = 3, EG_KINDS = 4 };
 
 // Sliding-window kernels: 0 = windows of <= 32 samples through LDS, 1 = <= 64 through LDS,
 // 2 = any size from the ring; exactly 32, 16 or 64 samples in registers: 3, 4, 5.
@@ -158,12 +162,17 @@ struct EgressJob {
     const float *cb, *cr;           // colour mode reconstructions
     const uint8_t *src_u8;          // source frame (RGB24) or null
     const float *src_f32;           // source frame (float RGB) or null
-    float2 *ab;                     // guided filter a, b scratch
     uint8_t *out_u8;                // RGB24 out (or null)
     float *out_f32;                 // float RGB out (or null)
     int w, h;
-    int tiles_x;
+    int tiles_x;                    // tiles (guided filter: strips) per row of tiles
     int tile_begin;
+    int seg;                        // guided filter: rows per segment (a multiple of 4)
 };
 
 __host__ __device__ static inline int wema_ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+static inline bool wema_gf_fast(const EgressJob &j) {
+    return j.src_u8 && j.out_u8 && !j.src_f32 && !j.out_f32 && (j.w & 3) == 0 && ((uintptr_t)j.gray & 15) == 0 &&
+           (((uintptr_t)j.src_u8 | (uintptr_t)j.out_u8) & 3) == 0;
+}

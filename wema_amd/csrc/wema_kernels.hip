@@ -18,8 +18,9 @@
 //   k_dwt_inv   one 2D synthesis level as strips (column synthesis as a register
 //               pipeline, then rows across lanes); k_dwt_inv0 level 0.
 //   k_dwt_inv0_ycc  colour mode: level 0 of Y, Cb, Cr + YCbCr -> RGB24 in one pass.
-//   k_gf1/k_gf2 guided filter (r = 4, eps = 0.01) of the luma delta with 9x9 clipped box
-//               means in LDS; k_gf2 adds the smoothed delta to RGB and writes RGB24.
+//   k_gf        guided filter (r = 4, eps = 0.01) of the luma delta with 9x9 clipped box
+//               means, streamed down column strips with a and b kept on the chip; adds
+//               the smoothed delta to RGB and writes RGB24.
 //   k_gauss     5x5 binomial alternative (--no-edge-aware).
 //   k_passthru  window mode before the window is full: output = input.
 //   k_synth     synthetic frames (deterministic integer generator).
@@ -1190,10 +1191,17 @@ WEMA_WIN_LDS_KERNEL(64)
 // Box means over the 9x9 window clipped to the frame. The reference divides each row's
 // window sums by their count and then the column sums of those by theirs (a mean of means
 // over FP32 prefix sums, src/wema.c:114-171). Here the window sum is the vertical 9-tap sum
-// of horizontal 9-tap sums, each formed in blocks of 4 outputs (sums4), and the mean is that
-// sum times the correctly rounded reciprocal of the clipped window's area: the same mean in
-// real arithmetic, with one multiply where the reference divides twice (the oracle restates
-// this order exactly).
+// of horizontal 9-tap sums, and the mean is that sum times the correctly rounded reciprocal
+// of the clipped window's area: the same mean in real arithmetic, with one multiply where
+// the reference divides twice. The order of the additions (the oracle restates it exactly):
+//   horizontal: blocks of 4 outputs starting at multiples of 4 of the image; the first sum
+//     of a block adds its 9 samples left to right, each next one is
+//     (previous - leaving sample) + entering sample;
+//   vertical: rows in groups of 4 starting at multiples of 4; a group h0..h3 has the prefix
+//     sums P1 = h0, P2 = P1 + h1, P3 = P2 + h2, F = P3 + h3 and the suffix sums U1 = h3,
+//     U2 = h2 + h3, U3 = h1 + U2, U4 = F; the sum of row 4b + j (j = 0..3) is
+//     (U_{b-1}(4 - j) + F_b) + P_{b+1}(j + 1), with P(4) = F.
+// Samples outside the frame are 0 in both directions.
 
 __device__ __forceinline__ float src_luma(const EgressJob &J, size_t idx) {
     if (J.src_u8) {
@@ -1227,485 +1235,329 @@ __device__ __forceinline__ float div_mid(float a, float b) {
     return fmaf(fmaf(-b, q, a), y, q);
 }
 
-// Correctly rounded 1/c for c = 0..81 (the areas of clipped 9x9 windows) in LDS (entry 0
-// unused), filled by the first threads.
-constexpr int GF_RCP = 82;
-__device__ __forceinline__ void fill_rcp(float *t) {
-    if (threadIdx.x < GF_RCP) t[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 0.0f;
+// One kernel streams the whole filter. A workgroup of 4 waves owns a strip of GF_OUT = 240
+// output columns of one frame and a segment of its rows, and walks down the segment one
+// group of 4 rows per step; a and b never leave the chip. Step m:
+//   A  (row layout: wave w takes input row 4m + w, lane l the 4 columns x0 - 8 + 4l ..)
+//      guide I = luma(src), p = rec - I, I^2 and I p; their horizontal sums, the
+//      neighbouring lanes' samples taken by DPP lane shifts; valid in lanes 1-62.
+//   B  (column layout: thread t takes column x0 - 8 + t of all 4 rows) the vertical sums,
+//      which complete the window sums of row group m - 1: means, a and b (0 outside the
+//      frame).
+//   C  (row layout) horizontal sums of a and b, valid in lanes 2-61.
+//   D  (column layout) their vertical sums: mean a and mean b of row group m - 2.
+//   E  (row layout) q = mean(a) I + mean(b) and RGB out = clamp(src + q) of row group
+//      m - 2, from the source pixels this lane loaded two steps earlier.
+// The input rows of step m + 1 are loaded while step m computes. A segment starts 2 groups
+// above its first output row (the filter reaches 8 rows up) and ends 2 groups below its last.
+constexpr int GF_OUT = WEMA_GF_OUT;   // output columns of a strip
+constexpr int GF_W = GF_OUT + 16;     // with the 8-column halo on each side: 64 lanes x 4
+
+// Horizontal 9-tap sums of the 4 columns of this lane from its own samples o and those of
+// the lanes on either side, in the block order above: v[0..3] = the previous lane's o,
+// v[4..7] = o, v[8..11] = the next lane's o; the previous lane's ((o0 + o1) + o2) + o3 is
+// the block's first partial sum.
+__device__ __forceinline__ void hsum_lanes(const float (&o)[4], float (&s)[4]) {
+    const float a = ((o[0] + o[1]) + o[2]) + o[3];
+    float s0 = lane_prev(a);
+    s0 = (((s0 + o[0]) + o[1]) + o[2]) + o[3];
+    s[0] = s0 + lane_next(o[0]);
+    s[1] = (s[0] - lane_prev(o[0])) + lane_next(o[1]);
+    s[2] = (s[1] - lane_prev(o[1])) + lane_next(o[2]);
+    s[3] = (s[2] - lane_prev(o[2])) + lane_next(o[3]);
 }
 
-// 1 / (area of the clipped window) at column count cx and row y of h: inside (IN: every
-// window of the tile lies in the frame) the constant 1/81, correctly rounded as in the table.
-template <bool IN>
-__device__ __forceinline__ float inv_area(int cx, int y, int h, const float *sRcp) {
-    if (IN) return 1.0f / 81.0f;
-    return sRcp[cx * clip_count(y, h)];
-}
-template <bool B> using bool_c = std::integral_constant<bool, B>;
-
-constexpr int GF_TX = WEMA_GF_TX;        // output columns of a tile
-// Threads per block of the guided-filter kernels: the 32-row tiles take 8 waves, so that
-// their larger LDS tile (2 blocks per CU) still gives every SIMD 4 waves.
-#define GF_NT(TY) ((TY) == 32 ? 512 : 256)
-constexpr int GF_EX = GF_TX + 8;         // with the 4-column halo on each side
-constexpr int GF_PITCH = GF_EX + 4;      // multiple of 4: rows start 16-byte aligned
-
-// 9-tap sums of a block of 4 consecutive outputs (starting at a multiple of 4 of the image)
-// from 12 samples held in registers: the first adds its window left to right, starting from
-// its first sample, each next one is (previous - leaving sample) + entering sample. Samples
-// outside the image are 0 (exact in both steps). The blocks are anchored to the image, so
-// results do not depend on the tile shape.
-__device__ __forceinline__ void sums4(const float (&v)[12], float (&out)[4]) {
-    float a = v[0];
-#pragma unroll
-    for (int k = 1; k < 9; ++k) a += v[k];
-    out[0] = a;
-#pragma unroll
-    for (int j = 1; j < 4; ++j) out[j] = (out[j - 1] - v[j - 1]) + v[j + 8];
-}
-
-// Source pixel of the frame as three floats in [0, 1] (RGB24 or float RGB).
-template <bool U8>
-struct SrcPix {
-    unsigned r, g, b;   // raw bytes or float bits
-    __device__ __forceinline__ void load(const EgressJob &J, size_t idx) {
-        if (U8) {
-            const GLOBAL_AS unsigned char *p = gp(J.src_u8) + idx * 3;
-            r = p[0]; g = p[1]; b = p[2];
-        } else {
-            const GLOBAL_AS float *p = gp(J.src_f32) + idx * 3;
-            r = __float_as_uint(p[0]); g = __float_as_uint(p[1]); b = __float_as_uint(p[2]);
-        }
+// Vertical window sums of one column of one quantity, fed one group of 4 rows at a time.
+struct VSum {
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;   // U_{b-1}(4 - j) + F_b for the next group's outputs
+    float u1 = 0.0f, u2 = 0.0f, u3 = 0.0f, f = 0.0f;    // suffix sums and sum of the last group
+    // Takes group b + 1 (rows h0..h3); gives the window sums of the 4 rows of group b.
+    __device__ __forceinline__ void step(float h0, float h1, float h2, float h3, float (&s)[4]) {
+        const float p2 = h0 + h1, p3 = p2 + h2, F = p3 + h3;
+        const float U2 = h2 + h3, U3 = h1 + U2;
+        s[0] = t0 + h0;
+        s[1] = t1 + p2;
+        s[2] = t2 + p3;
+        s[3] = t3 + F;
+        t0 = f + F;
+        t1 = u3 + F;
+        t2 = u2 + F;
+        t3 = u1 + F;
+        u1 = h3;
+        u2 = U2;
+        u3 = U3;
+        f = F;
     }
-    __device__ __forceinline__ float ch(unsigned v) const { return U8 ? u8f(v) : __uint_as_float(v); }
-    __device__ __forceinline__ float luma() const { return luma_of(ch(r), ch(g), ch(b)); }
 };
 
-// Guide I (source luma), input p (reconstruction - guide) and the products I^2 and I p of
-// the extended tile (TY + 8 rows) into LDS, zero outside the frame. Coordinates are clamped
-// into the frame so that every load is valid and all of them are issued before any is used
-// (one memory round trip).
-template <bool U8, int TY>
-__device__ __forceinline__ void gf1_load(const EgressJob &J, int x0, int y0, float (*sI)[GF_PITCH],
-                                         float (*sP)[GF_PITCH], float (*sII)[GF_PITCH],
-                                         float (*sIP)[GF_PITCH]) {
-    constexpr int NT = GF_NT(TY);
-    constexpr int EY = TY + 8, NLD = (EY * GF_EX + NT - 1) / NT;
-    const int w = J.w, h = J.h;
-    const GLOBAL_AS float *gray = gp(J.gray);
-    SrcPix<U8> px[NLD];
-    float g[NLD];
-#pragma unroll
-    for (int j = 0; j < NLD; ++j) {
-        const int i = threadIdx.x + NT * j;
-        const int ey = i / GF_EX, ex = i - ey * GF_EX;
-        const int x = min(max(x0 - 4 + ex, 0), w - 1), y = min(max(y0 - 4 + ey, 0), h - 1);
-        const size_t idx = (size_t)y * w + x;
-        px[j].load(J, idx);
-        g[j] = gray[idx];
+// The source pixels of 4 consecutive columns: 3 dwords of RGB24 or 12 floats.
+template <bool U8>
+struct Src4 {
+    unsigned d[U8 ? 3 : 12];
+    __device__ __forceinline__ float ch(int i) const {   // channel i (0..11) in [0, 1]
+        return U8 ? u8f((d[i / 4] >> (8 * (i % 4))) & 0xFFu) : __uint_as_float(d[i]);
     }
-#pragma unroll
-    for (int j = 0; j < NLD; ++j) {
-        const int i = threadIdx.x + NT * j;
-        if (i < EY * GF_EX) {
-            const int ey = i / GF_EX, ex = i - ey * GF_EX;
-            const int x = x0 - 4 + ex, y = y0 - 4 + ey;
-            const bool in = x >= 0 && x < w && y >= 0 && y < h;
-            const float l = px[j].luma();
-            const float I = in ? l : 0.0f, pv = in ? g[j] - l : 0.0f;
-            sI[ey][ex] = I;
-            sP[ey][ex] = pv;
-            sII[ey][ex] = I * I;
-            sIP[ey][ex] = I * pv;
-        }
-    }
-}
+};
 
-// The same, 4 pixels per load: with the width a multiple of 4 and the buffers aligned, the
-// tile's rows are groups of 4 pixels that lie wholly inside or outside the frame; a group's
-// RGB24 is 3 dwords (float RGB: 3 x 16 bytes) and its guide a float4.
-template <bool U8, int TY>
-__device__ __forceinline__ void gf1_load_vec(const EgressJob &J, int x0, int y0, float (*sI)[GF_PITCH],
-                                             float (*sP)[GF_PITCH], float (*sII)[GF_PITCH],
-                                             float (*sIP)[GF_PITCH]) {
-    constexpr int NT = GF_NT(TY);
-    constexpr int GX = GF_EX / 4, NG = (TY + 8) * GX, NIT = (NG + NT - 1) / NT;
+// Input row y of the 4 columns from x: the source pixels and the reconstruction. VEC (width
+// a multiple of 4, aligned buffers): the 4 columns lie wholly inside or outside the frame
+// and load as 3 dwords (or 3 x 16 bytes) plus 16 bytes; coordinates are clamped into the
+// frame so every load is valid (samples outside are zeroed by the caller).
+template <bool U8, bool VEC>
+__device__ __forceinline__ void gf_load(const EgressJob &J, int x, int y, Src4<U8> &s, v4f &g) {
     const int w = J.w, h = J.h;
-    unsigned raw[NIT][12];
-    v4f g[NIT];
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int i = threadIdx.x + NT * it;
-        const int ey = i / GX, gx = i - ey * GX;
-        const int x = min(max(x0 - 4 + 4 * gx, 0), w - 4), y = min(max(y0 - 4 + ey, 0), h - 1);
-        const size_t idx = (size_t)y * w + x;
+    const int yc = min(max(y, 0), h - 1);
+    if (VEC) {
+        const int xc = min(max(x, 0), w - 4);
+        const unsigned idx = (unsigned)yc * (unsigned)w + (unsigned)xc;
         if (U8) {
-            const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(gp(J.src_u8) + idx * 3);
-            raw[it][0] = q[0]; raw[it][1] = q[1]; raw[it][2] = q[2];
+            const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(gp(J.src_u8) + (size_t)idx * 3);
+            s.d[0] = q[0]; s.d[1] = q[1]; s.d[2] = q[2];
         } else {
-            const GLOBAL_AS v4f *q = (const GLOBAL_AS v4f *)(gp(J.src_f32) + idx * 3);
+            const GLOBAL_AS v4f *q = (const GLOBAL_AS v4f *)(gp(J.src_f32) + (size_t)idx * 3);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const v4f f = q[k];
-                raw[it][4 * k] = __float_as_uint(f.x); raw[it][4 * k + 1] = __float_as_uint(f.y);
-                raw[it][4 * k + 2] = __float_as_uint(f.z); raw[it][4 * k + 3] = __float_as_uint(f.w);
+                s.d[4 * k] = __float_as_uint(f.x); s.d[4 * k + 1] = __float_as_uint(f.y);
+                s.d[4 * k + 2] = __float_as_uint(f.z); s.d[4 * k + 3] = __float_as_uint(f.w);
             }
         }
-        g[it] = *(const GLOBAL_AS v4f *)(gp(J.gray) + idx);
-    }
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int i = threadIdx.x + NT * it;
-        if (i >= NG) break;
-        const int ey = i / GX, gx = i - ey * GX;
-        const int x = x0 - 4 + 4 * gx, y = y0 - 4 + ey;
-        const bool in = x >= 0 && x < w && y >= 0 && y < h;
-        float c[12];
-#pragma unroll
-        for (int k = 0; k < 12; ++k)
-            c[k] = U8 ? u8f((raw[it][k / 4] >> (8 * (k % 4))) & 0xFFu) : __uint_as_float(raw[it][k]);
-        const float gv[4] = {g[it].x, g[it].y, g[it].z, g[it].w};
-        float I[4], pv[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float l = luma_of(c[3 * j], c[3 * j + 1], c[3 * j + 2]);
-            I[j] = in ? l : 0.0f;
-            pv[j] = in ? gv[j] - l : 0.0f;
-        }
-        *(v4f *)&sI[ey][4 * gx] = v4f{I[0], I[1], I[2], I[3]};
-        *(v4f *)&sP[ey][4 * gx] = v4f{pv[0], pv[1], pv[2], pv[3]};
-        *(v4f *)&sII[ey][4 * gx] = v4f{I[0] * I[0], I[1] * I[1], I[2] * I[2], I[3] * I[3]};
-        *(v4f *)&sIP[ey][4 * gx] = v4f{I[0] * pv[0], I[1] * pv[1], I[2] * pv[2], I[3] * pv[3]};
-    }
-}
-
-// Tile of a guided-filter job: GF_TX x TY outputs.
-template <int TY>
-__device__ __forceinline__ const EgressJob &gf_tile(const EgressJob *__restrict__ jobs, int njobs, int &x0,
-                                                  int &y0) {
-    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int jid = find_job(jobs, njobs, bid);
-    const EgressJob &J = jobs[jid];
-    const int t = bid - J.tile_begin;
-    x0 = (t % J.tiles_x) * GF_TX;
-    y0 = (t / J.tiles_x) * TY;
-    return J;
-}
-
-// Horizontal window sums of NQ quantities over the extended tile, 4 outputs per item (taps
-// read as three 16-byte vectors per quantity), written over the samples they come from: every
-// thread computes all of its items first, then a barrier separates the last read of the tile
-// from the first write.
-template <int TY, int NQ>
-__device__ __forceinline__ void gf_hsums(float (*const (&S)[NQ])[GF_PITCH]) {
-    constexpr int NT = GF_NT(TY);
-    constexpr int EY = TY + 8, NI = (EY * (GF_TX / 4) + NT - 1) / NT;   // items per thread
-    float hv[NI][NQ][4];
-#pragma unroll
-    for (int it = 0; it < NI; ++it) {
-        const int i = threadIdx.x + NT * it;
-        if (i >= EY * (GF_TX / 4)) break;
-        const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            float v[12];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float4 a = *reinterpret_cast<const float4 *>(&S[q][ey][x4 + 4 * k]);
-                v[4 * k] = a.x; v[4 * k + 1] = a.y; v[4 * k + 2] = a.z; v[4 * k + 3] = a.w;
-            }
-            sums4(v, hv[it][q]);
-        }
-    }
-    __syncthreads();   // (every read of the tile is done)
-#pragma unroll
-    for (int it = 0; it < NI; ++it) {
-        const int i = threadIdx.x + NT * it;
-        if (i >= EY * (GF_TX / 4)) break;
-        const int ey = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
-#pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            *reinterpret_cast<float4 *>(&S[q][ey][x4]) = make_float4(hv[it][q][0], hv[it][q][1], hv[it][q][2], hv[it][q][3]);
-    }
-}
-
-// First pass: box means of I, p, I^2, I p over the tile, then a and b (src/wema.c:173-219).
-// IN: every window of the tile lies in the frame (area 81 everywhere).
-template <int TY, bool IN>
-__device__ __forceinline__ void gf1_vert(const EgressJob &J, int x0, int y0, float (*const (&S)[4])[GF_PITCH],
-                                         const float *sRcp) {
-    constexpr int NT = GF_NT(TY);
-    const int w = J.w, h = J.h;
-    GLOBAL_AS v2f *ab = (GLOBAL_AS v2f *)gp((float *)J.ab);
-    // 4 consecutive rows of one column per item
-    for (int i = threadIdx.x; i < (TY / 4) * GF_TX; i += NT) {
-        const int y4 = 4 * (i / GF_TX), x = i % GF_TX;
-        const int gx = x0 + x;
-        if (!IN && gx >= w) continue;
-        float sm[4][4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float v[12];
-#pragma unroll
-            for (int k = 0; k < 12; ++k) v[k] = S[q][y4 + k][x];
-            sums4(v, sm[q]);
-        }
-        const int cx = IN ? 9 : clip_count(gx, w);
-        const unsigned row0 = (unsigned)(y0 + y4) * (unsigned)w + (unsigned)gx;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int gy = y0 + y4 + j;
-            if (!IN && gy >= h) break;
-            const float k = inv_area<IN>(cx, gy, h, sRcp);
-            const float mI = sm[0][j] * k, mp = sm[1][j] * k, cI = sm[2][j] * k, cIp = sm[3][j] * k;
-            const float var = fmaf(-mI, mI, cI);
-            const float cov = fmaf(-mI, mp, cIp);
-            // var + eps lies in [0.0099, 1.02): I is in [0, 1]
-            const float av = div_mid(cov, var + 0.01f);
-            const float bv = fmaf(-av, mI, mp);
-            // written once, read by the second pass's next launch: nontemporal
-            __builtin_nontemporal_store(v2f{av, bv}, ab + (row0 + (unsigned)(j * w)));
-        }
-    }
-}
-
-template <int TY>
-__device__ __forceinline__ void gf1_run(const EgressJob *__restrict__ jobs, int njobs, float (*sI)[GF_PITCH],
-                                        float (*sP)[GF_PITCH], float (*sII)[GF_PITCH], float (*sIP)[GF_PITCH],
-                                        float *sRcp) {
-    fill_rcp(sRcp);
-    int x0, y0;
-    const EgressJob &J = gf_tile<TY>(jobs, njobs, x0, y0);
-    const int w = J.w, h = J.h;
-    const uintptr_t al = (uintptr_t)J.gray | (J.src_u8 ? 0 : (uintptr_t)J.src_f32);
-    const bool vec = (w & 3) == 0 && (al & 15) == 0 && (!J.src_u8 || ((uintptr_t)J.src_u8 & 3) == 0);
-    if (vec) {
-        if (J.src_u8) gf1_load_vec<true, TY>(J, x0, y0, sI, sP, sII, sIP);
-        else gf1_load_vec<false, TY>(J, x0, y0, sI, sP, sII, sIP);
+        g = *(const GLOBAL_AS v4f *)(gp(J.gray) + idx);
     } else {
-        if (J.src_u8) gf1_load<true, TY>(J, x0, y0, sI, sP, sII, sIP);
-        else gf1_load<false, TY>(J, x0, y0, sI, sP, sII, sIP);
-    }
-    __syncthreads();
-    float (*const S[4])[GF_PITCH] = {sI, sP, sII, sIP};
-    gf_hsums<TY, 4>(S);
-    __syncthreads();
-    const bool in = x0 >= 4 && x0 + GF_TX + 4 <= w && y0 >= 4 && y0 + TY + 4 <= h;
-    if (in) gf1_vert<TY, true>(J, x0, y0, S, sRcp);
-    else gf1_vert<TY, false>(J, x0, y0, S, sRcp);
-}
-
-// RGB out = clamp(source + q) (src/wema.c:329-334).
-template <bool U8>
-__device__ __forceinline__ void write_rgb_px(const EgressJob &J, size_t idx, const SrcPix<U8> &sp, float q) {
-    const float s[3] = {sp.ch(sp.r), sp.ch(sp.g), sp.ch(sp.b)};
+        float gv[4];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float v = clamp01(s[k] + q);
-        if (J.out_u8) gp(J.out_u8)[idx * 3 + k] = egress_u8(v);
-        if (J.out_f32) gp(J.out_f32)[idx * 3 + k] = v;
-    }
-}
-
-// Second pass: box means of a and b, q = mean(a) I + mean(b), RGB out (src/wema.c:221-229,
-// 289-335).
-// VOUT (RGB24 in and out only, width a multiple of 4, frames on a dword): the vertical pass
-// leaves mean(a) and mean(b) in LDS, and the output stage takes 4 consecutive pixels of a
-// row per thread, reading and writing their RGB24 as 3 dwords; otherwise each thread
-// finishes its own column of 4 rows pixel by pixel.
-// As in the first pass, the horizontal window sums of a and b overwrite the a, b tile once
-// every thread has read it, and (VOUT) mean(a), mean(b) overwrite those once every thread
-// has read them.
-template <bool U8, int TY, bool VOUT>
-__device__ __forceinline__ void gf2_body(const EgressJob &J, int x0, int y0, float (*sA)[GF_PITCH],
-                                         float (*sB)[GF_PITCH], const float *sRcp) {
-    float (*hA)[GF_PITCH] = sA;
-    float (*hB)[GF_PITCH] = sB;
-    constexpr int NT = GF_NT(TY);
-    constexpr int EY = TY + 8, NV = (TY / 4) * GF_TX, VI = (NV + NT - 1) / NT;
-    const int w = J.w, h = J.h;
-    // the source pixels of this thread's outputs (VI columns of 4 rows) and the a, b tile,
-    // all loaded first from coordinates clamped into the frame
-    SrcPix<U8> src[VOUT ? 1 : VI][4];
-    if (!VOUT) {
+        for (int k = 0; k < 4; ++k) {
+            const int xc = min(max(x + k, 0), w - 1);
+            const size_t idx = (size_t)yc * w + xc;
+            gv[k] = gp(J.gray)[idx];
+            if (U8) {
+                const GLOBAL_AS unsigned char *p = gp(J.src_u8) + idx * 3;
+                const unsigned b0 = p[0], b1 = p[1], b2 = p[2];
+                // channel 3k + c of the group sits in byte (3k + c) % 4 of dword (3k + c) / 4
+                const unsigned v[3] = {b0, b1, b2};
 #pragma unroll
-        for (int v = 0; v < VI; ++v) {
-            const int i = threadIdx.x + NT * v;
-            const int y4 = 4 * (i / GF_TX), gx = x0 + i % GF_TX;
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                src[v][j].load(J, (size_t)min(y0 + y4 + j, h - 1) * w + min(gx, w - 1));
-        }
-    }
-    const GLOBAL_AS float2 *ab = gp(J.ab);
-    if ((w & 3) == 0 && ((uintptr_t)J.ab & 15) == 0) {
-        // groups of 4 pixels wholly inside or outside the frame, two 16-byte loads each
-        constexpr int GX = GF_EX / 4, NG = EY * GX, NIT = (NG + NT - 1) / NT;
-        v4f q[NIT][2];
-#pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int i = threadIdx.x + NT * it;
-            const int ey = i / GX, gx = i - ey * GX;
-            const int x = min(max(x0 - 4 + 4 * gx, 0), w - 4), y = min(max(y0 - 4 + ey, 0), h - 1);
-            const GLOBAL_AS v4f *p = (const GLOBAL_AS v4f *)(ab + (size_t)y * w + x);
-            q[it][0] = p[0];
-            q[it][1] = p[1];
-        }
-#pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int i = threadIdx.x + NT * it;
-            if (i >= NG) break;
-            const int ey = i / GX, gx = i - ey * GX;
-            const int x = x0 - 4 + 4 * gx, y = y0 - 4 + ey;
-            const bool in = x >= 0 && x < w && y >= 0 && y < h;
-            const v4f a = {q[it][0].x, q[it][0].z, q[it][1].x, q[it][1].z};
-            const v4f b = {q[it][0].y, q[it][0].w, q[it][1].y, q[it][1].w};
-            *(v4f *)&sA[ey][4 * gx] = in ? a : v4f{0.0f, 0.0f, 0.0f, 0.0f};
-            *(v4f *)&sB[ey][4 * gx] = in ? b : v4f{0.0f, 0.0f, 0.0f, 0.0f};
-        }
-    } else {
-        constexpr int NLD = (EY * GF_EX + NT - 1) / NT;
-        float2 lab[NLD];
-#pragma unroll
-        for (int j = 0; j < NLD; ++j) {
-            const int i = threadIdx.x + NT * j;
-            const int ey = i / GF_EX, ex = i - ey * GF_EX;
-            const int x = min(max(x0 - 4 + ex, 0), w - 1), y = min(max(y0 - 4 + ey, 0), h - 1);
-            const GLOBAL_AS float *q = (const GLOBAL_AS float *)(ab + (size_t)y * w + x);
-            lab[j].x = q[0];
-            lab[j].y = q[1];
-        }
-#pragma unroll
-        for (int j = 0; j < NLD; ++j) {
-            const int i = threadIdx.x + NT * j;
-            if (i < EY * GF_EX) {
-                const int ey = i / GF_EX, ex = i - ey * GF_EX;
-                const int x = x0 - 4 + ex, y = y0 - 4 + ey;
-                const bool in = x >= 0 && x < w && y >= 0 && y < h;
-                sA[ey][ex] = in ? lab[j].x : 0.0f;
-                sB[ey][ex] = in ? lab[j].y : 0.0f;
-            }
-        }
-    }
-    __syncthreads();
-    float (*const S[2])[GF_PITCH] = {sA, sB};
-    gf_hsums<TY, 2>(S);
-    __syncthreads();
-    // every window of the tile inside the frame (area 81 everywhere)
-    const bool in = x0 >= 4 && x0 + GF_TX + 4 <= w && y0 >= 4 && y0 + TY + 4 <= h;
-    float mv[VOUT ? VI : 1][2][4];
-    auto vpass = [&](auto in_c) {
-    constexpr bool IN = decltype(in_c)::value;
-#pragma unroll
-    for (int v = 0; v < VI; ++v) {
-        const int i = threadIdx.x + NT * v;
-        const int y4 = 4 * (i / GF_TX), xo = i % GF_TX;
-        const int gx = x0 + xo;
-        if (i >= NV || (!IN && gx >= w)) break;
-        float va[12], vb[12];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            va[k] = hA[y4 + k][xo];
-            vb[k] = hB[y4 + k][xo];
-        }
-        float sa[4], sb[4];
-        sums4(va, sa);
-        sums4(vb, sb);
-        const int cx = IN ? 9 : clip_count(gx, w);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int gy = y0 + y4 + j;
-            if (!IN && gy >= h) break;
-            const float k = inv_area<IN>(cx, gy, h, sRcp);
-            if (VOUT) {
-                mv[v][0][j] = sa[j] * k;
-                mv[v][1][j] = sb[j] * k;
+                for (int c = 0; c < 3; ++c) {
+                    const int i = 3 * k + c;
+                    if (i % 4 == 0) s.d[i / 4] = v[c];
+                    else s.d[i / 4] |= v[c] << (8 * (i % 4));
+                }
             } else {
-                const float q = fmaf(sa[j] * k, src[v][j].luma(), sb[j] * k);
-                write_rgb_px<U8>(J, (size_t)gy * w + gx, src[v][j], q);
+                const GLOBAL_AS float *p = gp(J.src_f32) + idx * 3;
+                s.d[3 * k] = __float_as_uint(p[0]);
+                s.d[3 * k + 1] = __float_as_uint(p[1]);
+                s.d[3 * k + 2] = __float_as_uint(p[2]);
             }
         }
+        g = v4f{gv[0], gv[1], gv[2], gv[3]};
     }
-    };
-    if (in) vpass(bool_c<true>{});
-    else vpass(bool_c<false>{});
-    if (VOUT) {
-        __syncthreads();   // (every read of the horizontal means is done)
+}
+
+// 1 / (area of the clipped window) at column count cx and row y of h: the correctly rounded
+// reciprocal from the table (kx9: the entry of cx rows of 9, for rows whose window lies in
+// the frame; the row test is the same for the whole workgroup).
+__device__ __forceinline__ float inv_area(int cx, float kx9, int y, int h, const float *sRcp) {
+    const int cy = clip_count(y, h);
+    return cy == 9 ? kx9 : sRcp[cx * cy];
+}
+
+constexpr int GF_RCP = 82;   // correctly rounded 1/c for c = 0..81 (entry 0 unused)
+
+// OUTF: float RGB output possible (otherwise RGB24 only).
+template <bool U8, bool VEC, bool OUTF>
+__device__ __forceinline__ void gf_strip(const EgressJob &J, int x0, int ybeg, int yend, float (*hs)[4][GF_W],
+                                         float (*ab)[2][GF_W], const float *sRcp) {
+    const int w = J.w, h = J.h;
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int xr = x0 - 8 + 4 * ln;            // row layout: first of this lane's 4 columns
+    const int xcol = x0 - 8 + (int)threadIdx.x;   // column layout: this thread's column
+    const bool col_in = xcol >= 0 && xcol < w;
+    const int cx = clip_count(xcol, w);
+    const float kx9 = sRcp[cx * 9];
+    const bool out_lane = ln >= 2 && ln < 62;
+    VSum v1[4], v2[2];
+    const int m0 = (ybeg >> 2) - 2, m1 = ((yend - 1) >> 2) + 2;
+    Src4<U8> nxt, cur, old1, old2;
+    v4f gn;
+    gf_load<U8, VEC>(J, xr, 4 * m0 + wv, nxt, gn);
+    for (int m = m0; m <= m1; ++m) {
+        cur = nxt;
+        const v4f g = gn;
+        if (m < m1) gf_load<U8, VEC>(J, xr, 4 * (m + 1) + wv, nxt, gn);
+        // A: the four quantities of input row 4m + wv and their horizontal sums
+        {
+            const int y = 4 * m + wv;
+            bool in = (unsigned)y < (unsigned)h;
+            if (VEC) in = in && xr >= 0 && xr < w;
+            const float gv[4] = {g.x, g.y, g.z, g.w};
+            float q[4][4];
 #pragma unroll
-        for (int v = 0; v < VI; ++v) {
-            const int i = threadIdx.x + NT * v;
-            const int y4 = 4 * (i / GF_TX), xo = i % GF_TX;
-            if (i >= NV || x0 + xo >= w) break;
+            for (int k = 0; k < 4; ++k) {
+                const float l = luma_of(cur.ch(3 * k), cur.ch(3 * k + 1), cur.ch(3 * k + 2));
+                const bool ink = VEC ? in : (in && (unsigned)(xr + k) < (unsigned)w);
+                const float I = ink ? l : 0.0f, p = ink ? gv[k] - l : 0.0f;
+                q[0][k] = I;
+                q[1][k] = p;
+                q[2][k] = I * I;
+                q[3][k] = I * p;
+            }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (y0 + y4 + j >= h) break;
-                sA[y4 + j][xo] = mv[v][0][j];
-                sB[y4 + j][xo] = mv[v][1][j];
+            for (int c = 0; c < 4; ++c) {
+                float s[4];
+                hsum_lanes(q[c], s);
+                *(v4f *)&hs[wv][c][4 * ln] = v4f{s[0], s[1], s[2], s[3]};
             }
         }
         __syncthreads();
-        // 4 pixels of a row per thread: q = mean(a) I + mean(b), out = clamp(src + q)
-        for (int i = threadIdx.x; i < TY * (GF_TX / 4); i += NT) {
-            const int r = i / (GF_TX / 4), x4 = 4 * (i % (GF_TX / 4));
-            const int gy = y0 + r, gx = x0 + x4;
-            if (gy >= h || gx >= w) continue;
-            const size_t b0 = ((size_t)gy * w + gx) * 3;
-            const GLOBAL_AS unsigned *qs = (const GLOBAL_AS unsigned *)(gp(J.src_u8) + b0);
-            const unsigned d[3] = {qs[0], qs[1], qs[2]};
-            const float4 ma = *reinterpret_cast<const float4 *>(&sA[r][x4]);
-            const float4 mb = *reinterpret_cast<const float4 *>(&sB[r][x4]);
-            const float mav[4] = {ma.x, ma.y, ma.z, ma.w}, mbv[4] = {mb.x, mb.y, mb.z, mb.w};
-            unsigned o[3] = {0u, 0u, 0u};
+        // B: vertical sums, means, a and b of row group m - 1
+        {
+            float hv[4][4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) hv[c][j] = hs[j][c][threadIdx.x];
+            float S[4][4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v1[c].step(hv[c][0], hv[c][1], hv[c][2], hv[c][3], S[c]);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                float c[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) c[k] = u8f((d[(3 * j + k) / 4] >> (8 * ((3 * j + k) % 4))) & 0xFFu);
-                const float q = fmaf(mav[j], luma_of(c[0], c[1], c[2]), mbv[j]);
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-                    o[(3 * j + k) / 4] |= (unsigned)egress_u8(clamp01(c[k] + q)) << (8 * ((3 * j + k) % 4));
+                const int y = 4 * (m - 1) + j;
+                const float k = inv_area(cx, kx9, y, h, sRcp);
+                const float mI = S[0][j] * k, mp = S[1][j] * k, cI = S[2][j] * k, cIp = S[3][j] * k;
+                const float var = fmaf(-mI, mI, cI);
+                const float cov = fmaf(-mI, mp, cIp);
+                // var + eps lies in [0.0099, 1.02): I is in [0, 1]
+                const float av = div_mid(cov, var + 0.01f);
+                const float bv = fmaf(-av, mI, mp);
+                const bool in = col_in && (unsigned)y < (unsigned)h;
+                ab[j][0][threadIdx.x] = in ? av : 0.0f;
+                ab[j][1][threadIdx.x] = in ? bv : 0.0f;
             }
-            GLOBAL_AS unsigned *os = (GLOBAL_AS unsigned *)(gp(J.out_u8) + b0);
-            os[0] = o[0];
-            os[1] = o[1];
-            os[2] = o[2];
         }
+        __syncthreads();
+        // C: horizontal sums of a and b of row 4(m - 1) + wv, in place (each lane rewrites
+        // only the 4 columns it read)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const v4f a = *(const v4f *)&ab[wv][c][4 * ln];
+            const float o[4] = {a.x, a.y, a.z, a.w};
+            float s[4];
+            hsum_lanes(o, s);
+            *(v4f *)&ab[wv][c][4 * ln] = v4f{s[0], s[1], s[2], s[3]};
+        }
+        __syncthreads();
+        // D: vertical sums of a and b: their means at row group m - 2, in place
+        {
+            float hv[2][4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                hv[0][j] = ab[j][0][threadIdx.x];
+                hv[1][j] = ab[j][1][threadIdx.x];
+            }
+            float S[2][4];
+            v2[0].step(hv[0][0], hv[0][1], hv[0][2], hv[0][3], S[0]);
+            v2[1].step(hv[1][0], hv[1][1], hv[1][2], hv[1][3], S[1]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float k = inv_area(cx, kx9, 4 * (m - 2) + j, h, sRcp);
+                ab[j][0][threadIdx.x] = S[0][j] * k;
+                ab[j][1][threadIdx.x] = S[1][j] * k;
+            }
+        }
+        __syncthreads();
+        // E: output row 4(m - 2) + wv
+        {
+            const int y = 4 * (m - 2) + wv;
+            if (y >= ybeg && y < yend && out_lane && xr < w) {
+                const v4f ma = *(const v4f *)&ab[wv][0][4 * ln];
+                const v4f mb = *(const v4f *)&ab[wv][1][4 * ln];
+                const float mav[4] = {ma.x, ma.y, ma.z, ma.w}, mbv[4] = {mb.x, mb.y, mb.z, mb.w};
+                float o[12];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float c0 = old2.ch(3 * k), c1 = old2.ch(3 * k + 1), c2 = old2.ch(3 * k + 2);
+                    const float q = fmaf(mav[k], luma_of(c0, c1, c2), mbv[k]);
+                    o[3 * k] = clamp01(c0 + q);
+                    o[3 * k + 1] = clamp01(c1 + q);
+                    o[3 * k + 2] = clamp01(c2 + q);
+                }
+                const size_t base = ((size_t)y * w + xr) * 3;
+                if (!OUTF || J.out_u8) {
+                    if (VEC) {
+                        unsigned d[3] = {0u, 0u, 0u};
+#pragma unroll
+                        for (int i = 0; i < 12; ++i) d[i / 4] |= (unsigned)egress_u8(o[i]) << (8 * (i % 4));
+                        GLOBAL_AS unsigned *os = (GLOBAL_AS unsigned *)(gp(J.out_u8) + base);
+                        os[0] = d[0];
+                        os[1] = d[1];
+                        os[2] = d[2];
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 12; ++i)
+                            if (xr + i / 3 < w) gp(J.out_u8)[base + i] = egress_u8(o[i]);
+                    }
+                }
+                if (OUTF && J.out_f32) {
+                    if (VEC) {
+                        GLOBAL_AS v4f *os = (GLOBAL_AS v4f *)(gp(J.out_f32) + base);
+                        os[0] = v4f{o[0], o[1], o[2], o[3]};
+                        os[1] = v4f{o[4], o[5], o[6], o[7]};
+                        os[2] = v4f{o[8], o[9], o[10], o[11]};
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 12; ++i)
+                            if (xr + i / 3 < w) gp(J.out_f32)[base + i] = o[i];
+                    }
+                }
+            }
+        }
+        old2 = old1;
+        old1 = cur;
     }
 }
 
-template <int TY>
-__device__ __forceinline__ void gf2_run(const EgressJob *__restrict__ jobs, int njobs, float (*sA)[GF_PITCH],
-                                        float (*sB)[GF_PITCH], float *sRcp) {
-    fill_rcp(sRcp);
-    int x0, y0;
-    const EgressJob &J = gf_tile<TY>(jobs, njobs, x0, y0);
-    const bool vout = J.src_u8 && J.out_u8 && !J.out_f32 && (J.w & 3) == 0 &&
-                      (((uintptr_t)J.src_u8 | (uintptr_t)J.out_u8) & 3) == 0;
-    if (vout) gf2_body<true, TY, true>(J, x0, y0, sA, sB, sRcp);
-    else if (J.src_u8) gf2_body<true, TY, false>(J, x0, y0, sA, sB, sRcp);
-    else gf2_body<false, TY, false>(J, x0, y0, sA, sB, sRcp);
+// The workgroup's job, strip and segment.
+struct GfTile {
+    const EgressJob *J;
+    int x0, ybeg, yend;
+};
+__device__ __forceinline__ GfTile gf_tile(const EgressJob *__restrict__ jobs, int njobs, float *sRcp) {
+    if (threadIdx.x < GF_RCP) sRcp[threadIdx.x] = threadIdx.x ? 1.0f / (float)threadIdx.x : 0.0f;
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const EgressJob &J = jobs[find_job(jobs, njobs, bid)];
+    const int t = bid - J.tile_begin;
+    const int ybeg = (t / J.tiles_x) * J.seg;
+    __syncthreads();
+    return GfTile{&J, (t % J.tiles_x) * GF_OUT, ybeg, min(ybeg + J.seg, J.h)};
 }
 
-// One kernel per tile height (GF_TX x TY outputs per block); the session's tuner picks one.
-#define WEMA_GF_KERNELS(TY)                                                                    \
-    __global__ void __launch_bounds__(GF_NT(TY)) k_gf1_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
-        __shared__ __attribute__((aligned(16))) float sI[TY + 8][GF_PITCH];                    \
-        __shared__ __attribute__((aligned(16))) float sP[TY + 8][GF_PITCH];                    \
-        __shared__ __attribute__((aligned(16))) float sII[TY + 8][GF_PITCH];                   \
-        __shared__ __attribute__((aligned(16))) float sIP[TY + 8][GF_PITCH];                   \
-        __shared__ float sRcp[GF_RCP];                                                         \
-        gf1_run<TY>(jobs, njobs, sI, sP, sII, sIP, sRcp);                                      \
-    }                                                                                          \
-    __global__ void __launch_bounds__(GF_NT(TY)) k_gf2_r##TY(const EgressJob *__restrict__ jobs, int njobs) { \
-        __shared__ __attribute__((aligned(16))) float sA[TY + 8][GF_PITCH];                    \
-        __shared__ __attribute__((aligned(16))) float sB[TY + 8][GF_PITCH];                    \
-        __shared__ float sRcp[GF_RCP];                                                         \
-        gf2_run<TY>(jobs, njobs, sA, sB, sRcp);                                                \
+#define WEMA_GF_SHARED                                              \
+    __shared__ __attribute__((aligned(16))) float hs[4][4][GF_W];   \
+    __shared__ __attribute__((aligned(16))) float ab[4][2][GF_W];   \
+    __shared__ float sRcp[GF_RCP];
+
+// RGB24 in and out, width a multiple of 4, frames and planes aligned (the host checks every
+// job of the launch): the pipeline's path, with the fewest registers.
+__global__ void __launch_bounds__(256) k_gf(const EgressJob *__restrict__ jobs, int njobs) {
+    WEMA_GF_SHARED
+    const GfTile T = gf_tile(jobs, njobs, sRcp);
+    gf_strip<true, true, false>(*T.J, T.x0, T.ybeg, T.yend, hs, ab, sRcp);
+}
+
+// Any job: float RGB in or out, any width.
+__global__ void __launch_bounds__(256) k_gf_any(const EgressJob *__restrict__ jobs, int njobs) {
+    WEMA_GF_SHARED
+    const GfTile T = gf_tile(jobs, njobs, sRcp);
+    const EgressJob &J = *T.J;
+    const uintptr_t al = (uintptr_t)J.gray | (J.src_u8 ? 0 : (uintptr_t)J.src_f32) | (J.out_f32 ? (uintptr_t)J.out_f32 : 0);
+    const bool vec = (J.w & 3) == 0 && (al & 15) == 0 && (((uintptr_t)J.src_u8 | (uintptr_t)J.out_u8) & 3) == 0;
+    if (J.src_u8) {
+        if (vec) gf_strip<true, true, true>(J, T.x0, T.ybeg, T.yend, hs, ab, sRcp);
+        else gf_strip<true, false, true>(J, T.x0, T.ybeg, T.yend, hs, ab, sRcp);
+    } else {
+        if (vec) gf_strip<false, true, true>(J, T.x0, T.ybeg, T.yend, hs, ab, sRcp);
+        else gf_strip<false, false, true>(J, T.x0, T.ybeg, T.yend, hs, ab, sRcp);
     }
-WEMA_GF_KERNELS(8)
-WEMA_GF_KERNELS(16)
-WEMA_GF_KERNELS(32)
+}
 
 // RGB out = clamp(source + q) for the 5x5 path (src/wema.c:329-334).
 __device__ __forceinline__ void write_rgb(const EgressJob &J, size_t idx, float q) {
@@ -1871,20 +1723,11 @@ hipError_t wema_launch_win(int variant, const TemporalJob *jobs, int njobs, int 
     else hipLaunchKernelGGL(k_win, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError();
 }
-hipError_t wema_launch_egress(int kind, int rows, const EgressJob *jobs, int njobs, int nblocks,
-                              hipStream_t s) {
+hipError_t wema_launch_egress(int kind, const EgressJob *jobs, int njobs, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     switch (kind) {
-    case EG_GF1:
-        if (rows == 8) hipLaunchKernelGGL(k_gf1_r8, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
-        else if (rows == 32) hipLaunchKernelGGL(k_gf1_r32, dim3(nblocks), dim3(GF_NT(32)), 0, s, jobs, njobs);
-        else hipLaunchKernelGGL(k_gf1_r16, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
-        break;
-    case EG_GF2:
-        if (rows == 8) hipLaunchKernelGGL(k_gf2_r8, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
-        else if (rows == 32) hipLaunchKernelGGL(k_gf2_r32, dim3(nblocks), dim3(GF_NT(32)), 0, s, jobs, njobs);
-        else hipLaunchKernelGGL(k_gf2_r16, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
-        break;
+    case EG_GF: hipLaunchKernelGGL(k_gf, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
+    case EG_GF_ANY: hipLaunchKernelGGL(k_gf_any, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     case EG_GAUSS: hipLaunchKernelGGL(k_gauss, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     default: hipLaunchKernelGGL(k_passthru, dim3(nblocks), dim3(256), 0, s, jobs, njobs); break;
     }

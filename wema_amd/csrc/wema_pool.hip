@@ -18,7 +18,6 @@
 #include "wema_host.h"
 
 int wema_session_setup(wema_dev *s, int with_filt, bool pooled);
-int wema_dev_open_ex(wema_dev_t **out, const wema_dev_params *p, bool pooled);
 
 #define HIP_OK(call)                                                                       \
     do {                                                                                   \
@@ -208,10 +207,8 @@ extern "C" int wema_pool_timing_report(wema_pool_t *p, int kind, char *name, int
 static int egress_kind(const wema_dev *s) {
     if (!s->p.use_iir && s->frames + 1 < s->p.window) return EG_PASSTHRU;
     if (s->p.color) return EG_KINDS;
-    // (the guided filter's two tile heights, as one code)
-    return (s->p.edge_aware || s->p.batch_path)
-               ? EG_KINDS + 1 + wema_rows_index(s->gf_rows) + 3 * wema_rows_index(s->gf_rows2)
-               : EG_GAUSS;
+    // (the guided filter with its segment height, 0: the launch's split)
+    return (s->p.edge_aware || s->p.batch_path) ? EG_KINDS + 1 + s->gf_seg : EG_GAUSS;
 }
 
 extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8_t *const *d_in,

@@ -234,11 +234,16 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
             }
     };
 
-    // guided-filter tile heights of the mix's sizes, tuned before the clock starts (the
-    // first session of a size would otherwise time the candidate tiles while ticks run)
+    // DWT strip heights of the mix's sizes, tuned before the clock starts (the first session
+    // of a size would otherwise time the candidate strips while ticks run): one session of
+    // each size is opened and closed
     for (const ResClass &c : kMix) {
-        int r1, r2;
-        wema_gf_rows_for(cfg->device, c.w, c.h, &r1, &r2);
+        wema_dev_params tp;
+        wema_dev_params_default(&tp, c.w, c.h, 30.0f);
+        tp.device = cfg->device;
+        wema_dev_t *ts = nullptr;
+        if (wema_dev_open_ex(&ts, &tp, true) < 0) { free_frames(); return -1; }
+        wema_dev_close(ts);
     }
 
     wema_pool_t *pool = nullptr;

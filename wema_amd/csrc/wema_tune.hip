@@ -1,11 +1,13 @@
 // Tile-shape autotuners.
 //
-// Guided filter: its kernels come in tile heights of 8, 16 and 32 rows (64 columns each);
-// which is fastest depends on the frame size and the pass (edge tiles, blocks per XCD, LDS
-// per block). The first session of a frame size on a device times the three variants of
-// each pass on frames of that size and keeps the fastest of each for the process. Tile
-// shapes never change results (the box sums are anchored to the image), so the two passes
-// may use different heights.
+// Guided filter: its kernel streams strips of 240 columns down segments of rows; every
+// segment recomputes the 16 rows above it, so tall segments waste the least work and short
+// ones give more workgroups. The first plain session of a shape (frame size, frames per
+// call) on a device times the strips split into 1, 2, 4, .. 64 equal segments (each at
+// least 32 rows) on its own per-call buffers when it is opened, and keeps the fastest.
+// Pooled sessions leave the split to each launch (JobBuilder::segments_of), whose size
+// depends on the other sessions of the tick. The split never changes a result: the window
+// sums are anchored to the image.
 //
 // DWT strips: every forward and inverse level runs as strips 60 pair columns wide and 8, 16
 // or 32 pair rows tall; taller strips re-read fewer halo rows, shorter ones give more waves
@@ -27,8 +29,7 @@
 #include "../../include/wema_dev.h"
 #include "wema_host.h"
 
-extern "C" hipError_t wema_launch_egress(int kind, int rows, const EgressJob *jobs, int njobs, int nblocks,
-                                         hipStream_t s);
+extern "C" hipError_t wema_launch_egress(int kind, const EgressJob *jobs, int njobs, int nblocks, hipStream_t s);
 extern "C" hipError_t wema_launch_dwt_fwd(const DwtfJob *, int, int, int, hipStream_t);
 extern "C" hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, int, hipStream_t);
 extern "C" hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *, int, int, hipStream_t);
@@ -36,12 +37,11 @@ extern "C" hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *, int, int, hipStre
 namespace {
 
 std::mutex g_mu;
-std::map<std::tuple<int, int, int>, std::pair<int, int>> g_rows;   // (device, w, h) -> rows of each pass
+std::map<std::tuple<int, int, int, int>, int> g_segs;   // (device, w, h, frames per call) -> segment rows
 // (device, kind, w, h, nch, frames per call) -> strip height; kind: 0 forward level 0,
 // 1 forward coarser level, 2 inverse coarser level, 3 inverse level 0, 4 colour level-0 inverse
 std::map<std::tuple<int, int, int, int, int, int>, int> g_strips;
 int g_mask = WEMA_TUNE_GF | WEMA_TUNE_DWT;
-const int kRows[3] = {8, 16, 32};
 
 struct Scratch {
     void *p[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -57,92 +57,7 @@ struct Scratch {
     }
 };
 
-// Mean time of k_gf1 (ms1) and of k_gf2 (ms2) over F frames at each tile height; -1 on
-// error.
-int time_variants(int w, int h, double *ms1, double *ms2) {
-    const size_t P = (size_t)w * h;
-    const int F = (int)std::max<size_t>(1, std::min<size_t>(32, ((size_t)4 << 20) / P));
-    Scratch sc;
-    float *gray = nullptr;
-    float2 *ab = nullptr;
-    uint8_t *src = nullptr, *out = nullptr;
-    EgressJob *jobs_d = nullptr;
-    if (hipMalloc((void **)&gray, sizeof(float) * P * F) != hipSuccess) return -1;
-    sc.p[0] = gray;
-    if (hipMalloc((void **)&ab, sizeof(float2) * P * F) != hipSuccess) return -1;
-    sc.p[1] = ab;
-    if (hipMalloc((void **)&src, 3 * P * F) != hipSuccess) return -1;
-    sc.p[2] = src;
-    if (hipMalloc((void **)&out, 3 * P * F) != hipSuccess) return -1;
-    sc.p[3] = out;
-    if (hipMalloc((void **)&jobs_d, sizeof(EgressJob) * 2 * F) != hipSuccess) return -1;
-    sc.p[4] = jobs_d;
-    if (hipStreamCreateWithFlags(&sc.st, hipStreamNonBlocking) != hipSuccess) return -1;
-    if (hipEventCreate(&sc.e0) != hipSuccess || hipEventCreate(&sc.e1) != hipSuccess) return -1;
-    if (hipMemsetAsync(gray, 0, sizeof(float) * P * F, sc.st) != hipSuccess ||
-        hipMemsetAsync(src, 0x80, 3 * P * F, sc.st) != hipSuccess)
-        return -1;
-    for (int v = 0; v < 3; v++) {
-        const int rows = kRows[v];
-        const int tiles_x = wema_ceil_div(w, WEMA_GF_TX), tiles = tiles_x * wema_ceil_div(h, rows);
-        std::vector<EgressJob> jobs((size_t)F);
-        for (int f = 0; f < F; f++) {
-            EgressJob j{};
-            j.gray = gray + (size_t)f * P;
-            j.src_u8 = src + (size_t)f * P * 3;
-            j.ab = ab + (size_t)f * P;
-            j.out_u8 = out + (size_t)f * P * 3;
-            j.w = w;
-            j.h = h;
-            j.tiles_x = tiles_x;
-            j.tile_begin = f * tiles;
-            jobs[(size_t)f] = j;
-        }
-        if (hipMemcpyAsync(jobs_d, jobs.data(), sizeof(EgressJob) * F, hipMemcpyHostToDevice, sc.st) != hipSuccess)
-            return -1;
-        const int nb = F * tiles;
-        for (int k = 0; k < 2; k++) {
-            const int kind = k == 0 ? EG_GF1 : EG_GF2;
-            if (wema_launch_egress(kind, rows, jobs_d, F, nb, sc.st) != hipSuccess) return -1;   // warm-up
-            const int reps = 3;
-            if (hipEventRecord(sc.e0, sc.st) != hipSuccess) return -1;
-            for (int r = 0; r < reps; r++)
-                if (wema_launch_egress(kind, rows, jobs_d, F, nb, sc.st) != hipSuccess) return -1;
-            if (hipEventRecord(sc.e1, sc.st) != hipSuccess || hipEventSynchronize(sc.e1) != hipSuccess)
-                return -1;
-            float t = 0.0f;
-            if (hipEventElapsedTime(&t, sc.e0, sc.e1) != hipSuccess) return -1;
-            (k == 0 ? ms1 : ms2)[v] = t / reps;
-        }
-    }
-    return 0;
-}
-
 }  // namespace
-
-// Tile rows of each guided-filter pass at this frame size (timed once per size and device).
-void wema_gf_rows_for(int device, int w, int h, int *rows1, int *rows2) {
-    *rows1 = *rows2 = 16;
-    if (!(g_mask & WEMA_TUNE_GF) || w < 4 || h < 4) return;
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto key = std::make_tuple(device, w, h);
-    auto it = g_rows.find(key);
-    if (it == g_rows.end()) {
-        double ms1[3] = {0, 0, 0}, ms2[3] = {0, 0, 0};
-        std::pair<int, int> rows(16, 16);
-        if (time_variants(w, h, ms1, ms2) == 0) {
-            int b1 = 1, b2 = 1;
-            for (int v = 0; v < 3; v++) {
-                if (ms1[v] < ms1[b1]) b1 = v;
-                if (ms2[v] < ms2[b2]) b2 = v;
-            }
-            rows = std::make_pair(kRows[b1], kRows[b2]);
-        }
-        it = g_rows.emplace(key, rows).first;
-    }
-    *rows1 = it->second.first;
-    *rows2 = it->second.second;
-}
 
 extern "C" void wema_dev_autotune(int mask) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -283,20 +198,83 @@ extern "C" int wema_dev_set_strip_rows(wema_dev_t *s, int rows) {
     return 0;
 }
 
-extern "C" int wema_dev_tile_rows(const wema_dev_t *s) { return s ? s->gf_rows : -1; }
-
-extern "C" int wema_dev_tile_rows_pass(const wema_dev_t *s, int pass) {
-    if (!s || (pass != 1 && pass != 2)) return -1;
-    return pass == 1 ? s->gf_rows : s->gf_rows2;
-}
-
-extern "C" int wema_dev_set_tile_rows_pass(wema_dev_t *s, int pass, int rows) {
-    if (!s) { wema_set_error("null session"); return -1; }
-    if (pass < 0 || pass > 2) { wema_set_error("pass must be 0 (both), 1 or 2"); return -1; }
-    if (rows != 8 && rows != 16 && rows != 32) { wema_set_error("tile rows must be 8, 16 or 32"); return -1; }
-    if (pass != 2) s->gf_rows = rows;
-    if (pass != 1) s->gf_rows2 = rows;
+// Guided filter segments of a session: times the candidate splits on the session's own
+// per-call buffers (the reconstruction planes as the guide input, the coefficient planes as
+// source and output frames; their contents do not matter to the timing).
+int wema_gf_tune(wema_dev *s) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!(g_mask & WEMA_TUNE_GF) || s->pooled || s->p.color || !(s->p.edge_aware || s->p.batch_path)) return 0;
+    const int n = s->p.max_batch, dev = s->p.device, w = s->g.W, h = s->g.H;
+    const auto key = std::make_tuple(dev, w, h, n);
+    auto it = g_segs.find(key);
+    if (it == g_segs.end()) {
+        const size_t P = s->g.P;
+        std::vector<int> cand;
+        for (int nseg = 1; nseg <= 64; nseg *= 2) {
+            const int seg = JobBuilder::gf_seg_rows(h, nseg);
+            if (cand.empty() || seg < cand.back()) cand.push_back(seg);
+        }
+        Scratch sc;
+        EgressJob *jobs_d = nullptr;
+        if (hipStreamCreateWithFlags(&sc.st, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&sc.e0) != hipSuccess || hipEventCreate(&sc.e1) != hipSuccess ||
+            hipMalloc((void **)&jobs_d, sizeof(EgressJob) * (size_t)n) != hipSuccess) {
+            wema_set_error("guided-filter autotuner set-up failed");
+            return -1;
+        }
+        sc.p[0] = jobs_d;
+        std::vector<EgressJob> jobs((size_t)n);
+        double best = 0.0;
+        int best_seg = cand[0];
+        for (int seg : cand) {
+            const int strips = wema_ceil_div(w, WEMA_GF_OUT), tiles = strips * wema_ceil_div(h, seg);
+            for (int f = 0; f < n; f++) {
+                EgressJob j{};
+                j.gray = s->rec + (size_t)f * P;
+                j.src_u8 = (const uint8_t *)s->coef + (size_t)f * P * 3;
+                j.out_u8 = (uint8_t *)s->coef + (size_t)f * P * 3;
+                j.w = w;
+                j.h = h;
+                j.tiles_x = strips;
+                j.tile_begin = f * tiles;
+                j.seg = seg;
+                jobs[(size_t)f] = j;
+            }
+            const int kind = wema_gf_fast(jobs[0]) ? EG_GF : EG_GF_ANY;
+            if (hipMemcpyAsync(jobs_d, jobs.data(), sizeof(EgressJob) * (size_t)n, hipMemcpyHostToDevice, sc.st) != hipSuccess ||
+                wema_launch_egress(kind, jobs_d, n, n * tiles, sc.st) != hipSuccess ||   // warm-up
+                hipEventRecord(sc.e0, sc.st) != hipSuccess) {
+                wema_set_error("guided-filter autotuner launch failed");
+                return -1;
+            }
+            const int reps = 3;
+            for (int r = 0; r < reps; r++)
+                if (wema_launch_egress(kind, jobs_d, n, n * tiles, sc.st) != hipSuccess) {
+                    wema_set_error("guided-filter autotuner launch failed");
+                    return -1;
+                }
+            float t = 0.0f;
+            if (hipEventRecord(sc.e1, sc.st) != hipSuccess || hipEventSynchronize(sc.e1) != hipSuccess ||
+                hipEventElapsedTime(&t, sc.e0, sc.e1) != hipSuccess) {
+                wema_set_error("guided-filter autotuner timing failed");
+                return -1;
+            }
+            if (seg == cand[0] || t < best) {
+                best = t;
+                best_seg = seg;
+            }
+        }
+        it = g_segs.emplace(key, best_seg).first;
+    }
+    s->gf_seg = it->second;
     return 0;
 }
 
-extern "C" int wema_dev_set_tile_rows(wema_dev_t *s, int rows) { return wema_dev_set_tile_rows_pass(s, 0, rows); }
+extern "C" int wema_dev_tile_rows(const wema_dev_t *s) { return s ? s->gf_seg : -1; }
+
+extern "C" int wema_dev_set_tile_rows(wema_dev_t *s, int rows) {
+    if (!s) { wema_set_error("null session"); return -1; }
+    if (rows < 0 || rows % 4 != 0) { wema_set_error("segment rows must be 0 (the launch's split) or a positive multiple of 4"); return -1; }
+    s->gf_seg = rows;
+    return 0;
+}

@@ -34,12 +34,12 @@ def _one(pattern):
 
 
 def _short(name):
-    """Kernel name without its argument list; the guided-filter tile variants (k_gf1_r16 ...)
-    are reported under the pass name bench.py times them by (k_gf1, k_gf2)."""
+    """Kernel name without its argument list; the guided filter's general kernel (k_gf_any)
+    is reported under the name bench.py times both by (k_gf)."""
     n = name.split("(")[0]
     if n.startswith("void "):
         n = n[5:]
-    return re.sub(r"^(k_gf[12])_r\d+$", r"\1", n)
+    return "k_gf" if n == "k_gf_any" else n
 
 
 def kernel_stats(trace_dir):
