@@ -144,13 +144,15 @@ __device__ __forceinline__ float threshold_of(unsigned long long acc, float n6f)
     return q * 0.2f;
 }
 
-// Value of lane i+1 / lane i-1 of the wave (DPP wave_shl:1 / wave_shr:1; the end lanes get
-// unspecified values, which only reach halo lanes).
+// Value of lane i+1 / lane i-1 of the wave (DPP wave_shl:1 / wave_shr:1; the end lane gets
+// 0, a value that only reaches halo lanes). With the zero fill (bound_ctrl) the compiler
+// folds the shift into the VALU instruction that uses it, as that instruction's DPP source
+// operand, instead of issuing a separate move.
 __device__ __forceinline__ float lane_next(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float lane_prev(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 
 // One lifting pass over a row held one (even, odd) pair per lane; lanes 0-1 and 62-63 are
@@ -1259,8 +1261,8 @@ constexpr int GF_W = GF_OUT + 16;     // with the 8-column halo on each side: 64
 // the block's first partial sum.
 __device__ __forceinline__ void hsum_lanes(const float (&o)[4], float (&s)[4]) {
     const float a = ((o[0] + o[1]) + o[2]) + o[3];
-    float s0 = lane_prev(a);
-    s0 = (((s0 + o[0]) + o[1]) + o[2]) + o[3];
+    float s0 = lane_prev(a) + o[0];
+    s0 = ((s0 + o[1]) + o[2]) + o[3];
     s[0] = s0 + lane_next(o[0]);
     s[1] = (s[0] - lane_prev(o[0])) + lane_next(o[1]);
     s[2] = (s[1] - lane_prev(o[1])) + lane_next(o[2]);
@@ -1274,7 +1276,6 @@ struct VSum {
     // Takes group b + 1 (rows h0..h3); gives the window sums of the 4 rows of group b.
     __device__ __forceinline__ void step(float h0, float h1, float h2, float h3, float (&s)[4]) {
         const float p2 = h0 + h1, p3 = p2 + h2, F = p3 + h3;
-        const float U2 = h2 + h3, U3 = h1 + U2;
         s[0] = t0 + h0;
         s[1] = t1 + p2;
         s[2] = t2 + p3;
@@ -1283,9 +1284,9 @@ struct VSum {
         t1 = u3 + F;
         t2 = u2 + F;
         t3 = u1 + F;
+        u2 = h2 + h3;   // (the last uses of the previous group's sums are above)
+        u3 = h1 + u2;
         u1 = h3;
-        u2 = U2;
-        u3 = U3;
         f = F;
     }
 };
@@ -1471,21 +1472,26 @@ __device__ __forceinline__ void gf_strip(const EgressJob &J, int x0, int ybeg, i
                 const v4f ma = *(const v4f *)&ab[wv][0][4 * ln];
                 const v4f mb = *(const v4f *)&ab[wv][1][4 * ln];
                 const float mav[4] = {ma.x, ma.y, ma.z, ma.w}, mbv[4] = {mb.x, mb.y, mb.z, mb.w};
+                // clamp to [0, 1]: as clamp01 for the float output (which keeps a -0); the
+                // bytes take the median of (v, 0, 1), one instruction, with the same
+                // egress (fmaf(-0, 255, 0.5) = fmaf(+0, 255, 0.5))
                 float o[12];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const float c0 = old2.ch(3 * k), c1 = old2.ch(3 * k + 1), c2 = old2.ch(3 * k + 2);
                     const float q = fmaf(mav[k], luma_of(c0, c1, c2), mbv[k]);
-                    o[3 * k] = clamp01(c0 + q);
-                    o[3 * k + 1] = clamp01(c1 + q);
-                    o[3 * k + 2] = clamp01(c2 + q);
+                    o[3 * k] = c0 + q;
+                    o[3 * k + 1] = c1 + q;
+                    o[3 * k + 2] = c2 + q;
                 }
                 const size_t base = ((size_t)y * w + xr) * 3;
                 if (!OUTF || J.out_u8) {
                     if (VEC) {
                         unsigned d[3] = {0u, 0u, 0u};
 #pragma unroll
-                        for (int i = 0; i < 12; ++i) d[i / 4] |= (unsigned)egress_u8(o[i]) << (8 * (i % 4));
+                        for (int i = 0; i < 12; ++i)
+                            d[i / 4] |= (unsigned)(unsigned char)fmaf(__builtin_amdgcn_fmed3f(o[i], 0.0f, 1.0f), 255.0f, 0.5f)
+                                        << (8 * (i % 4));
                         GLOBAL_AS unsigned *os = (GLOBAL_AS unsigned *)(gp(J.out_u8) + base);
                         os[0] = d[0];
                         os[1] = d[1];
@@ -1493,19 +1499,20 @@ __device__ __forceinline__ void gf_strip(const EgressJob &J, int x0, int ybeg, i
                     } else {
 #pragma unroll
                         for (int i = 0; i < 12; ++i)
-                            if (xr + i / 3 < w) gp(J.out_u8)[base + i] = egress_u8(o[i]);
+                            if (xr + i / 3 < w)
+                                gp(J.out_u8)[base + i] = (unsigned char)fmaf(__builtin_amdgcn_fmed3f(o[i], 0.0f, 1.0f), 255.0f, 0.5f);
                     }
                 }
                 if (OUTF && J.out_f32) {
                     if (VEC) {
                         GLOBAL_AS v4f *os = (GLOBAL_AS v4f *)(gp(J.out_f32) + base);
-                        os[0] = v4f{o[0], o[1], o[2], o[3]};
-                        os[1] = v4f{o[4], o[5], o[6], o[7]};
-                        os[2] = v4f{o[8], o[9], o[10], o[11]};
+                        os[0] = v4f{clamp01(o[0]), clamp01(o[1]), clamp01(o[2]), clamp01(o[3])};
+                        os[1] = v4f{clamp01(o[4]), clamp01(o[5]), clamp01(o[6]), clamp01(o[7])};
+                        os[2] = v4f{clamp01(o[8]), clamp01(o[9]), clamp01(o[10]), clamp01(o[11])};
                     } else {
 #pragma unroll
                         for (int i = 0; i < 12; ++i)
-                            if (xr + i / 3 < w) gp(J.out_f32)[base + i] = o[i];
+                            if (xr + i / 3 < w) gp(J.out_f32)[base + i] = clamp01(o[i]);
                     }
                 }
             }
