@@ -42,11 +42,33 @@ def _short(name):
     return "k_gf" if n == "k_gf_any" else n
 
 
+def pipeline_dispatches(rows):
+    """Dispatch ids of the pipeline's own launches: from the first step (its level-0 forward
+    DWT, the launch before its forward levels and temporal stage) on, which leaves out the
+    autotuners' timing launches at session open (they run no temporal stage); device copies
+    (the counter calibration) are always kept."""
+    order = sorted({int(r["Dispatch_Id"]): _short(r["Kernel_Name"]) for r in rows}.items())
+    names = [n for _, n in order]
+    first_t = next((i for i, n in enumerate(names) if n == "k_iir" or n.startswith("k_win")), None)
+    cut = 0
+    if first_t is not None:
+        cut = first_t
+        while cut > 0 and names[cut - 1].startswith("k_dwt_fwd"):
+            cut -= 1
+            if names[cut] == "k_dwt_fwd0":
+                break
+    return {d for i, (d, n) in enumerate(order) if i >= cut or n.startswith("__amd_rocclr_copy")}
+
+
 def kernel_stats(trace_dir):
     """Per (kernel, grid size): calls, mean / min / max duration in us, share of GPU time.
     The DWT kernels run one grid size per level, so each level gets its own row."""
     per = collections.defaultdict(list)
-    for r in csv.DictReader(open(_one(os.path.join(trace_dir, "*_kernel_trace.csv")))):
+    rows = list(csv.DictReader(open(_one(os.path.join(trace_dir, "*_kernel_trace.csv")))))
+    keep = pipeline_dispatches(rows)
+    for r in rows:
+        if int(r["Dispatch_Id"]) not in keep:
+            continue
         grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
         per[(_short(r["Kernel_Name"]), grid)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     total = sum(sum(v) for v in per.values())
@@ -56,9 +78,10 @@ def kernel_stats(trace_dir):
 
 def counters(pmc_dir, counter):
     rows = list(csv.DictReader(open(_one(os.path.join(pmc_dir, "*_counter_collection.csv")))))
+    keep = pipeline_dispatches(rows)
     per = collections.defaultdict(list)
     for r in rows:
-        if r["Counter_Name"] == counter:
+        if r["Counter_Name"] == counter and int(r["Dispatch_Id"]) in keep:
             per[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
     return per
 
@@ -159,12 +182,12 @@ def main():
                   "VALU issue fraction = SQ_INSTS_VALU x 4 cycles / (%d SIMDs x %.1f GHz x mean duration)."
                   % (SIMDS, CLOCK_HZ / 1e9), "",
                   "| kernel | launches | waves per launch | VALU / wave | LDS / wave | SALU / wave |%s VALU issue fraction |"
-                  % (" VALU / pixel |" if a.pixels_per_launch else ""),
+                  % (" VALU lane-ops / pixel |" if a.pixels_per_launch else ""),
                   "|---|---:|---:|---:|---:|---:|%s---:|" % ("---:|" if a.pixels_per_launch else "")]
         for n, v in sorted(vs.items(), key=lambda kv: -kv[1]["mean_us"]):
             ppx = ""
             if a.pixels_per_launch:
-                ppx = " %.1f |" % (v["valu_per_wave"] * v["waves"] / a.pixels_per_launch)
+                ppx = " %.1f |" % (v["valu_per_wave"] * v["waves"] * 64 / a.pixels_per_launch)
             lines.append("| %s | %d | %d | %.1f | %.1f | %.1f |%s %.3f |" % (
                 n, v["launches"], v["waves"], v["valu_per_wave"], v["lds_per_wave"], v["salu_per_wave"],
                 ppx, v["valu_frac"]))
