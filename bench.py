@@ -81,13 +81,16 @@ def read_traffic(kernel, workload):
 
 def read_valu(kernel, workload):
     """VALU issue fraction of a kernel from profiles/valu_<workload>.json (rocprofv3 SQ
-    counters of the same workload, wema_amd/profile_summary.py), or None."""
+    counters of the same workload, wema_amd/profile_summary.py), or None. The sliding
+    window's variants (k_win_r32 ...) are timed here under one name, k_win."""
     path = os.path.join(ROOT, "profiles", "valu_%s.json" % workload)
     try:
         with open(path) as f:
-            return json.load(f)["kernels"][kernel]["valu_frac"]
+            ks = json.load(f)["kernels"]
     except (OSError, KeyError, ValueError):
         return None
+    names = [n for n in ks if n == kernel or (kernel == "k_win" and n.startswith("k_win"))]
+    return ks[names[0]]["valu_frac"] if len(names) == 1 else None
 
 
 def run_gpu(cfg, name, steps, warmup, batch, gf_seg=0):
