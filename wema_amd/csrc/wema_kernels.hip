@@ -1121,18 +1121,88 @@ __device__ __forceinline__ void win_lds_body(const TemporalJob *__restrict__ job
     }
 }
 
-// Windows of exactly WS samples held in registers: the frame loop shifts the window down
-// one register and appends the newest sample (loaded one frame ahead), so every access is
-// static: no LDS column, no wrap-around index, and the time weights are uniform loads.
-template <int WS>
-struct WinReg {
-    const float (&x)[WS];
-    template <class F>
-    __device__ __forceinline__ void each(F &&fn) const {
+// Windows of exactly WS samples held in registers, frames in groups of WIN_U: frame j of a
+// group sees x[j .. j + WS - 1], so every access is static; after the group the window moves
+// down WIN_U registers (WS - 1 moves per WIN_U frames) and takes the WIN_U newest samples,
+// which were loaded while the group computed. The bilateral pass keeps each sample's
+// deviation from the centre sample from the first pass. The sums are those of win_delta,
+// in the same order.
+constexpr int WIN_U = 4;
+
+template <int WS, bool BIL>
+__device__ __forceinline__ float win_frame_reg(const float *x, float inv_ws, const GLOBAL_AS float *ltw) {
+    const float cur = x[WS / 2];
+    float mean = 0.0f, mad = 0.0f;
+    float d[BIL ? WS : 1];
 #pragma unroll
-        for (int t = 0; t < WS; ++t) fn(t, x[t]);
+    for (int t = 0; t < WS; ++t) {
+        mean += x[t];
+        if (BIL) {
+            d[t] = x[t] - cur;
+            mad += fabsf(d[t]);   // d > 0 ? d : -d, the same sum from +0
+        }
     }
-};
+    mean *= inv_ws;
+    float delta = cur - mean;
+    if (BIL) {
+        const float sigma = fmaxf(mad * inv_ws * 0.5f, 0.001f);
+        const float inv2 = 1.0f / (2.0f * sigma * sigma);
+        const float c = inv2 * -1.4426950408889634f;   // -log2(e) / (2 sigma^2)
+        float wsum = 0.0f, acc = 0.0f;
+#pragma unroll
+        for (int t = 0; t < WS; ++t) {
+            const float wt = __builtin_amdgcn_exp2f(fmaf(d[t] * d[t], c, ltw[t]));
+            wsum += wt;
+            acc = fmaf(wt, x[t] - mean, acc);
+        }
+        if (wsum > 1e-8f) delta = acc * (1.0f / wsum);
+    }
+    return delta;
+}
+
+template <int WS, bool BIL>
+__device__ __forceinline__ void win_reg_run(const TemporalJob &J, long long p, const float *sT, const float *sA) {
+    const int nframes = J.nframes;
+    const float inv_ws = 1.0f / (float)WS;
+    const float lscale = p < J.level0_end ? 0.5f : 1.0f;
+    const size_t plane = (size_t)J.coef_stride;
+    const GLOBAL_AS float *ring = gp(J.ring);
+    const GLOBAL_AS float *ltw = gp(J.ltw);
+    float x[WS + WIN_U - 1];
+    // the first WS - 1 samples of frame 0's window, then the newest samples of frames
+    // 0 .. WIN_U - 1 (ring slot ring_first + f + WS - 1 for frame f)
+    int slot = J.ring_first;
+#pragma unroll
+    for (int t = 0; t < WS - 1; ++t) {
+        x[t] = ring[(size_t)slot * plane + p];
+        if (++slot == J.ring_slots) slot = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < WIN_U; ++j) {
+        if (j < nframes) x[WS - 1 + j] = __builtin_nontemporal_load(&ring[(size_t)slot * plane + p]);
+        if (++slot == J.ring_slots) slot = 0;
+    }
+    for (int f0 = 0; f0 < nframes; f0 += WIN_U) {
+        float nxt[WIN_U];
+#pragma unroll
+        for (int j = 0; j < WIN_U; ++j) {
+            if (f0 + WIN_U + j < nframes) nxt[j] = __builtin_nontemporal_load(&ring[(size_t)slot * plane + p]);
+            if (++slot == J.ring_slots) slot = 0;
+        }
+#pragma unroll
+        for (int j = 0; j < WIN_U; ++j) {
+            const int f = f0 + j;
+            if (f < nframes) {
+                const float delta = win_frame_reg<WS, BIL>(x + j, inv_ws, ltw);
+                win_store(J, f, p, x[j + WS - 1], delta, sT, sA, lscale);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < WS - 1; ++t) x[t] = x[t + WIN_U];
+#pragma unroll
+        for (int j = 0; j < WIN_U; ++j) x[WS - 1 + j] = nxt[j];
+    }
+}
 
 template <int WS>
 __device__ __forceinline__ void win_reg_body(const TemporalJob *__restrict__ jobs, int njobs, float *sT,
@@ -1142,32 +1212,8 @@ __device__ __forceinline__ void win_reg_body(const TemporalJob *__restrict__ job
     frame_params(J, sT, sA);
     const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
     if (p >= J.n) return;
-    const int nframes = J.nframes;
-    const float inv_ws = 1.0f / (float)WS;
-    const float lscale = p < J.level0_end ? 0.5f : 1.0f;
-    const size_t plane = (size_t)J.coef_stride;
-    const GLOBAL_AS float *ring = gp(J.ring);
-    const GLOBAL_AS float *ltw = gp(J.ltw);
-    float x[WS];
-    // the first WS - 1 samples of frame 0's window, then its newest
-    int slot = J.ring_first;
-#pragma unroll
-    for (int t = 1; t < WS; ++t) {
-        x[t] = ring[(size_t)slot * plane + p];
-        if (++slot == J.ring_slots) slot = 0;
-    }
-    float next = ring[(size_t)slot * plane + p];
-    for (int f = 0; f < nframes; ++f) {
-#pragma unroll
-        for (int t = 0; t < WS - 1; ++t) x[t] = x[t + 1];
-        x[WS - 1] = next;
-        if (++slot == J.ring_slots) slot = 0;
-        if (f + 1 < nframes) next = __builtin_nontemporal_load(&ring[(size_t)slot * plane + p]);
-        const WinReg<WS> win{x};
-        float mean;
-        const float delta = win_delta(win, WS, inv_ws, x[WS / 2], J.bilateral != 0, ltw, mean);
-        win_store(J, f, p, x[WS - 1], delta, sT, sA, lscale);
-    }
+    if (J.bilateral) win_reg_run<WS, true>(J, p, sT, sA);
+    else win_reg_run<WS, false>(J, p, sT, sA);
 }
 
 #define WEMA_WIN_REG_KERNEL(WS)                                                                 \
