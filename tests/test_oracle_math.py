@@ -245,6 +245,35 @@ def test_box_and_guided_vs_brute_force(prefix):
     np.testing.assert_allclose(q32, q, atol=2e-6)
 
 
+def box_mean_f64(x, w, h, r=4):
+    """Clipped (2r + 1)^2 window means in f64 from an integral image (exact enough in f64)."""
+    x = x.reshape(h, w).astype(np.float64)
+    S = np.zeros((h + 1, w + 1))
+    S[1:, 1:] = x.cumsum(0).cumsum(1)
+    ys, xs = np.arange(h), np.arange(w)
+    y0, y1 = np.maximum(ys - r, 0), np.minimum(ys + r + 1, h)
+    x0, x1 = np.maximum(xs - r, 0), np.minimum(xs + r + 1, w)
+    tot = S[y1][:, x1] - S[y0][:, x1] - S[y1][:, x0] + S[y0][:, x0]
+    area = np.outer(y1 - y0, x1 - x0)
+    return (tot / area).reshape(-1)
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (1, 9), (9, 1), (5, 3), (4, 7), (37, 29), (64, 48), (131, 67)])
+def test_box_kernel_order_at_edge_sizes(w, h):
+    """The kernel's box order (horizontal blocks of 4, vertical groups of 4 rows with prefix and
+    suffix sums) at heights and widths below, at and past the window and the group size, on
+    values in [0, 1] and on signed ones: within 2e-6 of the f64 means. A row or column taken
+    from the wrong group, or one too many or too few, is off by far more."""
+    rng = np.random.default_rng(w * 1000 + h)
+    for x in (rng.random(w * h), rng.standard_normal(w * h) * 0.05):
+        x = x.astype(np.float32)
+        out = np.zeros(w * h, np.float32)
+        assert L.orc_box_f32(ptr(x), ptr(out), w, h, 4, 0) == 0
+        np.testing.assert_allclose(out, box_mean_f64(x, w, h), atol=2e-6)
+    # only radius 4 has the kernel's order
+    assert L.orc_box_f32(ptr(x), ptr(out), w, h, 3, 0) == -1
+
+
 def test_fp32_entry_points_flush_denormals():
     """The reference runs with FTZ/DAZ (its -Ofast build), the kernels flush denormals: the
     oracle's FP32 entry points do too, whatever the calling thread's mode. A denormal input
