@@ -165,12 +165,12 @@ int wema_dev_import_state(wema_dev_t *s, const void *host, size_t bytes);
  * segment height an autotuner picked for its shape: the first session of a frame size and
  * frames per call on a device times the strips split into 1, 2, 4, .. 64 equal segments on
  * its own buffers when it is opened and keeps the fastest for the process. Pooled sessions,
- * and sessions opened with that tuner off, leave the split to each launch (the fewest
- * segments that give the launch 2048 workgroups). The DWT levels run as strips 8, 16 or 32
- * pair rows tall: the first session of a shape (level size, channels, frames per call) on a
- * device times the three heights on its own buffers when it is opened and keeps the
- * fastest for each level (wema_dev_strip_rows: a level's forward or inverse height, 0 = the
- * fixed rule). Results never depend on either shape.
+ * and sessions opened with that tuner off, leave the split to each launch (one segment
+ * height for all its frames, from a cost model of its rows and workgroups). The DWT levels
+ * run as strips 8, 16 or 32 pair rows tall: the first session of a shape (level size,
+ * channels, frames per call) on a device times the three heights on its own buffers when
+ * it is opened and keeps the fastest for each level (wema_dev_strip_rows: a level's
+ * forward or inverse height, 0 = the fixed rule). Results never depend on either shape.
  * wema_dev_autotune(mask) selects the tuners for sessions opened later: WEMA_TUNE_GF,
  * WEMA_TUNE_DWT (both by default; 0: the launch's split and the strips' fixed rule).
  * wema_dev_tile_rows: the session's guided-filter segment height (0: the launch's split);

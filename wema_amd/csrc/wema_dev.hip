@@ -524,8 +524,10 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
             }
     }
 
-    // temporal stage: one job per channel over all n frames
-    const int tiles_t = (int)((nu + 255) / 256);
+    // temporal stage: one job per channel over all n frames (an IIR job of one frame takes
+    // WEMA_IIR_PPT positions per thread)
+    const int ppt = p.use_iir && n == 1 ? WEMA_IIR_PPT : 1;
+    const int tiles_t = (int)((nu + 256 * ppt - 1) / (256 * ppt));
     for (int c = 0; c < nch; c++) {
         TemporalJob j{};
         j.n = (long long)nu;
@@ -547,6 +549,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
             memcpy(j.c, s->c10, sizeof(j.c));
             j.preseed = frame0 == 0;
             j.warmup = s->warmup;
+            j.ppt = ppt;
         } else {
             j.coef = nullptr;
             j.ring = s->coef + (size_t)c * s->ring_slots * nus;

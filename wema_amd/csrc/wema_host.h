@@ -19,10 +19,10 @@
 #endif
 
 #define WEMA_MAX_LEVELS 12
-// Guided filter: workgroups a launch should have before its strips are split into more
-// segments (8 per CU).
-#ifndef WEMA_GF_WG_TARGET
-#define WEMA_GF_WG_TARGET 2048
+// Guided filter: workgroups that run at a time (256 CUs x 4: each takes 4 waves, and the
+// kernel's registers allow 4 waves per SIMD), for the launch's segment height.
+#ifndef WEMA_GF_CONCURRENT
+#define WEMA_GF_CONCURRENT 1024
 #endif
 #define WEMA_STAGE_BYTES (8 << 20)
 
@@ -231,24 +231,39 @@ struct JobBuilder {
         jl.blocks = blocks;
     }
     // Segment height of each guided-filter job: its session's tuned height (seg set), else
-    // one split for the rest of the launch: the fewest equal segments per strip (1, 2, 4,
-    // .. 64, each at least 32 rows) that give WEMA_GF_WG_TARGET workgroups. Every segment
-    // recomputes the 16 rows above it; results never depend on the split.
+    // one height for the rest of the launch, picked by a cost model: a launch with R rows to
+    // process over all its workgroups (each segment recomputes the 16 rows above it) and C
+    // workgroups running at a time takes about max(R / C, the rows of its longest
+    // workgroup). Frames of different sizes get the same segment height, so a small frame is
+    // not cut into small segments to make up a large one's count. Results never depend on
+    // the split.
     static int gf_seg_rows(int h, int nseg) {
         const int seg = 4 * wema_ceil_div(h, 4 * nseg), whole = 4 * wema_ceil_div(h, 4);
         return std::max(seg, std::min(32, whole));
     }
-    static void segments_of(std::vector<EgressJob> &v, JobList &jl, long target) {
+    static void segments_of(std::vector<EgressJob> &v, JobList &jl, long concurrent) {
         if (v.empty()) return;
-        int nseg = 1;
-        for (; nseg < 64; nseg *= 2) {
-            long work = 0;
-            for (const auto &j : v) work += (long)j.tiles_x * wema_ceil_div(j.h, j.seg ? j.seg : gf_seg_rows(j.h, nseg));
-            if (work >= target) break;
+        int hmax = 0;
+        for (const auto &j : v) hmax = std::max(hmax, j.h);
+        const int top = 4 * wema_ceil_div(hmax, 4);
+        int best = top;
+        double best_t = 0.0;
+        for (int seg = 32; ; seg = seg < 64 ? seg + 16 : seg + seg / 2) {
+            const int sg = std::min(4 * wema_ceil_div(seg, 4), top);
+            double rows = 0.0;
+            int longest = 0;
+            for (const auto &j : v) {
+                const int js = j.seg ? j.seg : sg, ns = wema_ceil_div(j.h, js);
+                rows += (double)j.tiles_x * (j.h + 16.0 * ns);
+                longest = std::max(longest, std::min(js, j.h) + 16);
+            }
+            const double t = std::max(rows / (double)concurrent, (double)longest);
+            if (best_t == 0.0 || t < best_t) { best_t = t; best = sg; }
+            if (sg >= top) break;
         }
         int blocks = 0;
         for (auto &j : v) {
-            if (!j.seg) j.seg = gf_seg_rows(j.h, nseg);
+            if (!j.seg) j.seg = best;
             j.tile_begin = blocks;
             blocks += j.tiles_x * wema_ceil_div(j.h, j.seg);
         }
@@ -261,8 +276,8 @@ struct JobBuilder {
             strips_of(dwti_v[st], dwti[st],
                       st < WEMA_MAX_LEVELS - 1 ? WEMA_STRIP_TARGET_INV_COARSE : WEMA_STRIP_TARGET_INV);
         strips_of(dwti_ycc_v, dwti_ycc, WEMA_STRIP_TARGET_INV);
-        segments_of(egress_v[EG_GF], egress[EG_GF], WEMA_GF_WG_TARGET);
-        segments_of(egress_v[EG_GF_ANY], egress[EG_GF_ANY], WEMA_GF_WG_TARGET);
+        segments_of(egress_v[EG_GF], egress[EG_GF], WEMA_GF_CONCURRENT);
+        segments_of(egress_v[EG_GF_ANY], egress[EG_GF_ANY], WEMA_GF_CONCURRENT);
     }
     int pack(void *dst_v, size_t cap, size_t *used) {
         assign_strips();

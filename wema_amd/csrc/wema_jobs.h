@@ -47,11 +47,8 @@
 #define WEMA_GF_TX 64
 #define WEMA_GF_TY 16
 // Guided filter: output columns of a strip (a workgroup's 64 lanes x 4 columns less the
-// 8-column halo on each side), and the segment heights (rows per workgroup, multiples of 4)
-// the autotuner chooses from.
+// 8-column halo on each side).
 #define WEMA_GF_OUT 240
-#define WEMA_GF_SEGS 6
-__host__ __device__ static inline int wema_gf_seg_rows(int i) { return 32 << i; }   // 32 .. 1024
 
 enum SrcKind {
     SRC_F32_PLANE = 0,   // float image (the previous level's LL)
@@ -154,7 +151,13 @@ struct TemporalJob {
     int bilateral;
     int tile_begin;
     int tiles;
+    int ppt;                        // IIR, one frame: positions per thread (256 ppt per block)
 };
+
+// IIR jobs of one frame take this many positions per thread.
+#ifndef WEMA_IIR_PPT
+#define WEMA_IIR_PPT 4
+#endif
 
 // Guided filter / Gaussian / colour egress over one output frame.
 struct EgressJob {

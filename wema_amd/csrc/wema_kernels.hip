@@ -937,16 +937,70 @@ __device__ __forceinline__ void iir_frames(const TemporalJob &J, const float *c,
     }
 }
 
+// One frame, PPT positions per thread (p0 + 256 k), the path of one frame per call and of
+// the serving loop's ticks: 40 bytes move per position, so a block of one position per
+// thread would spend most of its life on its set-up (finding its job, the frame's threshold)
+// and on one memory round trip; here the loads of all PPT positions are in flight together
+// and the set-up serves PPT times the positions.
+template <int PPT, bool SEP, bool FILT>
+__device__ __forceinline__ void iir_one_frame(const TemporalJob &J, const float *c, const float *sT,
+                                              const float *sA, long long p0) {
+    GLOBAL_AS float *in = gp(J.coef);
+    GLOBAL_AS float *out = SEP ? gp(J.amp_out) : gp(J.coef);
+    GLOBAL_AS float *sp = gp((float *)J.state);
+    float x[PPT];
+    float4 st[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const long long p = p0 + 256 * k < J.n ? p0 + 256 * k : p0;   // (a valid address)
+        x[k] = __builtin_nontemporal_load(&in[p]);
+        if (!J.preseed)
+            st[k] = make_float4(__builtin_nontemporal_load(&sp[4 * p]), __builtin_nontemporal_load(&sp[4 * p + 1]),
+                                __builtin_nontemporal_load(&sp[4 * p + 2]), __builtin_nontemporal_load(&sp[4 * p + 3]));
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const long long p = p0 + 256 * k;
+        if (J.preseed) {
+            // warm-up on frame 0 (src/iir_filter.c:376-471), register-resident
+            st[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int it = 0; it < J.warmup; ++it) (void)biquad2(c, st[k], x[k]);
+        }
+        float y;
+        const float o = iir_amp(c, st[k], x[k], sT[0], sA[0] * (p < J.level0_end ? 0.5f : 1.0f), y);
+        if (p < J.n) {
+            __builtin_nontemporal_store(o, &out[p]);
+            if (FILT) gp(J.filt_out)[p] = y;
+            __builtin_nontemporal_store(st[k].x, &sp[4 * p]);
+            __builtin_nontemporal_store(st[k].y, &sp[4 * p + 1]);
+            __builtin_nontemporal_store(st[k].z, &sp[4 * p + 2]);
+            __builtin_nontemporal_store(st[k].w, &sp[4 * p + 3]);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ jobs, int njobs) {
     __shared__ float sT[WEMA_MAX_BATCH], sA[WEMA_MAX_BATCH];
     const int jid = find_job(jobs, njobs, (int)blockIdx.x);
     const TemporalJob J = jobs[jid];
     frame_params(J, sT, sA);
-    const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
-    if (p >= J.n) return;
     float c[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) c[i] = J.c[i];
+    if (J.ppt == WEMA_IIR_PPT && J.nframes == 1) {
+        const long long p0 = (long long)((int)blockIdx.x - J.tile_begin) * (256 * WEMA_IIR_PPT) + threadIdx.x;
+        if (p0 >= J.n) return;
+        if (J.amp_out) {
+            if (J.filt_out) iir_one_frame<WEMA_IIR_PPT, true, true>(J, c, sT, sA, p0);
+            else iir_one_frame<WEMA_IIR_PPT, true, false>(J, c, sT, sA, p0);
+        } else {
+            if (J.filt_out) iir_one_frame<WEMA_IIR_PPT, false, true>(J, c, sT, sA, p0);
+            else iir_one_frame<WEMA_IIR_PPT, false, false>(J, c, sT, sA, p0);
+        }
+        return;
+    }
+    const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
+    if (p >= J.n) return;
     GLOBAL_AS float *sp = gp((float *)J.state) + 4 * p;
     float4 st;
     if (J.preseed) {
