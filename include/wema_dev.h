@@ -191,15 +191,18 @@ void  wema_dev_host_free(void *p);
 /* ---- multi-stream pool (no counterpart in the reference: it processes one stream) --------
  * Sessions of many streams share one GPU. wema_pool_add admits a session only if the
  * pool's device bytes stay within hbm_budget (-1 and "HBM budget exceeded" otherwise).
- * wema_pool_tick processes one frame of each listed session with one launch per pipeline
- * stage covering all of them; each session's results are identical to running it alone
- * (no computation crosses sessions). With graphs enabled, the launch sequence of each tick
- * shape (set of sessions and their output stage) is captured once in a hipGraph and
- * replayed. NULL stream = the pool's own stream (graphs run only there). */
+ * wema_pool_tick processes the listed frames with one launch per pipeline stage covering
+ * all of them: each entry of ids is one frame of that session, and a session listed k
+ * times in a row (k <= its max_batch) has its next k frames processed in order, as one
+ * call of k frames would; a session may appear in one such run per tick. Each session's
+ * results are identical to running it alone (no computation crosses sessions). With graphs
+ * enabled, the launch sequence of each tick shape (sessions, their frame counts and output
+ * stage) is captured once in a hipGraph and replayed. NULL stream = the pool's own stream
+ * (graphs run only there). */
 typedef struct wema_pool wema_pool_t;
 int    wema_pool_create(wema_pool_t **out, int device, size_t hbm_budget);
 void   wema_pool_destroy(wema_pool_t *p);
-int    wema_pool_add(wema_pool_t *p, const wema_dev_params *prm, int *id);   /* max_batch 1 */
+int    wema_pool_add(wema_pool_t *p, const wema_dev_params *prm, int *id);
 int    wema_pool_remove(wema_pool_t *p, int id);
 int    wema_pool_session(wema_pool_t *p, int id, wema_dev_t **s);          /* for queries */
 size_t wema_pool_bytes_in_use(const wema_pool_t *p);
@@ -220,8 +223,10 @@ int    wema_pool_timing_report(wema_pool_t *p, int kind, char *name, int name_le
  * 1280x720, 1920x1080, 3840x2160, a frame rate from that mix (fps_from_mix) or 30/60 at
  * random, a band in 0.3..5 Hz and amplification in 20..150. They are admitted first come
  * first served while the pool's bytes stay within hbm_budget (and max_sessions, if > 0),
- * and deliver frames at their frame rate. batched: all due frames run as one pool tick;
- * otherwise one session at a time (the plain path). saturate: `sessions` sessions admitted
+ * and deliver frames at their frame rate. batched: all due frames run as one pool tick, up
+ * to 2 frames of a session that has fallen behind (as one call of 2 frames; sessions are
+ * opened with max_batch 2); otherwise one frame of one session at a time (the plain path,
+ * max_batch 1). saturate: `sessions` sessions admitted
  * at once, taking the mix in its exact 8:8:6:2 proportions, with a frame always due, for
  * duration_s (capacity). warm_start: the Poisson run starts in steady state (the sessions
  * that arrived in the life_max_s before the start and are still alive are admitted before

@@ -85,6 +85,45 @@ def test_mixed_ticks_equal_alone(graphs):
     pool.close()
 
 
+@pytest.mark.parametrize("graphs", [False, True])
+def test_tick_runs_of_frames_equal_alone(graphs):
+    """A session listed k times in a row in one tick (k <= its max_batch) has its next k
+    frames processed as one call of k frames: with random runs of 1-3 frames per session
+    per tick, every stream's bytes equal the stream alone; a session in two runs of one
+    tick, or a run longer than max_batch, is refused."""
+    n = 18
+    frames = clips(n)
+    ref = alone(n, frames)
+    pool = wema_amd.Pool(4e9, graphs=graphs)
+    ids = [pool.add(s["w"], s["h"], s["fps"], max_batch=3, **s["kw"]) for s in STREAMS]
+    outs = [torch.empty_like(f) for f in frames]
+    rng = np.random.default_rng(7)
+    done = [0] * len(STREAMS)
+    while min(done) < n:
+        t_ids, t_in, t_out, took = [], [], [], {}
+        for i in range(len(STREAMS)):
+            k = min(int(rng.integers(0, 4)), n - done[i])
+            for j in range(k):
+                t_ids.append(ids[i])
+                t_in.append(frames[i][done[i] + j])
+                t_out.append(outs[i][done[i] + j])
+            if k:
+                took[i] = k
+        if not t_ids:
+            continue
+        pool.tick(t_ids, t_in, t_out)
+        for i, k in took.items():
+            done[i] += k
+    pool.sync()
+    for i in range(len(STREAMS)):
+        assert np.array_equal(outs[i].cpu().numpy(), ref[i]), STREAMS[i]
+    with pytest.raises(wema_amd.WemaError, match="two runs"):
+        pool.tick([ids[0], ids[1], ids[0]], [frames[0][0]] * 3, [outs[0][0]] * 3)
+    with pytest.raises(wema_amd.WemaError, match="max_batch"):
+        pool.tick([ids[0]] * 4, [frames[0][0]] * 4, [outs[0][0]] * 4)
+    pool.close()
+
+
 def test_c4_mix_full_size_vs_alone_and_oracle():
     """The 24-stream mix at its full sizes (8 x 640x480@30, 8 x 1280x720@30, 6 x 1920x1080@60,
     2 x 3840x2160@30; per-stream band, amplification and window; automatic levels), 20 ticks

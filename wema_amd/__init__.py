@@ -464,7 +464,8 @@ class Pool:
         return lib.wema_pool_stream(self._h)
 
     def tick(self, ids, ins, outs, stream=None):
-        """One frame of each session ids[i]: device RGB24 buffers ins[i] -> outs[i]."""
+        """One frame per entry: device RGB24 buffers ins[i] -> outs[i] of session ids[i]; a
+        session listed k times in a row has its next k frames processed in order."""
         n = len(ids)
         a_ids = (C.c_int * n)(*ids)
         a_in = (C.c_void_p * n)(*[t.data_ptr() for t in ins])

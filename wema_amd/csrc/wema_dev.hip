@@ -459,7 +459,8 @@ extern "C" long wema_dev_frames(const wema_dev_t *s) { return s ? s->frames : -1
 // job tables for n frames of one session, appended to a JobBuilder (also used by the pool)
 
 void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,
-                     float *out_f32, int n, const CallBufs *bufs) {
+                     float *out_f32, int n, const CallBufs *bufs, const void *const *in_list,
+                     uint8_t *const *out_list) {
     const WemaGeom &g = s->g;
     const wema_dev_params &p = s->p;
     const CallBufs b = bufs ? *bufs : CallBufs{s->coef, s->ampc, s->ll, s->rec, s->acc};
@@ -467,6 +468,14 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
     const size_t P = g.P, nu = g.nu, nus = g.nus;
     const size_t in_stride = P * 3 * (in_f32 ? sizeof(float) : 1);
     const long frame0 = s->frames;
+    // frame f's source and RGB24 output: consecutive frames from in / out_u8, or one pointer
+    // per frame (in_list / out_list: a pool tick's frames of one session)
+    auto in_of = [&](int f) -> const char * {
+        return in_list ? (const char *)in_list[f] : (const char *)in + (size_t)f * in_stride;
+    };
+    auto out_u8_of = [&](int f) -> uint8_t * {
+        return out_list ? out_list[f] : (out_u8 ? out_u8 + (size_t)f * P * 3 : nullptr);
+    };
 
     // per-frame effective amplification (channel-major) and threshold overrides
     for (int c = 0; c < nch; c++)
@@ -498,7 +507,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
             for (int c = 0; c < nch; c += jch) {
                 DwtfJob j{};
                 if (l == 0) {
-                    j.src = (const char *)in + (size_t)f * in_stride;
+                    j.src = in_of(f);
                     j.src_kind = in_f32 ? SRC_F32_Y : SRC_U8_Y;
                     j.prev = nullptr;
                 } else {
@@ -580,7 +589,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         if (!b.ampc) return coef_ptr(f, c) + g.plane_off[l];
         return b.ampc + ((size_t)f * nch + c) * nus + g.plane_off[l];
     };
-    const size_t out_b = (out_u8 ? 3 * P : 0) + (out_f32 ? 12 * P : 0);
+    const size_t out_b = (out_u8 || out_list ? 3 * P : 0) + (out_f32 ? 12 * P : 0);
 
     // inverse DWT levels, coarsest first
     for (int l = L - 1; l >= 0; l--) {
@@ -597,7 +606,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.ll_cstride = (long long)(ll_ptr(f, 1, 0) - ll_ptr(f, 0, 0));
                 j.planes_cstride = (long long)(planes_ptr(f, 1, 0) - planes_ptr(f, 0, 0));
                 j.nch = nch;
-                j.out_u8 = out_u8 ? out_u8 + (size_t)f * P * 3 : nullptr;
+                j.out_u8 = out_u8_of(f);
                 j.out_f32 = out_f32 ? out_f32 + (size_t)f * P * 3 : nullptr;
                 j.acc_clear = L == 1 ? b.acc + (size_t)f * nch : nullptr;
                 j.w = w;
@@ -633,9 +642,9 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         j.gray = rec_ptr(f, 0);
         j.cb = p.color ? rec_ptr(f, 1) : nullptr;
         j.cr = p.color ? rec_ptr(f, 2) : nullptr;
-        if (in_f32) j.src_f32 = (const float *)((const char *)in + (size_t)f * in_stride);
-        else j.src_u8 = (const uint8_t *)in + (size_t)f * in_stride;
-        j.out_u8 = out_u8 ? out_u8 + (size_t)f * P * 3 : nullptr;
+        if (in_f32) j.src_f32 = (const float *)in_of(f);
+        else j.src_u8 = (const uint8_t *)in_of(f);
+        j.out_u8 = out_u8_of(f);
         j.out_f32 = out_f32 ? out_f32 + (size_t)f * P * 3 : nullptr;
         j.w = g.W;
         j.h = g.H;

@@ -307,7 +307,8 @@ int wema_dev_open_ex(wema_dev_t **out, const wema_dev_params *p, bool pooled);
 int wema_dwt_tune(wema_dev *s);
 void wema_geometry_init(WemaGeom *g, const wema_dev_params *p);
 void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, uint8_t *out_u8,
-                     float *out_f32, int n, const CallBufs *bufs = nullptr);
+                     float *out_f32, int n, const CallBufs *bufs = nullptr,
+                     const void *const *in_list = nullptr, uint8_t *const *out_list = nullptr);
 // Injected failures (test hook, wema_dev_debug_inject_error): true when the armed site is
 // reached for the armed time.
 enum InjectSite { INJ_NONE = 0, INJ_TABLE_UPLOAD = 1, INJ_LAUNCH = 2, INJ_FRAME_UPLOAD = 3 };

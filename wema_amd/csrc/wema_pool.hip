@@ -109,7 +109,6 @@ extern "C" void wema_pool_destroy(wema_pool_t *p) {
 
 extern "C" int wema_pool_add(wema_pool_t *p, const wema_dev_params *prm, int *id) {
     if (!p || !prm || !id) { wema_set_error("null argument"); return -1; }
-    if (prm->max_batch != 1) { wema_set_error("pool sessions take one frame per tick (max_batch 1)"); return -1; }
     if (prm->device != p->device) { wema_set_error("session device differs from the pool's"); return -1; }
     size_t bytes = 0;
     if (wema_dev_query_bytes(prm, &bytes) < 0) return -1;
@@ -216,19 +215,33 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
     if (!p || n < 1 || !ids || !d_in || !d_out) { wema_set_error("bad argument"); return -1; }
     HIP_OK(hipSetDevice(p->device));
     hipStream_t st = hip_stream ? (hipStream_t)hip_stream : p->stream;
-    std::vector<wema_dev *> ss((size_t)n);
+    // runs of equal ids: consecutive frames of one session (at most its max_batch)
+    struct Run {
+        wema_dev *s;
+        int first, len;
+    };
+    std::vector<Run> runs;
     std::string key;
-    for (int i = 0; i < n; i++) {
-        ss[(size_t)i] = get(p, ids[i]);
-        if (!ss[(size_t)i]) return -1;
-        if (ss[(size_t)i]->failed) { wema_set_error("session %d failed in an earlier tick; remove it", ids[i]); return -1; }
-        for (int j = 0; j < i; j++)
-            if (ids[j] == ids[i]) { wema_set_error("session %d twice in one tick", ids[i]); return -1; }
-        key += "|" + std::to_string(ids[i]) + ":" + std::to_string(egress_kind(ss[(size_t)i]));
+    for (int i = 0; i < n;) {
+        int k = i + 1;
+        while (k < n && ids[k] == ids[i]) k++;
+        wema_dev *s = get(p, ids[i]);
+        if (!s) return -1;
+        if (s->failed) { wema_set_error("session %d failed in an earlier tick; remove it", ids[i]); return -1; }
+        for (const Run &r : runs)
+            if (r.s == s) { wema_set_error("session %d in two runs of one tick", ids[i]); return -1; }
+        if (k - i > s->p.max_batch) {
+            wema_set_error("session %d: %d frames in one tick, max_batch %d", ids[i], k - i, s->p.max_batch);
+            return -1;
+        }
+        runs.push_back({s, i, k - i});
+        key += "|" + std::to_string(ids[i]) + ":" + std::to_string(egress_kind(s)) + "x" + std::to_string(k - i);
+        i = k;
     }
     JobBuilder jb;
-    for (int i = 0; i < n; i++)
-        wema_build_jobs(ss[(size_t)i], jb, d_in[i], 0, d_out[i], nullptr, 1);
+    for (const Run &r : runs)
+        wema_build_jobs(r.s, jb, d_in[r.first], 0, d_out[r.first], nullptr, r.len, nullptr, (const void *const *)&d_in[r.first],
+                        &d_out[r.first]);
 
     // a shape is captured the second time it occurs (one-off shapes run directly), and the
     // cache keeps at most kMaxGraphs shapes
@@ -247,7 +260,7 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
         jb.fix_amp(b.amp);
         const int rc = wema_run_jobs(jb, b.h, b.d, b.ev, b.amp, st, &p->timer);
         if (rc < 0) {
-            for (wema_dev *s : ss) s->failed = s->failed || rc == -2;
+            for (const Run &r : runs) r.s->failed = r.s->failed || rc == -2;
             return -1;
         }
     } else {
@@ -288,11 +301,11 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
         HIP_OK(hipEventRecord(it->second.buf.ev, st));
         p->graph_ticks++;
     }
-    for (wema_dev *s : ss) {
-        s->frames += 1;
-        s->last_n = 1;
-        s->last_stream = st;
-        s->thr_ovr_active = 0;
+    for (const Run &r : runs) {
+        r.s->frames += r.len;
+        r.s->last_n = r.len;
+        r.s->last_stream = st;
+        r.s->thr_ovr_active = 0;
     }
     p->ticks++;
     return 0;

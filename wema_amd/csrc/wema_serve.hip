@@ -21,6 +21,13 @@
 #include "../../include/wema_dev.h"
 #include "wema_host.h"
 
+// Frames of one session a batched tick takes at most: a session that has fallen behind its
+// frame rate catches up that many frames at a time, as one call of that many frames (its
+// IIR state is then read and written once for them).
+#ifndef WEMA_SERVE_RUN
+#define WEMA_SERVE_RUN 2
+#endif
+
 namespace {
 
 struct Rng {
@@ -89,6 +96,7 @@ std::vector<Planned> make_plan(const wema_serve_config *cfg) {
         const int wsz[3] = {16, 32, 64};
         pl.p.window = wsz[rng.next() % 3];
         pl.p.device = cfg->device;
+        pl.p.max_batch = cfg->batched ? WEMA_SERVE_RUN : 1;
         pl.arrive = fixed ? 0.0 : t;
         pl.life = fixed ? 1e30 : rng.uniform(cfg->life_min_s, cfg->life_max_s);
         pl.phase = rng.uniform();
@@ -103,14 +111,17 @@ std::vector<Planned> make_plan(const wema_serve_config *cfg) {
     return plan;
 }
 
-// Per-session frame buffers: two device inputs (host_io), two device outputs and two
-// pinned host outputs (host_io), alternating by frame parity, so that the next frame's
-// upload overlaps the running tick. Buffers of departed sessions are kept per resolution
-// class and handed to the next session of that class.
+constexpr int kRun = WEMA_SERVE_RUN;
+// Per-session frame buffers: 2 kRun device inputs (host_io), device outputs and pinned host
+// outputs (host_io), frame f in slot f mod 2 kRun, so that the frames of the next tick
+// (launched before this one is waited for) never share a slot with this one's. Buffers of
+// departed sessions are kept per resolution class and handed to the next session of that
+// class.
+constexpr int kSlots = 2 * kRun;
 struct Slots {
-    uint8_t *din[2] = {nullptr, nullptr};
-    uint8_t *dout[2] = {nullptr, nullptr};
-    uint8_t *hout[2] = {nullptr, nullptr};
+    uint8_t *din[kSlots] = {};
+    uint8_t *dout[kSlots] = {};
+    uint8_t *hout[kSlots] = {};
 };
 
 struct SlotPool {
@@ -125,7 +136,7 @@ struct SlotPool {
         }
         const size_t bytes = (size_t)kMix[c].w * kMix[c].h * 3;
         Slots s;
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < kSlots; k++) {
             if (hipMalloc(&s.dout[k], bytes) != hipSuccess ||
                 (host_io && (hipMalloc(&s.din[k], bytes) != hipSuccess ||
                              hipHostMalloc(&s.hout[k], bytes, hipHostMallocDefault) != hipSuccess))) {
@@ -141,7 +152,7 @@ struct SlotPool {
     void give(int c, const Slots &s) { free[c].push_back(s); }
     void release() {
         for (Slots &s : all)
-            for (int k = 0; k < 2; k++) {
+            for (int k = 0; k < kSlots; k++) {
                 if (s.din[k]) hipFree(s.din[k]);
                 if (s.dout[k]) hipFree(s.dout[k]);
                 if (s.hout[k]) hipHostFree(s.hout[k]);
@@ -241,6 +252,7 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         wema_dev_params tp;
         wema_dev_params_default(&tp, c.w, c.h, 30.0f);
         tp.device = cfg->device;
+        tp.max_batch = cfg->batched ? WEMA_SERVE_RUN : 1;
         wema_dev_t *ts = nullptr;
         if (wema_dev_open_ex(&ts, &tp, true) < 0) { free_frames(); return -1; }
         wema_dev_close(ts);
@@ -279,6 +291,7 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
     std::vector<uint8_t *> outs;
     std::vector<double> due;
     std::vector<size_t> who;
+    std::vector<long> fidx;       // frame index of each entry
 
     // Ticks are pipelined: a tick is launched before the previous one is waited for, so the
     // host gathers the next tick (and, with host_io, uploads its frames) while the GPU runs
@@ -368,7 +381,7 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         for (size_t k = k0; k < k1; k++) {
             const Live &L = live[who[k]];
             const size_t bytes = (size_t)kMix[L.cls].w * kMix[L.cls].h * 3;
-            if (hipMemcpyAsync((void *)ins[k], in_frames[L.cls][(size_t)(L.next % kFrames)], bytes,
+            if (hipMemcpyAsync((void *)ins[k], in_frames[L.cls][(size_t)(fidx[k] % kFrames)], bytes,
                                hipMemcpyHostToDevice, s_up) != hipSuccess)
                 return -1;
         }
@@ -380,7 +393,7 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         for (size_t k = k0; k < k1; k++) {
             const Live &L = live[who[k]];
             const size_t bytes = (size_t)kMix[L.cls].w * kMix[L.cls].h * 3;
-            const int b = (int)(L.next & 1);
+            const int b = (int)(fidx[k] % kSlots);
             if (hipMemcpyAsync(L.sl.hout[b], outs[k], bytes, hipMemcpyDeviceToHost, s_down) != hipSuccess) return -1;
         }
         return hipEventRecord(q.ev_done, s_down) == hipSuccess ? 0 : -1;
@@ -390,8 +403,8 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         if (!cfg->on_frame) return;
         for (size_t k = k0; k < k1; k++) {
             const Live &L = live[who[k]];
-            const int b = (int)(L.next & 1);
-            q.done.push_back({L.plan_idx, L.cls, L.next, host_io ? L.sl.hout[b] : L.sl.dout[b]});
+            const int b = (int)(fidx[k] % kSlots);
+            q.done.push_back({L.plan_idx, L.cls, fidx[k], host_io ? L.sl.hout[b] : L.sl.dout[b]});
         }
     };
 
@@ -420,21 +433,28 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
         if (live.empty() && waiting.empty() && next_plan == plan.size() && (quota || fixed)) break;
         out->peak_sessions = std::max(out->peak_sessions, (int)live.size());
         out->peak_bytes = std::max(out->peak_bytes, wema_pool_bytes_in_use(pool));
-        // frames due now (one per session per tick; a backlog keeps its oldest frame)
-        ids.clear(); ins.clear(); outs.clear(); due.clear(); who.clear();
+        // frames due now: batched, up to kRun of a session (a backlog is taken oldest first);
+        // plain, one per session per round
+        ids.clear(); ins.clear(); outs.clear(); due.clear(); who.clear(); fidx.clear();
         double next_due = 1e30;
+        const int run = cfg->batched ? kRun : 1;
         for (size_t i = 0; i < live.size(); i++) {
             Live &L = live[i];
-            const double d = fixed ? now : L.start + (double)L.next / L.fps;
-            if (d <= now) {
-                const int b = (int)(L.next & 1);
+            for (int k = 0; k < run; k++) {
+                const long f = L.next + k;
+                if (quota && f >= quota) break;
+                const double d = fixed ? now : L.start + (double)f / L.fps;
+                if (d > now || (fixed && k > 0)) {
+                    next_due = std::min(next_due, d);
+                    break;
+                }
+                const int b = (int)(f % kSlots);
                 ids.push_back(L.id);
-                ins.push_back(host_io ? L.sl.din[b] : in_frames[L.cls][(size_t)(L.next % kFrames)]);
+                ins.push_back(host_io ? L.sl.din[b] : in_frames[L.cls][(size_t)(f % kFrames)]);
                 outs.push_back(L.sl.dout[b]);
                 due.push_back(d);
                 who.push_back(i);
-            } else {
-                next_due = std::min(next_due, d);
+                fidx.push_back(f);
             }
         }
         if (ids.empty()) {
