@@ -204,7 +204,7 @@ static void plan_layout(const WemaGeom &g, const wema_dev_params &p, int with_fi
     L->state = take(p.use_iir ? sizeof(float) * 4 * nu * nch : 0);
     L->ll = take(sizeof(float) * g.ll_total * nch * B);
     L->rec = take(sizeof(float) * g.P * nch * B);
-    L->acc = take(sizeof(unsigned long long) * nch * B);
+    L->acc = take(sizeof(unsigned long long) * wema_acc_entries((int)nch, (int)B));
     L->amp = take(sizeof(float) * nch * B);
     L->thr = take(sizeof(float) * 3 * B);
     L->thr_ovr = take(sizeof(float) * 3 * B);
@@ -465,6 +465,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
     const wema_dev_params &p = s->p;
     const CallBufs b = bufs ? *bufs : CallBufs{s->coef, s->ampc, s->ll, s->rec, s->acc};
     const int nch = g.nch, L = g.L;
+    const int acc_s = wema_acc_slots(n);   // Σ|c| slots per frame and channel
     const size_t P = g.P, nu = g.nu, nus = g.nus;
     const size_t in_stride = P * 3 * (in_f32 ? sizeof(float) : 1);
     const long frame0 = s->frames;
@@ -517,7 +518,8 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 }
                 j.ll = ll_ptr(f, c, l);
                 j.planes = coef_ptr(f, c) + g.plane_off[l];
-                j.acc = b.acc + (size_t)f * nch + c;
+                j.acc = b.acc + ((size_t)f * nch + c) * acc_s;
+                j.acc_slots = acc_s;
                 j.nch = jch;
                 j.ll_cstride = jch > 1 ? (long long)(ll_ptr(f, 1, l) - ll_ptr(f, 0, l)) : 0;
                 j.planes_cstride = jch > 1 ? (long long)(coef_ptr(f, 1) - coef_ptr(f, 0)) : 0;
@@ -541,8 +543,9 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
         TemporalJob j{};
         j.n = (long long)nu;
         j.level0_end = (long long)g.plane_off[1];
-        j.acc = b.acc + c;
-        j.acc_stride = nch;
+        j.acc = b.acc + (size_t)c * acc_s;
+        j.acc_stride = nch * acc_s;
+        j.acc_slots = acc_s;
         j.nframes = n;
         j.n6f = (float)(2 * nu);
         j.thr_out = s->thr_d + c;
@@ -608,7 +611,8 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.nch = nch;
                 j.out_u8 = out_u8_of(f);
                 j.out_f32 = out_f32 ? out_f32 + (size_t)f * P * 3 : nullptr;
-                j.acc_clear = L == 1 ? b.acc + (size_t)f * nch : nullptr;
+                j.acc_clear = L == 1 ? b.acc + (size_t)f * nch * acc_s : nullptr;
+                j.acc_n = nch * acc_s;
                 j.w = w;
                 j.h = h;
                 j.tiles_x = tiles_x;
@@ -621,7 +625,8 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.ll = ll_ptr(f, c, l);
                 j.planes = planes_ptr(f, c, l);
                 j.out = l == 0 ? rec_ptr(f, c) : ll_ptr(f, c, l - 1);
-                j.acc_clear = l == L - 1 ? b.acc + (size_t)f * nch + c : nullptr;
+                j.acc_clear = l == L - 1 ? b.acc + ((size_t)f * nch + c) * acc_s : nullptr;
+                j.acc_n = acc_s;
                 j.nch = 1;
                 j.w = w;
                 j.h = h;
@@ -789,7 +794,7 @@ static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, flo
     s->stage_cur ^= 1;
     HIP_OK(hipEventSynchronize(s->stage_ev[k]));   // staging buffer k free again
     if (bufs)   // the call's accumulators start at zero (the scratch keeps the last call's contents)
-        HIP_OK(wema_launch_zero_u64(bufs->acc, (long long)n * s->g.nch, st));
+        HIP_OK(wema_launch_zero_u64(bufs->acc, (long long)n * s->g.nch * wema_acc_slots(n), st));
     JobBuilder jb;
     wema_build_jobs(s, jb, in, in_f32, out_u8, out_f32, n, bufs);
     jb.fix_amp(s->amp_d);
@@ -825,7 +830,7 @@ static size_t scratch_plan(const wema_dev *s, int n, char *base, CallBufs *b) {
     b->ampc = s->p.use_iir ? s->ampc : (float *)take(sizeof(float) * g.nus * nch * nf);
     b->ll = (float *)take(sizeof(float) * g.ll_total * nch * nf);
     b->rec = (float *)take(sizeof(float) * g.P * nch * nf);
-    b->acc = (unsigned long long *)take(sizeof(unsigned long long) * nch * nf);
+    b->acc = (unsigned long long *)take(sizeof(unsigned long long) * wema_acc_entries((int)nch, n));
     return off;
 }
 

@@ -48,7 +48,7 @@ struct CallBufs {
     float *ampc;                // amplified coefficients when kept apart (or null)
     float *ll;                  // [B][nch][ll_total]
     float *rec;                 // [B][nch][P]
-    unsigned long long *acc;    // [B][nch], zero at the start of the call
+    unsigned long long *acc;    // [B][nch][acc slots], zero at the start of the call
 };
 
 struct JobList {
@@ -106,7 +106,7 @@ struct wema_dev {
     float4 *state = nullptr;    // IIR state [nch][nus]
     float *ll = nullptr;        // [B][nch][ll_total]
     float *rec = nullptr;       // level-0 reconstructions [B][nch][P]
-    unsigned long long *acc = nullptr;   // [B][nch]
+    unsigned long long *acc = nullptr;   // [B][nch][acc slots] (wema_acc_entries)
     float *amp_d = nullptr;     // [nch][B] effective amplification
     float *thr_d = nullptr;     // [B][3] thresholds used
     float *thr_ovr_d = nullptr; // [B][3] overrides (NaN = none)

@@ -60,18 +60,35 @@ enum SrcKind {
     SRC_F32_CR = 6,
 };
 
+// Sum of |c| of a frame and channel: acc_slots Q32 integers that strips add into (strip s
+// into slot s mod acc_slots) and whose total is the sum. One address per frame and channel
+// would take every strip's atomic in turn: a one-frame 4K launch has 4,320 strips. A call of
+// n frames uses the largest power of two up to 64 / n slots (at least 1), so the temporal
+// kernel reads at most max(n, 64) of them per block.
+#define WEMA_ACC_SLOTS_MAX 64
+__host__ __device__ static inline int wema_acc_slots(int nframes) {
+    int s = WEMA_ACC_SLOTS_MAX;
+    while (s > 1 && s * nframes > WEMA_ACC_SLOTS_MAX) s >>= 1;
+    return s;
+}
+// Accumulator entries a call of up to nframes frames of nch channels uses.
+__host__ __device__ static inline int wema_acc_entries(int nch, int nframes) {
+    return nch * (nframes > WEMA_ACC_SLOTS_MAX ? nframes : WEMA_ACC_SLOTS_MAX);
+}
+
 // A forward job covers nch channels of one frame that share the source (level 0 of the
 // colour mode: Y, Cb, Cr from one RGB frame); channel c uses ll + c * ll_cstride,
-// planes + c * planes_cstride and acc + c. Each wave computes its strip of every channel
-// from one load of the source.
+// planes + c * planes_cstride and acc + c * acc_slots. Each wave computes its strip of
+// every channel from one load of the source.
 struct DwtfJob {
     const void *src;                // level 0: RGB frame; else previous level's LL (w x h)
     float *ll;                      // LL of this level (sw x sh), channel 0
     float *planes;                  // LH, HH, HL of this level (3 x sw x sh), channel 0
     const float *prev;              // previous level's planes (3 x w x h) or null (level 0)
-    unsigned long long *acc;        // sum of |c| over unique positions, Q32 (or null)
+    unsigned long long *acc;        // sum of |c| over unique positions, Q32, in acc_slots parts (or null)
     long long ll_cstride, planes_cstride;
     int w, h;                       // input size of this level
+    int acc_slots;                  // a power of two
     int src_kind;
     int tiles_x;
     int tile_begin;
@@ -86,7 +103,7 @@ struct DwtiJob {
     const float *ll;                // approximation at this level (sw x sh)
     const float *planes;            // LH, HH, HL (amplified) of this level
     float *out;                     // reconstruction (w x h)
-    unsigned long long *acc_clear;  // accumulators to reset once read (nch of them, or null)
+    unsigned long long *acc_clear;  // accumulators to reset once read (acc_n of them, or null)
     uint8_t *out_u8;                // colour output job: RGB24 frame (or null)
     float *out_f32;                 // colour output job: float RGB frame (or null)
     long long ll_cstride, planes_cstride;
@@ -95,6 +112,7 @@ struct DwtiJob {
     int tile_begin;
     int nch;
     int sp;                         // strip height in coefficient pair rows
+    int acc_n;
 };
 
 #define WEMA_MAX_BATCH 256
@@ -129,7 +147,7 @@ struct TemporalJob {
     float *filt_out;                // optional [nframes][coef_stride] temporal output
     float4 *state;                  // IIR: per position HP w1, HP w2, LP w1, LP w2
     const float *ring;              // window mode: ring [ring_slots][coef_stride]
-    const unsigned long long *acc;  // [nframes] stride acc_stride
+    const unsigned long long *acc;  // [nframes] stride acc_stride, acc_slots parts each
     const float *thr_override;      // [nframes] stride 3, NaN = compute (or null)
     float *thr_out;                 // [nframes] stride 3 (or null)
     const float *amp;               // [nframes] effective amplification of each frame
@@ -141,6 +159,7 @@ struct TemporalJob {
     long long level0_end;           // positions below this are level 0 (alpha * 0.5)
     long long coef_stride;
     int acc_stride;
+    int acc_slots;                  // a power of two, acc_slots * nframes <= max(nframes, 64)
     int nframes;
     int preseed;                    // run the warm-up on frame 0 first
     int warmup;

@@ -208,7 +208,7 @@ enum { FAM_U8_RGB = 0, FAM_F32_RGB = 1, FAM_F32_PLANE = 2 };
 
 // One input row's data for a lane's sample pair (xa, xb).
 template <int FAM> struct StripRaw;
-template <> struct StripRaw<FAM_U8_RGB> { unsigned d0, d1, d2, sh; };   // 6 bytes at byte sh of d0
+template <> struct StripRaw<FAM_U8_RGB> { unsigned d0, d1, d2; };   // 6 bytes from byte L.sh of d0 (0: d0, d1)
 template <> struct StripRaw<FAM_F32_RGB> { float ra, ga, ba, rb, gb, bb; };
 template <> struct StripRaw<FAM_F32_PLANE> { float a, b; };
 
@@ -223,7 +223,6 @@ struct StripLane {
     int xa, xb;        // reflected sample columns of the lane's pair
     unsigned boff;     // fast path: byte offset of the pair in its row (RGB24: aligned down)
     unsigned sh;       // fast path, RGB24: byte offset of the pair in its first dword
-    bool fast;         // the pair is two adjacent samples (no reflection)
 };
 
 // Loads row y. FAST: every lane's pair is two adjacent samples and its wide load stays
@@ -240,27 +239,15 @@ __device__ __forceinline__ StripRaw<FAM> strip_load(const DwtfJob &J, const Stri
             d[0] = q[0];
             d[1] = q[1];
             d[2] = q[2];
-            d[3] = L.sh;
         } else {
+            // reflected columns, rows that need not start on a dword, the end of the frame:
+            // the 6 bytes one by one, the same loads in every lane (no branch, so the loads
+            // of later rows stay in flight while this one is computed)
             const GLOBAL_AS unsigned char *row = gp((const unsigned char *)J.src) + (size_t)y * w * 3;
-            if (L.fast && !(y == J.h - 1 && L.xa + 4 > w)) {
-                // (rows need not start on a dword: the alignment is taken per row)
-                const size_t a = (size_t)(row + (size_t)L.xa * 3);
-                const unsigned sh = (unsigned)(a & 3u);
-                const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(a - sh);
-                d[0] = q[0];
-                d[1] = q[1];
-                d[2] = q[2];
-                d[3] = sh;
-            } else {
-                // reflected columns, or the end of the frame: bytes one by one
-                const GLOBAL_AS unsigned char *pa = row + (size_t)L.xa * 3, *pb = row + (size_t)L.xb * 3;
-                d[0] = (unsigned)pa[0] | ((unsigned)pa[1] << 8) | ((unsigned)pa[2] << 16) |
-                       ((unsigned)pb[0] << 24);
-                d[1] = (unsigned)pb[1] | ((unsigned)pb[2] << 8);
-                d[2] = 0u;
-                d[3] = 0u;
-            }
+            const GLOBAL_AS unsigned char *pa = row + (size_t)L.xa * 3, *pb = row + (size_t)L.xb * 3;
+            d[0] = (unsigned)pa[0] | ((unsigned)pa[1] << 8) | ((unsigned)pa[2] << 16) | ((unsigned)pb[0] << 24);
+            d[1] = (unsigned)pb[1] | ((unsigned)pb[2] << 8);
+            d[2] = 0u;
         }
     } else if (FAM == FAM_F32_PLANE) {
         float *d = (float *)&r;
@@ -288,12 +275,12 @@ __device__ __forceinline__ StripRaw<FAM> strip_load(const DwtfJob &J, const Stri
     return r;
 }
 
-// Float RGB of both samples of a row (RGB families).
+// Float RGB of both samples of a row (RGB families); sh: byte offset of an RGB24 pair in d0.
 template <int FAM>
-__device__ __forceinline__ void strip_rgb(const StripRaw<FAM> &r, float (&a)[3], float (&b)[3]) {
+__device__ __forceinline__ void strip_rgb(const StripRaw<FAM> &r, unsigned sh, float (&a)[3], float (&b)[3]) {
     if constexpr (FAM == FAM_U8_RGB) {
-        const unsigned v0 = __builtin_amdgcn_alignbyte(r.d1, r.d0, r.sh);
-        const unsigned v1 = __builtin_amdgcn_alignbyte(r.d2, r.d1, r.sh);
+        const unsigned v0 = __builtin_amdgcn_alignbyte(r.d1, r.d0, sh);
+        const unsigned v1 = __builtin_amdgcn_alignbyte(r.d2, r.d1, sh);
         a[0] = u8f(v0 & 0xFFu);
         a[1] = u8f((v0 >> 8) & 0xFFu);
         a[2] = u8f((v0 >> 16) & 0xFFu);
@@ -347,7 +334,6 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
     StripLane L;
     L.xa = reflect_idx(sx0 + 2 * lane, w);
     L.xb = reflect_idx(sx0 + 2 * lane + 1, w);
-    L.fast = L.xb == L.xa + 1;
     if (FAM == FAM_U8_RGB) {   // (FAST strips: rows start on a dword)
         const unsigned b0 = 3u * (unsigned)L.xa;
         L.sh = b0 & 3u;
@@ -412,8 +398,8 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
     auto step = [&](int m, bool out, const StripRaw<FAM> &ra, const StripRaw<FAM> &rb) {
         float a0[3], b0[3], a1[3], b1[3];
         if constexpr (FAM != FAM_F32_PLANE) {
-            strip_rgb<FAM>(ra, a0, b0);
-            strip_rgb<FAM>(rb, a1, b1);
+            strip_rgb<FAM>(ra, FAST ? L.sh : 0u, a0, b0);
+            strip_rgb<FAM>(rb, FAST ? L.sh : 0u, a1, b1);
         }
         const int py = py0 + m - 4;
 #pragma unroll
@@ -456,7 +442,7 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
         // both rows of the pair are row 0
         const StripRaw<FAM> r = strip_load<FAM, FAST>(J, L, 0);
         float a0[3], b0[3];
-        if constexpr (FAM != FAM_F32_PLANE) strip_rgb<FAM>(r, a0, b0);
+        if constexpr (FAM != FAM_F32_PLANE) strip_rgb<FAM>(r, FAST ? L.sh : 0u, a0, b0);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             float e0, o0;
@@ -476,8 +462,12 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
             emit(c, 0, l0, l0, g0, g0);
         }
     } else {
-        // input pairs py0 - 2 + m for m < M; output pair rows py0 .. py0 + M - 5
-        const int M = min(SP, sh - py0) + 4;
+        // input pairs py0 - 2 + m for m < M; output pair rows py0 .. py0 + nout - 1
+        constexpr int SLOTS = SLOTS_OF(NCH);
+        const int nout = min(SP, sh - py0);
+        // FAST strips have nout >= 2 * SLOTS; shorter strips run on to 2 * SLOTS steps past
+        // the warm-up, the extra steps reading reflected rows and writing nothing
+        const int M = (FAST ? nout : max(nout, 2 * SLOTS)) + 4;
         auto load = [&](int m, int r) {
             int y = 2 * (py0 - 2 + m) + r;
             // (FAST strips: h >= 8, so one reflection reaches every row a strip reads)
@@ -486,15 +476,16 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
         };
         // SLOTS input pairs in flight: a slot is reloaded right after its pair is used, so
         // its registers are rewritten in place while the other slots' pairs are computed.
-        // Pairs 0-3 only fill the column pipelines. FAST strips have M >= 4 + 2 * SLOTS, so
-        // the loads and steps of the warm-up and of the main loop need no conditions (which
-        // keeps the loads in flight across iterations); the tail takes the rest.
+        // Pairs 0-3 only fill the column pipelines. Every strip has M >= 4 + 2 * SLOTS, so
+        // the loads of the warm-up and of the main loop need no conditions (which keeps them
+        // in flight across iterations: a load issued under a condition makes the wait for it
+        // a wait for every load); the tail takes the rest.
         auto reload = [&](StripRaw<FAM> &a, StripRaw<FAM> &b, int m) {
             a = load(m, 0);
             b = load(m, 1);
         };
-        StripRaw<FAM> sa[SLOTS_OF(NCH)], sb[SLOTS_OF(NCH)];
-        constexpr int SLOTS = SLOTS_OF(NCH);
+        auto emits = [&](int m) { return FAST || m - 4 < nout; };
+        StripRaw<FAM> sa[SLOTS], sb[SLOTS];
         // (after the 4 warm-up pairs, pair m + k of the loops below is in slot k)
         static_assert(4 % SLOTS == 0, "the slot count must divide the 4 warm-up pairs");
 #pragma unroll
@@ -502,23 +493,21 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             step(m, false, sa[m % SLOTS], sb[m % SLOTS]);
-            if (FAST || m + SLOTS < M) reload(sa[m % SLOTS], sb[m % SLOTS], m + SLOTS);
+            reload(sa[m % SLOTS], sb[m % SLOTS], m + SLOTS);
         }
         int m = 4;
-        if (FAST) {
-            for (; m + 2 * SLOTS <= M; m += SLOTS) {
+        for (; m + 2 * SLOTS <= M; m += SLOTS) {
 #pragma unroll
-                for (int k = 0; k < SLOTS; ++k) {
-                    step(m + k, true, sa[k], sb[k]);
-                    reload(sa[k], sb[k], m + k + SLOTS);
-                }
+            for (int k = 0; k < SLOTS; ++k) {
+                step(m + k, emits(m + k), sa[k], sb[k]);
+                reload(sa[k], sb[k], m + k + SLOTS);
             }
         }
         for (; m < M; m += SLOTS) {
 #pragma unroll
             for (int k = 0; k < SLOTS; ++k) {
                 if (m + k < M) {
-                    step(m + k, true, sa[k], sb[k]);
+                    step(m + k, emits(m + k), sa[k], sb[k]);
                     if (m + k + SLOTS < M) reload(sa[k], sb[k], m + k + SLOTS);
                 }
             }
@@ -528,7 +517,9 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const unsigned long long s = wave_sum_u64(part[c]);
-            if (lane == 0 && s) atomicAdd(J.acc + c, s);
+            // (also when s is 0: with a branch on s here the compiler waits for every
+            // load in flight at the top of each iteration of the strip loop above)
+            if (lane == 0) atomicAdd(J.acc + c * J.acc_slots + (strip & (J.acc_slots - 1)), s);
         }
     }
 }
@@ -783,8 +774,14 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
         put_row(0, r.ll, r.hl);
         return;
     }
-    const int M = min(SP, sh - py0) + 4;
+    // coefficient pairs py0 - 2 + m for m < M; image rows of pairs py0 .. py0 + nout - 1.
+    // FAST strips have nout >= 2 * SLOTS; shorter strips run on to 2 * SLOTS steps past the
+    // warm-up (reflected rows, nothing stored), so that no load of the warm-up or of the
+    // main loop is issued under a condition and the loads stay in flight across steps.
     constexpr int SLOTS = ISLOTS_OF(NCH);
+    const int nout = min(SP, sh - py0);
+    const int M = (FAST ? nout : max(nout, 2 * SLOTS)) + 4;
+    auto emits = [&](int m) { return FAST || m - 4 < nout; };
     // (after the 4 warm-up pairs, pair m + k of the loops below is in slot k)
     static_assert(4 % SLOTS == 0, "the slot count must divide the 4 warm-up pairs");
     IStripRaw<NCH> s[SLOTS];
@@ -793,23 +790,21 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         step(m, false, s[m % SLOTS]);
-        if (FAST || m + SLOTS < M) s[m % SLOTS] = load(py0 - 2 + m + SLOTS);
+        s[m % SLOTS] = load(py0 - 2 + m + SLOTS);
     }
     int m = 4;
-    if (FAST) {
-        for (; m + 2 * SLOTS <= M; m += SLOTS) {
+    for (; m + 2 * SLOTS <= M; m += SLOTS) {
 #pragma unroll
-            for (int k = 0; k < SLOTS; ++k) {
-                step(m + k, true, s[k]);
-                s[k] = load(py0 - 2 + m + k + SLOTS);
-            }
+        for (int k = 0; k < SLOTS; ++k) {
+            step(m + k, emits(m + k), s[k]);
+            s[k] = load(py0 - 2 + m + k + SLOTS);
         }
     }
     for (; m < M; m += SLOTS) {
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
             if (m + k < M) {
-                step(m + k, true, s[k]);
+                step(m + k, emits(m + k), s[k]);
                 if (m + k + SLOTS < M) s[k] = load(py0 - 2 + m + k + SLOTS);
             }
         }
@@ -824,7 +819,8 @@ __device__ __forceinline__ void dwt_inv_strips(const DwtiJob *__restrict__ jobs,
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int strip = (bid - J.tile_begin) * WEMA_STRIPS_PER_BLOCK + wave;
-    if (strip == 0 && lane < J.nch && J.acc_clear) J.acc_clear[lane] = 0ull;
+    if (strip == 0 && J.acc_clear)
+        for (int i = lane; i < J.acc_n; i += 64) J.acc_clear[i] = 0ull;
     const int w = J.w, h = J.h, sw = (w + 1) >> 1, sh = (h + 1) >> 1;
     if (strip >= J.tiles_x * wema_ceil_div(sh, J.sp)) return;
     const int px0 = (strip % J.tiles_x) * WEMA_TILE_PAIRS_X, py0 = (strip / J.tiles_x) * J.sp;
@@ -865,17 +861,26 @@ __device__ __forceinline__ float biquad2(const float *c, float4 &st, float x) {
     return y;
 }
 
-// Thresholds and per-level amplification of the job's frames, computed once per block.
+// Thresholds and per-level amplification of the job's frames, computed once per block:
+// thread t reads slot t mod S of frame t / S (S = acc_slots, a power of two up to 64, so the
+// S slots of a frame sit in S consecutive lanes of one wave), and the S lanes add them up.
 __device__ __forceinline__ void frame_params(const TemporalJob &J, float *sT, float *sA) {
-    for (int f = threadIdx.x; f < J.nframes; f += blockDim.x) {
-        float T = threshold_of(J.acc[(size_t)f * J.acc_stride], J.n6f);
-        if (J.thr_override) {
-            const float o = J.thr_override[3 * f];
-            if (o == o) T = o;
+    const int S = J.acc_slots, total = J.nframes * S;
+    for (int base = 0; base < total; base += (int)blockDim.x) {   // (the same trip count in every lane)
+        const int i = base + (int)threadIdx.x;
+        const int f = i / S, k = i & (S - 1);
+        unsigned long long v = i < total ? J.acc[(size_t)f * J.acc_stride + k] : 0ull;
+        for (int off = 1; off < S; off <<= 1) v += __shfl_xor(v, off);
+        if (i < total && k == 0) {
+            float T = threshold_of(v, J.n6f);
+            if (J.thr_override) {
+                const float o = J.thr_override[3 * f];
+                if (o == o) T = o;
+            }
+            sT[f] = T;
+            sA[f] = J.amp[f];
+            if (J.thr_out && blockIdx.x == (unsigned)J.tile_begin) J.thr_out[3 * f] = T;
         }
-        sT[f] = T;
-        sA[f] = J.amp[f];
-        if (J.thr_out && blockIdx.x == (unsigned)J.tile_begin) J.thr_out[3 * f] = T;
     }
     __syncthreads();
 }

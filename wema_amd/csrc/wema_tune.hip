@@ -172,7 +172,7 @@ int wema_dwt_tune(wema_dev *s) {
         }
     }
     if (timed && rc == 0) {
-        if (hipMemsetAsync(s->acc, 0, sizeof(unsigned long long) * (size_t)s->g.nch * (size_t)n, sc.st) != hipSuccess ||
+        if (hipMemsetAsync(s->acc, 0, sizeof(unsigned long long) * (size_t)wema_acc_entries(s->g.nch, n), sc.st) != hipSuccess ||
             hipStreamSynchronize(sc.st) != hipSuccess) {
             wema_set_error("strip autotuner clean-up failed");
             rc = -1;
