@@ -557,21 +557,25 @@ def test_covariance_division_equals_ieee_on_the_gpu():
         assert r[-1] == np.float32(1.0) / top, (top, r[-1])
 
 
-@pytest.mark.parametrize("w,h,levels,color,batch", [(642, 481, 4, True, 3), (1000, 701, 5, False, 2)])
+@pytest.mark.parametrize("w,h,levels,color,batch", [(642, 481, 4, True, 3), (1000, 701, 5, False, 2),
+                                                   (70, 45, 4, False, 2), (130, 17, 3, True, 1),
+                                                   (260, 390, 3, False, 1)])
 def test_strip_rows_do_not_change_results(w, h, levels, color, batch):
-    """The DWT strip height, forced to 8, 16 or 32 pair rows for every level, or picked per
-    level by the autotuner, or by the fixed rule, gives the same bytes, equal to the
-    oracle's."""
+    """The DWT strip height, forced to 8, 16 or 32 pair rows for every level, LDS tiles for
+    the levels below full resolution (1), picked per level by the autotuner, or the fixed
+    rule, gives the same bytes, equal to the oracle's; the small and odd sizes reach the
+    tiles' reflections across several tile widths and past both ends of a level."""
     fps, n = 30.0, 2 * batch
     kw = dict(fps=fps, amp=100.0, f_low=0.5, f_high=3.0, levels=levels, color=color)
     rng = np.random.default_rng(w * h)
     frames = rng.integers(0, 256, size=(n, h, w, 3), dtype=np.uint8)
     ref = oracle_clip(frames, **kw)
     d_in = torch.from_numpy(frames).cuda()
-    for rows in (None, 0, 8, 16, 32):
+    for rows in (None, 0, 8, 16, 32, 1):
         sess = wema_amd.Session(w, h, max_batch=batch, **kw)
-        tuned = sess.strip_rows() + sess.strip_rows(inverse=True)
-        assert all(r in (8, 16, 32) for r in tuned), tuned
+        fwd, inv = sess.strip_rows(), sess.strip_rows(inverse=True)
+        assert fwd[0] in (8, 16, 32) and inv[0] in (8, 16, 32), (fwd, inv)
+        assert all(r in (1, 8, 16, 32) for r in fwd + inv), (fwd, inv)
         if rows is not None:
             sess.set_strip_rows(rows)
         d_out = torch.empty_like(d_in)

@@ -22,6 +22,8 @@
 extern "C" {
 hipError_t wema_launch_dwt_fwd(const DwtfJob *, int, int, int, hipStream_t);
 hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, int, hipStream_t);
+hipError_t wema_launch_dwt_fwd_tile(const DwtfJob *, int, int, hipStream_t);
+hipError_t wema_launch_dwt_inv_tile(const DwtiJob *, int, int, hipStream_t);
 hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *, int, int, hipStream_t);
 hipError_t wema_launch_iir(const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_win(int, const TemporalJob *, int, int, hipStream_t);
@@ -527,6 +529,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.h = h;
                 j.tiles_x = tiles_x;
                 j.sp = s->sp_fwd[l];
+                if (j.sp == WEMA_SP_TILE && (l == 0 || w < 2 || h < 2)) j.sp = 0;   // (tiles: coarser levels of 2 x 2 samples or more)
                 // input read once (RGB24 shared by the channels at level 0) + LL and three
                 // subbands written per channel
                 const double in_b = l == 0 ? 3.0 * w * h * (in_f32 ? 4 : 1) : 4.0 * w * h;
@@ -632,6 +635,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.h = h;
                 j.tiles_x = tiles_x;
                 j.sp = s->sp_inv[l];
+                if (j.sp == WEMA_SP_TILE && (l == 0 || w < 2 || h < 2)) j.sp = 0;
                 jb.add_dwti(WEMA_MAX_LEVELS - 1 - l, j, tiles, in_b + 4.0 * w * h);
             }
         }
@@ -737,9 +741,15 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
     if (timer_mark(timer, -1, 0, st) < 0) return -2;
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
         const JobList &jl = jb.dwtf[l];
-        if (!jl.count) continue;
-        LAUNCH_OK(wema_launch_dwt_fwd((const DwtfJob *)(base + jl.offset), jl.count, jl.blocks, l == 0, st));
-        if (timer_mark(timer, l == 0 ? TK_DWT_FWD0 : TK_DWT_FWD, jl.bytes, st) < 0) return -2;
+        if (jl.count) {
+            LAUNCH_OK(wema_launch_dwt_fwd((const DwtfJob *)(base + jl.offset), jl.count, jl.blocks, l == 0, st));
+            if (timer_mark(timer, l == 0 ? TK_DWT_FWD0 : TK_DWT_FWD, jl.bytes, st) < 0) return -2;
+        }
+        const JobList &tl = jb.dwtf_t[l];
+        if (tl.count) {
+            LAUNCH_OK(wema_launch_dwt_fwd_tile((const DwtfJob *)(base + tl.offset), tl.count, tl.blocks, st));
+            if (timer_mark(timer, TK_DWT_FWD, tl.bytes, st) < 0) return -2;
+        }
     }
     const JobList &tl = jb.temporal;
     if (tl.count) {
@@ -757,9 +767,15 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
     const int last_inv = WEMA_MAX_LEVELS - 1;
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
         const JobList &jl = jb.dwti[l];
-        if (!jl.count) continue;
-        LAUNCH_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, l == last_inv, st));
-        if (timer_mark(timer, l == last_inv ? TK_DWT_INV0 : TK_DWT_INV, jl.bytes, st) < 0) return -2;
+        if (jl.count) {
+            LAUNCH_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, l == last_inv, st));
+            if (timer_mark(timer, l == last_inv ? TK_DWT_INV0 : TK_DWT_INV, jl.bytes, st) < 0) return -2;
+        }
+        const JobList &tl = jb.dwti_t[l];
+        if (tl.count) {
+            LAUNCH_OK(wema_launch_dwt_inv_tile((const DwtiJob *)(base + tl.offset), tl.count, tl.blocks, st));
+            if (timer_mark(timer, TK_DWT_INV, tl.bytes, st) < 0) return -2;
+        }
     }
     if (jb.dwti_ycc.count) {
         const JobList &jl = jb.dwti_ycc;

@@ -153,12 +153,22 @@ struct JobBuilder {
     std::vector<TemporalJob> window_v[WEMA_WIN_KINDS];   // sliding-window jobs by kernel (wema_win_kind)
     std::vector<DwtiJob> dwti_v[WEMA_MAX_LEVELS];
     std::vector<DwtiJob> dwti_ycc_v;          // colour output jobs (level 0 + YCbCr -> RGB)
+    std::vector<DwtfJob> dwtf_t_v[WEMA_MAX_LEVELS];   // coarser levels as LDS tiles
+    std::vector<DwtiJob> dwti_t_v[WEMA_MAX_LEVELS];
     std::vector<EgressJob> egress_v[EG_KINDS];
     JobList dwtf[WEMA_MAX_LEVELS], temporal, window[WEMA_WIN_KINDS], dwti[WEMA_MAX_LEVELS], dwti_ycc, egress[EG_KINDS];
+    JobList dwtf_t[WEMA_MAX_LEVELS], dwti_t[WEMA_MAX_LEVELS];
     std::vector<float> host_amp;
     size_t amp_offset = 0;
 
     void add_dwtf(int l, DwtfJob j, int tiles, double bytes) {
+        if (j.sp == WEMA_SP_TILE) {   // one workgroup per tile
+            j.tile_begin = dwtf_t[l].blocks;
+            dwtf_t[l].blocks += wema_dt_tiles(j.w, j.h);
+            dwtf_t[l].bytes += bytes;
+            dwtf_t_v[l].push_back(j);
+            return;
+        }
         j.tile_begin = dwtf[l].blocks;
         dwtf[l].blocks += tiles;
         dwtf[l].bytes += bytes;
@@ -174,6 +184,13 @@ struct JobBuilder {
         (window_mode ? window_v[wv] : temporal_v).push_back(j);
     }
     void add_dwti(int stage, DwtiJob j, int tiles, double bytes) {
+        if (j.sp == WEMA_SP_TILE) {
+            j.tile_begin = dwti_t[stage].blocks;
+            dwti_t[stage].blocks += wema_dt_tiles(j.w, j.h);
+            dwti_t[stage].bytes += bytes;
+            dwti_t_v[stage].push_back(j);
+            return;
+        }
         j.tile_begin = dwti[stage].blocks;
         dwti[stage].blocks += tiles;
         dwti[stage].bytes += bytes;
@@ -289,6 +306,8 @@ struct JobBuilder {
         for (int k = 0; k < WEMA_WIN_KINDS; k++) ok = ok && put(dst, cap, off, window_v[k], window[k]);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_v[l], dwti[l]);
         ok = ok && put(dst, cap, off, dwti_ycc_v, dwti_ycc);
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwtf_t_v[l], dwtf_t[l]);
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_t_v[l], dwti_t[l]);
         for (int k = 0; k < EG_KINDS; k++) ok = ok && put(dst, cap, off, egress_v[k], egress[k]);
         JobList amp_list;
         ok = ok && put(dst, cap, off, host_amp, amp_list);

@@ -25,6 +25,17 @@
 #define WEMA_SP_MAX 32
 #define WEMA_SP_MIN 8
 #define WEMA_STRIPS_PER_BLOCK 4
+// The levels below full resolution can also run as LDS tiles of WEMA_DT_TP x WEMA_DT_TP
+// output pairs, one per 256-thread workgroup (a job's strip height WEMA_SP_TILE selects
+// them); levels of fewer than 2 samples in either direction always run as strips.
+#ifndef WEMA_DT_TP
+#define WEMA_DT_TP 16
+#endif
+#define WEMA_SP_TILE 1
+__host__ __device__ static inline int wema_dt_tiles(int w, int h) {
+    const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
+    return ((sw + WEMA_DT_TP - 1) / WEMA_DT_TP) * ((sh + WEMA_DT_TP - 1) / WEMA_DT_TP);
+}
 // Strip height per launch: the tallest that still gives this many channel-strips (a colour
 // strip counts 3). Measured on 1080p colour and 4K luma: the forward DWT balances halo
 // re-reads against parallelism at about 64 K; the inverse is fastest with the shortest

@@ -843,6 +843,223 @@ __global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv0_ycc(con
 }
 
 // ---------------------------------------------------------------------------------------
+// Coarser DWT levels as LDS tiles (one float plane per job; src/dwt.c:369-582).
+//
+// A workgroup computes WEMA_DT_TP x WEMA_DT_TP output pairs of one job from the
+// (WEMA_DT_TP + 4)^2 pairs around them (2 pairs of halo on each side, the reach of the four
+// lifting steps), staged in LDS. The samples come from the same whole-sample symmetric
+// extension the strips read, every lifting step is one pass over the tile with a barrier
+// after it, and every value takes the same fused multiply-add of the same operands as in
+// the strips (forward: rows, then columns; inverse: columns, then rows), so tiles and
+// strips give identical results. A strip walks down its columns one pair row per step; a
+// tile has all its loads in flight at once and a depth of eight short passes, which suits
+// levels too small to fill the chip with strips. Valid extents after each step (pairs of
+// the tile): forward rows/columns step 1 [0, R-1), 2 [1, R-1), 3 [1, R-2), 4 [2, R-2);
+// inverse step 1 [1, R), 2 [1, R-1), 3 [2, R-1), 4 [2, R-2): the central WEMA_DT_TP.
+
+constexpr int DT_TP = WEMA_DT_TP;
+constexpr int DT_R = DT_TP + 4;          // pairs per side with the halo
+constexpr int DT_S = 2 * DT_R;           // samples per side
+constexpr int DT_LOADS = (DT_S * DT_S + 255) / 256;   // sample loads per thread
+
+// |v| as Q32 for any float (abs_q32_small where it applies).
+__device__ __forceinline__ unsigned long long abs_q32(float v) {
+    return fabsf(v) < 1073741824.0f ? abs_q32_small(v) : abs_fixed(v);
+}
+
+__global__ void __launch_bounds__(256) k_dwt_fwd_tile(const DwtfJob *__restrict__ jobs, int njobs) {
+    __shared__ float sE[DT_S][DT_R];   // even samples of each row, then L
+    __shared__ float sO[DT_S][DT_R];   // odd samples of each row
+    __shared__ float sH[DT_S][DT_R];   // H (the odd samples after the row lifting, scaled)
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const DwtfJob J = jobs[find_job(jobs, njobs, bid)];
+    const int t = bid - J.tile_begin, tid = (int)threadIdx.x;
+    const int w = J.w, h = J.h, sw = (w + 1) >> 1, sh = (h + 1) >> 1;
+    const int ntx = wema_ceil_div(sw, DT_TP);
+    const int px0 = (t % ntx) * DT_TP, py0 = (t / ntx) * DT_TP;
+    const GLOBAL_AS float *src = gp((const float *)J.src);
+    {
+        // sample (r, x) of the tile = image sample (2 (py0 - 2) + r, 2 (px0 - 2) + x),
+        // reflected into the image; all loads issued before the first LDS store
+        float v[DT_LOADS];
+#pragma unroll
+        for (int k = 0; k < DT_LOADS; ++k) {
+            const int q = min(tid + 256 * k, DT_S * DT_S - 1);
+            const int r = q / DT_S, x = q - r * DT_S;
+            const int yi = reflect_idx(2 * (py0 - 2) + r, h), xi = reflect_idx(2 * (px0 - 2) + x, w);
+            v[k] = src[(size_t)yi * w + xi];
+        }
+#pragma unroll
+        for (int k = 0; k < DT_LOADS; ++k) {
+            const int q = tid + 256 * k;
+            if (q < DT_S * DT_S) {
+                const int r = q / DT_S, x = q - r * DT_S;
+                (x & 1 ? sO : sE)[r][x >> 1] = v[k];
+            }
+        }
+    }
+    __syncthreads();
+    // row lifting, as lift_fwd_lanes: item (row r, pair i)
+#define DT_ROWS(BODY)                                                   \
+    for (int q = tid; q < DT_S * DT_R; q += 256) {                      \
+        const int r = q / DT_R, i = q - r * DT_R;                        \
+        BODY                                                            \
+    }                                                                   \
+    __syncthreads();
+    DT_ROWS(if (i < DT_R - 1) sO[r][i] = fmaf(WEMA_ALPHA, sE[r][i] + sE[r][i + 1], sO[r][i]);)
+    DT_ROWS(if (i > 0) sE[r][i] = fmaf(WEMA_BETA, sO[r][i - 1] + sO[r][i], sE[r][i]);)
+    DT_ROWS(if (i < DT_R - 1) sO[r][i] = fmaf(WEMA_GAMMA, sE[r][i] + sE[r][i + 1], sO[r][i]);)
+    DT_ROWS(if (i > 0) sE[r][i] = fmaf(WEMA_DELTA, sO[r][i - 1] + sO[r][i], sE[r][i]) * WEMA_KINV;
+            sH[r][i] = sO[r][i] * WEMA_K;)
+#undef DT_ROWS
+    // column lifting of the L (sE) and H (sH) columns, as vpipe_step: item (array, pair row
+    // j, column i); rows 2 j and 2 j + 1 are the pair's even and odd samples
+#define DT_COLS(BODY)                                                   \
+    for (int q = tid; q < 2 * DT_R * DT_R; q += 256) {                  \
+        const int a = q >= DT_R * DT_R, qq = q - (a ? DT_R * DT_R : 0); \
+        const int j = qq / DT_R, i = qq - j * DT_R;                      \
+        float(*X)[DT_R] = a ? sH : sE;                                  \
+        BODY                                                            \
+    }                                                                   \
+    __syncthreads();
+    DT_COLS(if (j < DT_R - 1) X[2 * j + 1][i] = fmaf(WEMA_ALPHA, X[2 * j][i] + X[2 * j + 2][i], X[2 * j + 1][i]);)
+    DT_COLS(if (j > 0) X[2 * j][i] = fmaf(WEMA_BETA, X[2 * j - 1][i] + X[2 * j + 1][i], X[2 * j][i]);)
+    DT_COLS(if (j < DT_R - 1) X[2 * j + 1][i] = fmaf(WEMA_GAMMA, X[2 * j][i] + X[2 * j + 2][i], X[2 * j + 1][i]);)
+#undef DT_COLS
+    // last column step and the outputs of the central pairs (as the strips' emit)
+    const int hw_valid = w >> 1, hh_valid = h >> 1;   // written extents (src/dwt.c:463-469)
+    const size_t n = (size_t)sw * sh, pn = (size_t)w * h;
+    const GLOBAL_AS float *prev = gp(J.prev);
+    unsigned long long part = 0ull;
+    for (int q = tid; q < DT_TP * DT_TP; q += 256) {
+        const int j = 2 + q / DT_TP, i = 2 + q % DT_TP;
+        const int px = px0 + i - 2, py = py0 + j - 2;
+        if (px >= sw || py >= sh) continue;
+        const float le4 = fmaf(WEMA_DELTA, sE[2 * j - 1][i] + sE[2 * j + 1][i], sE[2 * j][i]);
+        const float he4 = fmaf(WEMA_DELTA, sH[2 * j - 1][i] + sH[2 * j + 1][i], sH[2 * j][i]);
+        const float ll = le4 * WEMA_KINV, hl = he4 * WEMA_KINV;
+        float lh = sE[2 * j + 1][i] * WEMA_K, hh = sH[2 * j + 1][i] * WEMA_K;
+        const size_t o = (size_t)py * sw + px;
+        float hlv = hl;
+        if (prev) {
+            if (py >= hh_valid) {
+                lh = prev[o];
+                hh = prev[pn + o];
+            }
+            if (px >= hw_valid) {
+                hlv = prev[2 * pn + o];
+                hh = prev[pn + o];
+            }
+        }
+        gp(J.ll)[o] = ll;
+        gp(J.planes)[o] = lh;
+        gp(J.planes)[n + o] = hh;
+        gp(J.planes)[2 * n + o] = hlv;
+        part += abs_q32(lh) + abs_q32(hlv) + abs_q32(hh);
+    }
+    if (J.acc) {
+        const unsigned long long s = wave_sum_u64(part);
+        const int wave = tid >> 6;
+        if ((tid & 63) == 0) atomicAdd(J.acc + ((4 * t + wave) & (J.acc_slots - 1)), s);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dwt_inv_tile(const DwtiJob *__restrict__ jobs, int njobs) {
+    // column phase: even (e) and odd (o) coefficient rows of the low (L) and high (H)
+    // columns, by pair row; row phase: even (A) and odd (B) samples of each image row
+    __shared__ float sLe[DT_R][DT_R], sLo[DT_R][DT_R], sHe[DT_R][DT_R], sHo[DT_R][DT_R];
+    __shared__ float sA[DT_S][DT_R], sB[DT_S][DT_R];
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const DwtiJob J = jobs[find_job(jobs, njobs, bid)];
+    const int t = bid - J.tile_begin, tid = (int)threadIdx.x;
+    if (t == 0 && J.acc_clear)
+        for (int i = tid; i < J.acc_n; i += 256) J.acc_clear[i] = 0ull;
+    const int w = J.w, h = J.h, sw = (w + 1) >> 1, sh = (h + 1) >> 1;
+    const size_t n = (size_t)sw * sh;
+    const int ntx = wema_ceil_div(sw, DT_TP);
+    const int px0 = (t % ntx) * DT_TP, py0 = (t / ntx) * DT_TP;
+    {
+        // coefficient pair (j, i) of the tile = pair (py0 - 2 + j, px0 - 2 + i) of the
+        // symmetric extension of the interleaved signal, as the strips load it
+        constexpr int NP = DT_R * DT_R, NL = (NP + 255) / 256;
+        float v[NL][4];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int q = min(tid + 256 * k, NP - 1);
+            const int j = q / DT_R, i = q - j * DT_R;
+            const int I = px0 - 2 + i, Q = py0 - 2 + j;
+            const size_t colL = (size_t)(reflect_idx(2 * I, w) >> 1), colH = (size_t)(reflect_idx(2 * I + 1, w) >> 1);
+            const size_t re = (size_t)(reflect_idx(2 * Q, h) >> 1) * sw, ro = (size_t)(reflect_idx(2 * Q + 1, h) >> 1) * sw;
+            const GLOBAL_AS float *LL = gp(J.ll), *LH = gp(J.planes);
+            v[k][0] = LL[re + colL];
+            v[k][1] = LH[ro + colL];
+            v[k][2] = LH[2 * n + re + colH];
+            v[k][3] = LH[n + ro + colH];
+        }
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int q = tid + 256 * k;
+            if (q < NP) {
+                const int j = q / DT_R, i = q - j * DT_R;
+                sLe[j][i] = v[k][0] * WEMA_K;
+                sLo[j][i] = v[k][1] * WEMA_KINV;
+                sHe[j][i] = v[k][2] * WEMA_K;
+                sHo[j][i] = v[k][3] * WEMA_KINV;
+            }
+        }
+    }
+    __syncthreads();
+    // column synthesis, as ipipe_step: item (column kind, pair row j, column i)
+#define DT_ICOLS(BODY)                                                  \
+    for (int q = tid; q < 2 * DT_R * DT_R; q += 256) {                  \
+        const int a = q >= DT_R * DT_R, qq = q - (a ? DT_R * DT_R : 0); \
+        const int j = qq / DT_R, i = qq - j * DT_R;                      \
+        float(*Xe)[DT_R] = a ? sHe : sLe;                               \
+        float(*Xo)[DT_R] = a ? sHo : sLo;                               \
+        BODY                                                            \
+    }                                                                   \
+    __syncthreads();
+    DT_ICOLS(if (j > 0) Xe[j][i] = fmaf(-WEMA_DELTA, Xo[j - 1][i] + Xo[j][i], Xe[j][i]);)
+    DT_ICOLS(if (j < DT_R - 1) Xo[j][i] = fmaf(-WEMA_GAMMA, Xe[j][i] + Xe[j + 1][i], Xo[j][i]);)
+    DT_ICOLS(if (j > 0) Xe[j][i] = fmaf(-WEMA_BETA, Xo[j - 1][i] + Xo[j][i], Xe[j][i]);)
+    // last column step; image rows 2 j (e) and 2 j + 1 (o) to the row arrays, scaled as
+    // put_row scales them (low column: A, times K; high column: B, times 1/K)
+    DT_ICOLS(if (j >= 2 && j < DT_R - 2) {
+        const float o4 = fmaf(-WEMA_ALPHA, Xe[j][i] + Xe[j + 1][i], Xo[j][i]);
+        if (a) {
+            sB[2 * j][i] = Xe[j][i] * WEMA_KINV;
+            sB[2 * j + 1][i] = o4 * WEMA_KINV;
+        } else {
+            sA[2 * j][i] = Xe[j][i] * WEMA_K;
+            sA[2 * j + 1][i] = o4 * WEMA_K;
+        }
+    })
+#undef DT_ICOLS
+    // row synthesis of image rows 4 .. S - 5 (the central pair rows), as lift_inv_lanes
+#define DT_IROWS(BODY)                                                  \
+    for (int q = tid; q < (DT_S - 8) * DT_R; q += 256) {                \
+        const int r = 4 + q / DT_R, i = q % DT_R;                        \
+        BODY                                                            \
+    }                                                                   \
+    __syncthreads();
+    DT_IROWS(if (i > 0) sA[r][i] = fmaf(-WEMA_DELTA, sB[r][i - 1] + sB[r][i], sA[r][i]);)
+    DT_IROWS(if (i < DT_R - 1) sB[r][i] = fmaf(-WEMA_GAMMA, sA[r][i] + sA[r][i + 1], sB[r][i]);)
+    DT_IROWS(if (i > 0) sA[r][i] = fmaf(-WEMA_BETA, sB[r][i - 1] + sB[r][i], sA[r][i]);)
+#undef DT_IROWS
+    // last row step and the stores of the central pairs
+    GLOBAL_AS float *out = gp(J.out);
+    for (int q = tid; q < 2 * DT_TP * DT_TP; q += 256) {
+        const int r = 4 + q / DT_TP, i = 2 + q % DT_TP;
+        const int x = 2 * (px0 + i - 2), y = 2 * (py0 - 2) + r;
+        if (x >= w || y >= h) continue;
+        const float ev = sA[r][i];
+        const float od = fmaf(-WEMA_ALPHA, sA[r][i] + sA[r][i + 1], sB[r][i]);
+        out[(size_t)y * w + x] = ev;
+        if (x + 1 < w) out[(size_t)y * w + x + 1] = od;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // IIR band-pass + amplification (src/iir_filter.c:181-264, 266-370, 376-471;
 // src/wema.c:714-744, 783-800, 830-848; src/wema_batch.c:268-297, 420-440).
 __device__ __forceinline__ float biquad2(const float *c, float4 &st, float x) {
@@ -1812,6 +2029,16 @@ hipError_t wema_launch_dwt_inv(const DwtiJob *jobs, int njobs, int nblocks, int 
     if (nblocks <= 0) return hipSuccess;
     if (level0) hipLaunchKernelGGL(k_dwt_inv0, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
     else hipLaunchKernelGGL(k_dwt_inv, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+hipError_t wema_launch_dwt_fwd_tile(const DwtfJob *jobs, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dwt_fwd_tile, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+hipError_t wema_launch_dwt_inv_tile(const DwtiJob *jobs, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dwt_inv_tile, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *jobs, int njobs, int nblocks, hipStream_t s) {

@@ -33,6 +33,8 @@ extern "C" hipError_t wema_launch_egress(int kind, const EgressJob *jobs, int nj
 extern "C" hipError_t wema_launch_dwt_fwd(const DwtfJob *, int, int, int, hipStream_t);
 extern "C" hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, int, hipStream_t);
 extern "C" hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *, int, int, hipStream_t);
+extern "C" hipError_t wema_launch_dwt_fwd_tile(const DwtfJob *, int, int, hipStream_t);
+extern "C" hipError_t wema_launch_dwt_inv_tile(const DwtiJob *, int, int, hipStream_t);
 
 namespace {
 
@@ -66,25 +68,36 @@ extern "C" void wema_dev_autotune(int mask) {
 
 namespace {
 
-const int kStrips[3] = {8, 16, 32};
-
-// Mean launch time of one job list at each strip height (the jobs' buffers are a session's
-// own per-call buffers; their contents do not matter to the timing).
-template <typename Job, typename Launch>
-int time_strips(std::vector<Job> v, void *tab_h, void *tab_d, size_t cap, Scratch &sc, double *ms, Launch launch) {
-    for (int c = 0; c < 3; c++) {
+// Candidate shapes: strips 8, 16 and 32 pair rows tall, and (levels below full resolution)
+// LDS tiles.
+const int kStrips[4] = {8, 16, 32, WEMA_SP_TILE};
+// Mean launch time of one job list at each candidate shape (the jobs' buffers are a
+// session's own per-call buffers; their contents do not matter to the timing).
+template <typename Job, typename Launch, typename LaunchTile>
+int time_strips(std::vector<Job> v, void *tab_h, void *tab_d, size_t cap, Scratch &sc, int ncand, double *ms,
+                Launch launch, LaunchTile launch_tile) {
+    for (int c = 0; c < ncand; c++) {
+        const bool tile = kStrips[c] == WEMA_SP_TILE;
         for (auto &j : v) j.sp = kStrips[c];
         JobList jl;
-        JobBuilder::strips_of(v, jl, 0);
+        if (tile) {
+            for (auto &j : v) {
+                j.tile_begin = jl.blocks;
+                jl.blocks += wema_dt_tiles(j.w, j.h);
+            }
+        } else {
+            JobBuilder::strips_of(v, jl, 0);
+        }
         size_t off = 0;
         if (!JobBuilder::put((char *)tab_h, cap, off, v, jl)) return -1;
         if (hipMemcpyAsync(tab_d, tab_h, off, hipMemcpyHostToDevice, sc.st) != hipSuccess) return -1;
         const Job *jd = (const Job *)((char *)tab_d + jl.offset);
-        if (launch(jd, jl.count, jl.blocks) != hipSuccess) return -1;   // warm-up
+        auto run = [&]() { return tile ? launch_tile(jd, jl.count, jl.blocks) : launch(jd, jl.count, jl.blocks); };
+        if (run() != hipSuccess) return -1;   // warm-up
         const int reps = 3;
         if (hipEventRecord(sc.e0, sc.st) != hipSuccess) return -1;
         for (int r = 0; r < reps; r++)
-            if (launch(jd, jl.count, jl.blocks) != hipSuccess) return -1;
+            if (run() != hipSuccess) return -1;
         if (hipEventRecord(sc.e1, sc.st) != hipSuccess || hipEventSynchronize(sc.e1) != hipSuccess) return -1;
         float t = 0.0f;
         if (hipEventElapsedTime(&t, sc.e0, sc.e1) != hipSuccess) return -1;
@@ -92,7 +105,6 @@ int time_strips(std::vector<Job> v, void *tab_h, void *tab_d, size_t cap, Scratc
     }
     return 0;
 }
-
 }  // namespace
 
 // Strip heights of a session's DWT levels, from the shapes tuned so far or by timing the
@@ -122,12 +134,14 @@ int wema_dwt_tune(wema_dev *s) {
         sc.p[0] = tab_d;
         return 0;
     };
-    auto pick = [](const double *ms) {
+    auto pick = [](const double *ms, int ncand) {
         int b = 0;
-        for (int c = 1; c < 3; c++)
+        for (int c = 1; c < ncand; c++)
             if (ms[c] < ms[b]) b = c;
         return kStrips[b];
     };
+    // LDS tiles are a candidate for the levels below full resolution of 2 x 2 samples or more
+    auto cands = [](int l, int w, int h) { return l > 0 && w >= 2 && h >= 2 ? 4 : 3; };
     int rc = 0;
     for (int l = 0; l < L && rc == 0; l++) {
         // forward level l
@@ -136,14 +150,18 @@ int wema_dwt_tune(wema_dev *s) {
             const auto key = std::make_tuple(dev, l == 0 ? 0 : 1, fv[0].w, fv[0].h, fv[0].nch, n);
             auto it = g_strips.find(key);
             if (it == g_strips.end()) {
-                double ms[3];
+                double ms[4];
                 const bool lev0 = l == 0;
+                const int nc = cands(l, fv[0].w, fv[0].h);
                 if (ensure() < 0 ||
-                    time_strips(fv, tab_h, tab_d, cap, sc, ms, [&](const DwtfJob *j, int cnt, int nb) {
-                        return wema_launch_dwt_fwd(j, cnt, nb, lev0, sc.st);
-                    }) < 0) { rc = -1; break; }
+                    time_strips(fv, tab_h, tab_d, cap, sc, nc, ms,
+                                [&](const DwtfJob *j, int cnt, int nb) { return wema_launch_dwt_fwd(j, cnt, nb, lev0, sc.st); },
+                                [&](const DwtfJob *j, int cnt, int nb) { return wema_launch_dwt_fwd_tile(j, cnt, nb, sc.st); }) < 0) {
+                    rc = -1;
+                    break;
+                }
                 timed = true;
-                it = g_strips.emplace(key, pick(ms)).first;
+                it = g_strips.emplace(key, pick(ms, nc)).first;
             }
             s->sp_fwd[l] = it->second;
         }
@@ -155,18 +173,20 @@ int wema_dwt_tune(wema_dev *s) {
             const auto key = std::make_tuple(dev, kind, iv[0].w, iv[0].h, iv[0].nch, n);
             auto it = g_strips.find(key);
             if (it == g_strips.end()) {
-                double ms[3];
+                double ms[4];
                 const bool lev0 = l == 0;
+                const int nc = ycc ? 3 : cands(l, iv[0].w, iv[0].h);
                 if (ensure() < 0) { rc = -1; break; }
-                const int r = ycc ? time_strips(iv, tab_h, tab_d, cap, sc, ms, [&](const DwtiJob *j, int cnt, int nb) {
-                                        return wema_launch_dwt_inv_ycc(j, cnt, nb, sc.st);
-                                    })
-                                  : time_strips(iv, tab_h, tab_d, cap, sc, ms, [&](const DwtiJob *j, int cnt, int nb) {
-                                        return wema_launch_dwt_inv(j, cnt, nb, lev0, sc.st);
-                                    });
+                auto tile = [&](const DwtiJob *j, int cnt, int nb) { return wema_launch_dwt_inv_tile(j, cnt, nb, sc.st); };
+                const int r = ycc ? time_strips(iv, tab_h, tab_d, cap, sc, nc, ms,
+                                                [&](const DwtiJob *j, int cnt, int nb) { return wema_launch_dwt_inv_ycc(j, cnt, nb, sc.st); },
+                                                tile)
+                                  : time_strips(iv, tab_h, tab_d, cap, sc, nc, ms,
+                                                [&](const DwtiJob *j, int cnt, int nb) { return wema_launch_dwt_inv(j, cnt, nb, lev0, sc.st); },
+                                                tile);
                 if (r < 0) { rc = -1; break; }
                 timed = true;
-                it = g_strips.emplace(key, pick(ms)).first;
+                it = g_strips.emplace(key, pick(ms, nc)).first;
             }
             s->sp_inv[l] = it->second;
         }
@@ -190,11 +210,11 @@ extern "C" int wema_dev_strip_rows(const wema_dev_t *s, int level, int inverse) 
 
 extern "C" int wema_dev_set_strip_rows(wema_dev_t *s, int rows) {
     if (!s) { wema_set_error("null session"); return -1; }
-    if (rows != 0 && rows != 8 && rows != 16 && rows != 32) {
-        wema_set_error("strip rows must be 0 (the rule), 8, 16 or 32");
+    if (rows != 0 && rows != 8 && rows != 16 && rows != 32 && rows != WEMA_SP_TILE) {
+        wema_set_error("strip rows must be 0 (the rule), 8, 16, 32 or 1 (LDS tiles below full resolution)");
         return -1;
     }
-    for (int l = 0; l < WEMA_MAX_LEVELS; l++) s->sp_fwd[l] = s->sp_inv[l] = rows;
+    for (int l = 0; l < WEMA_MAX_LEVELS; l++) s->sp_fwd[l] = s->sp_inv[l] = (rows == WEMA_SP_TILE && l == 0) ? 0 : rows;
     return 0;
 }
 
