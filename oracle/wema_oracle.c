@@ -461,9 +461,14 @@ float orc_fade_amp(float amp, int global_frame) {
     const int fade_in_frames = 15;
     if (global_frame < 0) return 0.0f;
     if (global_frame < fade_in_frames) {
-        float t = (float)(global_frame + 1) / (float)fade_in_frames;
-        float at = amp * t;
-        return at * t;
+        /* src/wema.c:830-839 and src/wema_batch.c:426-435 write amp * t * t with
+         * t = (g + 1) / 15; built -Ofast (reciprocal math, reassociation) both compile to
+         * t = (g + 1) * fl(1/15), then amp * (t * t): a multiply by the rounded reciprocal
+         * and the square first, which differs from the division and (amp * t) * t by an ulp
+         * at some g (g = 2: 4.8000007 against 4.8) */
+        const float t = (float)(global_frame + 1) * (1.0f / (float)fade_in_frames);
+        const float tt = t * t;
+        return amp * tt;
     }
     return amp;
 }

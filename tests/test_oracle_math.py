@@ -210,10 +210,20 @@ def test_abs_fixed_and_threshold():
 
 
 def test_fade_closed_form():
-    for g in range(-2, 20):
-        a = L.orc_fade_amp(C.c_float(120.0), g)
-        want = 0.0 if g < 0 else (120.0 * ((g + 1) / 15.0) ** 2 if g < 15 else 120.0)
-        assert abs(a - want) <= 1e-4 * max(1.0, want), g
+    """((g + 1) / 15)^2 amp for g in 0..14, 0 before, amp after, in the FP32 operation order
+    of the reference's -Ofast build: t = (g + 1) * fl(1/15), amp * (t * t) (the compiled
+    reference's amplified coefficients pin that order: test_amplified_bitexact_vs_reference)."""
+    f32 = np.float32
+    for amp in (120.0, 50.0, 37.5):
+        for g in range(-2, 20):
+            a = L.orc_fade_amp(C.c_float(amp), g)
+            want = 0.0 if g < 0 else (amp * ((g + 1) / 15.0) ** 2 if g < 15 else amp)
+            assert abs(a - want) <= 1e-6 * max(1.0, want), g
+            if 0 <= g < 15:
+                t = f32(g + 1) * f32(f32(1.0) / f32(15.0))
+                assert a == f32(amp) * f32(t * t), (amp, g)
+    # the division form, (amp t) t, differs at g = 2: the test tells the two apart
+    assert L.orc_fade_amp(C.c_float(120.0), 2) != f32(f32(120.0) * f32(f32(3.0) / f32(15.0))) * f32(f32(3.0) / f32(15.0))
 
 
 def box_brute(x, w, h, r=4):

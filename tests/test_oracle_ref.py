@@ -203,6 +203,25 @@ def test_pipeline_vs_reference(mode):
     assert worst <= 1, worst
 
 
+@pytest.mark.parametrize("color,amp,fps,fl,fh", [(False, 120.0, 30.0, 0.5, 3.0), (True, 120.0, 30.0, 0.5, 3.0),
+                                                 (False, 50.0, 60.0, 0.8, 2.0)])
+def test_amplified_bitexact_vs_reference(color, amp, fps, fl, fh):
+    """Amplified coefficients (src/wema.c:714-744 with the fade-in of 830-839) bit-identical
+    to the compiled reference's over the 15 fade-in frames and past them, given its
+    thresholds: the fade's FP32 order is the -Ofast build's (t = (g + 1) * fl(1/15),
+    amp * (t * t)), and each masked coefficient is fma(alpha_l, y, c)."""
+    w, h, n, levels = 64, 48, 18, 3
+    kw = dict(fps=fps, amp=amp, f_low=fl, f_high=fh, levels=levels, color=color)
+    ref = O.RefPipe(w, h, **kw)
+    orc = O.OraclePipe(w, h, **kw)
+    for t, fr in enumerate(clip(w, h, n, 21, color)):
+        _, r_thr = ref.frame(fr)
+        orc.frame(fr, thr_override=r_thr)
+        for ch in range(3 if color else 1):
+            ra = O.flat6_to_planes(ref.amplified(ch), w, h, levels)
+            assert np.array_equal(ra, orc.buffer(2, ch)), (t, ch, int((ra != orc.buffer(2, ch)).sum()))
+
+
 def test_threshold_sum_relative_error():
     """The reference's FP32 running sum vs the exact sum on a 1080p-sized pyramid."""
     w, h, levels = 1920, 1080, 5

@@ -139,9 +139,11 @@ float wema_fade_amp(float amp, long global_frame) {
     const int fade_in_frames = 15;
     if (global_frame < 0) return 0.0f;
     if (global_frame < fade_in_frames) {
-        const float t = (float)(global_frame + 1) / (float)fade_in_frames;
-        const float at = amp * t;
-        return at * t;
+        // the reference's -Ofast build evaluates amp * t * t, t = (g + 1) / 15, as
+        // amp * (t * t) with t = (g + 1) * fl(1/15) (reciprocal math and reassociation)
+        const float t = (float)(global_frame + 1) * (1.0f / (float)fade_in_frames);
+        const float tt = t * t;
+        return amp * tt;
     }
     return amp;
 }
