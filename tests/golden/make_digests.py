@@ -5,6 +5,12 @@ oracle on at test time:
   c2      1920x1080 colour IIR, 5 levels, 0.5-3 Hz, amp 120, 20 frames (past the 15-frame
           fade-in of src/wema.c:830-839, so frames 15-19 run at full gain);
   c3      3840x2160 luma + guided filter, 6 levels, 0.8-2 Hz, amp 50, 20 frames;
+  c2w_nb  1920x1080 colour, 5 levels, sliding window of 32 frames without bilateral weights,
+          amp 120, 36 frames (the window is full from frame 31 on);
+  c2w     the same with bilateral weights (the window benchmark): the exponential differs
+          from libm's by an ulp, so output bytes may differ by 1 LSB and a digest cannot
+          hold them; instead the bytes at 2048 fixed positions of every frame are stored
+          (base64), with the exact thresholds;
   c4_*    the 640x480 and 1280x720 streams of the 24-stream mix (8:8:6:2 of 640x480@30,
           1280x720@30, 1920x1080@60, 3840x2160@30; band in 0.3-5 Hz, amp in 20-150 and
           window in {16, 32, 64} drawn per stream from a fixed seed; automatic levels;
@@ -18,6 +24,7 @@ and thresholds with these digests.
 
 Run from the repository root:  python tests/golden/make_digests.py   (8 processes, ~2 min)
 """
+import base64
 import hashlib
 import json
 import multiprocessing as mp
@@ -37,6 +44,12 @@ FULL = {
                kw=dict(levels=5, amp=120.0, f_low=0.5, f_high=3.0, color=True)),
     "c3": dict(preset=3, w=3840, h=2160, fps=60.0, seed_offset=0,
                kw=dict(levels=6, amp=50.0, f_low=0.8, f_high=2.0)),
+    "c2w_nb": dict(preset=2, w=1920, h=1080, fps=30.0, seed_offset=0, frames=36,
+                   kw=dict(levels=5, amp=120.0, f_low=0.5, f_high=3.0, color=True, use_iir=False,
+                           window=32)),
+    "c2w": dict(preset=2, w=1920, h=1080, fps=30.0, seed_offset=0, frames=36, samples=2048,
+                kw=dict(levels=5, amp=120.0, f_low=0.5, f_high=3.0, color=True, use_iir=False,
+                        window=32, bilateral=True)),
 }
 
 MIX = [(640, 480, 30.0, 8), (1280, 720, 30.0, 8), (1920, 1080, 60.0, 6), (3840, 2160, 30.0, 2)]
@@ -58,18 +71,30 @@ def c4_streams():
     return out
 
 
+def sample_positions(w, h, n):
+    """Fixed byte positions of an RGB24 frame for the sampled cases."""
+    return np.sort(np.random.default_rng(0x5A).choice(3 * w * h, size=n, replace=False))
+
+
 def run_case(args):
     name, c = args
     import wema_amd
     from oracle import oracle as O
     syn = wema_amd.Synth(c["preset"], c["w"], c["h"], c["fps"], c["seed_offset"])
     orc = O.OraclePipe(c["w"], c["h"], fps=c["fps"], **c["kw"])
-    sha, thr = [], []
-    for t in range(FRAMES):
+    frames = c.get("frames", FRAMES)
+    pos = sample_positions(c["w"], c["h"], c["samples"]) if c.get("samples") else None
+    sha, thr, smp = [], [], []
+    for t in range(frames):
         out, th = orc.frame(syn.frame(t))
         sha.append(hashlib.sha256(out.tobytes()).hexdigest())
         thr.append([int(x) for x in th.astype(np.float32).view(np.uint32)])
-    return name, dict(params=c, frames=FRAMES, out_sha256=sha, thr_bits=thr)
+        if pos is not None:
+            smp.append(base64.b64encode(out.reshape(-1)[pos].tobytes()).decode())
+    res = dict(params=c, frames=frames, out_sha256=sha, thr_bits=thr)
+    if pos is not None:
+        res["samples_b64"] = smp
+    return name, res
 
 
 def main():

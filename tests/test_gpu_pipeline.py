@@ -489,6 +489,48 @@ def test_full_size_full_gain_vs_oracle_digests(name, batch):
     sess.close()
 
 
+def sample_positions(w, h, n):
+    """The fixed byte positions of tests/golden/make_digests.py's sampled cases."""
+    return np.sort(np.random.default_rng(0x5A).choice(3 * w * h, size=n, replace=False))
+
+
+@pytest.mark.parametrize("name", ["c2w_nb", "c2w"])
+def test_full_size_window_vs_oracle(name):
+    """The 1080p colour window workload (32 frames, 5 levels) at full size for 36 frames in
+    calls of 12 (a ring of 32 + 11 slots, passthrough while the window fills, windowed from
+    frame 31 on). Without bilateral weights every frame's bytes and thresholds equal the
+    oracle's (digests). With them (the window benchmark) the thresholds are equal and the
+    bytes at 2048 fixed positions of every frame are within 1 LSB of the oracle's, the
+    exponential differing by an ulp (tests/golden/make_digests.py)."""
+    import base64
+    case = digest_case(name)
+    c = case["params"]
+    w, h, n, batch = c["w"], c["h"], case["frames"], 12
+    syn = wema_amd.Synth(c["preset"], w, h, c["fps"], c["seed_offset"])
+    d_in = syn.frames_device(0, n)
+    sess = wema_amd.Session(w, h, c["fps"], max_batch=batch, **c["kw"])
+    d_out = torch.empty_like(d_in)
+    thr = []
+    for t0 in range(0, n, batch):
+        k = min(batch, n - t0)
+        sess.process_device(d_in[t0:t0 + k], d_out_u8=d_out[t0:t0 + k], n=k)
+        sess.sync()
+        thr.append(sess.thresholds(k))
+    thr = np.concatenate(thr)
+    out = d_out.cpu().numpy()
+    pos = sample_positions(w, h, c["samples"]) if c.get("samples") else None
+    worst = 0
+    for t in range(n):
+        assert list(thr[t][:3].view(np.uint32)) == case["thr_bits"][t][:3], (name, t)
+        if pos is None or t + 1 < c["kw"]["window"]:
+            assert hashlib.sha256(out[t].tobytes()).hexdigest() == case["out_sha256"][t], (name, t)
+        else:
+            ref = np.frombuffer(base64.b64decode(case["samples_b64"][t]), np.uint8)
+            worst = max(worst, max_diff(out[t].reshape(-1)[pos], ref))
+    assert worst <= 1, worst
+    sess.close()
+
+
 def test_c1_clip_vs_oracle():
     """The 320x240 workload as the benchmark runs it: its whole 128-frame clip in one call
     (the IIR state in registers over all frames), bytes and thresholds equal to the oracle's
