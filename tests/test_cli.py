@@ -72,3 +72,28 @@ def test_probe_and_settings(tmp_path, extra, levels):
     assert "Frequency band: 0.60 - 14.98 Hz" in r.stderr     # --fh clamped to Nyquist
     if r.returncode != 0:
         assert "Failed to initialize WEMA" in r.stderr
+
+
+def test_option_paths_under_address_and_undefined_sanitizers(tmp_path):
+    """The CLI's host code (option parsing, the stream probe, the error paths, the set-up up
+    to the device session) built with AddressSanitizer and UndefinedBehaviorSanitizer, leak
+    detection on: no report on any of the option and probe cases above."""
+    exe = tmp_path / "wema_asan"
+    lib = os.path.join(ROOT, "wema_amd")
+    b = subprocess.run(["gcc", "-std=c11", "-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+                        "-fno-sanitize-recover=undefined", "-I" + os.path.join(ROOT, "include"),
+                        os.path.join(lib, "host", "wema_cli.c"), "-L" + lib, "-lwema_dev", "-Wl,-rpath," + lib,
+                        "-o", str(exe)], capture_output=True, text=True)
+    if b.returncode != 0:
+        pytest.skip("sanitizer build unavailable: " + b.stderr[-200:])
+    src = tmp_path / "clip.raw"
+    src.write_bytes(bytes(96 * 64 * 3 * 2))
+    env = dict(os.environ, PATH=FAKES + os.pathsep + os.environ["PATH"], FAKE_W="96", FAKE_H="64",
+               FAKE_FPS="30/1", FAKE_N="2", ASAN_OPTIONS="detect_leaks=1:exitcode=97", UBSAN_OPTIONS="exitcode=98")
+    cases = [["--help"], [], ["-i"], ["-i", "x", "--bogus"], ["-i", "x", "--levels", "13"],
+             ["-i", str(tmp_path / "missing.raw")], ["-i", str(src), "-v", "--levels", "5", "--fl", "0.6"],
+             ["-i", str(src), "--color", "--haar", "--bilateral-filter", "--temporal-window", "16"]]
+    for args in cases:
+        r = subprocess.run([str(exe)] + args, capture_output=True, text=True, timeout=120, env=env)
+        assert r.returncode not in (97, 98), (args, r.stderr[-2000:])
+        assert "ERROR: AddressSanitizer" not in r.stderr and "runtime error:" not in r.stderr, (args, r.stderr[-2000:])
