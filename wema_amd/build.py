@@ -43,7 +43,8 @@ def hipcc():
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp%d" % os.getpid()
-    cmd = [hipcc()] + HIPCC_FLAGS + srcs + ["-o", tmp]
+    # (rocprofiler-sdk-roctx: named host ranges of calls and stage launches for tracers)
+    cmd = [hipcc()] + HIPCC_FLAGS + srcs + ["-lrocprofiler-sdk-roctx", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -744,23 +744,27 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
         const JobList &jl = jb.dwtf[l];
         if (jl.count) {
+            TraceRange r_(l == 0 ? "k_dwt_fwd0" : "k_dwt_fwd");
             LAUNCH_OK(wema_launch_dwt_fwd((const DwtfJob *)(base + jl.offset), jl.count, jl.blocks, l == 0, st));
             if (timer_mark(timer, l == 0 ? TK_DWT_FWD0 : TK_DWT_FWD, jl.bytes, st) < 0) return -2;
         }
         const JobList &tl = jb.dwtf_t[l];
         if (tl.count) {
+            TraceRange r_("k_dwt_fwd_tile");
             LAUNCH_OK(wema_launch_dwt_fwd_tile((const DwtfJob *)(base + tl.offset), tl.count, tl.blocks, st));
             if (timer_mark(timer, TK_DWT_FWD, tl.bytes, st) < 0) return -2;
         }
     }
     const JobList &tl = jb.temporal;
     if (tl.count) {
+        TraceRange r_("k_iir");
         LAUNCH_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
         if (timer_mark(timer, TK_IIR, tl.bytes, st) < 0) return -2;
     }
     for (int v = 0; v < WEMA_WIN_KINDS; v++) {
         const JobList &wl = jb.window[v];
         if (!wl.count) continue;
+        TraceRange r_("k_win");
         LAUNCH_OK(wema_launch_win(v, (const TemporalJob *)(base + wl.offset), wl.count, wl.blocks, st));
         if (timer_mark(timer, TK_WIN, wl.bytes, st) < 0) return -2;
     }
@@ -770,17 +774,20 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
     for (int l = 0; l < WEMA_MAX_LEVELS; l++) {
         const JobList &jl = jb.dwti[l];
         if (jl.count) {
+            TraceRange r_(l == last_inv ? "k_dwt_inv0" : "k_dwt_inv");
             LAUNCH_OK(wema_launch_dwt_inv((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, l == last_inv, st));
             if (timer_mark(timer, l == last_inv ? TK_DWT_INV0 : TK_DWT_INV, jl.bytes, st) < 0) return -2;
         }
         const JobList &tl = jb.dwti_t[l];
         if (tl.count) {
+            TraceRange r_("k_dwt_inv_tile");
             LAUNCH_OK(wema_launch_dwt_inv_tile((const DwtiJob *)(base + tl.offset), tl.count, tl.blocks, st));
             if (timer_mark(timer, TK_DWT_INV, tl.bytes, st) < 0) return -2;
         }
     }
     if (jb.dwti_ycc.count) {
         const JobList &jl = jb.dwti_ycc;
+        TraceRange r_("k_dwt_inv0_ycc");
         LAUNCH_OK(wema_launch_dwt_inv_ycc((const DwtiJob *)(base + jl.offset), jl.count, jl.blocks, st));
         if (timer_mark(timer, TK_DWT_INV0_YCC, jl.bytes, st) < 0) return -2;
     }
@@ -788,6 +795,7 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
     for (int k = 0; k < EG_KINDS; k++) {
         const JobList &jl = jb.egress[k];
         if (!jl.count) continue;
+        TraceRange r_(wema_timer_names[eg_kind[k]]);
         LAUNCH_OK(wema_launch_egress(k, (const EgressJob *)(base + jl.offset), jl.count, jl.blocks, st));
         if (timer_mark(timer, eg_kind[k], jl.bytes, st) < 0) return -2;
     }
@@ -806,6 +814,7 @@ static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, flo
     }
     if (!in || (!out_u8 && !out_f32)) { wema_set_error("null frame buffer"); return -1; }
     HIP_OK(hipSetDevice(s->p.device));
+    TraceRange trace("wema_dev_process");
     hipStream_t st = (hipStream_t)stream;    // NULL: the default (null) stream
     s->last_stream = st;
     const int k = s->stage_cur;

@@ -1,6 +1,7 @@
 // Host-side session state and job-table builder shared by wema_dev.hip (one stream) and
 // wema_pool.hip (many streams per tick).
 #pragma once
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -145,6 +146,15 @@ struct wema_dev {
     };
     std::deque<QueuedFrame> fq;
     int fq_pending = 0;
+};
+
+// A named host-side range for tracers (rocprofv3 --marker-trace): a call, a pool tick, and
+// each stage launch within them.
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
 };
 
 struct JobBuilder {

@@ -214,6 +214,7 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
                               uint8_t *const *d_out, void *hip_stream) {
     if (!p || n < 1 || !ids || !d_in || !d_out) { wema_set_error("bad argument"); return -1; }
     HIP_OK(hipSetDevice(p->device));
+    TraceRange trace("wema_pool_tick");
     hipStream_t st = hip_stream ? (hipStream_t)hip_stream : p->stream;
     // runs of equal ids: consecutive frames of one session (at most its max_batch)
     struct Run {
