@@ -190,7 +190,7 @@ static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Byte layout of one session's arena.
 struct Layout {
-    size_t coef, ampc, filt, state, ll, rec, acc, amp, thr, thr_ovr, tw, total;
+    size_t coef, ampc, filt, state, ll, rec, acc, thr, thr_ovr, tw, total;
 };
 
 static void plan_layout(const WemaGeom &g, const wema_dev_params &p, int with_filt, Layout *L) {
@@ -209,7 +209,6 @@ static void plan_layout(const WemaGeom &g, const wema_dev_params &p, int with_fi
     L->ll = take(sizeof(float) * g.ll_total * nch * B);
     L->rec = take(sizeof(float) * g.P * nch * B);
     L->acc = take(sizeof(unsigned long long) * wema_acc_entries((int)nch, (int)B));
-    L->amp = take(sizeof(float) * nch * B);
     L->thr = take(sizeof(float) * 3 * B);
     L->thr_ovr = take(sizeof(float) * 3 * B);
     L->tw = take(sizeof(float) * 256);
@@ -383,7 +382,6 @@ int wema_session_alloc(wema_dev *s, int with_filt) {
     s->ll = (float *)(a + L.ll);
     s->rec = (float *)(a + L.rec);
     s->acc = (unsigned long long *)(a + L.acc);
-    s->amp_d = (float *)(a + L.amp);
     s->thr_d = (float *)(a + L.thr);
     s->thr_ovr_d = (float *)(a + L.thr_ovr);
     s->tw_d = (float *)(a + L.tw);
@@ -713,10 +711,10 @@ int wema_timer_collect(LaunchTimer *t) {
 }
 
 // Uploads the tables of a builder and launches all stages on stream st.
-int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage_ev,
-                  float *amp_d, hipStream_t st, LaunchTimer *timer) {
+int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage_ev, hipStream_t st,
+                  LaunchTimer *timer) {
     size_t bytes = 0;
-    if (jb.pack(stage_h, WEMA_STAGE_BYTES, &bytes) < 0) {
+    if (jb.pack(stage_h, stage_d, WEMA_STAGE_BYTES, &bytes) < 0) {
         wema_set_error("job tables exceed %d bytes", WEMA_STAGE_BYTES);
         return -1;
     }
@@ -725,9 +723,6 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
         return -1;
     }
     HIP_OK(hipMemcpyAsync(stage_d, stage_h, bytes, hipMemcpyHostToDevice, st));
-    if (!jb.host_amp.empty())
-        HIP_OK(hipMemcpyAsync(amp_d, (char *)stage_h + jb.amp_offset,
-                              jb.host_amp.size() * sizeof(float), hipMemcpyHostToDevice, st));
     if (stage_ev) HIP_OK(hipEventRecord(stage_ev, st));
     char *base = (char *)stage_d;
     // from here on part of the call is enqueued: a failure returns -2
@@ -824,8 +819,7 @@ static int process(wema_dev *s, const void *in, int in_f32, uint8_t *out_u8, flo
         HIP_OK(wema_launch_zero_u64(bufs->acc, (long long)n * s->g.nch * wema_acc_slots(n), st));
     JobBuilder jb;
     wema_build_jobs(s, jb, in, in_f32, out_u8, out_f32, n, bufs);
-    jb.fix_amp(s->amp_d);
-    const int rc = wema_run_jobs(jb, s->stage_h[k], s->stage_d[k], s->stage_ev[k], s->amp_d, st, &s->timer);
+    const int rc = wema_run_jobs(jb, s->stage_h[k], s->stage_d[k], s->stage_ev[k], st, &s->timer);
     if (rc < 0) {
         s->failed = rc == -2;
         return -1;

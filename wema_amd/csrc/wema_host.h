@@ -108,7 +108,6 @@ struct wema_dev {
     float *ll = nullptr;        // [B][nch][ll_total]
     float *rec = nullptr;       // level-0 reconstructions [B][nch][P]
     unsigned long long *acc = nullptr;   // [B][nch][acc slots] (wema_acc_entries)
-    float *amp_d = nullptr;     // [nch][B] effective amplification
     float *thr_d = nullptr;     // [B][3] thresholds used
     float *thr_ovr_d = nullptr; // [B][3] overrides (NaN = none)
     float *tw_d = nullptr;      // log2 of the bilateral time weights [256]
@@ -218,7 +217,10 @@ struct JobBuilder {
         egress[kind].bytes += bytes;
         egress_v[kind].push_back(j);
     }
-    void fix_amp(float *amp_d) {
+    // The temporal jobs read their frames' amplification from the table's own copy of
+    // host_amp (at amp_offset of the device table at dev_base).
+    void fix_amp(const void *dev_base) {
+        const float *amp_d = (const float *)((const char *)dev_base + amp_offset);
         for (auto &j : temporal_v) j.amp = amp_d + j.amp_index;
         for (auto &v : window_v)
             for (auto &j : v) j.amp = amp_d + j.amp_index;
@@ -306,11 +308,18 @@ struct JobBuilder {
         segments_of(egress_v[EG_GF], egress[EG_GF], WEMA_GF_CONCURRENT);
         segments_of(egress_v[EG_GF_ANY], egress[EG_GF_ANY], WEMA_GF_CONCURRENT);
     }
-    int pack(void *dst_v, size_t cap, size_t *used) {
+    // Packs every list into dst (pinned host memory) for one upload to dev_base: the
+    // amplification array first, so that its device address is known when the temporal
+    // jobs that point into it are written.
+    int pack(void *dst_v, const void *dev_base, size_t cap, size_t *used) {
         assign_strips();
         char *dst = (char *)dst_v;
         size_t off = 0;
         bool ok = true;
+        JobList amp_list;
+        ok = ok && put(dst, cap, off, host_amp, amp_list);
+        amp_offset = amp_list.offset;
+        fix_amp(dev_base);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwtf_v[l], dwtf[l]);
         ok = ok && put(dst, cap, off, temporal_v, temporal);
         for (int k = 0; k < WEMA_WIN_KINDS; k++) ok = ok && put(dst, cap, off, window_v[k], window[k]);
@@ -319,9 +328,6 @@ struct JobBuilder {
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwtf_t_v[l], dwtf_t[l]);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_t_v[l], dwti_t[l]);
         for (int k = 0; k < EG_KINDS; k++) ok = ok && put(dst, cap, off, egress_v[k], egress[k]);
-        JobList amp_list;
-        ok = ok && put(dst, cap, off, host_amp, amp_list);
-        amp_offset = amp_list.offset;
         *used = off;
         return ok ? 0 : -1;
     }
@@ -344,7 +350,7 @@ enum InjectSite { INJ_NONE = 0, INJ_TABLE_UPLOAD = 1, INJ_LAUNCH = 2, INJ_FRAME_
 bool wema_inject(int site);
 // Launches all stages of a builder; returns 0, -1 if it failed before enqueuing anything,
 // or -2 if it failed after enqueuing some stages.
-int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage_ev,
-                  float *amp_d, hipStream_t st, LaunchTimer *timer);
+int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage_ev, hipStream_t st,
+                  LaunchTimer *timer);
 int wema_timer_collect(LaunchTimer *t);
 extern const char *const wema_timer_names[TK_COUNT];

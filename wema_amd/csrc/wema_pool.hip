@@ -31,7 +31,6 @@ int wema_session_setup(wema_dev *s, int with_filt, bool pooled);
 struct StageBuf {
     void *h = nullptr;       // pinned job tables
     void *d = nullptr;       // device copy
-    float *amp = nullptr;    // device amplification array
     hipEvent_t ev = nullptr; // set when the last upload from h finished
 };
 
@@ -60,7 +59,6 @@ struct wema_pool {
 static int stage_alloc(StageBuf &b) {
     HIP_OK(hipHostMalloc(&b.h, WEMA_STAGE_BYTES, hipHostMallocDefault));
     HIP_OK(hipMalloc(&b.d, WEMA_STAGE_BYTES));
-    HIP_OK(hipMalloc(&b.amp, sizeof(float) * 65536));
     HIP_OK(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
     return 0;
 }
@@ -68,7 +66,6 @@ static int stage_alloc(StageBuf &b) {
 static void stage_free(StageBuf &b) {
     if (b.h) hipHostFree(b.h);
     if (b.d) hipFree(b.d);
-    if (b.amp) hipFree(b.amp);
     if (b.ev) hipEventDestroy(b.ev);
     b = StageBuf();
 }
@@ -258,8 +255,7 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
         StageBuf &b = p->direct[p->direct_cur];
         p->direct_cur ^= 1;
         HIP_OK(hipEventSynchronize(b.ev));
-        jb.fix_amp(b.amp);
-        const int rc = wema_run_jobs(jb, b.h, b.d, b.ev, b.amp, st, &p->timer);
+        const int rc = wema_run_jobs(jb, b.h, b.d, b.ev, st, &p->timer);
         if (rc < 0) {
             for (const Run &r : runs) r.s->failed = r.s->failed || rc == -2;
             return -1;
@@ -269,9 +265,8 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
         if (it == p->cache.end()) {
             GraphEntry g;
             if (stage_alloc(g.buf) < 0) return -1;
-            jb.fix_amp(g.buf.amp);
             HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            const int rc = wema_run_jobs(jb, g.buf.h, g.buf.d, nullptr, g.buf.amp, st, nullptr);
+            const int rc = wema_run_jobs(jb, g.buf.h, g.buf.d, nullptr, st, nullptr);
             hipGraph_t graph = nullptr;
             const hipError_t e = hipStreamEndCapture(st, &graph);
             if (rc < 0 || e != hipSuccess) {
@@ -291,9 +286,8 @@ extern "C" int wema_pool_tick(wema_pool_t *p, int n, const int *ids, const uint8
         } else {
             // tables of this tick into the graph's pinned buffer once its last upload ran
             HIP_OK(hipEventSynchronize(it->second.buf.ev));
-            jb.fix_amp(it->second.buf.amp);
             size_t used = 0;
-            if (jb.pack(it->second.buf.h, WEMA_STAGE_BYTES, &used) < 0) {
+            if (jb.pack(it->second.buf.h, it->second.buf.d, WEMA_STAGE_BYTES, &used) < 0) {
                 wema_set_error("job tables exceed %d bytes", WEMA_STAGE_BYTES);
                 return -1;
             }
