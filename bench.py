@@ -19,7 +19,9 @@ Workloads (the default is the single-stream drop-in case, 1920x1080 colour):
 
 roofline: the kernel with the largest share of the timed region; achieved = its
 algorithmic bytes per launch / its mean launch duration, from HIP events recorded on the
-launch stream between the launches of the timed region. traffic: HBM bytes per launch of
+launch stream between the launches of a second pass over the timed steps (the timed pass
+itself runs without them; ms_per_step_with_launch_markers is the second pass's time
+per step). traffic: HBM bytes per launch of
 that kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed in profiles/
 (scaled by the same counters on a copy kernel of known size), or null.
 cpu_baseline: the reference pipeline compiled from its sources (oracle/_ref), on this
@@ -114,7 +116,9 @@ def run_gpu(cfg, name, steps, warmup, batch, gf_seg=0):
     for k in range(warmup):
         sess.process_device(frames[k * batch:(k + 1) * batch], d_out_u8=out[k * batch:(k + 1) * batch], n=batch)
     torch.cuda.synchronize()
-    sess.timing(True)
+    # the timed steps run without per-launch event markers (an event recorded between two
+    # launches costs a few microseconds of stream time); a second pass over the same frames,
+    # with a marker after every launch, gives each kernel's mean launch duration
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t_wall = time.perf_counter()
@@ -125,13 +129,21 @@ def run_gpu(cfg, name, steps, warmup, batch, gf_seg=0):
     torch.cuda.synchronize()
     t_wall = time.perf_counter() - t_wall
     ms = e0.elapsed_time(e1)
+    out_sample = out[warmup * batch: warmup * batch + 2].cpu().numpy()
+    sess.timing(True)
+    e0.record()
+    for k in range(warmup, warmup + steps):
+        sess.process_device(frames[k * batch:(k + 1) * batch], d_out_u8=out[k * batch:(k + 1) * batch], n=batch)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_markers = e0.elapsed_time(e1)
     kernels = sess.timing_report()
     sess.timing(False)
     seg = sess.tile_rows
     frames_timed = steps * batch
     mpix = frames_timed * w * h / 1e6
     res = dict(ms=ms, wall_ms=t_wall * 1e3, frames=frames_timed, mpix_s=mpix / (ms / 1e3),
-               fps=frames_timed / (ms / 1e3), kernels=kernels, gf_seg=seg)
+               fps=frames_timed / (ms / 1e3), kernels=kernels, gf_seg=seg, ms_markers=ms_markers)
     dom = max(kernels, key=lambda k: k["ms"])
     avg_ms = dom["ms"] / dom["launches"]
     bytes_per_launch = dom["bytes"] / dom["launches"]
@@ -144,7 +156,7 @@ def run_gpu(cfg, name, steps, warmup, batch, gf_seg=0):
                            mean_launch_us=round(avg_ms * 1e3, 2))
     tot_bytes = sum(k["bytes"] for k in kernels)
     res["pipeline_gbs"] = round(tot_bytes / (ms / 1e3) / 1e9, 1)
-    res["out_sample"] = out[warmup * batch: warmup * batch + 2].cpu().numpy()
+    res["out_sample"] = out_sample
     res["first_timed_frame"] = warmup * batch
     return res
 
@@ -475,6 +487,7 @@ def main():
         "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": round(r["ms"] / steps, 3),
+        "ms_per_step_with_launch_markers": round(r["ms_markers"] / steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -507,6 +520,7 @@ def main():
         log("%s: %.1f Mpixel/s" % (name, rr["mpix_s"]))
         also[name] = {"desc": c["desc"], "value": round(rr["mpix_s"], 1), "unit": "Mpixel/s",
                       "fps": round(rr["fps"], 1), "steps": st_n, "ms_per_step": round(rr["ms"] / st_n, 3),
+                      "ms_per_step_with_launch_markers": round(rr["ms_markers"] / st_n, 3),
                       "frames_per_step": c["batch"], "pipeline_algorithmic_GBs": rr["pipeline_gbs"],
                       "gf_seg_rows": rr["gf_seg"] if not c["color"] else None,
                       "roofline": rr["roofline"],
