@@ -75,23 +75,33 @@ def read_traffic(kernel, workload):
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
     try:
         with open(path) as f:
-            d = json.load(f)
-        return d.get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
+            ks = json.load(f).get("kernels", {})
     except (OSError, ValueError):
         return None
+    names = [n for n in ks if n in _kernel_names(kernel)]
+    return ks[names[0]].get("hbm_bytes_per_launch") if len(names) == 1 else None
+
+
+def _kernel_names(kernel):
+    """rocprofv3 names of the kernels timed here under one name: the sliding window's
+    variants (k_win_r32 ...) as k_win, the one-frame IIR launches (k_iir1) as k_iir."""
+    if kernel == "k_win":
+        return ("k_win", "k_win_r16", "k_win_r32", "k_win_r64", "k_win_l32", "k_win_l64")
+    if kernel == "k_iir":
+        return ("k_iir", "k_iir1")
+    return (kernel,)
 
 
 def read_valu(kernel, workload):
     """VALU issue fraction of a kernel from profiles/valu_<workload>.json (rocprofv3 SQ
-    counters of the same workload, wema_amd/profile_summary.py), or None. The sliding
-    window's variants (k_win_r32 ...) are timed here under one name, k_win."""
+    counters of the same workload, wema_amd/profile_summary.py), or None."""
     path = os.path.join(ROOT, "profiles", "valu_%s.json" % workload)
     try:
         with open(path) as f:
             ks = json.load(f)["kernels"]
     except (OSError, KeyError, ValueError):
         return None
-    names = [n for n in ks if n == kernel or (kernel == "k_win" and n.startswith("k_win"))]
+    names = [n for n in ks if n in _kernel_names(kernel)]
     return ks[names[0]]["valu_frac"] if len(names) == 1 else None
 
 

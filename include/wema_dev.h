@@ -168,16 +168,17 @@ int wema_dev_import_state(wema_dev_t *s, const void *host, size_t bytes);
  * and sessions opened with that tuner off, leave the split to each launch (one segment
  * height for all its frames, from a cost model of its rows and workgroups). The DWT levels
  * run as strips 8, 16 or 32 pair rows tall, and the levels below full resolution also as
- * LDS tiles of 32 x 32 output pairs: the first session of a shape (level size, channels,
- * frames per call) on a device times the candidates on its own buffers when it is opened
- * and keeps the fastest for each level (wema_dev_strip_rows: a level's forward or inverse
- * height, 1 = LDS tiles, 0 = the fixed rule). Results never depend on either shape.
+ * LDS tiles of 16 x 16 output pairs or with one output pair per thread: the first session
+ * of a shape (level size, channels, frames per call) on a device times the candidates on
+ * its own buffers when it is opened and keeps the fastest for each level
+ * (wema_dev_strip_rows: a level's forward or inverse height, 1 = LDS tiles, 2 = one pair
+ * per thread, 0 = the fixed rule). Results never depend on either shape.
  * wema_dev_autotune(mask) selects the tuners for sessions opened later: WEMA_TUNE_GF,
  * WEMA_TUNE_DWT (both by default; 0: the launch's split and the strips' fixed rule).
  * wema_dev_tile_rows: the session's guided-filter segment height (0: the launch's split);
  * wema_dev_set_tile_rows forces one (a positive multiple of 4, or 0) for its next calls;
  * wema_dev_set_strip_rows one strip height for every level (0 = the rule; 1 = LDS tiles
- * below full resolution, level 0 then keeps the rule). */
+ * or 2 = one pair per thread below full resolution, level 0 then keeps the rule). */
 #define WEMA_TUNE_GF 1
 #define WEMA_TUNE_DWT 2
 void wema_dev_autotune(int mask);

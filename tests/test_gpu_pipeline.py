@@ -91,6 +91,36 @@ def test_iir_luma_bitexact_vs_oracle(w, h, levels):
         assert np.abs(g.astype(int) - o_out.astype(int)).max() == 0, t
 
 
+@pytest.mark.parametrize("rows", [1, 2])
+@pytest.mark.parametrize("w,h,levels", [(37, 29, 3), (50, 33, 4), (130, 17, 3), (70, 45, 4), (260, 390, 5)])
+def test_coarse_level_shapes_bitexact_vs_oracle(rows, w, h, levels):
+    """The levels below full resolution as LDS tiles (1) or with one output pair per thread
+    (2): coefficients, thresholds, IIR output and state, amplified coefficients and bytes
+    equal to the oracle's frame by frame, at odd sizes whose coarse levels are a few samples
+    across (every pair then reaches a reflection)."""
+    fps = 30.0
+    _, frames = clip(4, w, h, fps, 3, seed=11)
+    kw = dict(fps=fps, amp=120.0, f_low=0.5, f_high=3.0, levels=levels)
+    sess = wema_amd.Session(w, h, capture=True, **kw)
+    sess.set_strip_rows(rows)
+    assert sess.strip_rows()[1:] == [rows] * (levels - 1), sess.strip_rows()
+    orc = O.OraclePipe(w, h, **kw)
+    d_in = torch.from_numpy(frames).cuda()
+    d_out = torch.empty_like(d_in)
+    for t in range(frames.shape[0]):
+        sess.process_device(d_in[t:t + 1], d_out_u8=d_out[t:t + 1], n=1)
+        sess.sync()
+        o_out, o_thr = orc.frame(frames[t])
+        assert sess.thresholds(1)[0][0] == o_thr[0], t
+        np.testing.assert_array_equal(sess.read(0), orc.buffer(0), err_msg="coefficients t=%d" % t)
+        np.testing.assert_array_equal(sess.read(1), orc.buffer(1), err_msg="IIR output t=%d" % t)
+        np.testing.assert_array_equal(sess.read(2), orc.buffer(2), err_msg="amplified t=%d" % t)
+        np.testing.assert_array_equal(sess.read(3), orc.state(), err_msg="IIR state t=%d" % t)
+        assert np.array_equal(d_out[t].cpu().numpy(), o_out), t
+    sess.close()
+    orc.close()
+
+
 @pytest.mark.parametrize("mode", ["iir_color", "window", "window_bilateral", "window_color",
                                   "gauss"])
 def test_modes_vs_oracle(mode):
@@ -603,21 +633,22 @@ def test_covariance_division_equals_ieee_on_the_gpu():
                                                    (70, 45, 4, False, 2), (130, 17, 3, True, 1),
                                                    (260, 390, 3, False, 1)])
 def test_strip_rows_do_not_change_results(w, h, levels, color, batch):
-    """The DWT strip height, forced to 8, 16 or 32 pair rows for every level, LDS tiles for
-    the levels below full resolution (1), picked per level by the autotuner, or the fixed
-    rule, gives the same bytes, equal to the oracle's; the small and odd sizes reach the
-    tiles' reflections across several tile widths and past both ends of a level."""
+    """The DWT strip height, forced to 8, 16 or 32 pair rows for every level, LDS tiles (1)
+    or one output pair per thread (2) for the levels below full resolution, picked per level
+    by the autotuner, or the fixed rule, gives the same bytes, equal to the oracle's; the
+    small and odd sizes reach the tiles' reflections across several tile widths and past
+    both ends of a level."""
     fps, n = 30.0, 2 * batch
     kw = dict(fps=fps, amp=100.0, f_low=0.5, f_high=3.0, levels=levels, color=color)
     rng = np.random.default_rng(w * h)
     frames = rng.integers(0, 256, size=(n, h, w, 3), dtype=np.uint8)
     ref = oracle_clip(frames, **kw)
     d_in = torch.from_numpy(frames).cuda()
-    for rows in (None, 0, 8, 16, 32, 1):
+    for rows in (None, 0, 8, 16, 32, 1, 2):
         sess = wema_amd.Session(w, h, max_batch=batch, **kw)
         fwd, inv = sess.strip_rows(), sess.strip_rows(inverse=True)
         assert fwd[0] in (8, 16, 32) and inv[0] in (8, 16, 32), (fwd, inv)
-        assert all(r in (1, 8, 16, 32) for r in fwd + inv), (fwd, inv)
+        assert all(r in (1, 2, 8, 16, 32) for r in fwd + inv), (fwd, inv)
         if rows is not None:
             sess.set_strip_rows(rows)
         d_out = torch.empty_like(d_in)

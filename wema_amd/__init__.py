@@ -298,12 +298,13 @@ class Session:
 
     def strip_rows(self, inverse=False):
         """DWT strip height (pair rows) of each level, forward or inverse (0: the fixed rule;
-        1: LDS tiles, levels below full resolution)."""
+        1: LDS tiles, 2: one output pair per thread, levels below full resolution)."""
         return [lib.wema_dev_strip_rows(self._h, l, int(inverse)) for l in range(self.levels)]
 
     def set_strip_rows(self, rows):
         """Force one DWT strip height (8, 16 or 32 pair rows; 0: the fixed rule) for every level,
-        or 1: LDS tiles for the levels below full resolution (level 0 keeps the rule)."""
+        or for the levels below full resolution 1: LDS tiles, 2: one output pair per thread
+        (level 0 keeps the rule)."""
         _check(lib.wema_dev_set_strip_rows(self._h, int(rows)), "wema_dev_set_strip_rows")
 
     def iir_coeffs(self):

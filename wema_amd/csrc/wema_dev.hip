@@ -24,8 +24,10 @@ hipError_t wema_launch_dwt_fwd(const DwtfJob *, int, int, int, hipStream_t);
 hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, int, hipStream_t);
 hipError_t wema_launch_dwt_fwd_tile(const DwtfJob *, int, int, hipStream_t);
 hipError_t wema_launch_dwt_inv_tile(const DwtiJob *, int, int, hipStream_t);
+hipError_t wema_launch_dwt_fwd_point(const DwtfJob *, int, int, hipStream_t);
+hipError_t wema_launch_dwt_inv_point(const DwtiJob *, int, int, hipStream_t);
 hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *, int, int, hipStream_t);
-hipError_t wema_launch_iir(const TemporalJob *, int, int, hipStream_t);
+hipError_t wema_launch_iir(const TemporalJob *, int, int, int, hipStream_t);
 hipError_t wema_launch_win(int, const TemporalJob *, int, int, hipStream_t);
 hipError_t wema_launch_egress(int, const EgressJob *, int, int, hipStream_t);
 hipError_t wema_launch_synth(const SynthFrame *, const unsigned short *, unsigned char *, int,
@@ -529,7 +531,8 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.h = h;
                 j.tiles_x = tiles_x;
                 j.sp = s->sp_fwd[l];
-                if (j.sp == WEMA_SP_TILE && (l == 0 || w < 2 || h < 2)) j.sp = 0;   // (tiles: coarser levels of 2 x 2 samples or more)
+                if ((j.sp == WEMA_SP_TILE || j.sp == WEMA_SP_POINT) && (l == 0 || w < 2 || h < 2))
+                    j.sp = 0;   // (tiles and pairs: coarser levels of 2 x 2 samples or more)
                 // input read once (RGB24 shared by the channels at level 0) + LL and three
                 // subbands written per channel
                 const double in_b = l == 0 ? 3.0 * w * h * (in_f32 ? 4 : 1) : 4.0 * w * h;
@@ -635,7 +638,7 @@ void wema_build_jobs(wema_dev *s, JobBuilder &jb, const void *in, int in_f32, ui
                 j.h = h;
                 j.tiles_x = tiles_x;
                 j.sp = s->sp_inv[l];
-                if (j.sp == WEMA_SP_TILE && (l == 0 || w < 2 || h < 2)) j.sp = 0;
+                if ((j.sp == WEMA_SP_TILE || j.sp == WEMA_SP_POINT) && (l == 0 || w < 2 || h < 2)) j.sp = 0;
                 jb.add_dwti(WEMA_MAX_LEVELS - 1 - l, j, tiles, in_b + 4.0 * w * h);
             }
         }
@@ -749,11 +752,19 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
             LAUNCH_OK(wema_launch_dwt_fwd_tile((const DwtfJob *)(base + tl.offset), tl.count, tl.blocks, st));
             if (timer_mark(timer, TK_DWT_FWD, tl.bytes, st) < 0) return -2;
         }
+        const JobList &pl = jb.dwtf_p[l];
+        if (pl.count) {
+            TraceRange r_("k_dwt_fwd_point");
+            LAUNCH_OK(wema_launch_dwt_fwd_point((const DwtfJob *)(base + pl.offset), pl.count, pl.blocks, st));
+            if (timer_mark(timer, TK_DWT_FWD, pl.bytes, st) < 0) return -2;
+        }
     }
     const JobList &tl = jb.temporal;
     if (tl.count) {
         TraceRange r_("k_iir");
-        LAUNCH_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, st));
+        bool one_frame = true;
+        for (const TemporalJob &j : jb.temporal_v) one_frame = one_frame && j.nframes == 1 && j.ppt == WEMA_IIR_PPT;
+        LAUNCH_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, one_frame, st));
         if (timer_mark(timer, TK_IIR, tl.bytes, st) < 0) return -2;
     }
     for (int v = 0; v < WEMA_WIN_KINDS; v++) {
@@ -778,6 +789,12 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
             TraceRange r_("k_dwt_inv_tile");
             LAUNCH_OK(wema_launch_dwt_inv_tile((const DwtiJob *)(base + tl.offset), tl.count, tl.blocks, st));
             if (timer_mark(timer, TK_DWT_INV, tl.bytes, st) < 0) return -2;
+        }
+        const JobList &pl = jb.dwti_p[l];
+        if (pl.count) {
+            TraceRange r_("k_dwt_inv_point");
+            LAUNCH_OK(wema_launch_dwt_inv_point((const DwtiJob *)(base + pl.offset), pl.count, pl.blocks, st));
+            if (timer_mark(timer, TK_DWT_INV, pl.bytes, st) < 0) return -2;
         }
     }
     if (jb.dwti_ycc.count) {

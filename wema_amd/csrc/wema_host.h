@@ -164,13 +164,23 @@ struct JobBuilder {
     std::vector<DwtiJob> dwti_ycc_v;          // colour output jobs (level 0 + YCbCr -> RGB)
     std::vector<DwtfJob> dwtf_t_v[WEMA_MAX_LEVELS];   // coarser levels as LDS tiles
     std::vector<DwtiJob> dwti_t_v[WEMA_MAX_LEVELS];
+    std::vector<DwtfJob> dwtf_p_v[WEMA_MAX_LEVELS];   // coarser levels, one output pair per thread
+    std::vector<DwtiJob> dwti_p_v[WEMA_MAX_LEVELS];
     std::vector<EgressJob> egress_v[EG_KINDS];
     JobList dwtf[WEMA_MAX_LEVELS], temporal, window[WEMA_WIN_KINDS], dwti[WEMA_MAX_LEVELS], dwti_ycc, egress[EG_KINDS];
     JobList dwtf_t[WEMA_MAX_LEVELS], dwti_t[WEMA_MAX_LEVELS];
+    JobList dwtf_p[WEMA_MAX_LEVELS], dwti_p[WEMA_MAX_LEVELS];
     std::vector<float> host_amp;
     size_t amp_offset = 0;
 
     void add_dwtf(int l, DwtfJob j, int tiles, double bytes) {
+        if (j.sp == WEMA_SP_POINT) {   // 256 output pairs per workgroup
+            j.tile_begin = dwtf_p[l].blocks;
+            dwtf_p[l].blocks += wema_pt_blocks(j.w, j.h);
+            dwtf_p[l].bytes += bytes;
+            dwtf_p_v[l].push_back(j);
+            return;
+        }
         if (j.sp == WEMA_SP_TILE) {   // one workgroup per tile
             j.tile_begin = dwtf_t[l].blocks;
             dwtf_t[l].blocks += wema_dt_tiles(j.w, j.h);
@@ -193,6 +203,13 @@ struct JobBuilder {
         (window_mode ? window_v[wv] : temporal_v).push_back(j);
     }
     void add_dwti(int stage, DwtiJob j, int tiles, double bytes) {
+        if (j.sp == WEMA_SP_POINT) {
+            j.tile_begin = dwti_p[stage].blocks;
+            dwti_p[stage].blocks += wema_pt_blocks(j.w, j.h);
+            dwti_p[stage].bytes += bytes;
+            dwti_p_v[stage].push_back(j);
+            return;
+        }
         if (j.sp == WEMA_SP_TILE) {
             j.tile_begin = dwti_t[stage].blocks;
             dwti_t[stage].blocks += wema_dt_tiles(j.w, j.h);
@@ -327,6 +344,8 @@ struct JobBuilder {
         ok = ok && put(dst, cap, off, dwti_ycc_v, dwti_ycc);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwtf_t_v[l], dwtf_t[l]);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_t_v[l], dwti_t[l]);
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwtf_p_v[l], dwtf_p[l]);
+        for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_p_v[l], dwti_p[l]);
         for (int k = 0; k < EG_KINDS; k++) ok = ok && put(dst, cap, off, egress_v[k], egress[k]);
         *used = off;
         return ok ? 0 : -1;

@@ -36,6 +36,15 @@ __host__ __device__ static inline int wema_dt_tiles(int w, int h) {
     const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
     return ((sw + WEMA_DT_TP - 1) / WEMA_DT_TP) * ((sh + WEMA_DT_TP - 1) / WEMA_DT_TP);
 }
+// ... or with one thread per output pair, which recomputes the lifting of its own
+// neighbourhood in registers (no LDS, no barriers; strip height value WEMA_SP_POINT), for
+// levels so small that a strip's or a tile's chain of dependent steps is what their launch
+// waits for. 256 pairs per workgroup.
+#define WEMA_SP_POINT 2
+__host__ __device__ static inline int wema_pt_blocks(int w, int h) {
+    const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
+    return (sw * sh + 255) / 256;
+}
 // Strip height per launch: the tallest that still gives this many channel-strips (a colour
 // strip counts 3). Measured on 1080p colour and 4K luma: the forward DWT balances halo
 // re-reads against parallelism at about 64 K; the inverse is fastest with the shortest
@@ -184,9 +193,10 @@ struct TemporalJob {
     int ppt;                        // IIR, one frame: positions per thread (256 ppt per block)
 };
 
-// IIR jobs of one frame take this many positions per thread.
+// IIR jobs of one frame take this many positions per thread (measured against 1, 4 and 8,
+// with the loads issued ahead of the threshold sum: 1-3% faster per frame than 4).
 #ifndef WEMA_IIR_PPT
-#define WEMA_IIR_PPT 4
+#define WEMA_IIR_PPT 2
 #endif
 
 // Guided filter / Gaussian / colour egress over one output frame.

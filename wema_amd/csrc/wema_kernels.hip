@@ -12,12 +12,15 @@
 //               pipeline; level 0 converts RGB24 to Y (and Cb, Cr) on load; writes LL +
 //               LH/HH/HL and adds sum|c| (Q32 integers) to the frame's counter.
 //   k_iir       Butterworth HP+LP biquads (DF2T) per coefficient over the job's frames with
-//               the state kept in registers, warm-up preseed, threshold mask, amplification.
+//               the state kept in registers, warm-up preseed, threshold mask, amplification;
+//               k_iir1 the same for launches of one-frame jobs only.
 //   k_win       sliding-window temporal filter (mean removal, optional bilateral weights)
 //               + amplification.
 //   k_dwt_inv   one 2D synthesis level as strips (column synthesis as a register
 //               pipeline, then rows across lanes); k_dwt_inv0 level 0.
 //   k_dwt_inv0_ycc  colour mode: level 0 of Y, Cb, Cr + YCbCr -> RGB24 in one pass.
+//   k_dwt_fwd_tile/k_dwt_inv_tile, k_dwt_fwd_point/k_dwt_inv_point  the coarser levels as
+//               LDS tiles, or one output pair per thread (autotuner candidates).
 //   k_gf        guided filter (r = 4, eps = 0.01) of the luma delta with 9x9 clipped box
 //               means, streamed down column strips with a and b kept on the chip; adds
 //               the smoothed delta to RGB and writes RGB24.
@@ -1060,6 +1063,174 @@ __global__ void __launch_bounds__(256) k_dwt_inv_tile(const DwtiJob *__restrict_
 }
 
 // ---------------------------------------------------------------------------------------
+// Coarser DWT levels, one output pair per thread (src/dwt.c:369-582). Each thread recomputes
+// the lifting steps its pair depends on from the symmetric extension the strips and tiles
+// read, every value by the same fused multiply-add of the same operands, so the results
+// are identical to theirs; the neighbours' intermediate values are computed again instead
+// of shared, which costs a few times the arithmetic and leaves a chain of a few dozen
+// instructions per thread, with all 81 loads in flight at once.
+
+// Forward lifting of one pair p from the samples v[k] = x[2p - 4 + k], k = 0..8 (the reach
+// of the four steps): lo = the low-pass value of pair p, hi = the high-pass one. With E[m]
+// = v[2m + 4], O[m] = v[2m + 5] the samples of pair p + m, as lift_fwd_lanes and the tiles'
+// row and column passes.
+__device__ __forceinline__ void lift9(const float (&v)[9], float &lo, float &hi) {
+    float o1[4];   // O after step 1, m = -2..1
+#pragma unroll
+    for (int m = -2; m <= 1; ++m) o1[m + 2] = fmaf(WEMA_ALPHA, v[2 * m + 4] + v[2 * m + 6], v[2 * m + 5]);
+    float e2[3];   // E after step 2, m = -1..1
+#pragma unroll
+    for (int m = -1; m <= 1; ++m) e2[m + 1] = fmaf(WEMA_BETA, o1[m + 1] + o1[m + 2], v[2 * m + 4]);
+    float o3[2];   // O after step 3, m = -1..0
+#pragma unroll
+    for (int m = -1; m <= 0; ++m) o3[m + 1] = fmaf(WEMA_GAMMA, e2[m + 1] + e2[m + 2], o1[m + 2]);
+    lo = fmaf(WEMA_DELTA, o3[0] + o3[1], e2[1]) * WEMA_KINV;
+    hi = o3[1] * WEMA_K;
+}
+
+__global__ void __launch_bounds__(256) k_dwt_fwd_point(const DwtfJob *__restrict__ jobs, int njobs) {
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const DwtfJob J = jobs[find_job(jobs, njobs, bid)];
+    const int t = bid - J.tile_begin, tid = (int)threadIdx.x;
+    const int w = J.w, h = J.h, sw = (w + 1) >> 1, sh = (h + 1) >> 1;
+    const int q = t * 256 + tid;
+    const bool valid = q < sw * sh;
+    const int qv = valid ? q : 0;   // (lanes past the end compute pair 0 and store nothing)
+    const int py = qv / sw, px = qv - py * sw;
+    const GLOBAL_AS float *src = gp((const float *)J.src);
+    int xi[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) xi[k] = reflect_idx(2 * px - 4 + k, w);
+    // rows 2 py - 4 .. 2 py + 4, lifted along the row: the L and H columns of pair px
+    float L[9], H[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const GLOBAL_AS float *row = src + (size_t)reflect_idx(2 * py - 4 + r, h) * w;
+        float v[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] = row[xi[k]];
+        lift9(v, L[r], H[r]);
+    }
+    // the columns, lifted: (LL, LH) from the L column, (HL, HH) from the H column
+    float ll, lh, hl, hh;
+    lift9(L, ll, lh);
+    lift9(H, hl, hh);
+    unsigned long long part = 0ull;
+    if (valid) {
+        // stale entries of the reference's planes outside the written extents (as the tiles)
+        const int hw_valid = w >> 1, hh_valid = h >> 1;
+        const size_t n = (size_t)sw * sh, pn = (size_t)w * h, o = (size_t)py * sw + px;
+        const GLOBAL_AS float *prev = gp(J.prev);
+        if (prev) {
+            if (py >= hh_valid) {
+                lh = prev[o];
+                hh = prev[pn + o];
+            }
+            if (px >= hw_valid) {
+                hl = prev[2 * pn + o];
+                hh = prev[pn + o];
+            }
+        }
+        gp(J.ll)[o] = ll;
+        gp(J.planes)[o] = lh;
+        gp(J.planes)[n + o] = hh;
+        gp(J.planes)[2 * n + o] = hl;
+        part = abs_q32(lh) + abs_q32(hl) + abs_q32(hh);
+    }
+    if (J.acc) {
+        const unsigned long long s = wave_sum_u64(part);
+        const int wave = tid >> 6;
+        if ((tid & 63) == 0) atomicAdd(J.acc + ((4 * t + wave) & (J.acc_slots - 1)), s);
+    }
+}
+
+// Inverse lifting of one coefficient pair row (or column) m = 0 from the even (e) and odd
+// (o) coefficients of pairs m = -2..2 (index m + 2; e[0] is not read): the even and the odd
+// output sample, as ipipe_step and the tiles' column pass (the row pass has the same steps).
+__device__ __forceinline__ void ilift5(const float (&e)[5], const float (&o)[5], float &ev, float &od) {
+    float e1[5];   // E after step 1, index 1..4
+#pragma unroll
+    for (int j = 1; j <= 4; ++j) e1[j] = fmaf(-WEMA_DELTA, o[j - 1] + o[j], e[j]);
+    float o2[4];   // O after step 2, index 1..3
+#pragma unroll
+    for (int j = 1; j <= 3; ++j) o2[j] = fmaf(-WEMA_GAMMA, e1[j] + e1[j + 1], o[j]);
+    float e3[4];   // E after step 3, index 2..3
+#pragma unroll
+    for (int j = 2; j <= 3; ++j) e3[j] = fmaf(-WEMA_BETA, o2[j - 1] + o2[j], e1[j]);
+    ev = e3[2];
+    od = fmaf(-WEMA_ALPHA, e3[2] + e3[3], o2[2]);
+}
+
+__global__ void __launch_bounds__(256) k_dwt_inv_point(const DwtiJob *__restrict__ jobs, int njobs) {
+    const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const DwtiJob J = jobs[find_job(jobs, njobs, bid)];
+    const int t = bid - J.tile_begin, tid = (int)threadIdx.x;
+    if (t == 0 && J.acc_clear)
+        for (int i = tid; i < J.acc_n; i += 256) J.acc_clear[i] = 0ull;
+    const int w = J.w, h = J.h, sw = (w + 1) >> 1, sh = (h + 1) >> 1;
+    const size_t n = (size_t)sw * sh;
+    const int q = t * 256 + tid;
+    const bool valid = q < sw * sh;
+    const int qv = valid ? q : 0;
+    const int py = qv / sw, px = qv - py * sw;
+    // coefficient pair (py - 2 + j, px - 2 + c) of the symmetric extension of the
+    // interleaved signal, as the strips and tiles load it
+    int colL[5], colH[5];
+    size_t re[5], ro[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const int I = px - 2 + c, Q = py - 2 + c;
+        colL[c] = reflect_idx(2 * I, w) >> 1;
+        colH[c] = reflect_idx(2 * I + 1, w) >> 1;
+        re[c] = (size_t)(reflect_idx(2 * Q, h) >> 1) * sw;
+        ro[c] = (size_t)(reflect_idx(2 * Q + 1, h) >> 1) * sw;
+    }
+    const GLOBAL_AS float *LL = gp(J.ll), *LH = gp(J.planes), *HH = LH + n, *HL = LH + 2 * n;
+    // column synthesis: image rows 2 py (index 0) and 2 py + 1 (index 1) of the low (A, pair
+    // columns c = 1..4) and high (B, c = 0..4) columns, scaled as the row pass takes them
+    float A[2][5], B[2][5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        float e[5], o[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            e[j] = j ? HL[re[j] + colH[c]] * WEMA_K : 0.0f;
+            o[j] = HH[ro[j] + colH[c]] * WEMA_KINV;
+        }
+        float ev, od;
+        ilift5(e, o, ev, od);
+        B[0][c] = ev * WEMA_KINV;
+        B[1][c] = od * WEMA_KINV;
+        if (c == 0) {
+            A[0][0] = A[1][0] = 0.0f;   // (not read)
+            continue;
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            e[j] = j ? LL[re[j] + colL[c]] * WEMA_K : 0.0f;
+            o[j] = LH[ro[j] + colL[c]] * WEMA_KINV;
+        }
+        ilift5(e, o, ev, od);
+        A[0][c] = ev * WEMA_K;
+        A[1][c] = od * WEMA_K;
+    }
+    if (!valid) return;
+    // row synthesis of pair px of each of the two rows: even samples A, odd samples B
+    GLOBAL_AS float *out = gp(J.out);
+    const int x = 2 * px;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int y = 2 * py + r;
+        float ev, od;
+        ilift5(A[r], B[r], ev, od);
+        if (y < h) {
+            out[(size_t)y * w + x] = ev;
+            if (x + 1 < w) out[(size_t)y * w + x + 1] = od;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // IIR band-pass + amplification (src/iir_filter.c:181-264, 266-370, 376-471;
 // src/wema.c:714-744, 783-800, 830-848; src/wema_batch.c:268-297, 420-440).
 __device__ __forceinline__ float biquad2(const float *c, float4 &st, float x) {
@@ -1162,24 +1333,38 @@ __device__ __forceinline__ void iir_frames(const TemporalJob &J, const float *c,
 // One frame, PPT positions per thread (p0 + 256 k), the path of one frame per call and of
 // the serving loop's ticks: 40 bytes move per position, so a block of one position per
 // thread would spend most of its life on its set-up (finding its job, the frame's threshold)
-// and on one memory round trip; here the loads of all PPT positions are in flight together
-// and the set-up serves PPT times the positions.
+// and on one memory round trip; here the loads of all PPT positions are issued first, and
+// the frame's threshold is summed within each wave (lane l takes slot l of the frame's at
+// most 64 accumulator slots) while they are in flight: no barrier, one round trip.
 template <int PPT, bool SEP, bool FILT>
-__device__ __forceinline__ void iir_one_frame(const TemporalJob &J, const float *c, const float *sT,
-                                              const float *sA, long long p0) {
+__device__ __forceinline__ void iir_one_frame(const TemporalJob &J, const float *c, long long p0) {
     GLOBAL_AS float *in = gp(J.coef);
     GLOBAL_AS float *out = SEP ? gp(J.amp_out) : gp(J.coef);
     GLOBAL_AS float *sp = gp((float *)J.state);
+    // slot lane mod acc_slots of the frame, loaded first: the wait for it then leaves the
+    // loads below in flight (loads complete in issue order as the wait counter sees them)
+    const int lane = (int)threadIdx.x & 63;
+    unsigned long long v = gp(J.acc)[lane & (J.acc_slots - 1)];
     float x[PPT];
     float4 st[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
-        const long long p = p0 + 256 * k < J.n ? p0 + 256 * k : p0;   // (a valid address)
+        const long long p = min(p0 + 256 * k, J.n - 1);   // (a valid address)
         x[k] = __builtin_nontemporal_load(&in[p]);
-        if (!J.preseed)
-            st[k] = make_float4(__builtin_nontemporal_load(&sp[4 * p]), __builtin_nontemporal_load(&sp[4 * p + 1]),
-                                __builtin_nontemporal_load(&sp[4 * p + 2]), __builtin_nontemporal_load(&sp[4 * p + 3]));
+        // (loaded on the preseed frame too, and replaced: a load under a condition would
+        // make the wait below wait for every load)
+        st[k] = make_float4(__builtin_nontemporal_load(&sp[4 * p]), __builtin_nontemporal_load(&sp[4 * p + 1]),
+                            __builtin_nontemporal_load(&sp[4 * p + 2]), __builtin_nontemporal_load(&sp[4 * p + 3]));
     }
+    // (each group of acc_slots lanes adds the frame's slots; uniform trip count)
+    for (int off = 1; off < J.acc_slots; off <<= 1) v += __shfl_xor(v, off);
+    float T = threshold_of(v, J.n6f);
+    if (J.thr_override) {
+        const float o = gp(J.thr_override)[0];
+        if (o == o) T = o;
+    }
+    const float A = gp(J.amp)[0];
+    if (J.thr_out && blockIdx.x == (unsigned)J.tile_begin && threadIdx.x == 0) gp(J.thr_out)[0] = T;
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const long long p = p0 + 256 * k;
@@ -1189,7 +1374,7 @@ __device__ __forceinline__ void iir_one_frame(const TemporalJob &J, const float 
             for (int it = 0; it < J.warmup; ++it) (void)biquad2(c, st[k], x[k]);
         }
         float y;
-        const float o = iir_amp(c, st[k], x[k], sT[0], sA[0] * (p < J.level0_end ? 0.5f : 1.0f), y);
+        const float o = iir_amp(c, st[k], x[k], T, A * (p < J.level0_end ? 0.5f : 1.0f), y);
         if (p < J.n) {
             __builtin_nontemporal_store(o, &out[p]);
             if (FILT) gp(J.filt_out)[p] = y;
@@ -1201,26 +1386,41 @@ __device__ __forceinline__ void iir_one_frame(const TemporalJob &J, const float 
     }
 }
 
+__device__ __forceinline__ void iir_one_frame_job(const TemporalJob &J, const float *c) {
+    // (every lane of a wave takes part in the threshold sum; positions past the end load a
+    // valid address and store nothing)
+    const long long p0 = (long long)((int)blockIdx.x - J.tile_begin) * (256 * WEMA_IIR_PPT) + threadIdx.x;
+    if (J.amp_out) {
+        if (J.filt_out) iir_one_frame<WEMA_IIR_PPT, true, true>(J, c, p0);
+        else iir_one_frame<WEMA_IIR_PPT, true, false>(J, c, p0);
+    } else {
+        if (J.filt_out) iir_one_frame<WEMA_IIR_PPT, false, true>(J, c, p0);
+        else iir_one_frame<WEMA_IIR_PPT, false, false>(J, c, p0);
+    }
+}
+
+// Launches whose jobs are all of one frame: the one-frame path alone, so the waits the
+// compiler places in it are not merged with those of the many-frame path.
+__global__ void __launch_bounds__(256) k_iir1(const TemporalJob *__restrict__ jobs, int njobs) {
+    const TemporalJob J = jobs[find_job(jobs, njobs, (int)blockIdx.x)];
+    float c[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) c[i] = J.c[i];
+    iir_one_frame_job(J, c);
+}
+
 __global__ void __launch_bounds__(256) k_iir(const TemporalJob *__restrict__ jobs, int njobs) {
     __shared__ float sT[WEMA_MAX_BATCH], sA[WEMA_MAX_BATCH];
     const int jid = find_job(jobs, njobs, (int)blockIdx.x);
     const TemporalJob J = jobs[jid];
-    frame_params(J, sT, sA);
     float c[10];
 #pragma unroll
     for (int i = 0; i < 10; ++i) c[i] = J.c[i];
     if (J.ppt == WEMA_IIR_PPT && J.nframes == 1) {
-        const long long p0 = (long long)((int)blockIdx.x - J.tile_begin) * (256 * WEMA_IIR_PPT) + threadIdx.x;
-        if (p0 >= J.n) return;
-        if (J.amp_out) {
-            if (J.filt_out) iir_one_frame<WEMA_IIR_PPT, true, true>(J, c, sT, sA, p0);
-            else iir_one_frame<WEMA_IIR_PPT, true, false>(J, c, sT, sA, p0);
-        } else {
-            if (J.filt_out) iir_one_frame<WEMA_IIR_PPT, false, true>(J, c, sT, sA, p0);
-            else iir_one_frame<WEMA_IIR_PPT, false, false>(J, c, sT, sA, p0);
-        }
+        iir_one_frame_job(J, c);
         return;
     }
+    frame_params(J, sT, sA);
     const long long p = (long long)((int)blockIdx.x - J.tile_begin) * 256 + threadIdx.x;
     if (p >= J.n) return;
     GLOBAL_AS float *sp = gp((float *)J.state) + 4 * p;
@@ -2041,14 +2241,26 @@ hipError_t wema_launch_dwt_inv_tile(const DwtiJob *jobs, int njobs, int nblocks,
     hipLaunchKernelGGL(k_dwt_inv_tile, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError();
 }
+hipError_t wema_launch_dwt_fwd_point(const DwtfJob *jobs, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dwt_fwd_point, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+hipError_t wema_launch_dwt_inv_point(const DwtiJob *jobs, int njobs, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dwt_inv_point, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
 hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *jobs, int njobs, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dwt_inv0_ycc, dim3(nblocks), dim3(64 * WEMA_STRIPS_PER_BLOCK), 0, s, jobs, njobs);
     return hipGetLastError();
 }
-hipError_t wema_launch_iir(const TemporalJob *jobs, int njobs, int nblocks, hipStream_t s) {
+// one_frame: every job of the launch is of one frame with WEMA_IIR_PPT positions per thread
+hipError_t wema_launch_iir(const TemporalJob *jobs, int njobs, int nblocks, int one_frame, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_iir, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    if (one_frame) hipLaunchKernelGGL(k_iir1, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
+    else hipLaunchKernelGGL(k_iir, dim3(nblocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 // variant: 0 = windows up to 32 samples (LDS), 1 = up to 64 (LDS), 2 = any (ring reads)

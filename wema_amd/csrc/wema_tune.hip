@@ -12,10 +12,11 @@
 // DWT strips: every forward and inverse level runs as strips 60 pair columns wide and 8, 16
 // or 32 pair rows tall; taller strips re-read fewer halo rows, shorter ones give more waves
 // in flight, and which wins depends on the level's size, its channels and the frames per
-// launch. The first session of a shape (level size, channels, frames per call) on a device
-// times the three heights on launches of that shape, using the session's own per-call
-// buffers before its first frame, and keeps the fastest. The strip height never changes a
-// result: each output is computed from the same operands whatever strip it is in.
+// launch. The levels below full resolution can also run as LDS tiles or with one output
+// pair per thread. The first session of a shape (level size, channels, frames per call) on
+// a device times the candidates on launches of that shape, using the session's own per-call
+// buffers before its first frame, and keeps the fastest. The shape never changes a result:
+// each output is computed from the same operands whatever strip, tile or thread it is in.
 #include <hip/hip_runtime.h>
 
 #include <string.h>
@@ -35,6 +36,8 @@ extern "C" hipError_t wema_launch_dwt_inv(const DwtiJob *, int, int, int, hipStr
 extern "C" hipError_t wema_launch_dwt_inv_ycc(const DwtiJob *, int, int, hipStream_t);
 extern "C" hipError_t wema_launch_dwt_fwd_tile(const DwtfJob *, int, int, hipStream_t);
 extern "C" hipError_t wema_launch_dwt_inv_tile(const DwtiJob *, int, int, hipStream_t);
+extern "C" hipError_t wema_launch_dwt_fwd_point(const DwtfJob *, int, int, hipStream_t);
+extern "C" hipError_t wema_launch_dwt_inv_point(const DwtiJob *, int, int, hipStream_t);
 
 namespace {
 
@@ -69,40 +72,63 @@ extern "C" void wema_dev_autotune(int mask) {
 namespace {
 
 // Candidate shapes: strips 8, 16 and 32 pair rows tall, and (levels below full resolution)
-// LDS tiles.
-const int kStrips[4] = {8, 16, 32, WEMA_SP_TILE};
-// Mean launch time of one job list at each candidate shape (the jobs' buffers are a
-// session's own per-call buffers; their contents do not matter to the timing).
-template <typename Job, typename Launch, typename LaunchTile>
+// LDS tiles and one output pair per thread.
+const int kStrips[5] = {8, 16, 32, WEMA_SP_TILE, WEMA_SP_POINT};
+// Launch time of one job list at each candidate shape (the jobs' buffers are a session's own
+// per-call buffers; their contents do not matter to the timing). The candidates' tables are
+// uploaded together and timed in alternation over several rounds, keeping each one's best
+// round: launches of a few microseconds differ by less than one round's noise.
+template <typename Job, typename Launch, typename LaunchTile, typename LaunchPoint>
 int time_strips(std::vector<Job> v, void *tab_h, void *tab_d, size_t cap, Scratch &sc, int ncand, double *ms,
-                Launch launch, LaunchTile launch_tile) {
+                Launch launch, LaunchTile launch_tile, LaunchPoint launch_point) {
+    JobList jl[5];
+    size_t off = 0;
     for (int c = 0; c < ncand; c++) {
-        const bool tile = kStrips[c] == WEMA_SP_TILE;
+        const bool tile = kStrips[c] == WEMA_SP_TILE, point = kStrips[c] == WEMA_SP_POINT;
         for (auto &j : v) j.sp = kStrips[c];
-        JobList jl;
-        if (tile) {
+        if (tile || point) {
             for (auto &j : v) {
-                j.tile_begin = jl.blocks;
-                jl.blocks += wema_dt_tiles(j.w, j.h);
+                j.tile_begin = jl[c].blocks;
+                jl[c].blocks += tile ? wema_dt_tiles(j.w, j.h) : wema_pt_blocks(j.w, j.h);
             }
         } else {
-            JobBuilder::strips_of(v, jl, 0);
+            JobBuilder::strips_of(v, jl[c], 0);
         }
-        size_t off = 0;
-        if (!JobBuilder::put((char *)tab_h, cap, off, v, jl)) return -1;
-        if (hipMemcpyAsync(tab_d, tab_h, off, hipMemcpyHostToDevice, sc.st) != hipSuccess) return -1;
-        const Job *jd = (const Job *)((char *)tab_d + jl.offset);
-        auto run = [&]() { return tile ? launch_tile(jd, jl.count, jl.blocks) : launch(jd, jl.count, jl.blocks); };
-        if (run() != hipSuccess) return -1;   // warm-up
-        const int reps = 3;
+        if (!JobBuilder::put((char *)tab_h, cap, off, v, jl[c])) return -1;
+    }
+    if (hipMemcpyAsync(tab_d, tab_h, off, hipMemcpyHostToDevice, sc.st) != hipSuccess) return -1;
+    auto run = [&](int c) {
+        const Job *jd = (const Job *)((char *)tab_d + jl[c].offset);
+        if (kStrips[c] == WEMA_SP_TILE) return launch_tile(jd, jl[c].count, jl[c].blocks);
+        if (kStrips[c] == WEMA_SP_POINT) return launch_point(jd, jl[c].count, jl[c].blocks);
+        return launch(jd, jl[c].count, jl[c].blocks);
+    };
+    auto timed = [&](int c, int reps, double &t_ms) {
         if (hipEventRecord(sc.e0, sc.st) != hipSuccess) return -1;
         for (int r = 0; r < reps; r++)
-            if (run() != hipSuccess) return -1;
+            if (run(c) != hipSuccess) return -1;
         if (hipEventRecord(sc.e1, sc.st) != hipSuccess || hipEventSynchronize(sc.e1) != hipSuccess) return -1;
         float t = 0.0f;
         if (hipEventElapsedTime(&t, sc.e0, sc.e1) != hipSuccess) return -1;
-        ms[c] = t / reps;
+        t_ms = t / reps;
+        return 0;
+    };
+    // warm-up round (one launch each), which also sizes the rounds: launches under 0.2 ms
+    // get 4 rounds of 8, longer ones 2 rounds of 2
+    double longest = 0.0;
+    for (int c = 0; c < ncand; c++) {
+        if (timed(c, 1, ms[c]) < 0) return -1;
+        longest = std::max(longest, ms[c]);
     }
+    const bool small = longest < 0.2;
+    const int rounds = small ? 4 : 2, reps = small ? 8 : 2;
+    for (int c = 0; c < ncand; c++) ms[c] = 1e30;
+    for (int r = 0; r < rounds; r++)
+        for (int c = 0; c < ncand; c++) {
+            double t = 0.0;
+            if (timed(c, reps, t) < 0) return -1;
+            ms[c] = std::min(ms[c], t);
+        }
     return 0;
 }
 }  // namespace
@@ -140,8 +166,9 @@ int wema_dwt_tune(wema_dev *s) {
             if (ms[c] < ms[b]) b = c;
         return kStrips[b];
     };
-    // LDS tiles are a candidate for the levels below full resolution of 2 x 2 samples or more
-    auto cands = [](int l, int w, int h) { return l > 0 && w >= 2 && h >= 2 ? 4 : 3; };
+    // LDS tiles and pairs are candidates for the levels below full resolution of 2 x 2
+    // samples or more
+    auto cands = [](int l, int w, int h) { return l > 0 && w >= 2 && h >= 2 ? 5 : 3; };
     int rc = 0;
     for (int l = 0; l < L && rc == 0; l++) {
         // forward level l
@@ -150,13 +177,14 @@ int wema_dwt_tune(wema_dev *s) {
             const auto key = std::make_tuple(dev, l == 0 ? 0 : 1, fv[0].w, fv[0].h, fv[0].nch, n);
             auto it = g_strips.find(key);
             if (it == g_strips.end()) {
-                double ms[4];
+                double ms[5];
                 const bool lev0 = l == 0;
                 const int nc = cands(l, fv[0].w, fv[0].h);
                 if (ensure() < 0 ||
                     time_strips(fv, tab_h, tab_d, cap, sc, nc, ms,
                                 [&](const DwtfJob *j, int cnt, int nb) { return wema_launch_dwt_fwd(j, cnt, nb, lev0, sc.st); },
-                                [&](const DwtfJob *j, int cnt, int nb) { return wema_launch_dwt_fwd_tile(j, cnt, nb, sc.st); }) < 0) {
+                                [&](const DwtfJob *j, int cnt, int nb) { return wema_launch_dwt_fwd_tile(j, cnt, nb, sc.st); },
+                                [&](const DwtfJob *j, int cnt, int nb) { return wema_launch_dwt_fwd_point(j, cnt, nb, sc.st); }) < 0) {
                     rc = -1;
                     break;
                 }
@@ -173,17 +201,18 @@ int wema_dwt_tune(wema_dev *s) {
             const auto key = std::make_tuple(dev, kind, iv[0].w, iv[0].h, iv[0].nch, n);
             auto it = g_strips.find(key);
             if (it == g_strips.end()) {
-                double ms[4];
+                double ms[5];
                 const bool lev0 = l == 0;
                 const int nc = ycc ? 3 : cands(l, iv[0].w, iv[0].h);
                 if (ensure() < 0) { rc = -1; break; }
                 auto tile = [&](const DwtiJob *j, int cnt, int nb) { return wema_launch_dwt_inv_tile(j, cnt, nb, sc.st); };
+                auto point = [&](const DwtiJob *j, int cnt, int nb) { return wema_launch_dwt_inv_point(j, cnt, nb, sc.st); };
                 const int r = ycc ? time_strips(iv, tab_h, tab_d, cap, sc, nc, ms,
                                                 [&](const DwtiJob *j, int cnt, int nb) { return wema_launch_dwt_inv_ycc(j, cnt, nb, sc.st); },
-                                                tile)
+                                                tile, point)
                                   : time_strips(iv, tab_h, tab_d, cap, sc, nc, ms,
                                                 [&](const DwtiJob *j, int cnt, int nb) { return wema_launch_dwt_inv(j, cnt, nb, lev0, sc.st); },
-                                                tile);
+                                                tile, point);
                 if (r < 0) { rc = -1; break; }
                 timed = true;
                 it = g_strips.emplace(key, pick(ms, nc)).first;
@@ -210,11 +239,13 @@ extern "C" int wema_dev_strip_rows(const wema_dev_t *s, int level, int inverse) 
 
 extern "C" int wema_dev_set_strip_rows(wema_dev_t *s, int rows) {
     if (!s) { wema_set_error("null session"); return -1; }
-    if (rows != 0 && rows != 8 && rows != 16 && rows != 32 && rows != WEMA_SP_TILE) {
-        wema_set_error("strip rows must be 0 (the rule), 8, 16, 32 or 1 (LDS tiles below full resolution)");
+    if (rows != 0 && rows != 8 && rows != 16 && rows != 32 && rows != WEMA_SP_TILE && rows != WEMA_SP_POINT) {
+        wema_set_error("strip rows must be 0 (the rule), 8, 16, 32, 1 (LDS tiles below full resolution) "
+                       "or 2 (one output pair per thread below full resolution)");
         return -1;
     }
-    for (int l = 0; l < WEMA_MAX_LEVELS; l++) s->sp_fwd[l] = s->sp_inv[l] = (rows == WEMA_SP_TILE && l == 0) ? 0 : rows;
+    for (int l = 0; l < WEMA_MAX_LEVELS; l++)
+        s->sp_fwd[l] = s->sp_inv[l] = ((rows == WEMA_SP_TILE || rows == WEMA_SP_POINT) && l == 0) ? 0 : rows;
     return 0;
 }
 
