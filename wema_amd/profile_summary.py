@@ -49,7 +49,7 @@ def pipeline_dispatches(rows):
     (the counter calibration) are always kept."""
     order = sorted({int(r["Dispatch_Id"]): _short(r["Kernel_Name"]) for r in rows}.items())
     names = [n for _, n in order]
-    first_t = next((i for i, n in enumerate(names) if n == "k_iir" or n.startswith("k_win")), None)
+    first_t = next((i for i, n in enumerate(names) if n in ("k_iir", "k_iir1") or n.startswith("k_win")), None)
     cut = 0
     if first_t is not None:
         cut = first_t
