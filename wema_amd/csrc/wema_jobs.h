@@ -24,7 +24,9 @@
 #define WEMA_TILE_PAIRS_X 60
 #define WEMA_SP_MAX 32
 #define WEMA_SP_MIN 8
+#ifndef WEMA_STRIPS_PER_BLOCK
 #define WEMA_STRIPS_PER_BLOCK 4
+#endif
 // The levels below full resolution can also run as LDS tiles of WEMA_DT_TP x WEMA_DT_TP
 // output pairs, one per 256-thread workgroup (a job's strip height WEMA_SP_TILE selects
 // them); levels of fewer than 2 samples in either direction always run as strips.
