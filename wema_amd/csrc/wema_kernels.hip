@@ -1826,24 +1826,27 @@ struct Src4 {
 // a multiple of 4, aligned buffers): the 4 columns lie wholly inside or outside the frame
 // and load as 3 dwords (or 3 x 16 bytes) plus 16 bytes; coordinates are clamped into the
 // frame so every load is valid (samples outside are zeroed by the caller).
+// VEC: y is the same in every lane of the wave, and xo is the lane's clamped first column
+// (loop-invariant): the row base is a scalar, the lane's part a 32-bit offset.
 template <bool U8, bool VEC>
-__device__ __forceinline__ void gf_load(const EgressJob &J, int x, int y, Src4<U8> &s, v4f &g) {
+__device__ __forceinline__ void gf_load(const EgressJob &J, int x, int y, unsigned xo, Src4<U8> &s, v4f &g) {
     const int w = J.w, h = J.h;
     const int yc = min(max(y, 0), h - 1);
-    if (VEC) {
+    if constexpr (VEC && U8) {
+        const size_t row = (size_t)yc * (unsigned)w;
+        const GLOBAL_AS unsigned char *rgb = gp(J.src_u8) + row * 3;
+        const GLOBAL_AS unsigned *q = &at_b((const GLOBAL_AS unsigned *)rgb, 3u * xo);
+        s.d[0] = q[0]; s.d[1] = q[1]; s.d[2] = q[2];
+        g = at_b((const GLOBAL_AS v4f *)(gp(J.gray) + row), 4u * xo);
+    } else if constexpr (VEC) {
         const int xc = min(max(x, 0), w - 4);
         const unsigned idx = (unsigned)yc * (unsigned)w + (unsigned)xc;
-        if (U8) {
-            const GLOBAL_AS unsigned *q = (const GLOBAL_AS unsigned *)(gp(J.src_u8) + (size_t)idx * 3);
-            s.d[0] = q[0]; s.d[1] = q[1]; s.d[2] = q[2];
-        } else {
-            const GLOBAL_AS v4f *q = (const GLOBAL_AS v4f *)(gp(J.src_f32) + (size_t)idx * 3);
+        const GLOBAL_AS v4f *q = (const GLOBAL_AS v4f *)(gp(J.src_f32) + (size_t)idx * 3);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const v4f f = q[k];
-                s.d[4 * k] = __float_as_uint(f.x); s.d[4 * k + 1] = __float_as_uint(f.y);
-                s.d[4 * k + 2] = __float_as_uint(f.z); s.d[4 * k + 3] = __float_as_uint(f.w);
-            }
+        for (int k = 0; k < 3; ++k) {
+            const v4f f = q[k];
+            s.d[4 * k] = __float_as_uint(f.x); s.d[4 * k + 1] = __float_as_uint(f.y);
+            s.d[4 * k + 2] = __float_as_uint(f.z); s.d[4 * k + 3] = __float_as_uint(f.w);
         }
         g = *(const GLOBAL_AS v4f *)(gp(J.gray) + idx);
     } else {
@@ -1890,8 +1893,9 @@ template <bool U8, bool VEC, bool OUTF>
 __device__ __forceinline__ void gf_strip(const EgressJob &J, int x0, int ybeg, int yend, float (*hs)[4][GF_W],
                                          float (*ab)[2][GF_W], const float *sRcp) {
     const int w = J.w, h = J.h;
-    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
     const int xr = x0 - 8 + 4 * ln;            // row layout: first of this lane's 4 columns
+    const unsigned xo = (unsigned)min(max(xr, 0), w - 4);   // (VEC loads: clamped into the frame)
     const int xcol = x0 - 8 + (int)threadIdx.x;   // column layout: this thread's column
     const bool col_in = xcol >= 0 && xcol < w;
     const int cx = clip_count(xcol, w);
@@ -1901,11 +1905,11 @@ __device__ __forceinline__ void gf_strip(const EgressJob &J, int x0, int ybeg, i
     const int m0 = (ybeg >> 2) - 2, m1 = ((yend - 1) >> 2) + 2;
     Src4<U8> nxt, cur, old1, old2;
     v4f gn;
-    gf_load<U8, VEC>(J, xr, 4 * m0 + wv, nxt, gn);
+    gf_load<U8, VEC>(J, xr, 4 * m0 + wv, xo, nxt, gn);
     for (int m = m0; m <= m1; ++m) {
         cur = nxt;
         const v4f g = gn;
-        if (m < m1) gf_load<U8, VEC>(J, xr, 4 * (m + 1) + wv, nxt, gn);
+        if (m < m1) gf_load<U8, VEC>(J, xr, 4 * (m + 1) + wv, xo, nxt, gn);
         // A: the four quantities of input row 4m + wv and their horizontal sums
         {
             const int y = 4 * m + wv;
@@ -2014,7 +2018,9 @@ __device__ __forceinline__ void gf_strip(const EgressJob &J, int x0, int ybeg, i
                         for (int i = 0; i < 12; ++i)
                             d[i / 4] |= (unsigned)(unsigned char)fmaf(__builtin_amdgcn_fmed3f(o[i], 0.0f, 1.0f), 255.0f, 0.5f)
                                         << (8 * (i % 4));
-                        GLOBAL_AS unsigned *os = (GLOBAL_AS unsigned *)(gp(J.out_u8) + base);
+                        // (the row is the wave's; out_lane columns are inside the frame)
+                        GLOBAL_AS unsigned *os = &at_b((GLOBAL_AS unsigned *)(gp(J.out_u8) + (size_t)y * (unsigned)w * 3),
+                                                       3u * (unsigned)xr);
                         os[0] = d[0];
                         os[1] = d[1];
                         os[2] = d[2];
