@@ -167,7 +167,7 @@ int wema_dev_import_state(wema_dev_t *s, const void *host, size_t bytes);
  * its own buffers when it is opened and keeps the fastest for the process. Pooled sessions,
  * and sessions opened with that tuner off, leave the split to each launch (one segment
  * height for all its frames, from a cost model of its rows and workgroups). The DWT levels
- * run as strips 8, 16 or 32 pair rows tall, and the levels below full resolution also as
+ * run as strips 4, 8, 16 or 32 pair rows tall, and the levels below full resolution also as
  * LDS tiles of 16 x 16 output pairs or with one output pair per thread: the first session
  * of a shape (level size, channels, frames per call) on a device times the candidates on
  * its own buffers when it is opened and keeps the fastest for each level

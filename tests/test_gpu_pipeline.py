@@ -633,7 +633,7 @@ def test_covariance_division_equals_ieee_on_the_gpu():
                                                    (70, 45, 4, False, 2), (130, 17, 3, True, 1),
                                                    (260, 390, 3, False, 1)])
 def test_strip_rows_do_not_change_results(w, h, levels, color, batch):
-    """The DWT strip height, forced to 8, 16 or 32 pair rows for every level, LDS tiles (1)
+    """The DWT strip height, forced to 4, 8, 16 or 32 pair rows for every level, LDS tiles (1)
     or one output pair per thread (2) for the levels below full resolution, picked per level
     by the autotuner, or the fixed rule, gives the same bytes, equal to the oracle's; the
     small and odd sizes reach the tiles' reflections across several tile widths and past
@@ -644,11 +644,11 @@ def test_strip_rows_do_not_change_results(w, h, levels, color, batch):
     frames = rng.integers(0, 256, size=(n, h, w, 3), dtype=np.uint8)
     ref = oracle_clip(frames, **kw)
     d_in = torch.from_numpy(frames).cuda()
-    for rows in (None, 0, 8, 16, 32, 1, 2):
+    for rows in (None, 0, 4, 8, 16, 32, 1, 2):
         sess = wema_amd.Session(w, h, max_batch=batch, **kw)
         fwd, inv = sess.strip_rows(), sess.strip_rows(inverse=True)
-        assert fwd[0] in (8, 16, 32) and inv[0] in (8, 16, 32), (fwd, inv)
-        assert all(r in (1, 2, 8, 16, 32) for r in fwd + inv), (fwd, inv)
+        assert fwd[0] in (4, 8, 16, 32) and inv[0] in (4, 8, 16, 32), (fwd, inv)
+        assert all(r in (1, 2, 4, 8, 16, 32) for r in fwd + inv), (fwd, inv)
         if rows is not None:
             sess.set_strip_rows(rows)
         d_out = torch.empty_like(d_in)

@@ -302,7 +302,7 @@ class Session:
         return [lib.wema_dev_strip_rows(self._h, l, int(inverse)) for l in range(self.levels)]
 
     def set_strip_rows(self, rows):
-        """Force one DWT strip height (8, 16 or 32 pair rows; 0: the fixed rule) for every level,
+        """Force one DWT strip height (4, 8, 16 or 32 pair rows; 0: the fixed rule) for every level,
         or for the levels below full resolution 1: LDS tiles, 2: one output pair per thread
         (level 0 keeps the rule)."""
         _check(lib.wema_dev_set_strip_rows(self._h, int(rows)), "wema_dev_set_strip_rows")

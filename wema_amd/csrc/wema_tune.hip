@@ -9,8 +9,8 @@
 // depends on the other sessions of the tick. The split never changes a result: the window
 // sums are anchored to the image.
 //
-// DWT strips: every forward and inverse level runs as strips 60 pair columns wide and 8, 16
-// or 32 pair rows tall; taller strips re-read fewer halo rows, shorter ones give more waves
+// DWT strips: every forward and inverse level runs as strips 60 pair columns wide and 4, 8,
+// 16 or 32 pair rows tall; taller strips re-read fewer halo rows, shorter ones give more waves
 // in flight, and which wins depends on the level's size, its channels and the frames per
 // launch. The levels below full resolution can also run as LDS tiles or with one output
 // pair per thread. The first session of a shape (level size, channels, frames per call) on
@@ -71,9 +71,9 @@ extern "C" void wema_dev_autotune(int mask) {
 
 namespace {
 
-// Candidate shapes: strips 8, 16 and 32 pair rows tall, and (levels below full resolution)
-// LDS tiles and one output pair per thread.
-const int kStrips[5] = {8, 16, 32, WEMA_SP_TILE, WEMA_SP_POINT};
+// Candidate shapes: strips 8, 16, 32 and 4 pair rows tall, and (levels below full
+// resolution) LDS tiles and one output pair per thread.
+const int kStrips[6] = {8, 16, 32, 4, WEMA_SP_TILE, WEMA_SP_POINT};
 // Launch time of one job list at each candidate shape (the jobs' buffers are a session's own
 // per-call buffers; their contents do not matter to the timing). The candidates' tables are
 // uploaded together and timed in alternation over several rounds, keeping each one's best
@@ -81,7 +81,7 @@ const int kStrips[5] = {8, 16, 32, WEMA_SP_TILE, WEMA_SP_POINT};
 template <typename Job, typename Launch, typename LaunchTile, typename LaunchPoint>
 int time_strips(std::vector<Job> v, void *tab_h, void *tab_d, size_t cap, Scratch &sc, int ncand, double *ms,
                 Launch launch, LaunchTile launch_tile, LaunchPoint launch_point) {
-    JobList jl[5];
+    JobList jl[6];
     size_t off = 0;
     for (int c = 0; c < ncand; c++) {
         const bool tile = kStrips[c] == WEMA_SP_TILE, point = kStrips[c] == WEMA_SP_POINT;
@@ -168,7 +168,7 @@ int wema_dwt_tune(wema_dev *s) {
     };
     // LDS tiles and pairs are candidates for the levels below full resolution of 2 x 2
     // samples or more
-    auto cands = [](int l, int w, int h) { return l > 0 && w >= 2 && h >= 2 ? 5 : 3; };
+    auto cands = [](int l, int w, int h) { return l > 0 && w >= 2 && h >= 2 ? 6 : 4; };
     int rc = 0;
     for (int l = 0; l < L && rc == 0; l++) {
         // forward level l
@@ -177,7 +177,7 @@ int wema_dwt_tune(wema_dev *s) {
             const auto key = std::make_tuple(dev, l == 0 ? 0 : 1, fv[0].w, fv[0].h, fv[0].nch, n);
             auto it = g_strips.find(key);
             if (it == g_strips.end()) {
-                double ms[5];
+                double ms[6];
                 const bool lev0 = l == 0;
                 const int nc = cands(l, fv[0].w, fv[0].h);
                 if (ensure() < 0 ||
@@ -201,9 +201,9 @@ int wema_dwt_tune(wema_dev *s) {
             const auto key = std::make_tuple(dev, kind, iv[0].w, iv[0].h, iv[0].nch, n);
             auto it = g_strips.find(key);
             if (it == g_strips.end()) {
-                double ms[5];
+                double ms[6];
                 const bool lev0 = l == 0;
-                const int nc = ycc ? 3 : cands(l, iv[0].w, iv[0].h);
+                const int nc = ycc ? 4 : cands(l, iv[0].w, iv[0].h);
                 if (ensure() < 0) { rc = -1; break; }
                 auto tile = [&](const DwtiJob *j, int cnt, int nb) { return wema_launch_dwt_inv_tile(j, cnt, nb, sc.st); };
                 auto point = [&](const DwtiJob *j, int cnt, int nb) { return wema_launch_dwt_inv_point(j, cnt, nb, sc.st); };
@@ -239,8 +239,8 @@ extern "C" int wema_dev_strip_rows(const wema_dev_t *s, int level, int inverse) 
 
 extern "C" int wema_dev_set_strip_rows(wema_dev_t *s, int rows) {
     if (!s) { wema_set_error("null session"); return -1; }
-    if (rows != 0 && rows != 8 && rows != 16 && rows != 32 && rows != WEMA_SP_TILE && rows != WEMA_SP_POINT) {
-        wema_set_error("strip rows must be 0 (the rule), 8, 16, 32, 1 (LDS tiles below full resolution) "
+    if (rows != 0 && rows != 4 && rows != 8 && rows != 16 && rows != 32 && rows != WEMA_SP_TILE && rows != WEMA_SP_POINT) {
+        wema_set_error("strip rows must be 0 (the rule), 4, 8, 16, 32, 1 (LDS tiles below full resolution) "
                        "or 2 (one output pair per thread below full resolution)");
         return -1;
     }
