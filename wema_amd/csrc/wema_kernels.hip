@@ -325,7 +325,9 @@ __device__ __forceinline__ void vpipe_step(VPipe &p, float e, float o, float &e4
     p.o3 = o3;
 }
 
-template <int FAM, int NCH, bool FAST, bool STALE>
+// SL: input pairs in flight (the slot count of NCH; 2 for luma strips of 4 pair rows, whose
+// fast path needs 2 SL rows)
+template <int FAM, int NCH, bool FAST, bool STALE, int SL = SLOTS_OF(NCH)>
 __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int lane) {
     const int SP = J.sp;
     const int w = J.w, h = J.h;
@@ -466,7 +468,7 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
         }
     } else {
         // input pairs py0 - 2 + m for m < M; output pair rows py0 .. py0 + nout - 1
-        constexpr int SLOTS = SLOTS_OF(NCH);
+        constexpr int SLOTS = SL;
         const int nout = min(SP, sh - py0);
         // FAST strips have nout >= 2 * SLOTS; shorter strips run on to 2 * SLOTS steps past
         // the warm-up, the extra steps reading reflected rows and writing nothing
@@ -542,8 +544,9 @@ __device__ __forceinline__ void dwt_fwd_strip(const DwtfJob &J, int strip, int l
         const int px0 = (strip % J.tiles_x) * WEMA_TILE_PAIRS_X;
         const bool stale = px0 + WEMA_TILE_PAIRS_X > (w >> 1) || py0 + J.sp > (h >> 1);
         const bool fast = sx0 >= 0 && sx0 + 128 <= w && h >= 8 && (long long)w * h < (1ll << 29) &&
-                          min(J.sp, sh - py0) >= 2 * SLOTS_OF(NCH);
+                          min(J.sp, sh - py0) >= (NCH == 1 && J.sp == 4 ? 4 : 2 * SLOTS_OF(NCH));
         if (stale) dwt_fwd_strip_t<FAM, NCH, false, true>(J, strip, lane);
+        else if (fast && NCH == 1 && J.sp == 4) dwt_fwd_strip_t<FAM, NCH, true, false, 2>(J, strip, lane);
         else if (fast) dwt_fwd_strip_t<FAM, NCH, true, false>(J, strip, lane);
         else dwt_fwd_strip_t<FAM, NCH, false, false>(J, strip, lane);
     } else {
@@ -552,9 +555,10 @@ __device__ __forceinline__ void dwt_fwd_strip(const DwtfJob &J, int strip, int l
         // reach the frame's last bytes
         const int last_row = 2 * (min(J.sp, sh - py0) + py0 + 1) + 1;   // last input row (unreflected)
         const bool fast = sx0 >= 0 && sx0 + 128 <= w && h >= 8 && (long long)w * h < (1ll << 29) &&
-                          min(J.sp, sh - py0) >= 2 * SLOTS_OF(NCH) && (((size_t)J.src | (size_t)w) & 3) == 0 &&
-                          (sx0 + 132 <= w || last_row < h - 1);
-        if (fast) dwt_fwd_strip_t<FAM, NCH, true, false>(J, strip, lane);
+                          min(J.sp, sh - py0) >= (NCH == 1 && J.sp == 4 ? 4 : 2 * SLOTS_OF(NCH)) &&
+                          (((size_t)J.src | (size_t)w) & 3) == 0 && (sx0 + 132 <= w || last_row < h - 1);
+        if (fast && NCH == 1 && J.sp == 4) dwt_fwd_strip_t<FAM, NCH, true, false, 2>(J, strip, lane);
+        else if (fast) dwt_fwd_strip_t<FAM, NCH, true, false>(J, strip, lane);
         else dwt_fwd_strip_t<FAM, NCH, false, false>(J, strip, lane);
     }
 }
@@ -640,8 +644,9 @@ struct IStripRaw {
 #endif
 #define ISLOTS_OF(NCH) ((NCH) == 1 ? 4 : WEMA_ISLOTS_COLOUR)
 
-// OUT: 0 = one float plane (J.out), 1 = colour egress (RGB24 and/or float RGB).
-template <int NCH, int OUT, bool FAST>
+// OUT: 0 = one float plane (J.out), 1 = colour egress (RGB24 and/or float RGB). SL: pairs in
+// flight (the slot count of NCH; 2 for luma strips of 4 pair rows).
+template <int NCH, int OUT, bool FAST, int SL = ISLOTS_OF(NCH)>
 __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int lane) {
     const int SP = J.sp;
     const int w = J.w, h = J.h;
@@ -781,7 +786,7 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
     // FAST strips have nout >= 2 * SLOTS; shorter strips run on to 2 * SLOTS steps past the
     // warm-up (reflected rows, nothing stored), so that no load of the warm-up or of the
     // main loop is issued under a condition and the loads stay in flight across steps.
-    constexpr int SLOTS = ISLOTS_OF(NCH);
+    constexpr int SLOTS = SL;
     const int nout = min(SP, sh - py0);
     const int M = (FAST ? nout : max(nout, 2 * SLOTS)) + 4;
     auto emits = [&](int m) { return FAST || m - 4 < nout; };
@@ -830,8 +835,10 @@ __device__ __forceinline__ void dwt_inv_strips(const DwtiJob *__restrict__ jobs,
     // every lane's columns unreflected (I in [0, sw) with 2 I + 1 < w), one reflection
     // reaches every row, full-length strips, 32-bit plane offsets
     const bool fast = px0 >= 2 && 2 * (px0 + 61) + 1 < w && h >= 8 &&
-                      min(J.sp, sh - py0) >= 2 * ISLOTS_OF(NCH) && (long long)sw * sh < (1ll << 28);
-    if (fast) dwt_inv_strip_t<NCH, OUT, true>(J, strip, lane);
+                      min(J.sp, sh - py0) >= (NCH == 1 && J.sp == 4 ? 4 : 2 * ISLOTS_OF(NCH)) &&
+                      (long long)sw * sh < (1ll << 28);
+    if (fast && NCH == 1 && J.sp == 4) dwt_inv_strip_t<NCH, OUT, true, 2>(J, strip, lane);
+    else if (fast) dwt_inv_strip_t<NCH, OUT, true>(J, strip, lane);
     else dwt_inv_strip_t<NCH, OUT, false>(J, strip, lane);
 }
 
