@@ -499,9 +499,13 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
                 host_ms += hms;
             }
         } else {
-            // the plain path: each frame through the device alone, waited for
-            Pending &q = slots[0];
+            // the plain path: each frame through the device alone, one call per frame, two
+            // frames in flight as the batched ticks are (the next frame's launches are
+            // issued while the previous one runs)
             for (size_t k = 0; k < ids.size(); k++) {
+                Pending &q = slots[slot];
+                slot ^= 1;
+                if (!inflight.empty() && inflight.front() == &q && drain_one() < 0) { rc = -1; break; }
                 const auto h0 = std::chrono::steady_clock::now();
                 if (host_io && upload(q, k, k + 1) < 0) { wema_set_error("frame upload failed"); rc = -1; break; }
                 if (hipEventRecord(q.ev_begin, st) != hipSuccess ||
@@ -520,7 +524,8 @@ extern "C" int wema_serve_run(const wema_serve_config *cfg, wema_serve_stats *ou
                 q.due.assign(1, due[k]);
                 q.measured = measured;
                 record_done(q, k, k + 1);
-                if (complete(&q) < 0) { rc = -1; break; }
+                inflight.push_back(&q);
+                if (inflight.size() > 1 && drain_one() < 0) { rc = -1; break; }
                 if (measured) {
                     ticks++;
                     host_ms += hms;
