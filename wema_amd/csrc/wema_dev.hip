@@ -762,10 +762,14 @@ int wema_run_jobs(JobBuilder &jb, void *stage_h, void *stage_d, hipEvent_t stage
     const JobList &tl = jb.temporal;
     if (tl.count) {
         TraceRange r_("k_iir");
-        bool one_frame = true;
-        for (const TemporalJob &j : jb.temporal_v) one_frame = one_frame && j.nframes == 1 && j.ppt == WEMA_IIR_PPT;
-        LAUNCH_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, one_frame, st));
+        LAUNCH_OK(wema_launch_iir((const TemporalJob *)(base + tl.offset), tl.count, tl.blocks, 0, st));
         if (timer_mark(timer, TK_IIR, tl.bytes, st) < 0) return -2;
+    }
+    const JobList &t1 = jb.temporal1;
+    if (t1.count) {
+        TraceRange r_("k_iir1");
+        LAUNCH_OK(wema_launch_iir((const TemporalJob *)(base + t1.offset), t1.count, t1.blocks, 1, st));
+        if (timer_mark(timer, TK_IIR, t1.bytes, st) < 0) return -2;
     }
     for (int v = 0; v < WEMA_WIN_KINDS; v++) {
         const JobList &wl = jb.window[v];

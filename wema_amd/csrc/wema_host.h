@@ -158,7 +158,8 @@ struct TraceRange {
 
 struct JobBuilder {
     std::vector<DwtfJob> dwtf_v[WEMA_MAX_LEVELS];
-    std::vector<TemporalJob> temporal_v;      // IIR jobs
+    std::vector<TemporalJob> temporal_v;      // IIR jobs of several frames
+    std::vector<TemporalJob> temporal1_v;     // IIR jobs of one frame (k_iir1)
     std::vector<TemporalJob> window_v[WEMA_WIN_KINDS];   // sliding-window jobs by kernel (wema_win_kind)
     std::vector<DwtiJob> dwti_v[WEMA_MAX_LEVELS];
     std::vector<DwtiJob> dwti_ycc_v;          // colour output jobs (level 0 + YCbCr -> RGB)
@@ -167,7 +168,7 @@ struct JobBuilder {
     std::vector<DwtfJob> dwtf_p_v[WEMA_MAX_LEVELS];   // coarser levels, one output pair per thread
     std::vector<DwtiJob> dwti_p_v[WEMA_MAX_LEVELS];
     std::vector<EgressJob> egress_v[EG_KINDS];
-    JobList dwtf[WEMA_MAX_LEVELS], temporal, window[WEMA_WIN_KINDS], dwti[WEMA_MAX_LEVELS], dwti_ycc, egress[EG_KINDS];
+    JobList dwtf[WEMA_MAX_LEVELS], temporal, temporal1, window[WEMA_WIN_KINDS], dwti[WEMA_MAX_LEVELS], dwti_ycc, egress[EG_KINDS];
     JobList dwtf_t[WEMA_MAX_LEVELS], dwti_t[WEMA_MAX_LEVELS];
     JobList dwtf_p[WEMA_MAX_LEVELS], dwti_p[WEMA_MAX_LEVELS];
     std::vector<float> host_amp;
@@ -193,14 +194,17 @@ struct JobBuilder {
         dwtf[l].bytes += bytes;
         dwtf_v[l].push_back(j);
     }
+    // (IIR jobs of one frame go to a launch of their own: a tick of sessions that are
+    // behind mixes them with jobs of two frames)
     void add_temporal(TemporalJob j, int tiles, double bytes, bool window_mode) {
         const int wv = !window_mode ? 0 : wema_win_kind(j.window);
-        JobList &jl = window_mode ? window[wv] : temporal;
+        const bool one = !window_mode && j.nframes == 1 && j.ppt == WEMA_IIR_PPT;
+        JobList &jl = window_mode ? window[wv] : (one ? temporal1 : temporal);
         j.tile_begin = jl.blocks;
         j.tiles = tiles;
         jl.blocks += tiles;
         jl.bytes += bytes;
-        (window_mode ? window_v[wv] : temporal_v).push_back(j);
+        (window_mode ? window_v[wv] : (one ? temporal1_v : temporal_v)).push_back(j);
     }
     void add_dwti(int stage, DwtiJob j, int tiles, double bytes) {
         if (j.sp == WEMA_SP_POINT) {
@@ -239,6 +243,7 @@ struct JobBuilder {
     void fix_amp(const void *dev_base) {
         const float *amp_d = (const float *)((const char *)dev_base + amp_offset);
         for (auto &j : temporal_v) j.amp = amp_d + j.amp_index;
+        for (auto &j : temporal1_v) j.amp = amp_d + j.amp_index;
         for (auto &v : window_v)
             for (auto &j : v) j.amp = amp_d + j.amp_index;
     }
@@ -339,6 +344,7 @@ struct JobBuilder {
         fix_amp(dev_base);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwtf_v[l], dwtf[l]);
         ok = ok && put(dst, cap, off, temporal_v, temporal);
+        ok = ok && put(dst, cap, off, temporal1_v, temporal1);
         for (int k = 0; k < WEMA_WIN_KINDS; k++) ok = ok && put(dst, cap, off, window_v[k], window[k]);
         for (int l = 0; l < WEMA_MAX_LEVELS; l++) ok = ok && put(dst, cap, off, dwti_v[l], dwti[l]);
         ok = ok && put(dst, cap, off, dwti_ycc_v, dwti_ycc);
