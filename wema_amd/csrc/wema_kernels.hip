@@ -220,6 +220,16 @@ template <> struct StripRaw<FAM_F32_PLANE> { float a, b; };
 #define WEMA_FSLOTS_COLOUR 2
 #endif
 #define SLOTS_OF(NCH) ((NCH) == 1 ? 4 : WEMA_FSLOTS_COLOUR)
+// Pair columns of a workgroup's staged output rows (level-0 forward).
+#define WEMA_STAGE_W (WEMA_STRIPS_PER_BLOCK * WEMA_TILE_PAIRS_X)
+#ifndef WEMA_FWD_STAGE
+#define WEMA_FWD_STAGE 1
+#endif
+// ... in launches of at least this many workgroups (8 per CU): with fewer, the barrier per
+// row leaves the SIMDs idle (one frame per call: 1080p colour 133 -> 136 us, 4K 207 -> 210 us)
+#ifndef WEMA_STAGE_MIN_BLOCKS
+#define WEMA_STAGE_MIN_BLOCKS 2048
+#endif
 
 // Per-lane constants of a strip.
 struct StripLane {
@@ -327,8 +337,8 @@ __device__ __forceinline__ void vpipe_step(VPipe &p, float e, float o, float &e4
 
 // SL: input pairs in flight (the slot count of NCH; 2 for luma strips of 4 pair rows, whose
 // fast path needs 2 SL rows)
-template <int FAM, int NCH, bool FAST, bool STALE, int SL = SLOTS_OF(NCH)>
-__device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int lane) {
+template <int FAM, int NCH, bool FAST, bool STALE, bool STAGE, int SL = SLOTS_OF(NCH)>
+__device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int lane, float *stg) {
     const int SP = J.sp;
     const int w = J.w, h = J.h;
     const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
@@ -366,6 +376,34 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
 #pragma unroll
     for (int c = 0; c < NCH; ++c) part[c] = 0ull;
 
+    // Staged output rows (stg: the workgroup's LDS, its 4 strips side by side in one strip
+    // row): each wave puts its 60 pairs of a row into LDS, and after a barrier the workgroup
+    // stores the 240 pairs as contiguous rows, so that most 128-byte lines are written whole
+    // by one store instruction instead of in parts by two waves. Two buffers alternate, so a
+    // row is written while the one before it is still being read.
+    int sbuf = 0;
+    const int pxb = (sx - (int)(threadIdx.x >> 6)) * WEMA_TILE_PAIRS_X;   // the workgroup's first pair column
+    auto flush = [&](int py) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        const int t = (int)threadIdx.x, gx = pxb + t;
+        if (t < WEMA_STAGE_W && gx < sw) {
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const float *b = stg + (sbuf * NCH + c) * (4 * WEMA_STAGE_W) + t;
+                GLOBAL_AS float *LLc = gp(J.ll) + c * J.ll_cstride + (size_t)py * sw;
+                GLOBAL_AS float *LHc = gp(J.planes) + c * J.planes_cstride + (size_t)py * sw;
+                const unsigned xb = 4u * (unsigned)gx;
+                at_b(LLc, xb) = b[0];
+                at_b(LHc, xb) = b[WEMA_STAGE_W];
+                at_b(LHc + n, xb) = b[2 * WEMA_STAGE_W];
+                at_b(LHc + 2 * n, xb) = b[3 * WEMA_STAGE_W];
+            }
+        }
+        sbuf ^= 1;
+    };
+
     // output pair row py (uniform) of channel c from its four final values
     auto emit = [&](int c, int py, float ll, float lh, float hl, float hh) {
         // level 0 has no previous level (stale entries are 0); at the coarser levels only
@@ -381,14 +419,24 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
                 hh = FAM == FAM_F32_PLANE ? prev[pn + (size_t)py * sw + px] : 0.0f;
             }
         }
-        GLOBAL_AS float *LLc = gp(J.ll) + c * J.ll_cstride + (size_t)py * sw;
-        GLOBAL_AS float *LHc = gp(J.planes) + c * J.planes_cstride + (size_t)py * sw;
-        if (col_ok) {
-            const unsigned xb = 4u * (unsigned)px;
-            at_b(LLc, xb) = ll;
-            at_b(LHc, xb) = lh;
-            at_b(LHc + n, xb) = hh;
-            at_b(LHc + 2 * n, xb) = hl;
+        if (STAGE) {
+            if (lane >= 2 && lane < 62) {
+                float *b = stg + (sbuf * NCH + c) * (4 * WEMA_STAGE_W) + (threadIdx.x >> 6) * WEMA_TILE_PAIRS_X + lane - 2;
+                b[0] = ll;
+                b[WEMA_STAGE_W] = lh;
+                b[2 * WEMA_STAGE_W] = hh;
+                b[3 * WEMA_STAGE_W] = hl;
+            }
+        } else {
+            GLOBAL_AS float *LLc = gp(J.ll) + c * J.ll_cstride + (size_t)py * sw;
+            GLOBAL_AS float *LHc = gp(J.planes) + c * J.planes_cstride + (size_t)py * sw;
+            if (col_ok) {
+                const unsigned xb = 4u * (unsigned)px;
+                at_b(LLc, xb) = ll;
+                at_b(LHc, xb) = lh;
+                at_b(LHc + n, xb) = hh;
+                at_b(LHc + 2 * n, xb) = hl;
+            }
         }
         bool small = true;
         if (FAM != FAM_U8_RGB)   // (RGB24 samples give coefficients below 64)
@@ -440,6 +488,7 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
             vpipe_step(ph[c], g0, g1, he4, ho3);
             if (out) emit(c, py, le4 * WEMA_KINV, lo3 * WEMA_K, he4 * WEMA_KINV, ho3 * WEMA_K);
         }
+        if (out && STAGE) flush(py);
     };
 
     if (h < 2) {
@@ -466,6 +515,7 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
             }
             emit(c, 0, l0, l0, g0, g0);
         }
+        if (STAGE) flush(0);
     } else {
         // input pairs py0 - 2 + m for m < M; output pair rows py0 .. py0 + nout - 1
         constexpr int SLOTS = SL;
@@ -529,14 +579,14 @@ __device__ __forceinline__ void dwt_fwd_strip_t(const DwtfJob &J, int strip, int
     }
 }
 
-template <int FAM, int NCH>
-__device__ __forceinline__ void dwt_fwd_strip(const DwtfJob &J, int strip, int lane) {
+template <int FAM, int NCH, bool STAGE>
+__device__ __forceinline__ void dwt_fwd_strip(const DwtfJob &J, int strip, int lane, float *stg) {
     const int w = J.w, h = J.h, sh = (h + 1) >> 1;
     const int sx0 = 2 * ((strip % J.tiles_x) * WEMA_TILE_PAIRS_X - 2);
     const int py0 = (strip / J.tiles_x) * J.sp;
     if constexpr (FAM == FAM_F32_RGB) {
         // (float RGB frames: the general loads only)
-        dwt_fwd_strip_t<FAM, NCH, false, false>(J, strip, lane);
+        dwt_fwd_strip_t<FAM, NCH, false, false, STAGE>(J, strip, lane, stg);
     } else if constexpr (FAM == FAM_F32_PLANE) {
         // every lane's pair adjacent and inside the row, sources below 2^29 samples (32-bit
         // row offsets); strips that reach the unwritten last column or row of an odd-sized
@@ -545,10 +595,10 @@ __device__ __forceinline__ void dwt_fwd_strip(const DwtfJob &J, int strip, int l
         const bool stale = px0 + WEMA_TILE_PAIRS_X > (w >> 1) || py0 + J.sp > (h >> 1);
         const bool fast = sx0 >= 0 && sx0 + 128 <= w && h >= 8 && (long long)w * h < (1ll << 29) &&
                           min(J.sp, sh - py0) >= (NCH == 1 && J.sp == 4 ? 4 : 2 * SLOTS_OF(NCH));
-        if (stale) dwt_fwd_strip_t<FAM, NCH, false, true>(J, strip, lane);
-        else if (fast && NCH == 1 && J.sp == 4) dwt_fwd_strip_t<FAM, NCH, true, false, 2>(J, strip, lane);
-        else if (fast) dwt_fwd_strip_t<FAM, NCH, true, false>(J, strip, lane);
-        else dwt_fwd_strip_t<FAM, NCH, false, false>(J, strip, lane);
+        if (stale) dwt_fwd_strip_t<FAM, NCH, false, true, STAGE>(J, strip, lane, stg);
+        else if (fast && NCH == 1 && J.sp == 4) dwt_fwd_strip_t<FAM, NCH, true, false, STAGE, 2>(J, strip, lane, stg);
+        else if (fast) dwt_fwd_strip_t<FAM, NCH, true, false, STAGE>(J, strip, lane, stg);
+        else dwt_fwd_strip_t<FAM, NCH, false, false, STAGE>(J, strip, lane, stg);
     } else {
         // RGB24 as above, and rows that start on a dword (frame aligned, width a multiple
         // of 4); the 3-dword loads end at most 6 bytes past the pair, so the strip must not
@@ -557,9 +607,9 @@ __device__ __forceinline__ void dwt_fwd_strip(const DwtfJob &J, int strip, int l
         const bool fast = sx0 >= 0 && sx0 + 128 <= w && h >= 8 && (long long)w * h < (1ll << 29) &&
                           min(J.sp, sh - py0) >= (NCH == 1 && J.sp == 4 ? 4 : 2 * SLOTS_OF(NCH)) &&
                           (((size_t)J.src | (size_t)w) & 3) == 0 && (sx0 + 132 <= w || last_row < h - 1);
-        if (fast && NCH == 1 && J.sp == 4) dwt_fwd_strip_t<FAM, NCH, true, false, 2>(J, strip, lane);
-        else if (fast) dwt_fwd_strip_t<FAM, NCH, true, false>(J, strip, lane);
-        else dwt_fwd_strip_t<FAM, NCH, false, false>(J, strip, lane);
+        if (fast && NCH == 1 && J.sp == 4) dwt_fwd_strip_t<FAM, NCH, true, false, STAGE, 2>(J, strip, lane, stg);
+        else if (fast) dwt_fwd_strip_t<FAM, NCH, true, false, STAGE>(J, strip, lane, stg);
+        else dwt_fwd_strip_t<FAM, NCH, false, false, STAGE>(J, strip, lane, stg);
     }
 }
 
@@ -574,11 +624,24 @@ __device__ __forceinline__ void dwt_fwd_strips(const DwtfJob *__restrict__ jobs,
     const int strip = (bid - J.tile_begin) * WEMA_STRIPS_PER_BLOCK + wave;
     if (strip >= J.tiles_x * wema_ceil_div((J.h + 1) >> 1, J.sp)) return;
     // (level-0 jobs take Y, or Y, Cb and Cr, from RGB24 or float RGB frames)
-    if (!LEVEL0) dwt_fwd_strip<FAM_F32_PLANE, 1>(J, strip, lane);
-    else if (J.src_kind == SRC_U8_Y && J.nch == 3) dwt_fwd_strip<FAM_U8_RGB, 3>(J, strip, lane);
-    else if (J.src_kind == SRC_U8_Y) dwt_fwd_strip<FAM_U8_RGB, 1>(J, strip, lane);
-    else if (J.nch == 3) dwt_fwd_strip<FAM_F32_RGB, 3>(J, strip, lane);
-    else dwt_fwd_strip<FAM_F32_RGB, 1>(J, strip, lane);
+    // (level 0: staged rows where the job's strip rows are whole workgroups)
+    __shared__ float s_stg[LEVEL0 ? 2 * 3 * 4 * WEMA_STAGE_W : 1];
+    const bool stage = LEVEL0 && WEMA_FWD_STAGE && (J.tiles_x & (WEMA_STRIPS_PER_BLOCK - 1)) == 0 &&
+                       gridDim.x >= WEMA_STAGE_MIN_BLOCKS;
+    if (!LEVEL0) dwt_fwd_strip<FAM_F32_PLANE, 1, false>(J, strip, lane, s_stg);
+    else if (J.src_kind == SRC_U8_Y && J.nch == 3) {
+        if (stage) dwt_fwd_strip<FAM_U8_RGB, 3, true>(J, strip, lane, s_stg);
+        else dwt_fwd_strip<FAM_U8_RGB, 3, false>(J, strip, lane, s_stg);
+    } else if (J.src_kind == SRC_U8_Y) {
+        if (stage) dwt_fwd_strip<FAM_U8_RGB, 1, true>(J, strip, lane, s_stg);
+        else dwt_fwd_strip<FAM_U8_RGB, 1, false>(J, strip, lane, s_stg);
+    } else if (J.nch == 3) {
+        if (stage) dwt_fwd_strip<FAM_F32_RGB, 3, true>(J, strip, lane, s_stg);
+        else dwt_fwd_strip<FAM_F32_RGB, 3, false>(J, strip, lane, s_stg);
+    } else {
+        if (stage) dwt_fwd_strip<FAM_F32_RGB, 1, true>(J, strip, lane, s_stg);
+        else dwt_fwd_strip<FAM_F32_RGB, 1, false>(J, strip, lane, s_stg);
+    }
 }
 
 __global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_fwd0(const DwtfJob *__restrict__ jobs, int njobs) {
