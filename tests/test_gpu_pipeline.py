@@ -657,3 +657,37 @@ def test_strip_rows_do_not_change_results(w, h, levels, color, batch):
         sess.sync()
         assert np.array_equal(d_out.cpu().numpy(), ref), rows
         sess.close()
+
+
+@pytest.mark.parametrize("w,h,color", [(480, 272, True), (472, 271, True), (480, 272, False), (472, 271, False)])
+def test_staged_rows_bitexact_vs_oracle(w, h, color):
+    """Level-0 launches of 2,048 workgroups or more whose strip rows are whole workgroups
+    (the strips per row a multiple of 4) stage their output rows through LDS. Here every
+    level-0 strip is 4 pair rows tall and a call holds 64 frames: ceil(w / 2 / 60) = 4 strips
+    per row and ceil(ceil(h / 2) / 4) = 34 strip rows give 34 workgroups per frame and 2,176
+    per launch, forward and inverse. 472 leaves the workgroup's last 4 pair columns outside
+    the frame and 271 the last pair's second row; two calls carry the temporal state."""
+    fps, batch = 30.0, 64
+    n = 2 * batch
+    kw = dict(fps=fps, amp=100.0, f_low=0.5, f_high=3.0, levels=4, color=color)
+    strips = wema_amd_tiles(w) * (((h + 1) // 2 + 3) // 4)   # per frame, 4 pair rows each
+    assert wema_amd_tiles(w) % 4 == 0 and strips // 4 * batch >= 2048
+    rng = np.random.default_rng(w + h)
+    frames = rng.integers(0, 256, size=(n, h, w, 3), dtype=np.uint8)
+    ref = oracle_clip(frames, **kw)
+    d_in = torch.from_numpy(frames).cuda()
+    sess = wema_amd.Session(w, h, max_batch=batch, **kw)
+    sess.set_strip_rows(4)
+    d_out = torch.empty_like(d_in)
+    for t0 in range(0, n, batch):
+        sess.process_device(d_in[t0:t0 + batch], d_out_u8=d_out[t0:t0 + batch], n=batch)
+    sess.sync()
+    out = d_out.cpu().numpy()
+    bad = np.flatnonzero((out != ref).reshape(n, -1).any(axis=1))
+    assert bad.size == 0, (bad[:4], max_diff(out, ref))
+    sess.close()
+
+
+def wema_amd_tiles(w):
+    """Strips per row of a level-0 launch: 60 pair columns each."""
+    return ((w + 1) // 2 + 59) // 60

@@ -227,6 +227,9 @@ template <> struct StripRaw<FAM_F32_PLANE> { float a, b; };
 #endif
 // ... in launches of at least this many workgroups (8 per CU): with fewer, the barrier per
 // row leaves the SIMDs idle (one frame per call: 1080p colour 133 -> 136 us, 4K 207 -> 210 us)
+#ifndef WEMA_INV_STAGE
+#define WEMA_INV_STAGE 1
+#endif
 #ifndef WEMA_STAGE_MIN_BLOCKS
 #define WEMA_STAGE_MIN_BLOCKS 2048
 #endif
@@ -709,8 +712,8 @@ struct IStripRaw {
 
 // OUT: 0 = one float plane (J.out), 1 = colour egress (RGB24 and/or float RGB). SL: pairs in
 // flight (the slot count of NCH; 2 for luma strips of 4 pair rows).
-template <int NCH, int OUT, bool FAST, int SL = ISLOTS_OF(NCH)>
-__device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int lane) {
+template <int NCH, int OUT, bool FAST, bool STAGE, int SL = ISLOTS_OF(NCH)>
+__device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int lane, float *stg) {
     const int SP = J.sp;
     const int w = J.w, h = J.h;
     const int sw = (w + 1) >> 1, sh = (h + 1) >> 1;
@@ -752,6 +755,45 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
         return r;
     };
 
+    // Staged output rows (level 0 of large launches, the workgroup's 4 strips side by side
+    // in one strip row): the waves put the pixel pairs of a step's two image rows into LDS
+    // (float planes: 2 floats per pair; RGB24: 6 bytes, as an aligned dword and short), and
+    // after a barrier the workgroup stores its 480 pixels of each row as contiguous rows.
+    // Two buffers alternate.
+    int sbuf = 0;
+    const int pxb = (px0 / WEMA_TILE_PAIRS_X - (int)(threadIdx.x >> 6)) * WEMA_TILE_PAIRS_X;
+    auto flush = [&](int y0, int nrows) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        const int t = (int)threadIdx.x;
+        if constexpr (OUT == 0) {
+            const int gx = pxb + t;
+            if (t < WEMA_STAGE_W && 2 * gx < w) {
+                for (int r = 0; r < nrows; ++r) {
+                    if (y0 + r >= h) break;
+                    const v2f v = ((const v2f *)stg)[(sbuf * 2 + r) * WEMA_STAGE_W + t];
+                    at_b((GLOBAL_AS v2f *)(gp(J.out) + (size_t)(y0 + r) * w), 8u * (unsigned)gx) = v;
+                }
+            }
+        } else {
+            const int nb = 6 * min(WEMA_STAGE_W, sw - pxb);   // bytes of the workgroup's part of a row
+            const unsigned *sd = (const unsigned *)stg;
+            for (int r = 0; r < nrows; ++r) {
+                if (y0 + r >= h) break;
+                GLOBAL_AS unsigned char *rb = gp(J.out_u8) + ((size_t)(y0 + r) * w + 2 * pxb) * 3;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int d = t + 256 * k;
+                    const unsigned v = sd[(sbuf * 2 + r) * (6 * WEMA_STAGE_W / 4) + min(d, 6 * WEMA_STAGE_W / 4 - 1)];
+                    if (4 * d + 4 <= nb) at_b((GLOBAL_AS unsigned *)rb, 4u * (unsigned)d) = v;
+                    else if (4 * d < nb) at_b((GLOBAL_AS unsigned short *)rb, 4u * (unsigned)d) = (unsigned short)v;
+                }
+            }
+        }
+        sbuf ^= 1;
+    };
+
     // image row y (uniform) of every channel: row synthesis and store
     auto put_row = [&](int y, const float (&lo)[NCH], const float (&hi)[NCH]) {
         float ev[NCH], ov[NCH];
@@ -764,6 +806,30 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
                 ov[c] *= WEMA_KINV;
                 lift_inv_lanes(ev[c], ov[c]);
             }
+        }
+        if constexpr (STAGE) {
+            if (lane < 2 || lane >= 62) return;
+            const int k = (int)(threadIdx.x >> 6) * WEMA_TILE_PAIRS_X + lane - 2;   // pair in the workgroup's row
+            const int r = y & 1;
+            if constexpr (OUT == 0) {
+                ((v2f *)stg)[(sbuf * 2 + r) * WEMA_STAGE_W + k] = v2f{ev[0], ov[0]};
+            } else {
+                float v0[3], v1[3];
+                ycc_px(ev[0], ev[1], ev[2], v0);
+                ycc_px(ov[0], ov[1], ov[2], v1);
+                const unsigned lo = (unsigned)egress_u8(v0[0]) | ((unsigned)egress_u8(v0[1]) << 8) |
+                                    ((unsigned)egress_u8(v0[2]) << 16) | ((unsigned)egress_u8(v1[0]) << 24);
+                const unsigned hi = (unsigned)egress_u8(v1[1]) | ((unsigned)egress_u8(v1[2]) << 8);
+                unsigned char *sb = (unsigned char *)stg + (sbuf * 2 + r) * (6 * WEMA_STAGE_W) + 6 * k;
+                if ((k & 1) == 0) {   // (6 k on a dword)
+                    *(unsigned *)sb = lo;
+                    *(unsigned short *)(sb + 4) = (unsigned short)hi;
+                } else {
+                    *(unsigned short *)sb = (unsigned short)lo;
+                    *(unsigned *)(sb + 2) = (lo >> 16) | (hi << 16);
+                }
+            }
+            return;
         }
         if (!lane_ok || y >= h) return;
         if constexpr (OUT == 0) {
@@ -835,6 +901,7 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
             const int y = 2 * (py0 + m - 4);
             put_row(y, le, he);
             put_row(y + 1, lo, ho);
+            if (STAGE) flush(y, 2);
         }
     };
 
@@ -843,6 +910,7 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
         // image row 0 is the even sample
         const IStripRaw<NCH> r = load(0);
         put_row(0, r.ll, r.hl);
+        if (STAGE) flush(0, 1);
         return;
     }
     // coefficient pairs py0 - 2 + m for m < M; image rows of pairs py0 .. py0 + nout - 1.
@@ -882,7 +950,7 @@ __device__ __forceinline__ void dwt_inv_strip_t(const DwtiJob &J, int strip, int
     }
 }
 
-template <int NCH, int OUT>
+template <int NCH, int OUT, bool LEVEL0>
 __device__ __forceinline__ void dwt_inv_strips(const DwtiJob *__restrict__ jobs, int njobs) {
     const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int jid = find_job(jobs, njobs, bid);
@@ -900,19 +968,32 @@ __device__ __forceinline__ void dwt_inv_strips(const DwtiJob *__restrict__ jobs,
     const bool fast = px0 >= 2 && 2 * (px0 + 61) + 1 < w && h >= 8 &&
                       min(J.sp, sh - py0) >= (NCH == 1 && J.sp == 4 ? 4 : 2 * ISLOTS_OF(NCH)) &&
                       (long long)sw * sh < (1ll << 28);
-    if (fast && NCH == 1 && J.sp == 4) dwt_inv_strip_t<NCH, OUT, true, 2>(J, strip, lane);
-    else if (fast) dwt_inv_strip_t<NCH, OUT, true>(J, strip, lane);
-    else dwt_inv_strip_t<NCH, OUT, false>(J, strip, lane);
+    // (level 0: staged rows where the job's strip rows are whole workgroups; RGB24 rows on
+    // dwords, float rows of even width)
+    __shared__ float s_stg[LEVEL0 ? 2 * 2 * 2 * WEMA_STAGE_W : 1];
+    const bool stage = LEVEL0 && WEMA_INV_STAGE && (J.tiles_x & (WEMA_STRIPS_PER_BLOCK - 1)) == 0 &&
+                       gridDim.x >= WEMA_STAGE_MIN_BLOCKS &&
+                       (OUT == 0 ? (w & 1) == 0
+                                 : J.out_u8 && !J.out_f32 && ((w | (int)(size_t)J.out_u8) & 3) == 0);
+    if (stage) {
+        if (fast && NCH == 1 && J.sp == 4) dwt_inv_strip_t<NCH, OUT, true, true, 2>(J, strip, lane, s_stg);
+        else if (fast) dwt_inv_strip_t<NCH, OUT, true, true>(J, strip, lane, s_stg);
+        else dwt_inv_strip_t<NCH, OUT, false, true>(J, strip, lane, s_stg);
+    } else {
+        if (fast && NCH == 1 && J.sp == 4) dwt_inv_strip_t<NCH, OUT, true, false, 2>(J, strip, lane, s_stg);
+        else if (fast) dwt_inv_strip_t<NCH, OUT, true, false>(J, strip, lane, s_stg);
+        else dwt_inv_strip_t<NCH, OUT, false, false>(J, strip, lane, s_stg);
+    }
 }
 
 __global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv0(const DwtiJob *__restrict__ jobs, int njobs) {
-    dwt_inv_strips<1, 0>(jobs, njobs);
+    dwt_inv_strips<1, 0, true>(jobs, njobs);
 }
 __global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv(const DwtiJob *__restrict__ jobs, int njobs) {
-    dwt_inv_strips<1, 0>(jobs, njobs);
+    dwt_inv_strips<1, 0, false>(jobs, njobs);
 }
 __global__ void __launch_bounds__(64 * WEMA_STRIPS_PER_BLOCK) k_dwt_inv0_ycc(const DwtiJob *__restrict__ jobs, int njobs) {
-    dwt_inv_strips<3, 1>(jobs, njobs);
+    dwt_inv_strips<3, 1, true>(jobs, njobs);
 }
 
 // ---------------------------------------------------------------------------------------
