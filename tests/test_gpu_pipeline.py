@@ -659,14 +659,16 @@ def test_strip_rows_do_not_change_results(w, h, levels, color, batch):
         sess.close()
 
 
-@pytest.mark.parametrize("w,h,color", [(480, 272, True), (472, 271, True), (480, 272, False), (472, 271, False)])
+@pytest.mark.parametrize("w,h,color", [(480, 272, True), (472, 271, True), (480, 272, False), (472, 271, False),
+                                         (960, 544, True), (942, 541, False)])
 def test_staged_rows_bitexact_vs_oracle(w, h, color):
-    """Level-0 launches of 2,048 workgroups or more whose strip rows are whole workgroups
-    (the strips per row a multiple of 4) stage their output rows through LDS. Here every
-    level-0 strip is 4 pair rows tall and a call holds 64 frames: ceil(w / 2 / 60) = 4 strips
-    per row and ceil(ceil(h / 2) / 4) = 34 strip rows give 34 workgroups per frame and 2,176
-    per launch, forward and inverse. 472 leaves the workgroup's last 4 pair columns outside
-    the frame and 271 the last pair's second row; two calls carry the temporal state."""
+    """Level-0 strip launches of 2,048 workgroups or more whose strip rows are whole
+    workgroups (the strips per row a multiple of 4) stage their output rows through LDS.
+    Here every strip is 4 pair rows tall and a call holds 64 frames: at 480 x 272,
+    ceil(w / 2 / 60) = 4 strips per row and ceil(ceil(h / 2) / 4) = 34 strip rows give 34
+    workgroups per frame and 2,176 per launch, forward and inverse (8,704 at 960 x 544).
+    472 and 942 leave the last workgroup's last pair columns outside the frame, 271 and 541
+    the last pair's second row; two calls carry the temporal state."""
     fps, batch = 30.0, 64
     n = 2 * batch
     kw = dict(fps=fps, amp=100.0, f_low=0.5, f_high=3.0, levels=4, color=color)

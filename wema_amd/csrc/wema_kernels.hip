@@ -220,7 +220,7 @@ template <> struct StripRaw<FAM_F32_PLANE> { float a, b; };
 #define WEMA_FSLOTS_COLOUR 2
 #endif
 #define SLOTS_OF(NCH) ((NCH) == 1 ? 4 : WEMA_FSLOTS_COLOUR)
-// Pair columns of a workgroup's staged output rows (level-0 forward).
+// Pair columns of a workgroup's staged output rows (level-0 DWT strips).
 #define WEMA_STAGE_W (WEMA_STRIPS_PER_BLOCK * WEMA_TILE_PAIRS_X)
 #ifndef WEMA_FWD_STAGE
 #define WEMA_FWD_STAGE 1
